@@ -1,0 +1,107 @@
+"""Initial-data recipes (numpy, host side) for the patch workloads.
+
+These restate the reference's initial-data thorns as input generators:
+  balsara()        AsterSeeds/src/1D_tests.cxx:165-481 (Balsara 1-5 shock
+                   tubes, vector potential of :384-478)
+  post_initial()   AsterSeeds/src/SetTempEntropyYe.cxx (Ye, T, entropy)
+  minkowski()      ADMBaseX "Cartesian Minkowski"
+  kerr_schild(), fm_torus()  in torus.py
+They produce inputs only; nothing here is on the measured path.
+"""
+import math
+
+import numpy as np
+
+BALSARA = {
+    # rho, vx, vy, vz, p, Bx, By, Bz  (left, right)
+    "Balsara1": ((1.0, 0.0, 0.0, 0.0, 1.0, 0.5, 1.0, 0.0),
+                 (0.125, 0.0, 0.0, 0.0, 0.1, 0.5, -1.0, 0.0)),
+    "Balsara2": ((1.0, 0.0, 0.0, 0.0, 30.0, 5.0, 6.0, 6.0),
+                 (1.0, 0.0, 0.0, 0.0, 1.0, 5.0, 0.7, 0.7)),
+    "Balsara3": ((1.0, 0.0, 0.0, 0.0, 1000.0, 10.0, 7.0, 7.0),
+                 (1.0, 0.0, 0.0, 0.0, 0.1, 10.0, 0.7, 0.7)),
+    "Balsara4": ((1.0, 0.999, 0.0, 0.0, 0.1, 10.0, 7.0, 7.0),
+                 (1.0, -0.999, 0.0, 0.0, 0.1, 10.0, -7.0, -7.0)),
+    "Balsara5": ((1.08, 0.4, 0.3, 0.2, 0.95, 2.0, 0.3, 0.3),
+                 (1.0, -0.45, -0.2, 0.2, 1.0, 2.0, -0.7, 0.5)),
+}
+
+
+def _rot(th_x, th_y, th_z, seq):
+    """Rotation of 1D_tests.cxx:311-331: R = Rs[seq2] Rs[seq1] Rs[seq0]."""
+    cx, sx = math.cos(th_x), math.sin(th_x)
+    cy, sy = math.cos(th_y), math.sin(th_y)
+    cz, sz = math.cos(th_z), math.sin(th_z)
+    Rx = np.array([[1, 0, 0], [0, cx, sx], [0, -sx, cx]])
+    Ry = np.array([[cy, 0, -sy], [0, 1, 0], [sy, 0, cy]])
+    Rz = np.array([[cz, sz, 0], [-sz, cz, 0], [0, 0, 1]])
+    Rs = [Rx, Ry, Rz]
+    return Rs[seq[2]] @ Rs[seq[1]] @ Rs[seq[0]]
+
+
+def minkowski(patch):
+    for name in ("alp", "gxx", "gyy", "gzz"):
+        patch[name] = 1.0
+    for name in ("betax", "betay", "betaz", "gxy", "gxz", "gyz",
+                 "kxx", "kxy", "kxz", "kyy", "kyz", "kzz"):
+        patch[name] = 0.0
+
+
+def balsara(patch, case="Balsara1", shock_dir="x", gamma=2.0, perturb=None,
+            seed=100):
+    """Balsara shock tube on every point of the patch (ghosts included).
+
+    perturb=a multiplies every cell's rho and p by (1 + a*u), u ~ U[-1, 1]
+    from numpy's PCG64 seeded with `seed`, before eps is computed.
+    """
+    L, R = BALSARA[case]
+    th = {"x": (0.0, 0.0, 0.0), "y": (0.5 * math.pi, 0.0, 0.5 * math.pi),
+          "z": (0.0, -0.5 * math.pi, -0.5 * math.pi)}[shock_dir]
+    Rm = _rot(th[0], th[1], th[2], [2, 1, 0])
+    Rinv = _rot(-th[0], -th[1], -th[2], [0, 1, 2])
+    Bl, Br = Rinv @ np.array(L[5:8]), Rinv @ np.array(R[5:8])
+    vl, vr = Rinv @ np.array(L[1:4]), Rinv @ np.array(R[1:4])
+
+    def newx(z, y, x):
+        return Rm[0, 0] * x + Rm[0, 1] * y + Rm[0, 2] * z
+
+    z, y, x = patch.coords(0)
+    left = np.broadcast_to(newx(z, y, x) <= 0.0, patch.shape("rho"))
+    rho = np.where(left, L[0], R[0])
+    press = np.where(left, L[4], R[4])
+    if perturb:
+        rng = np.random.default_rng(seed)
+        u1 = rng.uniform(-1.0, 1.0, size=rho.shape)
+        u2 = rng.uniform(-1.0, 1.0, size=rho.shape)
+        rho = rho * (1.0 + perturb * u1)
+        press = press * (1.0 + perturb * u2)
+    patch["rho"] = rho
+    patch["press"] = press
+    for i, name in enumerate(("velx", "vely", "velz")):
+        patch[name] = np.where(left, vl[i], vr[i])
+    patch["eps"] = press / (rho * (gamma - 1.0))
+    for i, B in enumerate(("Bvecx", "Bvecy", "Bvecz")):
+        patch[B] = np.where(left, Bl[i], Br[i])
+
+    # 1D_tests.cxx:392-426 (rotation angles zero): A_x = B_y z, A_y = B_z x,
+    # A_z = B_x y, the side chosen by the rotated x of the A point.
+    def avec(name, comp, coord_axis):
+        z, y, x = patch.coords({"Avec_x": 6, "Avec_y": 5, "Avec_z": 3}[name])
+        lft = newx(z, y, x) <= 0.0
+        c = {0: x, 1: y, 2: z}[coord_axis]
+        patch[name] = np.broadcast_to(np.where(lft, Bl[comp], Br[comp]) * c,
+                                      patch.shape(name))
+
+    avec("Avec_x", 1, 2)
+    avec("Avec_y", 2, 0)
+    avec("Avec_z", 0, 1)
+
+
+def post_initial(patch, gamma, particle_mass=1.0, Ye_atmo=0.5):
+    """SetYe / SetTemp / SetEntropy (AsterSeeds/src/SetTempEntropyYe.cxx),
+    ideal gas: T = (gamma-1) m eps, entropy = kappa = (gamma-1) eps rho^(1-gamma)."""
+    patch["Ye"] = Ye_atmo
+    rho = patch.numpy("rho")
+    eps = patch.numpy("eps")
+    patch["temperature"] = (gamma - 1) * particle_mass * eps
+    patch["entropy"] = (gamma - 1.0) * eps * np.power(rho, 1.0 - gamma)

@@ -1,0 +1,343 @@
+/*
+ * asterx.h -- C ABI of the MI355X-native GRMHD method-of-lines substep.
+ *
+ * One CarpetX-style patch: N[3] interior cells, ng ghost zones on every side.
+ * Every grid function is a caller-owned, dense, x-fastest FP64 array over its
+ * centering's full extent including ghosts (cell axis: N+2ng points, vertex
+ * axis: N+1+2ng points), the layout AMReX/CarpetX allocate for a box.  The
+ * index (i,j,k) of a point counts from the first ghost point, so interior cells
+ * are ng..ng+N-1 and interior vertices ng..ng+N on each axis.  Cell i spans
+ * [x0+(i-ng)dx, x0+(i-ng+1)dx]; vertex i sits at x0+(i-ng)dx.
+ *
+ * Entry points mirror the reference's scheduled functions one to one
+ * (names in the comment of each declaration, file:line in the reference
+ * tree), take plain pointers and sizes, return an asx_status and never abort.
+ * Numerical failures are data (con2prim flags, counters), as in the reference.
+ * All work is stream-ordered on the given HIP stream (NULL = default stream);
+ * entry points are re-entrant per stream and hold no mutable global state.
+ */
+#ifndef ASTERX_H
+#define ASTERX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status codes                                                             */
+/* ------------------------------------------------------------------------ */
+typedef enum {
+  ASX_OK = 0,
+  ASX_ERR_NULL = 1,        /* required pointer missing                      */
+  ASX_ERR_LAYOUT = 2,      /* strides / extents do not match the patch      */
+  ASX_ERR_PARAM = 3,       /* invalid parameter value (reference: CCTK_ERROR) */
+  ASX_ERR_UNSUPPORTED = 4, /* valid in the reference, not built here        */
+  ASX_ERR_GHOSTS = 5,      /* too few ghost zones for the stencil           */
+  ASX_ERR_DEVICE = 6,      /* HIP runtime error                             */
+  ASX_ERR_SIZE = 7         /* size out of range                             */
+} asx_status;
+
+/* ------------------------------------------------------------------------ */
+/* Keyword parameters (reference param.ccl keywords as enums)               */
+/* ------------------------------------------------------------------------ */
+/* ReconX/param.ccl:5-21 reconstruction_method / loworder_method */
+enum {
+  ASX_RECON_GODUNOV = 0,
+  ASX_RECON_MINMOD = 1,
+  ASX_RECON_MONOCENTRAL = 2,
+  ASX_RECON_PPM = 3,
+  ASX_RECON_EPPM = 4, /* accepted by the reference; not built here */
+  ASX_RECON_WENOZ = 5,
+  ASX_RECON_MP5 = 6
+};
+/* AsterX/param.ccl:22-26 */
+enum { ASX_FLUX_LXF = 0, ASX_FLUX_HLLE = 1 };
+/* AsterX/param.ccl:28-33 */
+enum { ASX_RECVAR_V = 0, ASX_RECVAR_Z = 1, ASX_RECVAR_S = 2 };
+/* AsterX/param.ccl:3-7 */
+enum { ASX_GAUGE_ALGEBRAIC = 0, ASX_GAUGE_GENERALIZED_LORENZ = 1 };
+/* EOSX/param.ccl:14-20 (Hybrid is not built here) */
+enum { ASX_EOS_IDEALGAS = 0, ASX_EOS_HYBRID = 1, ASX_EOS_TABULATED = 2 };
+/* Con2PrimFactory/param.ccl:8-22 (Entropy is not built here) */
+enum {
+  ASX_C2P_NONE = 0,
+  ASX_C2P_NOBLE = 1,
+  ASX_C2P_PALENZUELA = 2,
+  ASX_C2P_ENTROPY = 3
+};
+
+/* AsterX/src/con2prim.cxx:27-35 -- written to con2prim_flag as FP64 */
+enum {
+  ASX_C2P_FLAG_INIT = 0,
+  ASX_C2P_FLAG_PRIME = 1,
+  ASX_C2P_FLAG_SECOND = 2,
+  ASX_C2P_FLAG_ENTROPY = 3,
+  ASX_C2P_FLAG_ATMO = 4,
+  ASX_C2P_FLAG_AVG = 5,
+  ASX_C2P_FLAG_FAIL = 6
+};
+
+/*
+ * Parameters: same names, meaning and defaults as the reference's param.ccl
+ * files (AsterX/param.ccl, ReconX/param.ccl, EOSX/param.ccl,
+ * Con2PrimFactory/param.ccl).  asx_params_default() sets the defaults.
+ */
+typedef struct {
+  /* ReconX */
+  int32_t reconstruction_method;        /* "Godunov"       */
+  int32_t loworder_method;              /* "Godunov"       */
+  int32_t reconstruct_with_temperature; /* no              */
+  int32_t ppm_shock_detection;          /* yes             */
+  int32_t ppm_zone_flattening;          /* no              */
+  int32_t pad0;
+  double ppm_eta1, ppm_eta2;            /* 20, 0.05        */
+  double ppm_eps, ppm_eps_shock;        /* 0.33, 0.01      */
+  double ppm_small;                     /* 1e-7            */
+  double ppm_omega1, ppm_omega2;        /* 0.75, 10        */
+  double enhanced_ppm_C2;               /* 1.25            */
+  double weno_eps;                      /* 1e-26           */
+  double mp5_alpha;                     /* 4               */
+  /* AsterX */
+  int32_t vector_potential_gauge; /* "generalized Lorentz" */
+  int32_t use_uct;                /* no                    */
+  int32_t use_entropy_fix;        /* no                    */
+  int32_t flux_type;              /* "LxF"                 */
+  int32_t recon_type;             /* "v_vec"               */
+  int32_t local_spatial_order;    /* 2                     */
+  int32_t excise;                 /* no                    */
+  int32_t use_mask;               /* no                    */
+  double lorenz_damp_fac;         /* 0                     */
+  /* EOSX */
+  int32_t evolution_eos; /* "IdealGas" */
+  int32_t pad1;
+  double poly_gamma, poly_k;   /* 2, 100 (initial-data polytrope) */
+  double gl_gamma;             /* 2                              */
+  double rho_max, rho_min;     /* 1, 1e-40                        */
+  double eps_max, eps_min;     /* 2, 0                            */
+  double ye_max, ye_min;       /* 1, 0                            */
+  double particle_mass;        /* 1                               */
+  /* Con2PrimFactory */
+  int32_t c2p_prime;        /* "Noble"      */
+  int32_t c2p_second;       /* "Palenzuela" */
+  int32_t max_iter;         /* 30           */
+  int32_t use_press_atmo;   /* no           */
+  int32_t thermal_eos_atmo; /* yes          */
+  int32_t Ye_lenient;       /* no           */
+  int32_t use_z;            /* no           */
+  int32_t use_temperature;  /* no           */
+  double rho_abs_min;       /* 1e-13        */
+  double atmo_tol;          /* 1e-3         */
+  double eps_atmo, p_atmo;  /* 1e-10, 1e-10 */
+  double t_atmo;            /* 1e-2         */
+  double Ye_atmo;           /* 0.5          */
+  double B_lim, vw_lim;     /* 100, 10      */
+  double c2p_tol;           /* 1e-10        */
+  double r_atmo;            /* 1e100        */
+  double n_rho_atmo, n_press_atmo, n_temp_atmo; /* 0, 0, 0 */
+  double alp_thresh;        /* 0            */
+  double rho_BH, eps_BH, vwlim_BH; /* 1e100 each */
+  double cons_error_limit;  /* -1           */
+} asx_params;
+
+/* ------------------------------------------------------------------------ */
+/* Grid functions                                                           */
+/* ------------------------------------------------------------------------ */
+/* One grid function: x-fastest, element strides for j and k (stride of i is
+ * 1).  The strides must equal the dense strides of the grid function's
+ * centering on the patch (checked; ASX_ERR_LAYOUT otherwise). */
+typedef struct {
+  double *data;
+  int64_t sj, sk;
+} asx_gf;
+
+/* Grid-function slots.  Names and groups follow AsterX/interface.ccl:16-165,
+ * HydroBaseX and ADMBaseX.  The centering of each slot is given by
+ * asx_gf_centering(). */
+enum {
+  /* ADMBaseX, vertex {vvv} */
+  ASX_ALP = 0, ASX_BETAX, ASX_BETAY, ASX_BETAZ,
+  ASX_GXX, ASX_GXY, ASX_GXZ, ASX_GYY, ASX_GYZ, ASX_GZZ,
+  ASX_KXX, ASX_KXY, ASX_KXZ, ASX_KYY, ASX_KYZ, ASX_KZZ,
+  /* HydroBaseX, cell {ccc} */
+  ASX_RHO, ASX_VELX, ASX_VELY, ASX_VELZ, ASX_EPS, ASX_PRESS,
+  ASX_TEMPERATURE, ASX_ENTROPY, ASX_YE, ASX_BVECX, ASX_BVECY, ASX_BVECZ,
+  /* AsterX conserved state and densitized B, cell */
+  ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ, ASX_TAU, ASX_DENT, ASX_DYE,
+  ASX_DBX, ASX_DBY, ASX_DBZ,
+  /* saved_prims, cell */
+  ASX_SAVED_RHO, ASX_SAVED_VELX, ASX_SAVED_VELY, ASX_SAVED_VELZ,
+  ASX_SAVED_EPS, ASX_SAVED_YE,
+  /* zvec, svec, cell */
+  ASX_ZVEC_X, ASX_ZVEC_Y, ASX_ZVEC_Z, ASX_SVEC_X, ASX_SVEC_Y, ASX_SVEC_Z,
+  /* flags and masks */
+  ASX_CON2PRIM_FLAG, /* cell   */
+  ASX_MASK_CC,       /* cell   (AsterMasks::aster_mask_cc) */
+  ASX_MASK_VC,       /* vertex (AsterMasks::aster_mask_vc) */
+  /* staggered densitized B: {vcc}, {cvc}, {ccv} */
+  ASX_DBX_STAG, ASX_DBY_STAG, ASX_DBZ_STAG,
+  /* vector potential: {cvv}, {vcv}, {vvc}; Psi {vvv} */
+  ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI,
+  /* face fluxes flux_x {vcc}: order as AsterX/interface.ccl:39-47 */
+  ASX_FXDENS, ASX_FXMOMX, ASX_FXMOMY, ASX_FXMOMZ, ASX_FXTAU, ASX_FXDYE,
+  ASX_FXDENT, ASX_FXBX, ASX_FXBY, ASX_FXBZ,
+  /* flux_y {cvc} */
+  ASX_FYDENS, ASX_FYMOMX, ASX_FYMOMY, ASX_FYMOMZ, ASX_FYTAU, ASX_FYDYE,
+  ASX_FYDENT, ASX_FYBX, ASX_FYBY, ASX_FYBZ,
+  /* flux_z {ccv} */
+  ASX_FZDENS, ASX_FZMOMX, ASX_FZMOMY, ASX_FZMOMZ, ASX_FZTAU, ASX_FZDYE,
+  ASX_FZDENT, ASX_FZBX, ASX_FZBY, ASX_FZBZ,
+  /* upwind-CT face data (AsterX/interface.ccl:137-165) */
+  ASX_VTILDE_Y_XFACE, ASX_VTILDE_Z_XFACE, /* {vcc} */
+  ASX_VTILDE_X_YFACE, ASX_VTILDE_Z_YFACE, /* {cvc} */
+  ASX_VTILDE_X_ZFACE, ASX_VTILDE_Y_ZFACE, /* {ccv} */
+  ASX_AMAX_XFACE, ASX_AMIN_XFACE,         /* {vcc} */
+  ASX_AMAX_YFACE, ASX_AMIN_YFACE,         /* {cvc} */
+  ASX_AMAX_ZFACE, ASX_AMIN_ZFACE,         /* {ccv} */
+  /* Aux_in_RHSof_A_Psi, vertex */
+  ASX_FX, ASX_FY, ASX_FZ, ASX_FBETAX, ASX_FBETAY, ASX_FBETAZ, ASX_G,
+  /* right-hand sides */
+  ASX_DENSRHS, ASX_MOMXRHS, ASX_MOMYRHS, ASX_MOMZRHS, ASX_TAURHS,
+  ASX_DENTRHS, ASX_DYE_RHS,                       /* cell            */
+  ASX_AVEC_X_RHS, ASX_AVEC_Y_RHS, ASX_AVEC_Z_RHS, /* {cvv},{vcv},{vvc} */
+  ASX_PSI_RHS,                                    /* vertex          */
+  ASX_NUM_GF
+};
+
+/* A patch: geometry plus one asx_gf per slot.  Unused slots may be NULL;
+ * each entry point lists the slots it reads and writes. */
+typedef struct {
+  int32_t n[3];  /* interior cells per axis                       */
+  int32_t ng;    /* ghost zones per side (same on every axis)     */
+  double x0[3];  /* coordinate of interior vertex ng on each axis */
+  double dx[3];  /* cell widths                                    */
+  asx_gf gf[ASX_NUM_GF];
+} asx_patch;
+
+/* The evolved state (the groups tagged rhs= in AsterX/interface.ccl:16-102),
+ * in this order: dens, momx, momy, momz, tau, DEnt, DYe, Avec_x, Avec_y,
+ * Avec_z, Psi.  Used for RK combinations and the fused substep. */
+enum { ASX_NUM_STATE = 11 };
+typedef struct {
+  asx_gf gf[ASX_NUM_STATE];
+} asx_state;
+
+/* Optional counters filled by the entry points that take them (NULL = skip).
+ * They replace the reference's device asserts / printf diagnostics
+ * (AsterX/src/fluxes.cxx:680-770, AsterX/src/rhs.cxx:199-206). */
+typedef struct {
+  int64_t nan_faces;         /* faces with NaN state/flux or negative rho, p  */
+  int64_t lo_faces;          /* face sides that fell back to low order        */
+  int64_t c2p_flags[7];      /* histogram of con2prim_flag over interior cells */
+  int64_t c2p_iters[64];     /* histogram of primary-solver iterations (0..63, last bin = overflow) */
+} asx_counters;
+
+/* ------------------------------------------------------------------------ */
+/* Setup                                                                    */
+/* ------------------------------------------------------------------------ */
+/* Defaults of every param.ccl entry used by the path. */
+void asx_params_default(asx_params *p);
+/* Centering of a slot: bit d set = vertex-centred along axis d. */
+int asx_gf_centering(int slot);
+/* Dense strides (sj, sk) of a slot on a patch with n[3], ng. */
+void asx_gf_strides(const int32_t n[3], int32_t ng, int slot, int64_t *sj,
+                    int64_t *sk);
+/* Library version string. */
+const char *asx_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Scheduled functions (drop-in for the reference's extern "C" entry points) */
+/* ------------------------------------------------------------------------ */
+/* AsterX_Fluxes (AsterX/src/fluxes.cxx:828-878): CalcFlux<0,1,2>
+ * (fluxes.cxx:38-796) + CalcAuxForAvecPsi (fluxes.cxx:798-826).
+ * Reads prims, dB*_stag, ADM lapse/shift/metric, Avec, Psi.
+ * Writes flux_x/y/z, UCT face data (when use_uct) and Aux on interior points.
+ * face_halo > 0 additionally computes faces and aux vertices that many layers
+ * into the ghost region (every axis), which is what a neighbour patch's SYNC
+ * delivers there; face_halo = 0 is the reference's interior-only write. */
+asx_status asx_fluxes(const asx_patch *patch, const asx_params *par,
+                      int32_t face_halo, asx_counters *cnt, void *stream);
+
+/* AsterX_SourceTerms (AsterX/src/source.cxx:170-185, SourceTerms<2|4>
+ * source.cxx:20-168). Writes densrhs..DYe_rhs on interior cells. */
+asx_status asx_source_terms(const asx_patch *patch, const asx_params *par,
+                            void *stream);
+
+/* AsterX_RHS (AsterX/src/rhs.cxx:18-253): adds the flux divergence to the
+ * hydro RHS, writes Avec_*_rhs (flux-CT or UCT EMF + gauge) and Psi_rhs. */
+asx_status asx_rhs(const asx_patch *patch, const asx_params *par, void *stream);
+
+/* AsterX_ComputedBstagFromA (AsterX/src/computeBfromA.cxx:58-65). */
+asx_status asx_bstag_from_A(const asx_patch *patch, void *stream);
+/* AsterX_ComputedBFromdBstag (AsterX/src/computeBfromA.cxx:67-85). */
+asx_status asx_dB_from_bstag(const asx_patch *patch, void *stream);
+/* AsterX_ComputeBFromdB (AsterX/src/computeBfromA.cxx:87-113). */
+asx_status asx_B_from_dB(const asx_patch *patch, void *stream);
+/* AsterX_Prim2Con_Initial (AsterX/src/prim2con.cxx:12-52). */
+asx_status asx_prim2con(const asx_patch *patch, void *stream);
+
+/* AsterX_Con2Prim (AsterX/src/con2prim.cxx:457-502 -> :37-455) for the
+ * ideal-gas EOS: atmosphere, seeds, cons floors, Noble / Palenzuela chain,
+ * flags, write-back of prims, zvec, svec, cons, saved_prims, mask. */
+asx_status asx_con2prim(const asx_patch *patch, const asx_params *par,
+                        asx_counters *cnt, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Driver pieces the reference takes from CarpetX (not vendored there)      */
+/* ------------------------------------------------------------------------ */
+/* ODESolvers state update: dst = scale*dst + sum_k fac[k]*src[k] on every
+ * point of every state grid function (ghosts included). nsrc <= 8. */
+asx_status asx_state_lincomb(const asx_patch *patch, const asx_state *dst,
+                             double scale, int32_t nsrc, const double *fac,
+                             const asx_state *const *src, void *stream);
+
+/* Single-patch physical boundary fill of one grid function's ghost zones
+ * (CarpetX "linear extrapolation": ghost = nearest interior point plus
+ * distance times the difference of the two nearest interior points; or
+ * periodic). kind: 0 = linear extrapolation, 1 = periodic, 2 = copy
+ * (zeroth order). */
+enum { ASX_BC_LINEAR = 0, ASX_BC_PERIODIC = 1, ASX_BC_COPY = 2 };
+asx_status asx_bc_fill(const asx_patch *patch, int32_t slot, int32_t kind,
+                       void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Fused substep (the performance path; no reference counterpart as one    */
+/* call: it is AsterX_Fluxes + AsterX_SourceTerms + AsterX_RHS + the        */
+/* ODESolvers update + AsterX_Con2PrimGroup on one patch)                   */
+/* ------------------------------------------------------------------------ */
+/* Stencil half of a substep.  Requires valid ghost data for prims, Avec and
+ * Psi on `stage` (as after a SYNC); dB*_stag is recomputed from Avec exactly
+ * as AsterX_ComputedBstagFromA does, so its ghosts are not needed.  Computes
+ * out = c0*u0 + c1*u + c2*rhs(u) on interior points, where u is the state
+ * held in `stage` and rhs is SourceTerms + RHS of the reference.  u0 is not
+ * read when c0 == 0.  Face scratch is allocated on first use per patch shape
+ * and kept (one buffer set per device). */
+asx_status asx_substep(const asx_patch *stage, const asx_state *u0,
+                       const asx_state *out, double c0, double c1, double c2,
+                       const asx_params *par, void *stream);
+
+/* PostStep half of a substep, fused: AsterX_ComputedBstagFromA +
+ * AsterX_ComputedBFromdBstag + AsterX_Con2Prim on interior cells
+ * (AsterX/schedule.ccl:119-165).  Writes dB*_stag only if those slots are
+ * set. */
+asx_status asx_con2prim_group(const asx_patch *patch, const asx_params *par,
+                              asx_counters *cnt, void *stream);
+
+/* Con2PrimFactory solve on a batch of independent zones (no grid): the
+ * c2p_2DNoble / c2p_1DPalenzuela call of con2prim.cxx:138-306 with the
+ * atmosphere and floors of con2prim.cxx:82-133 for r <= r_atmo.
+ * Inputs (length nz each): cons[8][nz] = dens, momx, momy, momz, tau, dBx,
+ * dBy, dBz (densitized); g[6][nz] = gxx, gxy, gxz, gyy, gyz, gzz;
+ * seed[5][nz] = saved rho, eps, velx, vely, velz.
+ * Outputs: prim[8][nz] = rho, eps, press, velx, vely, velz, w_lor, temperature;
+ * flag[nz] (C2PFlag) and iters[nz] (primary-solver iterations). */
+asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
+                         const double *seed, double *prim, double *flag,
+                         int32_t *iters, const asx_params *par, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASTERX_H */

@@ -1,0 +1,1240 @@
+// ORACLE -- test infrastructure only (see orc_physics.hpp).
+// Grid loops of the reference's scheduled functions, restated on the CPU,
+// with a C ABI (orc_*) mirroring include/asterx.h so tests drive the oracle
+// and the HIP library through the same descriptors.
+//
+//   orc_fluxes        AsterX/src/fluxes.cxx:38-826 (CalcFlux + CalcAux)
+//   orc_source_terms  AsterX/src/source.cxx:20-185
+//   orc_rhs           AsterX/src/rhs.cxx:18-253
+//   orc_bstag_from_A  AsterX/src/computeBfromA.cxx:20-65
+//   orc_dB_from_bstag AsterX/src/computeBfromA.cxx:67-85
+//   orc_B_from_dB     AsterX/src/computeBfromA.cxx:87-113
+//   orc_prim2con      AsterX/src/prim2con.cxx:12-52, prim2con.hxx:38-83
+//   orc_check_prims   AsterX/src/check_prims.cxx:18-228 (initial data)
+//   orc_con2prim      AsterX/src/con2prim.cxx:37-502
+//   orc_state_lincomb CarpetX ODESolvers state update (not vendored)
+//   orc_bc_fill       CarpetX boundary fill (not vendored): linear
+//                     extrapolation as observed in AsterX/test/*/ TSV ghosts
+// Parity: pinned end to end by the TSV goldens (tests/test_oracle_tsv.py);
+// grid-transfer details of CarpetX (BC corner order, lincomb rounding) are
+// parity unpinned below 1e-8.
+#include "asterx.h"
+#include "orc_c2p.hpp"
+#include "orc_physics.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace orc;
+
+namespace {
+
+struct GF {
+  double *p;
+  int64_t sj, sk;
+  double &operator()(int i, int j, int k) const { return p[i + sj * j + sk * k]; }
+  double &at(const int I[3]) const { return p[I[0] + sj * I[1] + sk * I[2]]; }
+  bool ok() const { return p != nullptr; }
+};
+
+struct View {
+  const asx_patch *pt;
+  GF g(int slot) const {
+    return GF{pt->gf[slot].data, pt->gf[slot].sj, pt->gf[slot].sk};
+  }
+  int n(int d) const { return pt->n[d]; }
+  int ng() const { return pt->ng; }
+};
+
+inline int cent(int slot) { return asx_gf_centering(slot); }
+
+// Offsets: I + a*e_d
+inline void sh(const int I[3], int d, int a, int O[3]) {
+  O[0] = I[0]; O[1] = I[1]; O[2] = I[2];
+  O[d] += a;
+}
+
+// AsterUtils/src/aster_interp.hxx:237-249 (vertex -> face normal to dir)
+inline double avg_v2f(const GF &g, const int I[3], int dir) {
+  const int j = (dir + 1) % 3, k = (dir + 2) % 3;
+  double s = 0.0;
+  for (int dk = 0; dk < 2; ++dk)
+    for (int dj = 0; dj < 2; ++dj) {
+      int O[3] = {I[0], I[1], I[2]};
+      O[j] += dj;
+      O[k] += dk;
+      s += g.at(O);
+    }
+  return s / 4.0;
+}
+// AsterUtils/src/aster_interp.hxx:191-205 (vertex -> cell)
+inline double avg_v2c(const GF &g, const int I[3]) {
+  double s = 0.0;
+  for (int dk = 0; dk < 2; ++dk)
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) s += g(I[0] + di, I[1] + dj, I[2] + dk);
+  return s / 8.0;
+}
+// AsterUtils/src/aster_interp.hxx:207-217 (edge along dir -> vertex)
+inline double avg_e2v(const GF &g, const int I[3], int dir) {
+  double s = 0.0;
+  for (int di = 0; di < 2; ++di) {
+    int O[3];
+    sh(I, dir, -di, O);
+    s += g.at(O);
+  }
+  return s / 2.0;
+}
+
+struct Ctx {
+  View v;
+  asx_params par;
+  ReconParams rp;
+  IdealGas eos;
+  Polytrope poly;
+};
+
+Ctx make_ctx(const asx_patch *pt, const asx_params *p) {
+  Ctx c;
+  c.v.pt = pt;
+  c.par = *p;
+  c.rp.ppm_shock_detection = p->ppm_shock_detection != 0;
+  c.rp.ppm_zone_flattening = p->ppm_zone_flattening != 0;
+  c.rp.poly_k = p->poly_k;
+  c.rp.poly_gamma = p->poly_gamma;
+  c.rp.ppm_eta1 = p->ppm_eta1;
+  c.rp.ppm_eta2 = p->ppm_eta2;
+  c.rp.ppm_eps = p->ppm_eps;
+  c.rp.ppm_eps_shock = p->ppm_eps_shock;
+  c.rp.ppm_small = p->ppm_small;
+  c.rp.ppm_omega1 = p->ppm_omega1;
+  c.rp.ppm_omega2 = p->ppm_omega2;
+  c.rp.weno_eps = p->weno_eps;
+  c.rp.mp5_alpha = p->mp5_alpha;
+  c.eos.init(p->gl_gamma, p->particle_mass, p->eps_min, p->eps_max,
+             p->rho_min, p->rho_max, p->ye_min, p->ye_max);
+  c.poly.init(p->poly_gamma, p->poly_k);
+  return c;
+}
+
+int check_par(const asx_params *p) {
+  if (!p) return ASX_ERR_NULL;
+  if (p->evolution_eos != ASX_EOS_IDEALGAS) return ASX_ERR_UNSUPPORTED;
+  if (p->reconstruction_method < 0 || p->reconstruction_method > 6 ||
+      p->reconstruction_method == ASX_RECON_EPPM)
+    return ASX_ERR_UNSUPPORTED;
+  if (p->loworder_method < 0 || p->loworder_method > 2) return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CalcFlux<dir> for one face, AsterX/src/fluxes.cxx:216-795
+// ---------------------------------------------------------------------------
+// Stencil points m of I + (m-3) e_dir read by each method (reference
+// reconstruct.hxx:44-49): Godunov 2..3, minmod/monocentral 1..4, else 0..5.
+inline void stencil_range(int method, int &m0, int &m1) {
+  if (method == R_GODUNOV) { m0 = 2; m1 = 3; }
+  else if (method == R_MINMOD || method == R_MONOCENTRAL) { m0 = 1; m1 = 4; }
+  else { m0 = 0; m1 = 5; }
+}
+struct Recon {
+  const Ctx &c;
+  int dir;
+  const int *I;
+  double ps[6], vs[6]; // press and vel_dir stencils (PPM only)
+  Recon(const Ctx &c_, int dir_, const int *I_) : c(c_), dir(dir_), I(I_) {
+    for (int m = 0; m < 6; ++m) ps[m] = vs[m] = std::nan("");
+    if (c.par.reconstruction_method == R_PPM) {
+      const GF pr = c.v.g(ASX_PRESS);
+      const GF vd = c.v.g(ASX_VELX + dir);
+      for (int m = 0; m < 6; ++m) {
+        int O[3];
+        sh(I, dir, m - 3, O);
+        ps[m] = pr.at(O);
+        vs[m] = vd.at(O);
+      }
+    }
+  }
+  void run(int method, int slot, bool is_rho, double out[2]) const {
+    const GF g = c.v.g(slot);
+    double s[6];
+    int m0, m1;
+    stencil_range(method, m0, m1);
+    for (int m = 0; m < 6; ++m) s[m] = std::nan("");
+    for (int m = m0; m <= m1; ++m) {
+      int O[3];
+      sh(I, dir, m - 3, O);
+      s[m] = g.at(O);
+    }
+    reconstruct(method, s, ps, vs, is_rho, c.rp, out);
+  }
+};
+
+void calc_flux_face(const Ctx &c, int dir, const int I[3]) {
+  const View &v = c.v;
+  const asx_params &P = c.par;
+  const int HO = P.reconstruction_method, LO = P.loworder_method;
+  const Recon R(c, dir, I);
+
+  const double alp_avg = avg_v2f(v.g(ASX_ALP), I, dir);
+  double betas_avg[3];
+  for (int i = 0; i < 3; ++i) betas_avg[i] = avg_v2f(v.g(ASX_BETAX + i), I, dir);
+  Sym3 g;
+  g.xx = avg_v2f(v.g(ASX_GXX), I, dir);
+  g.xy = avg_v2f(v.g(ASX_GXY), I, dir);
+  g.xz = avg_v2f(v.g(ASX_GXZ), I, dir);
+  g.yy = avg_v2f(v.g(ASX_GYY), I, dir);
+  g.yz = avg_v2f(v.g(ASX_GYZ), I, dir);
+  g.zz = avg_v2f(v.g(ASX_GZZ), I, dir);
+  const double detg = calc_det(g);
+  const double sqrtg = std::sqrt(detg);
+
+  bool lo0 = false, lo1 = false;
+  double rho[2], ent[2], ye[2], eps[2], press[2], temp[2];
+  R.run(HO, ASX_RHO, true, rho);
+  R.run(HO, ASX_ENTROPY, false, ent);
+  R.run(HO, ASX_YE, false, ye);
+  const bool with_T = P.reconstruct_with_temperature != 0;
+  const int thermo = with_T ? ASX_TEMPERATURE : ASX_PRESS;
+  double th[2];
+  R.run(HO, thermo, false, th);
+  if ((rho[0] <= 0.0) || (ent[0] <= 0.0) || (ye[0] <= 0.0) || (th[0] <= 0.0))
+    lo0 = true;
+  if ((rho[1] <= 0.0) || (ent[1] <= 0.0) || (ye[1] <= 0.0) || (th[1] <= 0.0))
+    lo1 = true;
+  for (int f = 0; f < 2; ++f) {
+    if (!(f == 0 ? lo0 : lo1)) continue;
+    double t[2];
+    R.run(LO, ASX_RHO, true, t);
+    rho[f] = t[f];
+    R.run(LO, ASX_ENTROPY, false, t);
+    ent[f] = t[f];
+    R.run(LO, ASX_YE, false, t);
+    ye[f] = t[f];
+    R.run(LO, thermo, false, t);
+    th[f] = t[f];
+  }
+  for (int f = 0; f < 2; ++f) {
+    if (with_T) { // fluxes.cxx:298-304
+      temp[f] = th[f];
+      eps[f] = c.eos.eps_from_rho_temp(rho[f], temp[f]);
+      press[f] = c.eos.press_from_rho_temp(rho[f], temp[f]);
+    } else { // fluxes.cxx:347-353
+      press[f] = th[f];
+      eps[f] = c.eos.eps_from_rho_press(rho[f], press[f]);
+      temp[f] = c.eos.temp_from_rho_eps(rho[f], eps[f]);
+    }
+  }
+  double rhoh[2];
+  for (int f = 0; f < 2; ++f) rhoh[f] = rho[f] + rho[f] * eps[f] + press[f];
+
+  // B: normal from dB_stag / sqrt(g), transverse reconstructed (:363-391)
+  double Bs[3][2];
+  const double val = v.g(ASX_DBX_STAG + dir).at(I) / sqrtg;
+  Bs[dir][0] = val;
+  Bs[dir][1] = val;
+  const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+  for (int d : {d1, d2}) {
+    double t[2];
+    R.run(HO, ASX_BVECX + d, false, t);
+    Bs[d][0] = t[0];
+    Bs[d][1] = t[1];
+    if (lo0) { R.run(LO, ASX_BVECX + d, false, t); Bs[d][0] = t[0]; }
+    if (lo1) { R.run(LO, ASX_BVECX + d, false, t); Bs[d][1] = t[1]; }
+  }
+
+  double vel[3][2], vlow[3][2], W[2];
+  auto lower2 = [&](double a[3][2], double out[3][2]) {
+    for (int i = 0; i < 3; ++i)
+      for (int f = 0; f < 2; ++f) {
+        double s = 0.0;
+        for (int j = 0; j < 3; ++j) s += g(i, j) * a[j][f];
+        out[i][f] = s;
+      }
+  };
+  auto dot2 = [&](double a[3][2], double b[3][2], int f) {
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i) s += a[i][f] * b[i][f];
+    return s;
+  };
+  auto recon_vec = [&](int slot0, double out[3][2]) {
+    for (int i = 0; i < 3; ++i) {
+      double t[2];
+      R.run(HO, slot0 + i, false, t);
+      out[i][0] = t[0];
+      out[i][1] = t[1];
+      if (lo0) { R.run(LO, slot0 + i, false, t); out[i][0] = t[0]; }
+      if (lo1) { R.run(LO, slot0 + i, false, t); out[i][1] = t[1]; }
+    }
+  };
+  if (P.recon_type == ASX_RECVAR_V) { // :397-425
+    recon_vec(ASX_VELX, vel);
+    lower2(vel, vlow);
+    W[0] = 1 / std::sqrt(1 - dot2(vlow, vel, 0));
+    W[1] = 1 / std::sqrt(1 - dot2(vlow, vel, 1));
+  } else if (P.recon_type == ASX_RECVAR_Z) { // :427-466
+    double z[3][2], zl[3][2];
+    recon_vec(ASX_ZVEC_X, z);
+    lower2(z, zl);
+    W[0] = std::sqrt(1 + dot2(zl, z, 0));
+    W[1] = std::sqrt(1 + dot2(zl, z, 1));
+    for (int i = 0; i < 3; ++i)
+      for (int f = 0; f < 2; ++f) {
+        vel[i][f] = z[i][f] / W[f];
+        vlow[i][f] = zl[i][f] / W[f];
+      }
+  } else { // s_vec :468-516
+    double sv[3][2], sl[3][2];
+    recon_vec(ASX_SVEC_X, sv);
+    lower2(sv, sl);
+    for (int f = 0; f < 2; ++f)
+      W[f] = std::sqrt(0.5 + std::sqrt(0.25 + dot2(sl, sv, f) / rhoh[f] / rhoh[f]));
+    for (int i = 0; i < 3; ++i)
+      for (int f = 0; f < 2; ++f) {
+        vel[i][f] = sv[i][f] / W[f] / W[f] / rhoh[f];
+        vlow[i][f] = sl[i][f] / W[f] / W[f] / rhoh[f];
+      }
+  }
+
+  double vt[3][2];
+  for (int i = 0; i < 3; ++i)
+    for (int f = 0; f < 2; ++f) vt[i][f] = alp_avg * vel[i][f] - betas_avg[i];
+  double ab0[2], Bl[3][2], B2[2], bl[3][2], bsq[2];
+  for (int f = 0; f < 2; ++f) ab0[f] = W[f] * dot2(Bs, vlow, f);
+  lower2(Bs, Bl);
+  for (int f = 0; f < 2; ++f) B2[f] = dot2(Bs, Bl, f);
+  for (int i = 0; i < 3; ++i)
+    for (int f = 0; f < 2; ++f) bl[i][f] = Bl[i][f] / W[f] + ab0[f] * vlow[i][f];
+  for (int f = 0; f < 2; ++f) bsq[f] = (B2[f] + pow2(ab0[f])) / pow2(W[f]);
+  const double beta_avg = betas_avg[dir];
+  double cs2[2], h[2];
+  for (int f = 0; f < 2; ++f)
+    cs2[f] = c.eos.csnd_from_rho_eps(rho[f], eps[f]) *
+             c.eos.csnd_from_rho_eps(rho[f], eps[f]);
+  for (int f = 0; f < 2; ++f) h[f] = 1 + eps[f] + press[f] / rho[f];
+  double dens[2], dent[2], dhW[2], dhWb[2], ppm[2], mom[3][2], tau[2], Bt[3][2];
+  for (int f = 0; f < 2; ++f) dens[f] = sqrtg * rho[f] * W[f];
+  for (int f = 0; f < 2; ++f) dent[f] = sqrtg * rho[f] * W[f] * ent[f];
+  for (int f = 0; f < 2; ++f) dhW[f] = dens[f] * h[f] * W[f];
+  for (int f = 0; f < 2; ++f) dhWb[f] = dhW[f] + sqrtg * (ab0[f] * ab0[f] + B2[f]);
+  for (int f = 0; f < 2; ++f) ppm[f] = press[f] + 0.5 * bsq[f];
+  for (int i = 0; i < 3; ++i)
+    for (int f = 0; f < 2; ++f)
+      mom[i][f] = dhWb[f] * vlow[i][f] - sqrtg * ab0[f] * bl[i][f];
+  for (int f = 0; f < 2; ++f) tau[f] = dhW[f] - dens[f] + sqrtg * (B2[f] - ppm[f]);
+  for (int i = 0; i < 3; ++i)
+    for (int f = 0; f < 2; ++f) Bt[i][f] = sqrtg * Bs[i][f];
+  double unit[3] = {0.0, 0.0, 0.0};
+  unit[dir] = 1.0;
+  const double alp_sqrtg = alp_avg * sqrtg;
+  double BoW[2], fdens[2], fdent[2], fmom[3][2], ftau[2], dye[2], fdye[2];
+  for (int f = 0; f < 2; ++f) BoW[f] = Bs[dir][f] / W[f];
+  for (int f = 0; f < 2; ++f) fdens[f] = dens[f] * vt[dir][f];
+  for (int f = 0; f < 2; ++f) fdent[f] = dent[f] * vt[dir][f];
+  for (int j = 0; j < 3; ++j)
+    for (int f = 0; f < 2; ++f)
+      fmom[j][f] = mom[j][f] * vt[dir][f] +
+                   alp_sqrtg * (ppm[f] * unit[j] - bl[j][f] * BoW[f]);
+  for (int f = 0; f < 2; ++f)
+    ftau[f] = tau[f] * vt[dir][f] +
+              alp_sqrtg * (ppm[f] * vel[dir][f] - ab0[f] * BoW[f]);
+  for (int f = 0; f < 2; ++f) dye[f] = dens[f] * ye[f];
+  for (int f = 0; f < 2; ++f) fdye[f] = dye[f] * vt[dir][f];
+  double E[3][2], fB[3][2];
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    for (int f = 0; f < 2; ++f) E[i][f] = Bt[j][f] * vt[k][f] - Bt[k][f] * vt[j][f];
+  }
+  for (int i = 0; i < 3; ++i) {
+    const int j = (i + 1) % 3, k = (i + 2) % 3;
+    for (int f = 0; f < 2; ++f) fB[i][f] = unit[j] * E[k][f] - unit[k] * E[j][f];
+  }
+  const double u_avg = calc_inv(g, detg)(dir, dir);
+  double velrc[2] = {vel[dir][0], vel[dir][1]};
+  double lam[2][4];
+  eigenvalues(alp_avg, beta_avg, u_avg, velrc, rho, cs2, W, h, bsq, lam);
+  auto num = [&](const double U[2], const double F[2]) {
+    return P.flux_type == ASX_FLUX_HLLE ? hlle(lam, U, F) : laxf(lam, U, F);
+  };
+  const int fb = ASX_FXDENS + 10 * dir;
+  v.g(fb + 0).at(I) = num(dens, fdens);
+  v.g(fb + 6).at(I) = num(dent, fdent);
+  for (int i = 0; i < 3; ++i) v.g(fb + 1 + i).at(I) = num(mom[i], fmom[i]);
+  v.g(fb + 4).at(I) = num(tau, ftau);
+  v.g(fb + 5).at(I) = num(dye, fdye);
+  for (int i = 0; i < 3; ++i) v.g(fb + 7 + i).at(I) = (dir != i) * num(Bt[i], fB[i]);
+
+  // Upwind-CT face data, :776-793
+  const GF amax = v.g(ASX_AMAX_XFACE + 2 * dir), amin = v.g(ASX_AMIN_XFACE + 2 * dir);
+  if (amax.ok() && amin.ok()) {
+    double mx = 0, mn = 0;
+    for (int f = 0; f < 2; ++f)
+      for (int k = 0; k < 4; ++k) {
+        mx = smax(mx, lam[f][k]);
+        mn = smin(mn, lam[f][k]);
+      }
+    const double amx = mx, amn = -1 * mn;
+    amax.at(I) = amx;
+    amin.at(I) = amn;
+    const GF vt1 = v.g(ASX_VTILDE_Y_XFACE + 2 * dir);
+    const GF vt2 = v.g(ASX_VTILDE_Y_XFACE + 2 * dir + 1);
+    if (vt1.ok())
+      vt1.at(I) = (amx * vt[d1][0] + amn * vt[d1][1]) / (amx + amn);
+    if (vt2.ok())
+      vt2.at(I) = (amx * vt[d2][0] + amn * vt[d2][1]) / (amx + amn);
+  }
+}
+
+// CalcAuxForAvecPsi, AsterX/src/fluxes.cxx:803-825
+void calc_aux_vertex(const Ctx &c, const int I[3]) {
+  const View &v = c.v;
+  double A[3];
+  for (int i = 0; i < 3; ++i) A[i] = avg_e2v(v.g(ASX_AVEC_X + i), I, i);
+  Sym3 g{v.g(ASX_GXX).at(I), v.g(ASX_GXY).at(I), v.g(ASX_GXZ).at(I),
+         v.g(ASX_GYY).at(I), v.g(ASX_GYZ).at(I), v.g(ASX_GZZ).at(I)};
+  const double beta[3] = {v.g(ASX_BETAX).at(I), v.g(ASX_BETAY).at(I),
+                          v.g(ASX_BETAZ).at(I)};
+  const double detg = calc_det(g);
+  const double sqrtg = std::sqrt(detg);
+  const Sym3 ug = calc_inv(g, detg);
+  double Aup[3];
+  lower(ug, A, Aup);
+  const double alp = v.g(ASX_ALP).at(I), psi = v.g(ASX_PSI).at(I);
+  v.g(ASX_FX).at(I) = alp * sqrtg * Aup[0];
+  v.g(ASX_FY).at(I) = alp * sqrtg * Aup[1];
+  v.g(ASX_FZ).at(I) = alp * sqrtg * Aup[2];
+  v.g(ASX_FBETAX).at(I) = beta[0] * psi;
+  v.g(ASX_FBETAY).at(I) = beta[1] * psi;
+  v.g(ASX_FBETAZ).at(I) = beta[2] * psi;
+  v.g(ASX_G).at(I) = alp * psi / sqrtg - dot3(beta, A);
+}
+
+// Loop over a box of points [lo, hi] (inclusive) in k, j, i order.
+template <typename F>
+void loop_box(const int lo[3], const int hi[3], const F &f) {
+#pragma omp parallel for schedule(static)
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) {
+        const int I[3] = {i, j, k};
+        f(I);
+      }
+}
+
+// Interior box of a centering, widened by halo[d] on axis d.
+void interior(const asx_patch *pt, int centering, const int halo[3], int lo[3],
+              int hi[3]) {
+  for (int d = 0; d < 3; ++d) {
+    const int c = (centering >> d) & 1;
+    lo[d] = pt->ng - halo[d];
+    hi[d] = pt->ng + pt->n[d] - 1 + c + halo[d];
+  }
+}
+
+// calc_fd_v2c<2> and <4>, AsterUtils/src/aster_fd.hxx:49-111
+double fd_v2c(const GF &g, const int I[3], int dir, int order, double dx) {
+  const int d1 = (dir + 1) % 3, d2 = (dir + 2) % 3;
+  auto at = [&](int a, int b1, int b2) {
+    int O[3] = {I[0], I[1], I[2]};
+    O[dir] += a;
+    O[d1] += b1;
+    O[d2] += b2;
+    return g.at(O);
+  };
+  if (order == 2) {
+    const double g1 = (at(1, 0, 0) - at(0, 0, 0)) / dx;
+    const double g2 = (at(1, 1, 0) - at(0, 1, 0)) / dx;
+    const double g3 = (at(1, 0, 1) - at(0, 0, 1)) / dx;
+    const double g4 = (at(1, 1, 1) - at(0, 1, 1)) / dx;
+    return 0.25 * (g1 + g2 + g3 + g4);
+  }
+  auto d4 = [&](int b1, int b2) {
+    return ((1.0 / 24.0) * at(-1, b1, b2) - (27.0 / 24.0) * at(0, b1, b2) +
+            (27.0 / 24.0) * at(1, b1, b2) - (1.0 / 24.0) * at(2, b1, b2)) /
+           dx;
+  };
+  const double g1 = d4(0, 0), g2 = d4(1, 0), g3 = d4(0, 1), g4 = d4(1, 1);
+  return 0.25 * (g1 + g2 + g3 + g4);
+}
+
+// SourceTerms<FDORDER>, AsterX/src/source.cxx:32-167
+void source_cell(const Ctx &c, const int I[3]) {
+  const View &v = c.v;
+  const asx_params &P = c.par;
+  const double alp = avg_v2c(v.g(ASX_ALP), I);
+  double beta[3];
+  for (int i = 0; i < 3; ++i) beta[i] = avg_v2c(v.g(ASX_BETAX + i), I);
+  Sym3 g{avg_v2c(v.g(ASX_GXX), I), avg_v2c(v.g(ASX_GXY), I),
+         avg_v2c(v.g(ASX_GXZ), I), avg_v2c(v.g(ASX_GYY), I),
+         avg_v2c(v.g(ASX_GYZ), I), avg_v2c(v.g(ASX_GZZ), I)};
+  Sym3 kk{avg_v2c(v.g(ASX_KXX), I), avg_v2c(v.g(ASX_KXY), I),
+          avg_v2c(v.g(ASX_KXZ), I), avg_v2c(v.g(ASX_KYY), I),
+          avg_v2c(v.g(ASX_KYZ), I), avg_v2c(v.g(ASX_KZZ), I)};
+  const double detg = calc_det(g);
+  const double sqrt_detg = std::sqrt(detg);
+  const Sym3 ug = calc_inv(g, detg);
+  const int ord = P.local_spatial_order;
+  double d_alp[3], d_beta[3][3];
+  Sym3 d_g[3];
+  for (int k = 0; k < 3; ++k) {
+    const double dx = c.v.pt->dx[k];
+    d_alp[k] = fd_v2c(v.g(ASX_ALP), I, k, ord, dx);
+    for (int i = 0; i < 3; ++i) d_beta[k][i] = fd_v2c(v.g(ASX_BETAX + i), I, k, ord, dx);
+    d_g[k] = Sym3{fd_v2c(v.g(ASX_GXX), I, k, ord, dx), fd_v2c(v.g(ASX_GXY), I, k, ord, dx),
+                  fd_v2c(v.g(ASX_GXZ), I, k, ord, dx), fd_v2c(v.g(ASX_GYY), I, k, ord, dx),
+                  fd_v2c(v.g(ASX_GYZ), I, k, ord, dx), fd_v2c(v.g(ASX_GZZ), I, k, ord, dx)};
+  }
+  double v_up[3], v_low[3], w;
+  if (P.recon_type == ASX_RECVAR_V) {
+    for (int i = 0; i < 3; ++i) v_up[i] = v.g(ASX_VELX + i).at(I);
+    lower(g, v_up, v_low);
+    w = 1.0 / std::sqrt(1.0 - dot3(v_low, v_up));
+  } else {
+    double z_up[3], z_low[3];
+    for (int i = 0; i < 3; ++i) z_up[i] = v.g(ASX_ZVEC_X + i).at(I);
+    lower(g, z_up, z_low);
+    w = std::sqrt(1.0 + dot3(z_low, z_up));
+    for (int i = 0; i < 3; ++i) v_up[i] = z_up[i] / w;
+    for (int i = 0; i < 3; ++i) v_low[i] = z_low[i] / w;
+  }
+  const double rho = v.g(ASX_RHO).at(I), eps = v.g(ASX_EPS).at(I),
+               press = v.g(ASX_PRESS).at(I);
+  const double rhW2 = (rho * (1.0 + eps) + press) * w * w;
+  double vs[3];
+  for (int i = 0; i < 3; ++i) vs[i] = v_up[i] - beta[i] / alp;
+  double B_up[3], B_low[3];
+  for (int i = 0; i < 3; ++i) B_up[i] = v.g(ASX_BVECX + i).at(I);
+  lower(g, B_up, B_low);
+  const double bs0 = w * dot3(B_up, v_low) / alp;
+  double bs[3];
+  for (int i = 0; i < 3; ++i) bs[i] = B_up[i] / w + alp * bs0 * vs[i];
+  const double bs2 = (dot3(B_up, B_low) + pow2(alp * bs0)) / pow2(w);
+  double gb[3], gbs[3], bs_low[3];
+  lower(g, beta, gb);
+  lower(g, bs, gbs);
+  for (int i = 0; i < 3; ++i) bs_low[i] = gb[i] * bs0 + gbs[i];
+  const double Eh = rhW2 + bs2 * pow2(w);
+  const double Ph = press + 0.5 * bs2;
+  const double t00 = (Eh - Ph) / pow2(alp) - pow2(bs0);
+  double t0[3], t0low[3];
+  for (int i = 0; i < 3; ++i)
+    t0[i] = Eh * vs[i] / alp + Ph * beta[i] / pow2(alp) - bs0 * bs[i];
+  auto t = [&](int i, int j) {
+    return Eh * vs[i] * vs[j] + Ph * (ug(i, j) - beta[i] * beta[j] / pow2(alp)) -
+           bs[i] * bs[j];
+  };
+  for (int i = 0; i < 3; ++i) t0low[i] = Eh * v_low[i] / alp - bs_low[i] * bs0;
+  double shiftk[3];
+  lower(kk, beta, shiftk);
+  const double shiftshiftk = dot3(shiftk, beta);
+  double tmp[3];
+  for (int i = 0; i < 3; ++i) tmp[i] = -d_alp[i] + 2.0 * shiftk[i];
+  const double tau_source =
+      t00 * (shiftshiftk - dot3(beta, d_alp)) + dot3(t0, tmp) +
+      sum_symm([&](int i, int j) { return kk(i, j) * t(i, j); });
+  double ms[3];
+  for (int k = 0; k < 3; ++k) {
+    const Sym3 &dg = d_g[k];
+    ms[k] = t00 * (0.5 * sum_symm([&](int i, int j) {
+                     return dg(i, j) * beta[i] * beta[j];
+                   }) - alp * d_alp[k]) +
+            sum2([&](int i, int j) { return dg(i, j) * t0[i] * beta[j]; }) +
+            dot3(t0low, d_beta[k]) +
+            0.5 * sum_symm([&](int i, int j) { return t(i, j) * dg(i, j); });
+  }
+  v.g(ASX_DENSRHS).at(I) = 0.0;
+  v.g(ASX_DENTRHS).at(I) = 0.0;
+  v.g(ASX_MOMXRHS).at(I) = alp * sqrt_detg * ms[0];
+  v.g(ASX_MOMYRHS).at(I) = alp * sqrt_detg * ms[1];
+  v.g(ASX_MOMZRHS).at(I) = alp * sqrt_detg * ms[2];
+  v.g(ASX_TAURHS).at(I) = alp * sqrt_detg * tau_source;
+  v.g(ASX_DYE_RHS).at(I) = 0.0;
+}
+
+// AsterX_RHS hydro part, AsterX/src/rhs.cxx:107-114,188-207
+void rhs_hydro_cell(const Ctx &c, const int I[3]) {
+  const View &v = c.v;
+  const double idx[3] = {1 / c.v.pt->dx[0], 1 / c.v.pt->dx[1], 1 / c.v.pt->dx[2]};
+  static const int rhs_slot[7] = {ASX_DENSRHS, ASX_DENTRHS, ASX_MOMXRHS, ASX_MOMYRHS,
+                                  ASX_MOMZRHS, ASX_TAURHS, ASX_DYE_RHS};
+  static const int flux_var[7] = {0, 6, 1, 2, 3, 4, 5};
+  for (int q = 0; q < 7; ++q) {
+    double df[3];
+    for (int d = 0; d < 3; ++d) {
+      const GF F = v.g(ASX_FXDENS + 10 * d + flux_var[q]);
+      int O[3];
+      sh(I, d, 1, O);
+      df[d] = F.at(O) - F.at(I);
+    }
+    v.g(rhs_slot[q]).at(I) += -dot3(idx, df);
+  }
+}
+
+// AsterX_RHS vector potential, AsterX/src/rhs.cxx:116-186,209-225
+void rhs_avec_edge(const Ctx &c, const int I[3], int i) {
+  const View &v = c.v;
+  const asx_params &P = c.par;
+  const int j = (i + 1) % 3, k = (i + 2) % 3;
+  double E;
+  if (P.use_uct) {
+    // face data arranged as in rhs.cxx:87-105
+    static const int dBone[3] = {ASX_DBY_STAG, ASX_DBZ_STAG, ASX_DBX_STAG};
+    static const int dBtwo[3] = {ASX_DBZ_STAG, ASX_DBX_STAG, ASX_DBY_STAG};
+    static const int vtone[3] = {ASX_VTILDE_Z_YFACE, ASX_VTILDE_X_ZFACE, ASX_VTILDE_Y_XFACE};
+    static const int vttwo[3] = {ASX_VTILDE_Y_ZFACE, ASX_VTILDE_Z_XFACE, ASX_VTILDE_X_YFACE};
+    static const int amaxone[3] = {ASX_AMAX_ZFACE, ASX_AMAX_XFACE, ASX_AMAX_YFACE};
+    static const int amaxtwo[3] = {ASX_AMAX_YFACE, ASX_AMAX_ZFACE, ASX_AMAX_XFACE};
+    static const int aminone[3] = {ASX_AMIN_ZFACE, ASX_AMIN_XFACE, ASX_AMIN_YFACE};
+    static const int amintwo[3] = {ASX_AMIN_YFACE, ASX_AMIN_ZFACE, ASX_AMIN_XFACE};
+    auto rc = [&](int slot, int dir, double out[2]) {
+      const Recon R(c, dir, I);
+      R.run(P.reconstruction_method, slot, false, out);
+    };
+    double dB1[2], dB2[2], vt1[2], vt2[2];
+    rc(dBone[i], k, dB1);
+    rc(dBtwo[i], j, dB2);
+    rc(vtone[i], k, vt1);
+    rc(vttwo[i], j, vt2);
+    const double ax1 = v.g(amaxone[i]).at(I), an1 = v.g(aminone[i]).at(I);
+    const double ax2 = v.g(amaxtwo[i]).at(I), an2 = v.g(amintwo[i]).at(I);
+    const double den1 = ax1 + an1;
+    E = (ax1 * vt1[0] * dB1[0] + an1 * vt1[1] * dB1[1] - ax1 * an1 * (dB1[1] - dB1[0])) / den1;
+    const double den2 = ax2 + an2;
+    E -= (ax2 * vt2[0] * dB2[0] + an2 * vt2[1] * dB2[1] - ax2 * an2 * (dB2[1] - dB2[0])) / den2;
+  } else {
+    // gf_fBs(j)(k) = flux of B_j in direction k
+    const GF Fjk = v.g(ASX_FXDENS + 10 * k + 7 + j);
+    const GF Fkj = v.g(ASX_FXDENS + 10 * j + 7 + k);
+    int Oj[3], Ok[3];
+    sh(I, j, -1, Oj);
+    sh(I, k, -1, Ok);
+    E = 0.25 * ((Fjk.at(I) + Fjk.at(Oj)) - (Fkj.at(I) + Fkj.at(Ok)));
+  }
+  double r;
+  if (P.vector_potential_gauge == ASX_GAUGE_ALGEBRAIC) {
+    r = -E;
+  } else {
+    const GF G = v.g(ASX_G);
+    int O[3];
+    sh(I, i, 1, O);
+    r = -E - (G.at(O) - G.at(I)) / c.v.pt->dx[i];
+  }
+  v.g(ASX_AVEC_X_RHS + i).at(I) = r;
+}
+
+// AsterX_RHS Psi part, AsterX/src/rhs.cxx:227-252 and aster_fd.hxx:20-36
+void rhs_psi_vertex(const Ctx &c, const int I[3]) {
+  const View &v = c.v;
+  const asx_params &P = c.par;
+  if (P.vector_potential_gauge == ASX_GAUGE_ALGEBRAIC) {
+    v.g(ASX_PSI_RHS).at(I) = 0.0;
+    return;
+  }
+  double dF = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    const double dx = c.v.pt->dx[i];
+    const GF F = v.g(ASX_FX + i), Fb = v.g(ASX_FBETAX + i);
+    int Op[3], Om[3];
+    sh(I, i, 1, Op);
+    sh(I, i, -1, Om);
+    const double c2c = (0.5 / dx) * (F.at(Op) - F.at(Om));
+    const int sign = v.g(ASX_BETAX + i).at(I) < 0 ? -1 : 1;
+    int O1[3], O2[3];
+    sh(I, i, sign, O1);
+    sh(I, i, 2 * sign, O2);
+    const double one = -sign * (Fb.at(O2) - 4.0 * Fb.at(O1) + 3.0 * Fb.at(I)) * (0.5 / dx);
+    dF += c2c - one;
+  }
+  v.g(ASX_PSI_RHS).at(I) =
+      -dF - P.lorenz_damp_fac * v.g(ASX_ALP).at(I) * v.g(ASX_PSI).at(I);
+}
+
+// ComputeStaggeredB<dir>, AsterX/src/computeBfromA.cxx:32-55
+void bstag_face(const View &v, int dir, const int I[3]) {
+  const double *dx = v.pt->dx;
+  const double idx[3] = {1 / dx[0], 1 / dx[1], 1 / dx[2]};
+  const GF Ax = v.g(ASX_AVEC_X), Ay = v.g(ASX_AVEC_Y), Az = v.g(ASX_AVEC_Z);
+  int ip[3], jp[3], kp[3];
+  sh(I, 0, 1, ip);
+  sh(I, 1, 1, jp);
+  sh(I, 2, 1, kp);
+  if (dir == 0)
+    v.g(ASX_DBX_STAG).at(I) = idx[1] * (Az.at(jp) - Az.at(I)) - idx[2] * (Ay.at(kp) - Ay.at(I));
+  else if (dir == 1)
+    v.g(ASX_DBY_STAG).at(I) = idx[2] * (Ax.at(kp) - Ax.at(I)) - idx[0] * (Az.at(ip) - Az.at(I));
+  else
+    v.g(ASX_DBZ_STAG).at(I) = idx[0] * (Ay.at(ip) - Ay.at(I)) - idx[1] * (Ax.at(jp) - Ax.at(I));
+}
+
+int check_patch(const asx_patch *pt, std::initializer_list<int> slots) {
+  if (!pt) return ASX_ERR_NULL;
+  for (int s : slots) {
+    if (!pt->gf[s].data) return ASX_ERR_NULL;
+    int64_t sj, sk;
+    asx_gf_strides(pt->n, pt->ng, s, &sj, &sk);
+    if (pt->gf[s].sj != sj || pt->gf[s].sk != sk) return ASX_ERR_LAYOUT;
+  }
+  return ASX_OK;
+}
+
+const std::initializer_list<int> kAdm = {ASX_ALP, ASX_BETAX, ASX_BETAY, ASX_BETAZ,
+                                         ASX_GXX, ASX_GXY,   ASX_GXZ,   ASX_GYY,
+                                         ASX_GYZ, ASX_GZZ};
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Slot table shared with the product header (duplicated here so the oracle
+// does not link the HIP library).
+// ---------------------------------------------------------------------------
+extern "C" int asx_gf_centering(int s) {
+  if (s >= ASX_ALP && s <= ASX_KZZ) return 7;
+  if (s >= ASX_RHO && s <= ASX_MASK_CC) return 0;
+  if (s == ASX_MASK_VC) return 7;
+  if (s == ASX_DBX_STAG) return 1;
+  if (s == ASX_DBY_STAG) return 2;
+  if (s == ASX_DBZ_STAG) return 4;
+  if (s == ASX_AVEC_X) return 6;
+  if (s == ASX_AVEC_Y) return 5;
+  if (s == ASX_AVEC_Z) return 3;
+  if (s == ASX_PSI) return 7;
+  if (s >= ASX_FXDENS && s <= ASX_FXBZ) return 1;
+  if (s >= ASX_FYDENS && s <= ASX_FYBZ) return 2;
+  if (s >= ASX_FZDENS && s <= ASX_FZBZ) return 4;
+  if (s == ASX_VTILDE_Y_XFACE || s == ASX_VTILDE_Z_XFACE) return 1;
+  if (s == ASX_VTILDE_X_YFACE || s == ASX_VTILDE_Z_YFACE) return 2;
+  if (s == ASX_VTILDE_X_ZFACE || s == ASX_VTILDE_Y_ZFACE) return 4;
+  if (s == ASX_AMAX_XFACE || s == ASX_AMIN_XFACE) return 1;
+  if (s == ASX_AMAX_YFACE || s == ASX_AMIN_YFACE) return 2;
+  if (s == ASX_AMAX_ZFACE || s == ASX_AMIN_ZFACE) return 4;
+  if (s >= ASX_FX && s <= ASX_G) return 7;
+  if (s >= ASX_DENSRHS && s <= ASX_DYE_RHS) return 0;
+  if (s == ASX_AVEC_X_RHS) return 6;
+  if (s == ASX_AVEC_Y_RHS) return 5;
+  if (s == ASX_AVEC_Z_RHS) return 3;
+  if (s == ASX_PSI_RHS) return 7;
+  return -1;
+}
+
+extern "C" void asx_gf_strides(const int32_t n[3], int32_t ng, int s,
+                               int64_t *sj, int64_t *sk) {
+  const int c = asx_gf_centering(s);
+  const int64_t ex = n[0] + 2 * ng + (c & 1);
+  const int64_t ey = n[1] + 2 * ng + ((c >> 1) & 1);
+  *sj = ex;
+  *sk = ex * ey;
+}
+
+extern "C" {
+
+int orc_fluxes(const asx_patch *pt, const asx_params *par, int face_halo) {
+  int st = check_par(par);
+  if (st) return st;
+  if (!pt) return ASX_ERR_NULL;
+  if (face_halo < 0 || face_halo > 1) return ASX_ERR_PARAM;
+  int m0, m1;
+  stencil_range(par->reconstruction_method, m0, m1);
+  if (pt->ng < 3 - m0 || (face_halo > 0 && pt->ng < 3)) return ASX_ERR_GHOSTS;
+  const Ctx c = make_ctx(pt, par);
+  for (int dir = 0; dir < 3; ++dir) {
+    int halo[3] = {face_halo, face_halo, face_halo};
+    halo[dir] = 0;
+    int lo[3], hi[3];
+    interior(pt, 1 << dir, halo, lo, hi);
+    loop_box(lo, hi, [&](const int I[3]) { calc_flux_face(c, dir, I); });
+  }
+  if (pt->gf[ASX_FX].data) {
+    const int h = 2 * face_halo;
+    int halo[3] = {h, h, h}, lo[3], hi[3];
+    interior(pt, 7, halo, lo, hi);
+    loop_box(lo, hi, [&](const int I[3]) { calc_aux_vertex(c, I); });
+  }
+  return ASX_OK;
+}
+
+int orc_source_terms(const asx_patch *pt, const asx_params *par) {
+  if (!pt || !par) return ASX_ERR_NULL;
+  if (par->local_spatial_order != 2 && par->local_spatial_order != 4)
+    return ASX_ERR_PARAM;
+  const Ctx c = make_ctx(pt, par);
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) { source_cell(c, I); });
+  return ASX_OK;
+}
+
+int orc_rhs(const asx_patch *pt, const asx_params *par) {
+  int st = check_par(par);
+  if (st) return st;
+  const Ctx c = make_ctx(pt, par);
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) { rhs_hydro_cell(c, I); });
+  for (int i = 0; i < 3; ++i) {
+    interior(pt, asx_gf_centering(ASX_AVEC_X + i), halo, lo, hi);
+    loop_box(lo, hi, [&](const int I[3]) { rhs_avec_edge(c, I, i); });
+  }
+  interior(pt, 7, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) { rhs_psi_vertex(c, I); });
+  return ASX_OK;
+}
+
+// halo < 0 means: every face whose curl stencil is inside the arrays.
+int orc_bstag_from_A_halo(const asx_patch *pt, int full) {
+  if (!pt) return ASX_ERR_NULL;
+  View v{pt};
+  for (int dir = 0; dir < 3; ++dir) {
+    int lo[3], hi[3];
+    if (full) {
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = 0;
+        // last index with a +1 neighbour on the transverse vertex axes
+        hi[d] = (d == dir) ? pt->n[d] + 2 * pt->ng : pt->n[d] + 2 * pt->ng - 1;
+      }
+    } else {
+      int halo[3] = {0, 0, 0};
+      interior(pt, 1 << dir, halo, lo, hi);
+    }
+    loop_box(lo, hi, [&](const int I[3]) { bstag_face(v, dir, I); });
+  }
+  return ASX_OK;
+}
+int orc_bstag_from_A(const asx_patch *pt) { return orc_bstag_from_A_halo(pt, 0); }
+
+int orc_dB_from_bstag(const asx_patch *pt) {
+  if (!pt) return ASX_ERR_NULL;
+  View v{pt};
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) {
+    for (int d = 0; d < 3; ++d) {
+      const GF s = v.g(ASX_DBX_STAG + d);
+      int O[3];
+      sh(I, d, 1, O);
+      v.g(ASX_DBX + d).at(I) = 0.5 * (s.at(I) + s.at(O));
+    }
+  });
+  return ASX_OK;
+}
+
+int orc_B_from_dB(const asx_patch *pt) {
+  if (!pt) return ASX_ERR_NULL;
+  View v{pt};
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) {
+    Sym3 g{avg_v2c(v.g(ASX_GXX), I), avg_v2c(v.g(ASX_GXY), I),
+           avg_v2c(v.g(ASX_GXZ), I), avg_v2c(v.g(ASX_GYY), I),
+           avg_v2c(v.g(ASX_GYZ), I), avg_v2c(v.g(ASX_GZZ), I)};
+    const double sq = std::sqrt(calc_det(g));
+    for (int d = 0; d < 3; ++d) v.g(ASX_BVECX + d).at(I) = v.g(ASX_DBX + d).at(I) / sq;
+  });
+  return ASX_OK;
+}
+
+// AsterX/src/prim2con.cxx:12-52 with prim2con.hxx:38-83 (recomputed W)
+int orc_prim2con(const asx_patch *pt) {
+  if (!pt) return ASX_ERR_NULL;
+  View v{pt};
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) {
+    Sym3 g{avg_v2c(v.g(ASX_GXX), I), avg_v2c(v.g(ASX_GXY), I),
+           avg_v2c(v.g(ASX_GXZ), I), avg_v2c(v.g(ASX_GYY), I),
+           avg_v2c(v.g(ASX_GYZ), I), avg_v2c(v.g(ASX_GZZ), I)};
+    Prim pv;
+    pv.rho = v.g(ASX_RHO).at(I);
+    for (int i = 0; i < 3; ++i) pv.vel[i] = v.g(ASX_VELX + i).at(I);
+    pv.eps = v.g(ASX_EPS).at(I);
+    pv.press = v.g(ASX_PRESS).at(I);
+    pv.entropy = v.g(ASX_ENTROPY).at(I);
+    pv.Ye = v.g(ASX_YE).at(I);
+    for (int i = 0; i < 3; ++i) pv.B[i] = v.g(ASX_BVECX + i).at(I);
+    double v_low[3];
+    lower(g, pv.vel, v_low);
+    pv.w_lor = 1.0 / std::sqrt(1.0 - dot3(v_low, pv.vel));
+    Cons cv;
+    cv.from_prim(pv, g);
+    v.g(ASX_DENS).at(I) = cv.dens;
+    for (int i = 0; i < 3; ++i) v.g(ASX_MOMX + i).at(I) = cv.mom[i];
+    v.g(ASX_TAU).at(I) = cv.tau;
+    v.g(ASX_DYE).at(I) = cv.DYe;
+    for (int i = 0; i < 3; ++i) v.g(ASX_DBX + i).at(I) = cv.dB[i];
+    v.g(ASX_DENT).at(I) = cv.DEnt;
+  });
+  return ASX_OK;
+}
+
+// AsterX_CheckPrims, AsterX/src/check_prims.cxx:24-228 (ideal gas,
+// use_temperature = no); initial data only.
+int orc_check_prims(const asx_patch *pt, const asx_params *par) {
+  if (!pt || !par) return ASX_ERR_NULL;
+  const Ctx c = make_ctx(pt, par);
+  const View &v = c.v;
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) {
+    Sym3 g{avg_v2c(v.g(ASX_GXX), I), avg_v2c(v.g(ASX_GXY), I),
+           avg_v2c(v.g(ASX_GXZ), I), avg_v2c(v.g(ASX_GYY), I),
+           avg_v2c(v.g(ASX_GYZ), I), avg_v2c(v.g(ASX_GZZ), I)};
+    double v_up[3];
+    for (int i = 0; i < 3; ++i) v_up[i] = v.g(ASX_VELX + i).at(I);
+    double rhoL = v.g(ASX_RHO).at(I), epsL = v.g(ASX_EPS).at(I),
+           pressL = v.g(ASX_PRESS).at(I), YeL = v.g(ASX_YE).at(I),
+           tempL = v.g(ASX_TEMPERATURE).at(I);
+    double entL = c.eos.kappa_from_rho_eps(rhoL, epsL);
+    const double x = pt->x0[0] + (I[0] - pt->ng + 0.5) * pt->dx[0];
+    const double y = pt->x0[1] + (I[1] - pt->ng + 0.5) * pt->dx[1];
+    const double z = pt->x0[2] + (I[2] - pt->ng + 0.5) * pt->dx[2];
+    AtmoParams ap{par->rho_abs_min, par->atmo_tol, par->p_atmo, par->t_atmo,
+                  par->Ye_atmo, par->r_atmo, par->n_rho_atmo, par->n_press_atmo,
+                  par->n_temp_atmo, par->thermal_eos_atmo != 0,
+                  par->use_press_atmo != 0};
+    const Atmo atmo = make_atmo(c.eos, c.poly, ap, std::sqrt(x * x + y * y + z * z));
+    if (YeL > c.eos.ye_max) YeL = c.eos.ye_max;
+    if (YeL < c.eos.ye_min) YeL = c.eos.ye_min;
+    double v_low[3];
+    lower(g, v_up, v_low);
+    const double w_lim = std::sqrt(1.0 + par->vw_lim * par->vw_lim);
+    const double v_lim = par->vw_lim / w_lim;
+    const double vsq = dot3(v_low, v_up);
+    const double sol_v = std::sqrt(vsq);
+    if (sol_v > v_lim) {
+      const double s = v_lim / sol_v;
+      for (int i = 0; i < 3; ++i) v_up[i] *= s;
+      for (int i = 0; i < 3; ++i) v_low[i] = v_low[i] * v_lim / sol_v;
+    }
+    if (rhoL > c.eos.rho_max) {
+      rhoL = c.eos.rho_max;
+      epsL = c.eos.eps_from_rho_press(rhoL, pressL);
+      entL = c.eos.kappa_from_rho_eps(rhoL, epsL);
+    }
+    if (rhoL < atmo.rho_cut) {
+      rhoL = atmo.rho;
+      epsL = c.eos.eps_from_rho_press(rhoL, pressL);
+      entL = c.eos.kappa_from_rho_eps(rhoL, epsL);
+    }
+    if (epsL > c.eos.eps_max) {
+      epsL = c.eos.eps_max;
+      tempL = c.eos.temp_from_rho_eps(rhoL, epsL);
+      pressL = c.eos.press_from_rho_eps(rhoL, epsL);
+      entL = c.eos.kappa_from_rho_eps(rhoL, epsL);
+    }
+    if (par->use_press_atmo) {
+      if (pressL < atmo.press) {
+        pressL = atmo.press;
+        epsL = c.eos.eps_from_rho_press(rhoL, pressL);
+        tempL = c.eos.temp_from_rho_eps(rhoL, epsL);
+        entL = c.eos.kappa_from_rho_eps(rhoL, epsL);
+      }
+    } else if (epsL < atmo.eps) {
+      epsL = atmo.eps;
+      tempL = c.eos.temp_from_rho_eps(rhoL, epsL);
+      pressL = c.eos.press_from_rho_eps(rhoL, epsL);
+      entL = c.eos.kappa_from_rho_eps(rhoL, epsL);
+    }
+    v.g(ASX_RHO).at(I) = rhoL;
+    for (int i = 0; i < 3; ++i) v.g(ASX_VELX + i).at(I) = v_up[i];
+    v.g(ASX_EPS).at(I) = epsL;
+    v.g(ASX_PRESS).at(I) = pressL;
+    v.g(ASX_ENTROPY).at(I) = entL;
+    v.g(ASX_YE).at(I) = YeL;
+    v.g(ASX_TEMPERATURE).at(I) = tempL;
+    v.g(ASX_SAVED_RHO).at(I) = rhoL;
+    for (int i = 0; i < 3; ++i) v.g(ASX_SAVED_VELX + i).at(I) = v_up[i];
+    v.g(ASX_SAVED_EPS).at(I) = epsL;
+    v.g(ASX_SAVED_YE).at(I) = YeL;
+    const double wlor = 1.0 / std::sqrt(1.0 - dot3(v_low, v_up));
+    for (int i = 0; i < 3; ++i) v.g(ASX_ZVEC_X + i).at(I) = wlor * v_up[i];
+    for (int i = 0; i < 3; ++i)
+      v.g(ASX_SVEC_X + i).at(I) = (rhoL + rhoL * epsL + pressL) * wlor * wlor * v_up[i];
+  });
+  return ASX_OK;
+}
+
+static C2PDriver make_driver(const asx_params *par) {
+  C2PDriver d;
+  d.prime = par->c2p_prime;
+  d.second = par->c2p_second;
+  d.excise = par->excise != 0;
+  d.ap = AtmoParams{par->rho_abs_min, par->atmo_tol, par->p_atmo, par->t_atmo,
+                    par->Ye_atmo, par->r_atmo, par->n_rho_atmo, par->n_press_atmo,
+                    par->n_temp_atmo, par->thermal_eos_atmo != 0,
+                    par->use_press_atmo != 0};
+  d.cp.max_iter = par->max_iter;
+  d.cp.tol = par->c2p_tol;
+  d.cp.alp_thresh = par->alp_thresh;
+  d.cp.cons_error = par->cons_error_limit;
+  d.cp.vw_lim = par->vw_lim;
+  d.cp.w_lim = std::sqrt(1.0 + par->vw_lim * par->vw_lim);
+  d.cp.v_lim = par->vw_lim / d.cp.w_lim;
+  d.cp.Bsq_lim = par->B_lim * par->B_lim;
+  d.cp.rho_BH = par->rho_BH;
+  d.cp.eps_BH = par->eps_BH;
+  d.cp.vwlim_BH = par->vwlim_BH;
+  d.cp.use_z = par->use_z != 0;
+  d.cp.use_temp = par->use_temperature != 0;
+  d.cp.use_press_atmo = par->use_press_atmo != 0;
+  return d;
+}
+
+static int check_c2p_par(const asx_params *par) {
+  int st = check_par(par);
+  if (st) return st;
+  if (par->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
+  if (par->c2p_prime != ASX_C2P_NOBLE && par->c2p_prime != ASX_C2P_PALENZUELA)
+    return ASX_ERR_UNSUPPORTED;
+  if (par->c2p_second != ASX_C2P_NOBLE && par->c2p_second != ASX_C2P_PALENZUELA &&
+      par->c2p_second != ASX_C2P_NONE)
+    return ASX_ERR_UNSUPPORTED;
+  if (par->max_iter < 1) return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+
+// AsterX_Con2Prim_typeEoS, AsterX/src/con2prim.cxx:73-454
+int orc_con2prim(const asx_patch *pt, const asx_params *par) {
+  int st = check_c2p_par(par);
+  if (st) return st;
+  if (!pt) return ASX_ERR_NULL;
+  const Ctx c = make_ctx(pt, par);
+  const View &v = c.v;
+  const C2PDriver dp = make_driver(par);
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) {
+    C2PCellIn in;
+    const double x = pt->x0[0] + (I[0] - pt->ng + 0.5) * pt->dx[0];
+    const double y = pt->x0[1] + (I[1] - pt->ng + 0.5) * pt->dx[1];
+    const double z = pt->x0[2] + (I[2] - pt->ng + 0.5) * pt->dx[2];
+    in.r = std::sqrt(x * x + y * y + z * z);
+    in.alp = avg_v2c(v.g(ASX_ALP), I);
+    in.glo = Sym3{avg_v2c(v.g(ASX_GXX), I), avg_v2c(v.g(ASX_GXY), I),
+                  avg_v2c(v.g(ASX_GXZ), I), avg_v2c(v.g(ASX_GYY), I),
+                  avg_v2c(v.g(ASX_GYZ), I), avg_v2c(v.g(ASX_GZZ), I)};
+    in.mask = par->use_mask ? avg_v2c(v.g(ASX_MASK_VC), I) : 1.0;
+    in.cv.dens = v.g(ASX_DENS).at(I);
+    for (int i = 0; i < 3; ++i) in.cv.mom[i] = v.g(ASX_MOMX + i).at(I);
+    in.cv.tau = v.g(ASX_TAU).at(I);
+    in.cv.DYe = v.g(ASX_DYE).at(I);
+    in.cv.DEnt = v.g(ASX_DENT).at(I);
+    for (int i = 0; i < 3; ++i) in.cv.dB[i] = v.g(ASX_DBX + i).at(I);
+    in.saved_rho = v.g(ASX_SAVED_RHO).at(I);
+    in.saved_eps = v.g(ASX_SAVED_EPS).at(I);
+    in.saved_Ye = v.g(ASX_SAVED_YE).at(I);
+    for (int i = 0; i < 3; ++i) in.saved_vel[i] = v.g(ASX_SAVED_VELX + i).at(I);
+    in.temperature = v.g(ASX_TEMPERATURE).at(I);
+    C2PCellOut out;
+    con2prim_cell(c.eos, c.poly, dp, in, out);
+    v.g(ASX_CON2PRIM_FLAG).at(I) = out.flag;
+    if (v.g(ASX_MASK_CC).ok()) v.g(ASX_MASK_CC).at(I) = out.mask_cc;
+    const Prim &pv = out.pv;
+    const double wlor = pv.w_lor;
+    v.g(ASX_RHO).at(I) = pv.rho;
+    v.g(ASX_EPS).at(I) = pv.eps;
+    v.g(ASX_YE).at(I) = pv.Ye;
+    v.g(ASX_PRESS).at(I) = pv.press;
+    v.g(ASX_TEMPERATURE).at(I) = pv.temperature;
+    v.g(ASX_ENTROPY).at(I) = pv.entropy;
+    for (int i = 0; i < 3; ++i) v.g(ASX_VELX + i).at(I) = pv.vel[i];
+    for (int i = 0; i < 3; ++i) v.g(ASX_BVECX + i).at(I) = pv.B[i];
+    for (int i = 0; i < 3; ++i) v.g(ASX_ZVEC_X + i).at(I) = wlor * pv.vel[i];
+    for (int i = 0; i < 3; ++i)
+      v.g(ASX_SVEC_X + i).at(I) =
+          (pv.rho + pv.rho * pv.eps + pv.press) * wlor * wlor * pv.vel[i];
+    const Cons &cv = out.cv;
+    v.g(ASX_DENS).at(I) = cv.dens;
+    for (int i = 0; i < 3; ++i) v.g(ASX_MOMX + i).at(I) = cv.mom[i];
+    v.g(ASX_TAU).at(I) = cv.tau;
+    v.g(ASX_DYE).at(I) = cv.DYe;
+    v.g(ASX_DENT).at(I) = cv.DEnt;
+    for (int i = 0; i < 3; ++i) v.g(ASX_DBX + i).at(I) = cv.dB[i];
+    v.g(ASX_SAVED_RHO).at(I) = pv.rho;
+    for (int i = 0; i < 3; ++i) v.g(ASX_SAVED_VELX + i).at(I) = pv.vel[i];
+    v.g(ASX_SAVED_EPS).at(I) = pv.eps;
+    v.g(ASX_SAVED_YE).at(I) = pv.Ye;
+  });
+  return ASX_OK;
+}
+
+// dst = scale*dst + sum_k fac[k]*src[k] over every point.
+int orc_state_lincomb(const asx_patch *pt, const asx_state *dst, double scale,
+                      int nsrc, const double *fac, const asx_state *const *src) {
+  if (!pt || !dst || (nsrc > 0 && (!fac || !src))) return ASX_ERR_NULL;
+  static const int slot[ASX_NUM_STATE] = {ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ,
+                                          ASX_TAU,  ASX_DENT, ASX_DYE,  ASX_AVEC_X,
+                                          ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI};
+  for (int q = 0; q < ASX_NUM_STATE; ++q) {
+    const int c = asx_gf_centering(slot[q]);
+    const int64_t ex = pt->n[0] + 2 * pt->ng + (c & 1);
+    const int64_t ey = pt->n[1] + 2 * pt->ng + ((c >> 1) & 1);
+    const int64_t ez = pt->n[2] + 2 * pt->ng + ((c >> 2) & 1);
+    const int64_t np = ex * ey * ez;
+    double *d = dst->gf[q].data;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < np; ++p) {
+      double acc = (scale == 0.0) ? 0.0 : scale * d[p];
+      for (int k = 0; k < nsrc; ++k) acc += fac[k] * src[k]->gf[q].data[p];
+      d[p] = acc;
+    }
+  }
+  return ASX_OK;
+}
+
+int orc_bc_fill(const asx_patch *pt, int slot, int kind) {
+  if (!pt || !pt->gf[slot].data) return ASX_ERR_NULL;
+  const int c = asx_gf_centering(slot);
+  GF g{pt->gf[slot].data, pt->gf[slot].sj, pt->gf[slot].sk};
+  int ext[3];
+  for (int d = 0; d < 3; ++d) ext[d] = pt->n[d] + 2 * pt->ng + ((c >> d) & 1);
+  const int ng = pt->ng;
+  for (int d = 0; d < 3; ++d) {
+    const int cd = (c >> d) & 1;
+    const int lo = ng, hi = ng + pt->n[d] - 1 + cd;
+    const int a = (d + 1) % 3, b = (d + 2) % 3;
+    for (int ib = 0; ib < ext[b]; ++ib)
+      for (int ia = 0; ia < ext[a]; ++ia) {
+        auto at = [&](int q) -> double & {
+          int O[3];
+          O[d] = q;
+          O[a] = ia;
+          O[b] = ib;
+          return g.at(O);
+        };
+        for (int q = lo - 1; q >= 0; --q) {
+          if (kind == ASX_BC_LINEAR)
+            at(q) = at(lo) + (lo - q) * (at(lo) - at(lo + 1));
+          else if (kind == ASX_BC_PERIODIC)
+            at(q) = at(q + pt->n[d]);
+          else
+            at(q) = at(lo);
+        }
+        for (int q = hi + 1; q < ext[d]; ++q) {
+          if (kind == ASX_BC_LINEAR)
+            at(q) = at(hi) + (q - hi) * (at(hi) - at(hi - 1));
+          else if (kind == ASX_BC_PERIODIC)
+            at(q) = at(q - pt->n[d]);
+          else
+            at(q) = at(hi);
+        }
+      }
+  }
+  return ASX_OK;
+}
+
+// Fused-substep equivalent on the CPU: dB_stag from A on every face, fluxes
+// with face_halo = 1, sources, RHS, then out = c0*u0 + c1*u + c2*rhs on
+// interior points (c0 == 0: u0 not read).  Same arithmetic as the unfused
+// chain; used as the CPU baseline of the stencil half of a substep.
+int orc_substep(const asx_patch *pt, const asx_state *u0, const asx_state *out,
+                double c0, double c1, double c2, const asx_params *par) {
+  int st = orc_bstag_from_A_halo(pt, 1);
+  if (st) return st;
+  st = orc_fluxes(pt, par, 1);
+  if (st) return st;
+  st = orc_source_terms(pt, par);
+  if (st) return st;
+  st = orc_rhs(pt, par);
+  if (st) return st;
+  static const int sslot[ASX_NUM_STATE] = {ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ,
+                                           ASX_TAU,  ASX_DENT, ASX_DYE,  ASX_AVEC_X,
+                                           ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI};
+  static const int rslot[ASX_NUM_STATE] = {
+      ASX_DENSRHS, ASX_MOMXRHS, ASX_MOMYRHS, ASX_MOMZRHS, ASX_TAURHS, ASX_DENTRHS,
+      ASX_DYE_RHS, ASX_AVEC_X_RHS, ASX_AVEC_Y_RHS, ASX_AVEC_Z_RHS, ASX_PSI_RHS};
+  View v{pt};
+  for (int q = 0; q < ASX_NUM_STATE; ++q) {
+    const GF u = v.g(sslot[q]), r = v.g(rslot[q]);
+    const GF o{out->gf[q].data, out->gf[q].sj, out->gf[q].sk};
+    const GF z = (c0 != 0.0) ? GF{u0->gf[q].data, u0->gf[q].sj, u0->gf[q].sk} : u;
+    int halo[3] = {0, 0, 0}, lo[3], hi[3];
+    interior(pt, asx_gf_centering(sslot[q]), halo, lo, hi);
+    loop_box(lo, hi, [&](const int I[3]) {
+      double acc = (c0 != 0.0) ? c0 * z.at(I) + c1 * u.at(I) : c1 * u.at(I);
+      o.at(I) = acc + c2 * r.at(I);
+    });
+  }
+  return ASX_OK;
+}
+
+// Con2PrimFactory call on independent zones, same layout as asx_c2p_batch.
+int orc_c2p_batch(int64_t nz, const double *cons, const double *g,
+                  const double *seed, double *prim, double *flag, int32_t *iters,
+                  const asx_params *par) {
+  int st = check_c2p_par(par);
+  if (st) return st;
+  IdealGas eos;
+  eos.init(par->gl_gamma, par->particle_mass, par->eps_min, par->eps_max,
+           par->rho_min, par->rho_max, par->ye_min, par->ye_max);
+  Polytrope poly;
+  poly.init(par->poly_gamma, par->poly_k);
+  const C2PDriver dp = make_driver(par);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int64_t z = 0; z < nz; ++z) {
+    C2PCellIn in;
+    in.r = 0.0;
+    in.alp = 1.0;
+    in.glo = Sym3{g[0 * nz + z], g[1 * nz + z], g[2 * nz + z],
+                  g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+    in.mask = 1.0;
+    in.cv.dens = cons[0 * nz + z];
+    for (int i = 0; i < 3; ++i) in.cv.mom[i] = cons[(1 + i) * nz + z];
+    in.cv.tau = cons[4 * nz + z];
+    for (int i = 0; i < 3; ++i) in.cv.dB[i] = cons[(5 + i) * nz + z];
+    in.cv.DYe = in.cv.dens * par->Ye_atmo;
+    in.cv.DEnt = 0.0;
+    in.saved_rho = seed[0 * nz + z];
+    in.saved_eps = seed[1 * nz + z];
+    in.saved_Ye = par->Ye_atmo;
+    for (int i = 0; i < 3; ++i) in.saved_vel[i] = seed[(2 + i) * nz + z];
+    in.temperature = 0.0;
+    C2PCellOut out;
+    con2prim_cell(eos, poly, dp, in, out);
+    const Prim &pv = out.pv;
+    prim[0 * nz + z] = pv.rho;
+    prim[1 * nz + z] = pv.eps;
+    prim[2 * nz + z] = pv.press;
+    for (int i = 0; i < 3; ++i) prim[(3 + i) * nz + z] = pv.vel[i];
+    prim[6 * nz + z] = pv.w_lor;
+    prim[7 * nz + z] = pv.temperature;
+    flag[z] = out.flag;
+    iters[z] = out.iters;
+  }
+  return ASX_OK;
+}
+
+// cons_vars::from_prim on independent zones (Con2PrimFactory/src/cons.hxx:49-91)
+// prim[9][nz] = rho, eps, velx, vely, velz, Bx, By, Bz (undensitized), w_lor
+// is recomputed from v; cons[8][nz] as in asx_c2p_batch.
+int orc_prim2con_batch(int64_t nz, const double *prim, const double *g,
+                       double *cons, const asx_params *par) {
+  IdealGas eos;
+  eos.init(par->gl_gamma, par->particle_mass, par->eps_min, par->eps_max,
+           par->rho_min, par->rho_max, par->ye_min, par->ye_max);
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < nz; ++z) {
+    Sym3 gl{g[0 * nz + z], g[1 * nz + z], g[2 * nz + z],
+            g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+    Prim pv;
+    pv.rho = prim[0 * nz + z];
+    pv.eps = prim[1 * nz + z];
+    for (int i = 0; i < 3; ++i) pv.vel[i] = prim[(2 + i) * nz + z];
+    for (int i = 0; i < 3; ++i) pv.B[i] = prim[(5 + i) * nz + z];
+    pv.press = eos.press_from_rho_eps(pv.rho, pv.eps);
+    pv.Ye = par->Ye_atmo;
+    pv.entropy = 0.0;
+    double v_low[3];
+    lower(gl, pv.vel, v_low);
+    pv.w_lor = 1.0 / std::sqrt(1.0 - dot3(v_low, pv.vel));
+    Cons cv;
+    cv.from_prim(pv, gl);
+    cons[0 * nz + z] = cv.dens;
+    for (int i = 0; i < 3; ++i) cons[(1 + i) * nz + z] = cv.mom[i];
+    cons[4 * nz + z] = cv.tau;
+    for (int i = 0; i < 3; ++i) cons[(5 + i) * nz + z] = cv.dB[i];
+  }
+  return ASX_OK;
+}
+
+} // extern "C"
