@@ -1,0 +1,216 @@
+// Con2prim kernels for gfx950: AsterX_Con2Prim on a patch (optionally fused
+// with ComputedBstagFromA + ComputedBFromdBstag) and Con2PrimFactory solves
+// on independent zones.  One work-item per cell; flag and iteration
+// histograms are reduced in LDS per block before one atomic per bin.
+#include "asx_kernels.hpp"
+
+namespace asx {
+
+struct HistLDS {
+  unsigned int flags[8];
+  unsigned int iters[64];
+};
+
+ASX_DEV void hist_init(HistLDS &h) {
+  for (int q = threadIdx.x; q < 72; q += blockDim.x) {
+    if (q < 8) h.flags[q] = 0;
+    else h.iters[q - 8] = 0;
+  }
+}
+ASX_DEV void hist_flush(HistLDS &h, unsigned long long *g) {
+  for (int q = threadIdx.x; q < 71; q += blockDim.x) {
+    const unsigned v = q < 7 ? h.flags[q] : h.iters[q - 7];
+    if (v) atomicAdd(&g[q], (unsigned long long)v);
+  }
+}
+
+// Densitized cell B from the vector potential: curl at the six faces, then
+// the face -> cell average (computeBfromA.cxx:32-55 and :71-84).
+ASX_DEV void dB_from_A(const KPatch &P, int i, int j, int k, double dB[3]) {
+  const double idx[3] = {1 / P.dx[0], 1 / P.dx[1], 1 / P.dx[2]};
+  const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
+  auto bx = [&](int ii, int jj, int kk) {
+    const int oz = off(P, 3, ii, jj, kk), oy = off(P, 5, ii, jj, kk);
+    return idx[1] * (Az[oz + P.sj[3]] - Az[oz]) - idx[2] * (Ay[oy + P.sk[5]] - Ay[oy]);
+  };
+  auto by = [&](int ii, int jj, int kk) {
+    const int ox = off(P, 6, ii, jj, kk), oz = off(P, 3, ii, jj, kk);
+    return idx[2] * (Ax[ox + P.sk[6]] - Ax[ox]) - idx[0] * (Az[oz + 1] - Az[oz]);
+  };
+  auto bz = [&](int ii, int jj, int kk) {
+    const int oy = off(P, 5, ii, jj, kk), ox = off(P, 6, ii, jj, kk);
+    return idx[0] * (Ay[oy + 1] - Ay[oy]) - idx[1] * (Ax[ox + P.sj[6]] - Ax[ox]);
+  };
+  dB[0] = 0.5 * (bx(i, j, k) + bx(i + 1, j, k));
+  dB[1] = 0.5 * (by(i, j, k) + by(i, j + 1, k));
+  dB[2] = 0.5 * (bz(i, j, k) + bz(i, j, k + 1));
+}
+
+template <bool FROM_A>
+__global__ __launch_bounds__(256) void k_con2prim(KPatch P, Box B, IdealGas eos, C2PArgs a,
+                                                  unsigned long long *hist) {
+  __shared__ HistLDS h;
+  if (hist) {
+    hist_init(h);
+    __syncthreads();
+  }
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = t < B.total();
+  if (live) {
+    int i, j, k;
+    B.ijk(t, i, j, k);
+    const int oc = off(P, 0, i, j, k), ov = off(P, 7, i, j, k);
+    const int sj = P.sj[7], sk = P.sk[7];
+    C2PIn in;
+    const double x = P.x0[0] + (i - P.ng + 0.5) * P.dx[0];
+    const double y = P.x0[1] + (j - P.ng + 0.5) * P.dx[1];
+    const double z = P.x0[2] + (k - P.ng + 0.5) * P.dx[2];
+    in.r = sqrt(x * x + y * y + z * z);
+    in.alp = avg_v2c(P.g[ASX_ALP], ov, sj, sk);
+    in.glo = Sym3{avg_v2c(P.g[ASX_GXX], ov, sj, sk), avg_v2c(P.g[ASX_GXY], ov, sj, sk),
+                  avg_v2c(P.g[ASX_GXZ], ov, sj, sk), avg_v2c(P.g[ASX_GYY], ov, sj, sk),
+                  avg_v2c(P.g[ASX_GYZ], ov, sj, sk), avg_v2c(P.g[ASX_GZZ], ov, sj, sk)};
+    in.mask = a.use_mask ? avg_v2c(P.g[ASX_MASK_VC], ov, sj, sk) : 1.0;
+    in.cv.dens = P.g[ASX_DENS][oc];
+    in.cv.mom[0] = P.g[ASX_MOMX][oc];
+    in.cv.mom[1] = P.g[ASX_MOMY][oc];
+    in.cv.mom[2] = P.g[ASX_MOMZ][oc];
+    in.cv.tau = P.g[ASX_TAU][oc];
+    in.cv.DYe = P.g[ASX_DYE][oc];
+    in.cv.DEnt = P.g[ASX_DENT][oc];
+    if (FROM_A) {
+      dB_from_A(P, i, j, k, in.cv.dB);
+    } else {
+      in.cv.dB[0] = P.g[ASX_DBX][oc];
+      in.cv.dB[1] = P.g[ASX_DBY][oc];
+      in.cv.dB[2] = P.g[ASX_DBZ][oc];
+    }
+    in.saved_rho = P.g[ASX_SAVED_RHO][oc];
+    in.saved_eps = P.g[ASX_SAVED_EPS][oc];
+    in.saved_Ye = P.g[ASX_SAVED_YE][oc];
+    in.saved_vel[0] = P.g[ASX_SAVED_VELX][oc];
+    in.saved_vel[1] = P.g[ASX_SAVED_VELY][oc];
+    in.saved_vel[2] = P.g[ASX_SAVED_VELZ][oc];
+    in.temperature = P.g[ASX_TEMPERATURE][oc];
+    C2POut o;
+    con2prim_cell(eos, a, in, o);
+    const Prim &pv = o.pv;
+    const double w = pv.w_lor;
+    P.g[ASX_CON2PRIM_FLAG][oc] = o.flag;
+    if (P.g[ASX_MASK_CC]) P.g[ASX_MASK_CC][oc] = o.mask_cc;
+    P.g[ASX_RHO][oc] = pv.rho;
+    P.g[ASX_EPS][oc] = pv.eps;
+    P.g[ASX_YE][oc] = pv.Ye;
+    P.g[ASX_PRESS][oc] = pv.press;
+    P.g[ASX_TEMPERATURE][oc] = pv.temperature;
+    P.g[ASX_ENTROPY][oc] = pv.entropy;
+    P.g[ASX_VELX][oc] = pv.vel[0];
+    P.g[ASX_VELY][oc] = pv.vel[1];
+    P.g[ASX_VELZ][oc] = pv.vel[2];
+    P.g[ASX_BVECX][oc] = pv.B[0];
+    P.g[ASX_BVECY][oc] = pv.B[1];
+    P.g[ASX_BVECZ][oc] = pv.B[2];
+    P.g[ASX_ZVEC_X][oc] = w * pv.vel[0];
+    P.g[ASX_ZVEC_Y][oc] = w * pv.vel[1];
+    P.g[ASX_ZVEC_Z][oc] = w * pv.vel[2];
+    const double hw = (pv.rho + pv.rho * pv.eps + pv.press) * w * w;
+    P.g[ASX_SVEC_X][oc] = hw * pv.vel[0];
+    P.g[ASX_SVEC_Y][oc] = hw * pv.vel[1];
+    P.g[ASX_SVEC_Z][oc] = hw * pv.vel[2];
+    const Cons &cv = o.cv;
+    P.g[ASX_DENS][oc] = cv.dens;
+    P.g[ASX_MOMX][oc] = cv.mom[0];
+    P.g[ASX_MOMY][oc] = cv.mom[1];
+    P.g[ASX_MOMZ][oc] = cv.mom[2];
+    P.g[ASX_TAU][oc] = cv.tau;
+    P.g[ASX_DYE][oc] = cv.DYe;
+    P.g[ASX_DENT][oc] = cv.DEnt;
+    P.g[ASX_DBX][oc] = cv.dB[0];
+    P.g[ASX_DBY][oc] = cv.dB[1];
+    P.g[ASX_DBZ][oc] = cv.dB[2];
+    P.g[ASX_SAVED_RHO][oc] = pv.rho;
+    P.g[ASX_SAVED_VELX][oc] = pv.vel[0];
+    P.g[ASX_SAVED_VELY][oc] = pv.vel[1];
+    P.g[ASX_SAVED_VELZ][oc] = pv.vel[2];
+    P.g[ASX_SAVED_EPS][oc] = pv.eps;
+    P.g[ASX_SAVED_YE][oc] = pv.Ye;
+    if (hist) {
+      atomicAdd(&h.flags[(int)o.flag], 1u);
+      atomicAdd(&h.iters[o.iters < 63 ? o.iters : 63], 1u);
+    }
+  }
+  if (hist) {
+    __syncthreads();
+    hist_flush(h, hist);
+  }
+}
+
+// Independent zones (layout of asx_c2p_batch in include/asterx.h).
+__global__ __launch_bounds__(256) void k_c2p_batch(int64_t nz, const double *cons,
+                                                   const double *g, const double *seed,
+                                                   double *prim, double *flag,
+                                                   int32_t *iters, IdealGas eos,
+                                                   C2PArgs a) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  C2PIn in;
+  in.r = 0.0;
+  in.alp = 1.0;
+  in.mask = 1.0;
+  in.glo = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  in.cv.dens = cons[z];
+  in.cv.mom[0] = cons[nz + z];
+  in.cv.mom[1] = cons[2 * nz + z];
+  in.cv.mom[2] = cons[3 * nz + z];
+  in.cv.tau = cons[4 * nz + z];
+  in.cv.dB[0] = cons[5 * nz + z];
+  in.cv.dB[1] = cons[6 * nz + z];
+  in.cv.dB[2] = cons[7 * nz + z];
+  in.cv.DYe = in.cv.dens * a.Ye_atmo;
+  in.cv.DEnt = 0.0;
+  in.saved_rho = seed[z];
+  in.saved_eps = seed[nz + z];
+  in.saved_Ye = a.Ye_atmo;
+  in.saved_vel[0] = seed[2 * nz + z];
+  in.saved_vel[1] = seed[3 * nz + z];
+  in.saved_vel[2] = seed[4 * nz + z];
+  in.temperature = 0.0;
+  C2POut o;
+  con2prim_cell(eos, a, in, o);
+  prim[z] = o.pv.rho;
+  prim[nz + z] = o.pv.eps;
+  prim[2 * nz + z] = o.pv.press;
+  prim[3 * nz + z] = o.pv.vel[0];
+  prim[4 * nz + z] = o.pv.vel[1];
+  prim[5 * nz + z] = o.pv.vel[2];
+  prim[6 * nz + z] = o.pv.w_lor;
+  prim[7 * nz + z] = o.pv.temperature;
+  flag[z] = o.flag;
+  iters[z] = o.iters;
+}
+
+hipError_t launch_con2prim(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
+                           bool from_A, unsigned long long *hist, hipStream_t s) {
+  Box b;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = P.ng;
+    b.cnt[d] = P.n[d];
+  }
+  if (from_A)
+    hipLaunchKernelGGL(k_con2prim<true>, b.grid(), dim3(256), 0, s, P, b, eos, a, hist);
+  else
+    hipLaunchKernelGGL(k_con2prim<false>, b.grid(), dim3(256), 0, s, P, b, eos, a, hist);
+  return hipGetLastError();
+}
+
+hipError_t launch_c2p_batch(int64_t nz, const double *cons, const double *g,
+                            const double *seed, double *prim, double *flag,
+                            int32_t *iters, const IdealGas &eos, const C2PArgs &a,
+                            hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_c2p_batch, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, s, nz,
+                     cons, g, seed, prim, flag, iters, eos, a);
+  return hipGetLastError();
+}
+
+} // namespace asx

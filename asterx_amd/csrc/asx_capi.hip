@@ -1,0 +1,718 @@
+// C ABI of the substep library (include/asterx.h): argument validation,
+// parameter resolution and kernel launches.  Host code only.
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "asx_kernels.hpp"
+
+using namespace asx;
+
+namespace {
+
+constexpr int kStateSlots[ASX_NUM_STATE] = {
+    ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ, ASX_TAU, ASX_DENT,
+    ASX_DYE,  ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI};
+
+int centering(int s) {
+  if (s >= ASX_ALP && s <= ASX_KZZ) return 7;
+  if (s >= ASX_RHO && s <= ASX_MASK_CC) return 0;
+  if (s == ASX_MASK_VC) return 7;
+  if (s == ASX_DBX_STAG) return 1;
+  if (s == ASX_DBY_STAG) return 2;
+  if (s == ASX_DBZ_STAG) return 4;
+  if (s == ASX_AVEC_X) return 6;
+  if (s == ASX_AVEC_Y) return 5;
+  if (s == ASX_AVEC_Z) return 3;
+  if (s == ASX_PSI) return 7;
+  if (s >= ASX_FXDENS && s <= ASX_FXBZ) return 1;
+  if (s >= ASX_FYDENS && s <= ASX_FYBZ) return 2;
+  if (s >= ASX_FZDENS && s <= ASX_FZBZ) return 4;
+  if (s == ASX_VTILDE_Y_XFACE || s == ASX_VTILDE_Z_XFACE) return 1;
+  if (s == ASX_VTILDE_X_YFACE || s == ASX_VTILDE_Z_YFACE) return 2;
+  if (s == ASX_VTILDE_X_ZFACE || s == ASX_VTILDE_Y_ZFACE) return 4;
+  if (s == ASX_AMAX_XFACE || s == ASX_AMIN_XFACE) return 1;
+  if (s == ASX_AMAX_YFACE || s == ASX_AMIN_YFACE) return 2;
+  if (s == ASX_AMAX_ZFACE || s == ASX_AMIN_ZFACE) return 4;
+  if (s >= ASX_FX && s <= ASX_G) return 7;
+  if (s >= ASX_DENSRHS && s <= ASX_DYE_RHS) return 0;
+  if (s == ASX_AVEC_X_RHS) return 6;
+  if (s == ASX_AVEC_Y_RHS) return 5;
+  if (s == ASX_AVEC_Z_RHS) return 3;
+  if (s == ASX_PSI_RHS) return 7;
+  return -1;
+}
+
+void strides(const int32_t n[3], int32_t ng, int c, int64_t &sj, int64_t &sk) {
+  const int64_t ex = n[0] + 2 * ng + (c & 1);
+  const int64_t ey = n[1] + 2 * ng + ((c >> 1) & 1);
+  sj = ex;
+  sk = ex * ey;
+}
+
+int64_t npoints(const int32_t n[3], int32_t ng, int c) {
+  int64_t p = 1;
+  for (int d = 0; d < 3; ++d) p *= n[d] + 2 * ng + ((c >> d) & 1);
+  return p;
+}
+
+asx_status check_geom(const asx_patch *pt) {
+  if (!pt) return ASX_ERR_NULL;
+  for (int d = 0; d < 3; ++d)
+    if (pt->n[d] < 1 || !(pt->dx[d] > 0.0)) return ASX_ERR_SIZE;
+  if (pt->ng < 1) return ASX_ERR_GHOSTS;
+  // int32 offsets inside kernels
+  if (npoints(pt->n, pt->ng, 7) >= (int64_t(1) << 31)) return ASX_ERR_SIZE;
+  return ASX_OK;
+}
+
+asx_status need(const asx_patch *pt, std::initializer_list<int> slots) {
+  for (int s : slots) {
+    const asx_gf &g = pt->gf[s];
+    if (!g.data) return ASX_ERR_NULL;
+    int64_t sj, sk;
+    strides(pt->n, pt->ng, centering(s), sj, sk);
+    if (g.sj != sj || g.sk != sk) return ASX_ERR_LAYOUT;
+  }
+  return ASX_OK;
+}
+asx_status need_range(const asx_patch *pt, int a, int b) {
+  for (int s = a; s <= b; ++s) {
+    asx_status st = need(pt, {s});
+    if (st) return st;
+  }
+  return ASX_OK;
+}
+asx_status opt(const asx_patch *pt, int s) {
+  if (!pt->gf[s].data) return ASX_OK;
+  return need(pt, {s});
+}
+
+KPatch kpatch(const asx_patch *pt) {
+  KPatch P;
+  for (int s = 0; s < ASX_NUM_GF; ++s) P.g[s] = pt->gf[s].data;
+  for (int c = 0; c < 8; ++c) {
+    int64_t sj, sk;
+    strides(pt->n, pt->ng, c, sj, sk);
+    P.sj[c] = (int)sj;
+    P.sk[c] = (int)sk;
+  }
+  for (int d = 0; d < 3; ++d) {
+    P.n[d] = pt->n[d];
+    P.dx[d] = pt->dx[d];
+    P.x0[d] = pt->x0[d];
+  }
+  P.ng = pt->ng;
+  return P;
+}
+
+Box box(const asx_patch *pt, int c, const int halo[3]) {
+  Box b;
+  for (int d = 0; d < 3; ++d) {
+    b.lo[d] = pt->ng - halo[d];
+    b.cnt[d] = pt->n[d] + ((c >> d) & 1) + 2 * halo[d];
+  }
+  return b;
+}
+
+RParams rparams(const asx_params *p) {
+  RParams r;
+  r.ppm_shock_detection = p->ppm_shock_detection;
+  r.ppm_zone_flattening = p->ppm_zone_flattening;
+  r.kgamma = p->poly_k * p->poly_gamma;
+  r.ppm_eta1 = p->ppm_eta1;
+  r.ppm_eta2 = p->ppm_eta2;
+  r.ppm_eps = p->ppm_eps;
+  r.ppm_eps_shock = p->ppm_eps_shock;
+  r.ppm_small = p->ppm_small;
+  r.ppm_omega1 = p->ppm_omega1;
+  r.ppm_omega2 = p->ppm_omega2;
+  r.weno_eps = p->weno_eps;
+  r.mp5_alpha = p->mp5_alpha;
+  return r;
+}
+
+// EOSX/src/setup_eos.cxx:284-305 + eos_3p_idealgas.hxx:117-135
+IdealGas ideal_gas(const asx_params *p) {
+  IdealGas e;
+  e.gamma = p->gl_gamma;
+  e.gm1 = p->gl_gamma - 1;
+  e.inv_gamma = 1 / p->gl_gamma;
+  e.eps_min = p->eps_min;
+  e.eps_max = p->eps_max;
+  if (e.gamma > 2) {
+    const double cap = 1 / (e.gamma * (e.gamma - 2));
+    e.eps_max = (cap < e.eps_max) ? cap : e.eps_max;
+  }
+  e.rho_min = p->rho_min;
+  e.rho_max = p->rho_max;
+  e.ye_min = p->ye_min;
+  e.ye_max = p->ye_max;
+  e.temp_over_eps = e.gm1 * p->particle_mass;
+  e.temp_min = e.temp_over_eps * e.eps_min;
+  e.temp_max = e.temp_over_eps * e.eps_max;
+  return e;
+}
+
+C2PArgs c2p_args(const asx_params *p) {
+  C2PArgs a;
+  a.prime = p->c2p_prime;
+  a.second = p->c2p_second;
+  a.max_iter = p->max_iter;
+  a.excise = p->excise;
+  a.use_mask = p->use_mask;
+  a.use_z = p->use_z;
+  a.use_temp = p->use_temperature;
+  a.use_press_atmo = p->use_press_atmo;
+  a.thermal_eos_atmo = p->thermal_eos_atmo;
+  a.tol = p->c2p_tol;
+  a.alp_thresh = p->alp_thresh;
+  a.cons_error = p->cons_error_limit;
+  a.vw_lim = p->vw_lim;
+  a.w_lim = std::sqrt(1.0 + p->vw_lim * p->vw_lim);
+  a.v_lim = p->vw_lim / a.w_lim;
+  a.Bsq_lim = p->B_lim * p->B_lim;
+  a.rho_BH = p->rho_BH;
+  a.eps_BH = p->eps_BH;
+  a.vwlim_BH = p->vwlim_BH;
+  a.rho_abs_min = p->rho_abs_min;
+  a.atmo_tol = p->atmo_tol;
+  a.p_atmo = p->p_atmo;
+  a.t_atmo = p->t_atmo;
+  a.Ye_atmo = p->Ye_atmo;
+  a.r_atmo = p->r_atmo;
+  a.n_rho_atmo = p->n_rho_atmo;
+  a.n_press_atmo = p->n_press_atmo;
+  a.n_temp_atmo = p->n_temp_atmo;
+  const double n = 1.0 / (p->poly_gamma - 1.0);
+  a.poly_gamma = p->poly_gamma;
+  a.poly_np1 = n + 1;
+  a.poly_invn = 1.0 / n;
+  a.poly_rho_p = std::pow(p->poly_k, -n);
+  // c2p_1DPalenzuela.hxx:388-390
+  a.minbits = int(std::fabs(std::log(p->c2p_tol)) / std::log(2.0));
+  a.tol0 = std::ldexp(1.0, -a.minbits);
+  return a;
+}
+
+asx_status check_par(const asx_params *p) {
+  if (!p) return ASX_ERR_NULL;
+  if (p->evolution_eos != ASX_EOS_IDEALGAS) return ASX_ERR_UNSUPPORTED;
+  if (p->reconstruction_method == ASX_RECON_EPPM) return ASX_ERR_UNSUPPORTED;
+  if (p->reconstruction_method < 0 || p->reconstruction_method > 6) return ASX_ERR_PARAM;
+  if (p->loworder_method < 0 || p->loworder_method > 2) return ASX_ERR_PARAM;
+  if (p->flux_type != ASX_FLUX_LXF && p->flux_type != ASX_FLUX_HLLE) return ASX_ERR_PARAM;
+  if (p->recon_type < 0 || p->recon_type > 2) return ASX_ERR_PARAM;
+  if (p->vector_potential_gauge != ASX_GAUGE_ALGEBRAIC &&
+      p->vector_potential_gauge != ASX_GAUGE_GENERALIZED_LORENZ)
+    return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+
+asx_status check_c2p_par(const asx_params *p) {
+  asx_status st = check_par(p);
+  if (st) return st;
+  if (p->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
+  if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA)
+    return ASX_ERR_UNSUPPORTED;
+  if (p->c2p_second != ASX_C2P_NOBLE && p->c2p_second != ASX_C2P_PALENZUELA &&
+      p->c2p_second != ASX_C2P_NONE)
+    return ASX_ERR_UNSUPPORTED;
+  if (p->max_iter < 1) return ASX_ERR_PARAM;
+  if (!(p->c2p_tol > 0.0)) return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+
+int recon_ghosts(int m) {
+  return m == ASX_RECON_GODUNOV ? 1
+         : (m == ASX_RECON_MINMOD || m == ASX_RECON_MONOCENTRAL) ? 2
+                                                                  : 3;
+}
+
+asx_status dev_status(hipError_t e) { return e == hipSuccess ? ASX_OK : ASX_ERR_DEVICE; }
+
+// Per-device buffers: counters, and face scratch per (stream, shape).
+struct Scratch {
+  std::vector<double *> bufs;
+  double *slot[ASX_NUM_GF] = {};
+};
+std::mutex g_mu;
+std::map<int, unsigned long long *> g_cnt;
+std::map<std::tuple<int, void *, int, int, int, int>, Scratch> g_scratch;
+
+unsigned long long *counter_buf(int dev) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_cnt.find(dev);
+  if (it != g_cnt.end()) return it->second;
+  unsigned long long *p = nullptr;
+  if (hipMalloc(&p, 80 * sizeof(unsigned long long)) != hipSuccess) return nullptr;
+  g_cnt[dev] = p;
+  return p;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// Scratch slots the fused substep computes (fluxes, UCT data, aux, dB_stag).
+bool scratch_slot(int s) {
+  return (s >= ASX_FXDENS && s <= ASX_AMIN_ZFACE) || (s >= ASX_FX && s <= ASX_G) ||
+         (s >= ASX_DBX_STAG && s <= ASX_DBZ_STAG);
+}
+
+Scratch *scratch_for(const asx_patch *pt, void *stream) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const auto key = std::make_tuple(dev, stream, pt->n[0], pt->n[1], pt->n[2], pt->ng);
+  auto it = g_scratch.find(key);
+  if (it != g_scratch.end()) return &it->second;
+  Scratch s;
+  for (int q = 0; q < ASX_NUM_GF; ++q) {
+    if (!scratch_slot(q)) continue;
+    double *p = nullptr;
+    const size_t bytes = (size_t)npoints(pt->n, pt->ng, centering(q)) * sizeof(double);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      for (double *b : s.bufs) (void)hipFree(b);
+      return nullptr;
+    }
+    s.bufs.push_back(p);
+    s.slot[q] = p;
+  }
+  return &(g_scratch[key] = s);
+}
+
+} // namespace
+
+extern "C" {
+
+const char *asx_version(void) { return "asterx-mi355x 0.1 (gfx950, FP64)"; }
+
+int asx_gf_centering(int slot) { return centering(slot); }
+
+void asx_gf_strides(const int32_t n[3], int32_t ng, int slot, int64_t *sj, int64_t *sk) {
+  strides(n, ng, centering(slot), *sj, *sk);
+}
+
+void asx_params_default(asx_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->reconstruction_method = ASX_RECON_GODUNOV;
+  p->loworder_method = ASX_RECON_GODUNOV;
+  p->reconstruct_with_temperature = 0;
+  p->ppm_shock_detection = 1;
+  p->ppm_zone_flattening = 0;
+  p->ppm_eta1 = 20.;
+  p->ppm_eta2 = 0.05;
+  p->ppm_eps = 0.33;
+  p->ppm_eps_shock = 0.01;
+  p->ppm_small = 1.e-7;
+  p->ppm_omega1 = 0.75;
+  p->ppm_omega2 = 10.;
+  p->enhanced_ppm_C2 = 1.25;
+  p->weno_eps = 1e-26;
+  p->mp5_alpha = 4.0;
+  p->vector_potential_gauge = ASX_GAUGE_GENERALIZED_LORENZ;
+  p->use_uct = 0;
+  p->use_entropy_fix = 0;
+  p->flux_type = ASX_FLUX_LXF;
+  p->recon_type = ASX_RECVAR_V;
+  p->local_spatial_order = 2;
+  p->excise = 0;
+  p->use_mask = 0;
+  p->lorenz_damp_fac = 0.0;
+  p->evolution_eos = ASX_EOS_IDEALGAS;
+  p->poly_gamma = 2.0;
+  p->poly_k = 100.0;
+  p->gl_gamma = 2.0;
+  p->rho_max = 1.0;
+  p->rho_min = 1.0e-40;
+  p->eps_max = 2.0;
+  p->eps_min = 0.0;
+  p->ye_max = 1.0;
+  p->ye_min = 0.0;
+  p->particle_mass = 1.0;
+  p->c2p_prime = ASX_C2P_NOBLE;
+  p->c2p_second = ASX_C2P_PALENZUELA;
+  p->max_iter = 30;
+  p->use_press_atmo = 0;
+  p->thermal_eos_atmo = 1;
+  p->Ye_lenient = 0;
+  p->use_z = 0;
+  p->use_temperature = 0;
+  p->rho_abs_min = 1e-13;
+  p->atmo_tol = 1e-3;
+  p->eps_atmo = 1e-10;
+  p->p_atmo = 1e-10;
+  p->t_atmo = 1e-2;
+  p->Ye_atmo = 0.5;
+  p->B_lim = 100.0;
+  p->vw_lim = 10.0;
+  p->c2p_tol = 1e-10;
+  p->r_atmo = 1.0e100;
+  p->n_rho_atmo = 0.0;
+  p->n_press_atmo = 0.0;
+  p->n_temp_atmo = 0.0;
+  p->alp_thresh = 0.0;
+  p->rho_BH = 1.0e100;
+  p->eps_BH = 1.0e100;
+  p->vwlim_BH = 1.0e100;
+  p->cons_error_limit = -1;
+}
+
+asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_halo,
+                      asx_counters *cnt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = check_par(par))) return st;
+  if (face_halo < 0 || face_halo > 1) return ASX_ERR_PARAM;
+  if (pt->ng < recon_ghosts(par->reconstruction_method) ||
+      pt->ng < recon_ghosts(par->loworder_method) || (face_halo > 0 && pt->ng < 3))
+    return ASX_ERR_GHOSTS;
+  if ((st = need_range(pt, ASX_ALP, ASX_GZZ))) return st;
+  if ((st = need(pt, {ASX_RHO, ASX_VELX, ASX_VELY, ASX_VELZ, ASX_PRESS, ASX_ENTROPY,
+                      ASX_YE, ASX_BVECX, ASX_BVECY, ASX_BVECZ, ASX_DBX_STAG,
+                      ASX_DBY_STAG, ASX_DBZ_STAG})))
+    return st;
+  if (par->reconstruct_with_temperature && (st = need(pt, {ASX_TEMPERATURE}))) return st;
+  if (par->recon_type == ASX_RECVAR_Z && (st = need_range(pt, ASX_ZVEC_X, ASX_ZVEC_Z)))
+    return st;
+  if (par->recon_type == ASX_RECVAR_S && (st = need_range(pt, ASX_SVEC_X, ASX_SVEC_Z)))
+    return st;
+  if ((st = need_range(pt, ASX_FXDENS, ASX_FZBZ))) return st;
+  const bool uct = pt->gf[ASX_AMAX_XFACE].data != nullptr;
+  if (uct && (st = need_range(pt, ASX_VTILDE_Y_XFACE, ASX_AMIN_ZFACE))) return st;
+  if (par->use_uct && !uct) return ASX_ERR_NULL;
+  const bool aux = pt->gf[ASX_FX].data != nullptr;
+  if (aux && ((st = need_range(pt, ASX_FX, ASX_G)) ||
+              (st = need(pt, {ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI}))))
+    return st;
+  hipStream_t s = (hipStream_t)stream;
+  const KPatch P = kpatch(pt);
+  unsigned long long *cd = nullptr;
+  if (cnt) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    cd = counter_buf(dev);
+    if (!cd) return ASX_ERR_DEVICE;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, cd, 80);
+  }
+  FluxArgs A;
+  A.rp = rparams(par);
+  A.eos = ideal_gas(par);
+  A.lo_method = par->loworder_method;
+  A.flux_type = par->flux_type;
+  A.rec_var = par->recon_type;
+  A.with_T = par->reconstruct_with_temperature;
+  A.write_uct = uct;
+  A.cnt_dev = cd;
+  for (int d = 0; d < 3; ++d) {
+    int halo[3] = {face_halo, face_halo, face_halo};
+    halo[d] = 0;
+    const Box b = box(pt, 1 << d, halo);
+    for (int q = 0; q < 3; ++q) {
+      A.lo[q] = b.lo[q];
+      A.cnt[q] = b.cnt[q];
+    }
+    if ((st = dev_status(launch_flux(d, P, A, par->reconstruction_method, s)))) return st;
+  }
+  if (aux) {
+    const int h = 2 * face_halo;
+    const int halo[3] = {h, h, h};
+    if ((st = dev_status(launch_aux(P, box(pt, 7, halo), s)))) return st;
+  }
+  if (cnt) {
+    unsigned long long hbuf[2];
+    if (hipMemcpyAsync(hbuf, cd, sizeof(hbuf), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return ASX_ERR_DEVICE;
+    cnt->nan_faces += (int64_t)hbuf[0];
+    cnt->lo_faces += (int64_t)hbuf[1];
+  }
+  return ASX_OK;
+}
+
+static RhsArgs rhs_args(const asx_params *par) {
+  RhsArgs A;
+  A.rp = rparams(par);
+  A.recon = par->reconstruction_method;
+  A.gauge = par->vector_potential_gauge;
+  A.use_uct = par->use_uct;
+  A.fd_order = par->local_spatial_order;
+  A.rec_var = par->recon_type;
+  A.lorenz_damp_fac = par->lorenz_damp_fac;
+  return A;
+}
+
+asx_status asx_source_terms(const asx_patch *pt, const asx_params *par, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (!par) return ASX_ERR_NULL;
+  if (par->local_spatial_order != 2 && par->local_spatial_order != 4) return ASX_ERR_PARAM;
+  if (par->local_spatial_order == 4 && pt->ng < 2) return ASX_ERR_GHOSTS;
+  if ((st = need_range(pt, ASX_ALP, ASX_KZZ))) return st;
+  if ((st = need(pt, {ASX_RHO, ASX_EPS, ASX_PRESS, ASX_BVECX, ASX_BVECY, ASX_BVECZ})))
+    return st;
+  if ((st = (par->recon_type == ASX_RECVAR_V) ? need(pt, {ASX_VELX, ASX_VELY, ASX_VELZ})
+                                              : need_range(pt, ASX_ZVEC_X, ASX_ZVEC_Z)))
+    return st;
+  if ((st = need_range(pt, ASX_DENSRHS, ASX_DYE_RHS))) return st;
+  const int halo[3] = {0, 0, 0};
+  return dev_status(launch_sources(kpatch(pt), box(pt, 0, halo), rhs_args(par),
+                                   (hipStream_t)stream));
+}
+
+asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = check_par(par))) return st;
+  if ((st = need_range(pt, ASX_FXDENS, ASX_FZBZ))) return st;
+  if ((st = need_range(pt, ASX_DENSRHS, ASX_PSI_RHS))) return st;
+  if ((st = need(pt, {ASX_ALP, ASX_BETAX, ASX_BETAY, ASX_BETAZ, ASX_PSI}))) return st;
+  if (par->vector_potential_gauge == ASX_GAUGE_GENERALIZED_LORENZ &&
+      (st = need_range(pt, ASX_FX, ASX_G)))
+    return st;
+  if (par->use_uct) {
+    if ((st = need_range(pt, ASX_VTILDE_Y_XFACE, ASX_AMIN_ZFACE))) return st;
+    if ((st = need(pt, {ASX_DBX_STAG, ASX_DBY_STAG, ASX_DBZ_STAG, ASX_PRESS, ASX_VELX,
+                        ASX_VELY, ASX_VELZ})))
+      return st;
+    if (pt->ng < recon_ghosts(par->reconstruction_method)) return ASX_ERR_GHOSTS;
+  }
+  if (pt->ng < 2) return ASX_ERR_GHOSTS;
+  return dev_status(launch_rhs(kpatch(pt), rhs_args(par), (hipStream_t)stream));
+}
+
+asx_status asx_bstag_from_A(const asx_patch *pt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = need(pt, {ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z, ASX_DBX_STAG, ASX_DBY_STAG,
+                      ASX_DBZ_STAG})))
+    return st;
+  const KPatch P = kpatch(pt);
+  const int halo[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d)
+    if ((st = dev_status(launch_bstag(d, P, box(pt, 1 << d, halo), pt->gf[ASX_DBX_STAG + d].data,
+                                      (hipStream_t)stream))))
+      return st;
+  return ASX_OK;
+}
+
+asx_status asx_dB_from_bstag(const asx_patch *pt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = need(pt, {ASX_DBX_STAG, ASX_DBY_STAG, ASX_DBZ_STAG, ASX_DBX, ASX_DBY, ASX_DBZ})))
+    return st;
+  const int halo[3] = {0, 0, 0};
+  return dev_status(launch_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream));
+}
+
+asx_status asx_B_from_dB(const asx_patch *pt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = need_range(pt, ASX_GXX, ASX_GZZ))) return st;
+  if ((st = need(pt, {ASX_DBX, ASX_DBY, ASX_DBZ, ASX_BVECX, ASX_BVECY, ASX_BVECZ}))) return st;
+  const int halo[3] = {0, 0, 0};
+  return dev_status(launch_B_from_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream));
+}
+
+asx_status asx_prim2con(const asx_patch *pt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = need_range(pt, ASX_GXX, ASX_GZZ))) return st;
+  if ((st = need(pt, {ASX_RHO, ASX_VELX, ASX_VELY, ASX_VELZ, ASX_EPS, ASX_PRESS,
+                      ASX_ENTROPY, ASX_YE, ASX_BVECX, ASX_BVECY, ASX_BVECZ})))
+    return st;
+  if ((st = need_range(pt, ASX_DENS, ASX_DBZ))) return st;
+  const int halo[3] = {0, 0, 0};
+  return dev_status(launch_prim2con(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream));
+}
+
+static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool from_A,
+                             asx_counters *cnt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((st = check_c2p_par(par))) return st;
+  if ((st = need(pt, {ASX_ALP, ASX_GXX, ASX_GXY, ASX_GXZ, ASX_GYY, ASX_GYZ, ASX_GZZ})))
+    return st;
+  if ((st = need_range(pt, ASX_RHO, ASX_DBZ))) return st;
+  if ((st = need_range(pt, ASX_SAVED_RHO, ASX_CON2PRIM_FLAG))) return st;
+  if ((st = opt(pt, ASX_MASK_CC))) return st;
+  if (par->use_mask && (st = need(pt, {ASX_MASK_VC}))) return st;
+  if (from_A && (st = need(pt, {ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z}))) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const KPatch P = kpatch(pt);
+  if (from_A && pt->gf[ASX_DBX_STAG].data) {
+    if ((st = need(pt, {ASX_DBY_STAG, ASX_DBZ_STAG}))) return st;
+    const int halo[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d)
+      if ((st = dev_status(launch_bstag(d, P, box(pt, 1 << d, halo),
+                                        pt->gf[ASX_DBX_STAG + d].data, s))))
+        return st;
+  }
+  unsigned long long *cd = nullptr;
+  if (cnt) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    cd = counter_buf(dev);
+    if (!cd) return ASX_ERR_DEVICE;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, cd, 80);
+  }
+  if ((st = dev_status(launch_con2prim(P, ideal_gas(par), c2p_args(par), from_A,
+                                       cd ? cd + 2 : nullptr, s))))
+    return st;
+  if (cnt) {
+    unsigned long long h[71];
+    if (hipMemcpyAsync(h, cd + 2, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return ASX_ERR_DEVICE;
+    for (int q = 0; q < 7; ++q) cnt->c2p_flags[q] += (int64_t)h[q];
+    for (int q = 0; q < 64; ++q) cnt->c2p_iters[q] += (int64_t)h[7 + q];
+  }
+  return ASX_OK;
+}
+
+asx_status asx_con2prim(const asx_patch *pt, const asx_params *par, asx_counters *cnt,
+                        void *stream) {
+  return c2p_common(pt, par, false, cnt, stream);
+}
+
+asx_status asx_con2prim_group(const asx_patch *pt, const asx_params *par,
+                              asx_counters *cnt, void *stream) {
+  return c2p_common(pt, par, true, cnt, stream);
+}
+
+asx_status asx_state_lincomb(const asx_patch *pt, const asx_state *dst, double scale,
+                             int32_t nsrc, const double *fac, const asx_state *const *src,
+                             void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (!dst || nsrc < 0 || nsrc > 8 || (nsrc > 0 && (!fac || !src))) return ASX_ERR_PARAM;
+  for (int q = 0; q < ASX_NUM_STATE; ++q) {
+    const int c = centering(kStateSlots[q]);
+    int64_t sj, sk;
+    strides(pt->n, pt->ng, c, sj, sk);
+    if (!dst->gf[q].data) return ASX_ERR_NULL;
+    if (dst->gf[q].sj != sj || dst->gf[q].sk != sk) return ASX_ERR_LAYOUT;
+    double *srcp[8];
+    for (int k = 0; k < nsrc; ++k) {
+      if (!src[k] || !src[k]->gf[q].data) return ASX_ERR_NULL;
+      if (src[k]->gf[q].sj != sj || src[k]->gf[q].sk != sk) return ASX_ERR_LAYOUT;
+      srcp[k] = src[k]->gf[q].data;
+    }
+    if ((st = dev_status(launch_lincomb(dst->gf[q].data, npoints(pt->n, pt->ng, c), scale,
+                                        nsrc, fac, srcp, (hipStream_t)stream))))
+      return st;
+  }
+  return ASX_OK;
+}
+
+asx_status asx_bc_fill(const asx_patch *pt, int32_t slot, int32_t kind, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (slot < 0 || slot >= ASX_NUM_GF) return ASX_ERR_PARAM;
+  if (kind < ASX_BC_LINEAR || kind > ASX_BC_COPY) return ASX_ERR_PARAM;
+  if ((st = need(pt, {slot}))) return st;
+  for (int d = 0; d < 3; ++d) {
+    const int c = (centering(slot) >> d) & 1;
+    if (kind == ASX_BC_LINEAR && pt->n[d] + c < 2) return ASX_ERR_SIZE;
+    if (kind == ASX_BC_PERIODIC && pt->n[d] < pt->ng) return ASX_ERR_SIZE;
+  }
+  return dev_status(launch_bc(kpatch(pt), slot, kind, (hipStream_t)stream));
+}
+
+asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_state *out,
+                       double c0, double c1, double c2, const asx_params *par,
+                       void *stream) {
+  asx_status st = check_geom(stage);
+  if (st) return st;
+  if ((st = check_par(par))) return st;
+  if (!out || (c0 != 0.0 && !u0)) return ASX_ERR_NULL;
+  // Upwind CT reconstructs face data three rows into the transverse ghost
+  // region, which the fused path does not compute; use the unfused entry
+  // points (with a SYNC of the face data) for use_uct.
+  if (par->use_uct) return ASX_ERR_UNSUPPORTED;
+  if (stage->ng < 3 || stage->ng < recon_ghosts(par->reconstruction_method))
+    return ASX_ERR_GHOSTS;
+  if ((st = need_range(stage, ASX_ALP, ASX_KZZ))) return st;
+  if ((st = need(stage, {ASX_RHO, ASX_VELX, ASX_VELY, ASX_VELZ, ASX_EPS, ASX_PRESS,
+                         ASX_ENTROPY, ASX_YE, ASX_BVECX, ASX_BVECY, ASX_BVECZ})))
+    return st;
+  if (par->reconstruct_with_temperature && (st = need(stage, {ASX_TEMPERATURE}))) return st;
+  if (par->recon_type == ASX_RECVAR_Z && (st = need_range(stage, ASX_ZVEC_X, ASX_ZVEC_Z)))
+    return st;
+  if (par->recon_type == ASX_RECVAR_S && (st = need_range(stage, ASX_SVEC_X, ASX_SVEC_Z)))
+    return st;
+  for (int q = 0; q < ASX_NUM_STATE; ++q) {
+    if ((st = need(stage, {kStateSlots[q]}))) return st;
+    const int c = centering(kStateSlots[q]);
+    int64_t sj, sk;
+    strides(stage->n, stage->ng, c, sj, sk);
+    if (!out->gf[q].data) return ASX_ERR_NULL;
+    if (out->gf[q].sj != sj || out->gf[q].sk != sk) return ASX_ERR_LAYOUT;
+    if (c0 != 0.0) {
+      if (!u0->gf[q].data) return ASX_ERR_NULL;
+      if (u0->gf[q].sj != sj || u0->gf[q].sk != sk) return ASX_ERR_LAYOUT;
+    }
+  }
+  hipStream_t s = (hipStream_t)stream;
+  Scratch *sc = scratch_for(stage, stream);
+  if (!sc) return ASX_ERR_DEVICE;
+  KPatch P = kpatch(stage);
+  for (int q = 0; q < ASX_NUM_GF; ++q)
+    if (sc->slot[q]) P.g[q] = sc->slot[q];
+  // dB_stag on every face whose curl stencil lies inside the arrays
+  for (int d = 0; d < 3; ++d) {
+    Box b;
+    for (int q = 0; q < 3; ++q) {
+      b.lo[q] = 0;
+      b.cnt[q] = (q == d) ? stage->n[q] + 2 * stage->ng + 1 : stage->n[q] + 2 * stage->ng;
+    }
+    if ((st = dev_status(launch_bstag(d, P, b, P.g[ASX_DBX_STAG + d], s)))) return st;
+  }
+  FluxArgs A;
+  A.rp = rparams(par);
+  A.eos = ideal_gas(par);
+  A.lo_method = par->loworder_method;
+  A.flux_type = par->flux_type;
+  A.rec_var = par->recon_type;
+  A.with_T = par->reconstruct_with_temperature;
+  A.write_uct = par->use_uct;
+  A.cnt_dev = nullptr;
+  for (int d = 0; d < 3; ++d) {
+    int halo[3] = {1, 1, 1};
+    halo[d] = 0;
+    const Box b = box(stage, 1 << d, halo);
+    for (int q = 0; q < 3; ++q) {
+      A.lo[q] = b.lo[q];
+      A.cnt[q] = b.cnt[q];
+    }
+    if ((st = dev_status(launch_flux(d, P, A, par->reconstruction_method, s)))) return st;
+  }
+  {
+    const int halo[3] = {2, 2, 2};
+    if ((st = dev_status(launch_aux(P, box(stage, 7, halo), s)))) return st;
+  }
+  StatePtrs pu0, pout;
+  for (int q = 0; q < ASX_NUM_STATE; ++q) {
+    pu0.g[q] = (c0 != 0.0) ? u0->gf[q].data : nullptr;
+    pout.g[q] = out->gf[q].data;
+  }
+  return dev_status(launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, s));
+}
+
+asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
+                         const double *seed, double *prim, double *flag, int32_t *iters,
+                         const asx_params *par, void *stream) {
+  asx_status st = check_c2p_par(par);
+  if (st) return st;
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !seed || !prim || !flag || !iters)) return ASX_ERR_NULL;
+  return dev_status(launch_c2p_batch(nz, cons, g, seed, prim, flag, iters, ideal_gas(par),
+                                     c2p_args(par), (hipStream_t)stream));
+}
+
+} // extern "C"

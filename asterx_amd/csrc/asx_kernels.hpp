@@ -1,0 +1,69 @@
+// Kernel argument structs and launcher declarations shared by the .hip
+// translation units and the C ABI layer.
+#pragma once
+
+#include "asx_c2p.hpp"
+#include "asx_device.hpp"
+
+namespace asx {
+
+// A box of points [lo, lo+cnt) flattened x-fastest.
+struct Box {
+  int lo[3], cnt[3];
+  ASX_DEV int64_t total() const { return (int64_t)cnt[0] * cnt[1] * cnt[2]; }
+  ASX_DEV void ijk(int64_t t, int &i, int &j, int &k) const {
+    i = lo[0] + (int)(t % cnt[0]);
+    j = lo[1] + (int)((t / cnt[0]) % cnt[1]);
+    k = lo[2] + (int)(t / ((int64_t)cnt[0] * cnt[1]));
+  }
+  __host__ int64_t htotal() const { return (int64_t)cnt[0] * cnt[1] * cnt[2]; }
+  __host__ dim3 grid() const { return dim3((unsigned)((htotal() + 255) / 256)); }
+};
+
+struct FluxArgs {
+  RParams rp;
+  IdealGas eos;
+  int lo_method, flux_type, rec_var, with_T, write_uct;
+  int lo[3], cnt[3];
+  unsigned long long *cnt_dev; // [nan_faces, lo_faces] or null
+};
+
+struct RhsArgs {
+  RParams rp;
+  int recon, gauge, use_uct, fd_order, rec_var;
+  double lorenz_damp_fac;
+};
+
+// Evolved state pointers (asx_state order) for the fused update.
+struct StatePtrs {
+  double *g[ASX_NUM_STATE];
+};
+
+hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method,
+                       hipStream_t s);
+hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s);
+hipError_t launch_bstag(int d, const KPatch &P, const Box &B, double *out,
+                        hipStream_t s);
+hipError_t launch_dB(const KPatch &P, const Box &B, hipStream_t s);
+hipError_t launch_B_from_dB(const KPatch &P, const Box &B, hipStream_t s);
+hipError_t launch_prim2con(const KPatch &P, const Box &B, hipStream_t s);
+
+hipError_t launch_sources(const KPatch &P, const Box &B, const RhsArgs &A,
+                          hipStream_t s);
+hipError_t launch_rhs(const KPatch &P, const RhsArgs &A, hipStream_t s);
+hipError_t launch_lincomb(double *dst, int64_t np, double scale, int nsrc,
+                          const double *fac, double *const *src, hipStream_t s);
+hipError_t launch_bc(const KPatch &P, int slot, int kind, hipStream_t s);
+hipError_t launch_update(const KPatch &P, const RhsArgs &A, const StatePtrs &u0,
+                         const StatePtrs &out, double c0, double c1, double c2,
+                         hipStream_t s);
+
+hipError_t launch_con2prim(const KPatch &P, const IdealGas &eos,
+                           const C2PArgs &a, bool with_bstag,
+                           unsigned long long *hist, hipStream_t s);
+hipError_t launch_c2p_batch(int64_t nz, const double *cons, const double *g,
+                            const double *seed, double *prim, double *flag,
+                            int32_t *iters, const IdealGas &eos,
+                            const C2PArgs &a, hipStream_t s);
+
+} // namespace asx

@@ -1,0 +1,71 @@
+"""Loader of the HIP library (libasterx_hip.so) with ctypes signatures.
+
+The product path runs only through this library: if it is missing the
+import fails loudly (there is no CPU fallback).
+"""
+import ctypes as C
+import os
+import subprocess
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(_HERE, "libasterx_hip.so")
+
+_P = C.POINTER(abi.asx_patch)
+_PP = C.POINTER(abi.asx_params)
+_S = C.POINTER(abi.asx_state)
+_CNT = C.POINTER(abi.asx_counters)
+_V = C.c_void_p
+
+SIGNATURES = {
+    "asx_fluxes": ([_P, _PP, C.c_int32, _CNT, _V], C.c_int),
+    "asx_source_terms": ([_P, _PP, _V], C.c_int),
+    "asx_rhs": ([_P, _PP, _V], C.c_int),
+    "asx_bstag_from_A": ([_P, _V], C.c_int),
+    "asx_dB_from_bstag": ([_P, _V], C.c_int),
+    "asx_B_from_dB": ([_P, _V], C.c_int),
+    "asx_prim2con": ([_P, _V], C.c_int),
+    "asx_con2prim": ([_P, _PP, _CNT, _V], C.c_int),
+    "asx_con2prim_group": ([_P, _PP, _CNT, _V], C.c_int),
+    "asx_state_lincomb": ([_P, _S, C.c_double, C.c_int32, C.POINTER(C.c_double),
+                           C.POINTER(_S), _V], C.c_int),
+    "asx_bc_fill": ([_P, C.c_int32, C.c_int32, _V], C.c_int),
+    "asx_substep": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, _V],
+                    C.c_int),
+    "asx_c2p_batch": ([C.c_int64, _V, _V, _V, _V, _V, _V, _PP, _V], C.c_int),
+    "asx_params_default": ([_PP], None),
+    "asx_gf_centering": ([C.c_int], C.c_int),
+    "asx_gf_strides": ([C.POINTER(C.c_int32), C.c_int32, C.c_int,
+                        C.POINTER(C.c_int64), C.POINTER(C.c_int64)], None),
+    "asx_version": ([], C.c_char_p),
+}
+
+_lib = None
+
+
+def build():
+    """Compile every HIP source for gfx950 into libasterx_hip.so (in-tree)."""
+    subprocess.check_call(["make", "-s", "-B", "-j8", "-C",
+                           os.path.join(_HERE, "csrc")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(
+                "libasterx_hip.so is not built (%s); run __graft_entry__.build()"
+                % LIBPATH)
+        L = C.CDLL(LIBPATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def backend(stream=None):
+    from .driver import Backend
+    return Backend(lib(), "asx_", stream)

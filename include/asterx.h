@@ -312,7 +312,8 @@ asx_status asx_bc_fill(const asx_patch *patch, int32_t slot, int32_t kind,
  * out = c0*u0 + c1*u + c2*rhs(u) on interior points, where u is the state
  * held in `stage` and rhs is SourceTerms + RHS of the reference.  u0 is not
  * read when c0 == 0.  Face scratch is allocated on first use per patch shape
- * and kept (one buffer set per device). */
+ * and kept (one buffer set per device and stream).  use_uct is not supported
+ * here (ASX_ERR_UNSUPPORTED): run the unfused entry points for it. */
 asx_status asx_substep(const asx_patch *stage, const asx_state *u0,
                        const asx_state *out, double c0, double c1, double c2,
                        const asx_params *par, void *stream);

@@ -4,8 +4,8 @@
 // the per-cell driver AsterX/src/con2prim.cxx:73-454, ideal-gas EOS.
 //
 // Algo::brent (CarpetX Algo/roots.hxx) is not in the reference tree; it is
-// restated from Brent's published method (Brent 1973, as in the Wikipedia
-// statement of the algorithm) returning the final bracket, with the
+// restated as Brent's ZERO (Brent 1973), the form the reference ships in
+// EOSX/src/utils/eos_brent.hxx, returning the final bracket, with the
 // termination rule the reference documents at c2p_1DPalenzuela.hxx:380-393.
 // Its iterate sequence is parity unpinned; Palenzuela results are pinned to
 // solver tolerance by the round-trip and TSV tests.
@@ -466,66 +466,73 @@ inline void noble_solve(const IdealGas &eos, const Atmo &atmo,
   }
 }
 
-// Brent's bracketing root finder (Brent 1973); returns the final bracket
-// {a, b} (b the best estimate).  Stops when |b-a| <= 2^-minbits*min(|a|,|b|),
-// f(b) == 0, or after maxiters iterations; iters = iterations done.
+// Brent's bracketing root finder in the form of Brent's ZERO (Brent 1973),
+// the structure the reference ships in EOSX/src/utils/eos_brent.hxx:7-165,
+// with the stopping rule of the Palenzuela call site
+// (c2p_1DPalenzuela.hxx:380-393): stop when the bracket [b, c] satisfies
+// |b - c| <= 2^-minbits min(|b|, |c|), when f(b) == 0, or after maxiters
+// evaluations.  Steps are at least half that width, so the bracket
+// collapses once b has converged.  Returns {b, c}; iters = evaluations.
 template <typename F>
 inline std::pair<double, double> brent(const F &fn, double a, double b,
                                        int minbits, int maxiters, int &iters) {
   const double rel = std::ldexp(1.0, -minbits);
-  double fa = fn(a), fb = fn(b);
+  double sa = a, sb = b;
+  double fa = fn(sa), fb = fn(sb);
   iters = 0;
-  if (fa == 0.0) return {a, a};
-  if (fb == 0.0) return {b, b};
-  if (fa * fb > 0.0) return {a, b}; // not bracketed: interval unchanged
-  if (std::fabs(fa) < std::fabs(fb)) {
-    std::swap(a, b);
-    std::swap(fa, fb);
-  }
-  double c = a, fc = fa, d = a;
-  bool mflag = true;
-  while (iters < maxiters) {
-    const double tol = rel * smin(std::fabs(a), std::fabs(b));
-    if (std::fabs(b - a) <= tol || fb == 0.0) break;
-    double s;
-    if (fa != fc && fb != fc) {
-      s = a * fb * fc / ((fa - fb) * (fa - fc)) +
-          b * fa * fc / ((fb - fa) * (fb - fc)) +
-          c * fa * fb / ((fc - fa) * (fc - fb));
-    } else {
-      s = b - fb * (b - a) / (fb - fa);
+  if (fa == 0.0) return {sa, sa};
+  if (fb == 0.0) return {sb, sb};
+  if (fa * fb > 0.0) return {sa, sb}; // not bracketed: interval unchanged
+  double c = sa, fc = fa, e = sb - sa, d = e;
+  for (;;) {
+    if (std::fabs(fc) < std::fabs(fb)) {
+      sa = sb; sb = c; c = sa;
+      fa = fb; fb = fc; fc = fa;
     }
-    const double q = (3 * a + b) / 4;
-    const bool c1 = !((s > smin(q, b)) && (s < smax(q, b)));
-    const bool c2 = mflag && (std::fabs(s - b) >= std::fabs(b - c) / 2);
-    const bool c3 = !mflag && (std::fabs(s - b) >= std::fabs(c - d) / 2);
-    const bool c4 = mflag && (std::fabs(b - c) < tol);
-    const bool c5 = !mflag && (std::fabs(c - d) < tol);
-    if (c1 || c2 || c3 || c4 || c5) {
-      s = (a + b) / 2;
-      mflag = true;
+    const double tol = 0.5 * rel * smin(std::fabs(sb), std::fabs(c));
+    const double m = 0.5 * (c - sb);
+    if (std::fabs(m) <= tol || fb == 0.0 || iters >= maxiters) break;
+    if (std::fabs(e) < tol || std::fabs(fa) <= std::fabs(fb)) {
+      e = m;
+      d = e;
     } else {
-      mflag = false;
+      double s = fb / fa, p, q;
+      if (sa == c) {
+        p = 2.0 * m * s;
+        q = 1.0 - s;
+      } else {
+        q = fa / fc;
+        const double r = fb / fc;
+        p = s * (2.0 * m * q * (q - r) - (sb - sa) * (r - 1.0));
+        q = (q - 1.0) * (r - 1.0) * (s - 1.0);
+      }
+      if (0.0 < p) q = -q;
+      else p = -p;
+      s = e;
+      e = d;
+      if (2.0 * p < 3.0 * m * q - std::fabs(tol * q) && p < std::fabs(0.5 * s * q)) {
+        d = p / q;
+      } else {
+        e = m;
+        d = e;
+      }
     }
-    const double fs = fn(s);
+    sa = sb;
+    fa = fb;
+    if (tol < std::fabs(d)) sb = sb + d;
+    else if (0.0 < m) sb = sb + tol;
+    else sb = sb - tol;
+    fb = fn(sb);
     ++iters;
-    d = c;
-    c = b;
-    fc = fb;
-    if (fa * fs < 0.0) {
-      b = s;
-      fb = fs;
-    } else {
-      a = s;
-      fa = fs;
-    }
-    if (std::fabs(fa) < std::fabs(fb)) {
-      std::swap(a, b);
-      std::swap(fa, fb);
+    if ((0.0 < fb && 0.0 < fc) || (fb <= 0.0 && fc <= 0.0)) {
+      c = sa;
+      fc = fa;
+      e = sb - sa;
+      d = e;
     }
   }
-  if (fb == 0.0) return {b, b};
-  return {a, b};
+  if (fb == 0.0) return {sb, sb};
+  return {sb, c};
 }
 
 // Con2PrimFactory/src/c2p_1DPalenzuela.hxx:260-304

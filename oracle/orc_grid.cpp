@@ -378,8 +378,10 @@ void calc_flux_face(const Ctx &c, int dir, const int I[3]) {
     const double amx = mx, amn = -1 * mn;
     amax.at(I) = amx;
     amin.at(I) = amn;
-    const GF vt1 = v.g(ASX_VTILDE_Y_XFACE + 2 * dir);
-    const GF vt2 = v.g(ASX_VTILDE_Y_XFACE + 2 * dir + 1);
+    // vtilde^(dir+1) and vtilde^(dir+2) on this face (fluxes.cxx:64-67);
+    // each face holds its two transverse components in increasing order.
+    const GF vt1 = v.g(ASX_VTILDE_Y_XFACE + 2 * dir + (d1 < d2 ? 0 : 1));
+    const GF vt2 = v.g(ASX_VTILDE_Y_XFACE + 2 * dir + (d1 < d2 ? 1 : 0));
     if (vt1.ok())
       vt1.at(I) = (amx * vt[d1][0] + amn * vt[d1][1]) / (amx + amn);
     if (vt2.ok())

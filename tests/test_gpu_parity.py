@@ -1,0 +1,253 @@
+"""HIP library vs CPU oracle, entry point by entry point, on seeded states.
+
+Bar: fluxes / RHS / sources agree to 1e-12 relative in max norm (most are
+bit-identical: both sides evaluate the same expressions without contraction);
+con2prim flags and atmosphere masks are bit-exact; primitives agree to the
+solver tolerance.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from asterx_amd import abi
+from states import make_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _gpu():
+    from asterx_amd import lib
+    return lib.backend()
+
+
+def _pair(**kw):
+    h = make_state(**kw)
+    d = h.to("cuda")
+    return h, d
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _cmp(h, d, names, halo=0, tol=TOL, centering_halo=None):
+    bad = {}
+    for n in names:
+        hh = halo if centering_halo is None else centering_halo(n)
+        sl = h.interior(n, 0)
+        if hh:
+            sl = _face_box(h, n, hh)
+        e = rel_err(d.numpy(n)[sl], h.numpy(n)[sl])
+        if not e <= tol:
+            bad[n] = e
+    assert not bad, bad
+
+
+def _face_box(p, name, halo):
+    """Interior box widened by `halo` on the transverse axes of a face field,
+    or on every axis for vertex fields."""
+    c = abi.centering(name)
+    sl = []
+    for d in (2, 1, 0):
+        v = (c >> d) & 1
+        h = halo
+        if c in (1, 2, 4) and v:
+            h = 0
+        lo = p.ng - h
+        hi = p.ng + p.n[d] + v + h
+        sl.append(slice(lo, hi))
+    return tuple(sl)
+
+
+RECONS = ["ppm", "wenoz", "mp5", "minmod", "monocentral", "Godunov"]
+
+
+@pytest.mark.parametrize("recon", RECONS)
+@pytest.mark.parametrize("flux", ["HLLE", "LxF"])
+def test_fluxes(recon, flux):
+    _torch()
+    h, d = _pair(zero_entropy_block=True)
+    par = abi.params(reconstruction_method=recon, flux_type=flux,
+                     gl_gamma=5.0 / 3.0, loworder_method="minmod")
+    g = _gpu()
+    cnt = abi.asx_counters()
+    g._call("fluxes", C.byref(d.struct()), C.byref(par), 1, C.byref(cnt), None)
+    _sync()
+    oracle.lib().orc_fluxes(C.byref(h.struct()), C.byref(par), 1)
+    assert cnt.lo_faces > 0  # the zeroed entropy block forces low order
+    _cmp(h, d, abi.FLUX + abi.UCT, halo=1)
+    _cmp(h, d, abi.AUX, halo=2)
+
+
+@pytest.mark.parametrize("rec_var", ["z_vec", "s_vec"])
+def test_fluxes_recvar(rec_var):
+    _torch()
+    h, d = _pair(curved=True)
+    par = abi.params(reconstruction_method="ppm", flux_type="HLLE",
+                     recon_type=rec_var, gl_gamma=5.0 / 3.0,
+                     reconstruct_with_temperature=1)
+    _gpu().fluxes(d.struct(), par, 0)
+    _sync()
+    oracle.lib().orc_fluxes(C.byref(h.struct()), C.byref(par), 0)
+    _cmp(h, d, abi.FLUX + abi.UCT)
+
+
+@pytest.mark.parametrize("order", [2, 4])
+@pytest.mark.parametrize("rec_var", ["v_vec", "z_vec"])
+def test_source_terms(order, rec_var):
+    _torch()
+    h, d = _pair(curved=True)
+    par = abi.params(local_spatial_order=order, recon_type=rec_var)
+    _gpu().source_terms(d.struct(), par)
+    _sync()
+    oracle.lib().orc_source_terms(C.byref(h.struct()), C.byref(par))
+    _cmp(h, d, ["densrhs", "momxrhs", "momyrhs", "momzrhs", "taurhs",
+                "DEntrhs", "DYe_rhs"])
+
+
+@pytest.mark.parametrize("uct", [0, 1])
+@pytest.mark.parametrize("gauge", ["generalized Lorentz", "algebraic"])
+def test_rhs(uct, gauge):
+    _torch()
+    h, d = _pair(curved=True)
+    par = abi.params(reconstruction_method="wenoz", flux_type="HLLE",
+                     use_uct=uct, vector_potential_gauge=gauge,
+                     lorenz_damp_fac=0.3)
+    g = _gpu()
+    for p, be in ((d, g), (h, oracle.backend())):
+        be.fluxes(p.struct(), par, 1)
+        be.source_terms(p.struct(), par)
+        be.rhs(p.struct(), par)
+    _sync()
+    _cmp(h, d, abi.RHS)
+
+
+def test_bfield_and_prim2con():
+    _torch()
+    h, d = _pair(curved=True)
+    g = _gpu()
+    for p, be in ((d, g), (h, oracle.backend())):
+        be.bstag_from_A(p.struct())
+        be.dB_from_bstag(p.struct())
+        be.B_from_dB(p.struct())
+        be.prim2con(p.struct())
+    _sync()
+    _cmp(h, d, abi.DBSTAG + abi.DB + ["Bvecx", "Bvecy", "Bvecz"] + abi.CONS,
+         tol=1e-14)
+
+
+C2P_OUT = ["rho", "velx", "vely", "velz", "eps", "press", "temperature",
+           "entropy", "Ye", "Bvecx", "Bvecy", "Bvecz"] + abi.CONS + abi.DB + \
+    abi.SAVED + abi.ZS
+
+
+@pytest.mark.parametrize("prime,second", [("Noble", "Palenzuela"),
+                                          ("Palenzuela", "None"),
+                                          ("Palenzuela", "Noble")])
+def test_con2prim(prime, second):
+    _torch()
+    h, d = _pair(curved=True)
+    # a few cells in the atmosphere and a few with poisoned state
+    for p in (h, d):
+        dens = p.numpy("dens").copy()
+        dens[5, 6, 7] = 1e-14
+        dens[6, 7, 8] = 2e-13
+        tau = p.numpy("tau").copy()
+        tau[7, 5, 6] = np.nan
+        p["dens"] = dens
+        p["tau"] = tau
+    par = abi.params(c2p_prime=prime, c2p_second=second, gl_gamma=5.0 / 3.0,
+                     rho_abs_min=1e-12)
+    cnt = abi.asx_counters()
+    _gpu()._call("con2prim", C.byref(d.struct()), C.byref(par), C.byref(cnt), None)
+    _sync()
+    oracle.lib().orc_con2prim(C.byref(h.struct()), C.byref(par))
+    fl_h = h.numpy("con2prim_flag")[h.interior("rho")]
+    fl_d = d.numpy("con2prim_flag")[h.interior("rho")]
+    assert np.array_equal(fl_h, fl_d)
+    assert (fl_h == 4).sum() >= 2 and (fl_h == 6).sum() >= 1
+    hist = np.bincount(fl_h.astype(int).ravel(), minlength=7)
+    assert list(hist) == list(cnt.c2p_flags)
+    tol = 1e-12 if prime == "Noble" else 1e-9
+    _cmp(h, d, C2P_OUT, tol=tol)
+
+
+def test_substep_fused_matches_unfused():
+    _torch()
+    from asterx_amd.patch import StateBuf
+    h, d = _pair(curved=True)
+    par = abi.params(reconstruction_method="mp5", flux_type="HLLE",
+                     gl_gamma=5.0 / 3.0)
+    c0, c1, c2 = 0.75, 0.25, 0.25 * 0.01
+    outs = []
+    for p, lib_, pre in ((d, _gpu().lib, "asx"), (h, oracle.lib(), "orc")):
+        u0 = StateBuf(p, fill=0.0)
+        out = StateBuf(p, fill=0.0)
+        # u0 = perturbed copy of the state
+        for name in abi.STATE:
+            u0.arr[name] += 0.0
+        for q, name in enumerate(abi.STATE):
+            src = p[name]
+            u0.arr[name] = src * 1.01 if pre == "orc" else src.clone() * 1.01
+        ps, u0s, outs_ = p.struct(), u0.struct(), out.struct()
+        if pre == "asx":
+            abi.check("asx_substep", lib_.asx_substep(C.byref(ps), C.byref(u0s),
+                                                      C.byref(outs_), c0, c1, c2,
+                                                      C.byref(par), None))
+            _sync()
+        else:
+            abi.check("orc_substep", lib_.orc_substep(C.byref(ps), C.byref(u0s),
+                                                      C.byref(outs_), c0, c1, c2,
+                                                      C.byref(par)))
+        outs.append(out)
+    dg, ho = outs
+    bad = {}
+    for name in abi.STATE:
+        sl = h.interior(name)
+        a = dg.arr[name].cpu().numpy()[sl]
+        b = ho.arr[name][sl]
+        e = rel_err(a, b)
+        if not e <= TOL:
+            bad[name] = e
+    assert not bad, bad
+
+
+def test_c2p_batch():
+    _torch()
+    import torch
+    from c2p_batch import make_batch
+    nz = 20000
+    cons, g, seed, _ = make_batch(nz, seed=3)
+    par = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8,
+                     rho_abs_min=1e-12, vw_lim=1e3, B_lim=1e3)
+    prim_h = np.zeros((8, nz))
+    flag_h = np.zeros(nz)
+    it_h = np.zeros(nz, dtype=np.int32)
+    oracle.lib().orc_c2p_batch(nz, cons.ctypes.data, g.ctypes.data, seed.ctypes.data,
+                               prim_h.ctypes.data, flag_h.ctypes.data,
+                               it_h.ctypes.data, C.byref(par))
+    t = {k: torch.from_numpy(v).cuda() for k, v in
+         dict(cons=cons, g=g, seed=seed).items()}
+    prim_d = torch.zeros((8, nz), dtype=torch.float64, device="cuda")
+    flag_d = torch.zeros(nz, dtype=torch.float64, device="cuda")
+    it_d = torch.zeros(nz, dtype=torch.int32, device="cuda")
+    from asterx_amd import lib
+    abi.check("asx_c2p_batch", lib.lib().asx_c2p_batch(
+        nz, t["cons"].data_ptr(), t["g"].data_ptr(), t["seed"].data_ptr(),
+        prim_d.data_ptr(), flag_d.data_ptr(), it_d.data_ptr(), C.byref(par), None))
+    _sync()
+    assert np.array_equal(flag_d.cpu().numpy(), flag_h)
+    assert np.array_equal(it_d.cpu().numpy(), it_h)
+    assert rel_err(prim_d.cpu().numpy(), prim_h) <= 1e-12
