@@ -177,7 +177,12 @@ def test_con2prim(prime, second):
     fl_h = h.numpy("con2prim_flag")[h.interior("rho")]
     fl_d = d.numpy("con2prim_flag")[h.interior("rho")]
     assert np.array_equal(fl_h, fl_d)
-    assert (fl_h == 4).sum() >= 2 and (fl_h == 6).sum() >= 1
+    assert (fl_h == 4).sum() >= 2
+    if prime == "Noble":
+        # Noble checks every conserved variable for NaN; Palenzuela's check
+        # (c2p_1DPalenzuela.hxx:363-364) omits tau, so there the poisoned
+        # cell is not flagged -- on both sides alike.
+        assert (fl_h == 6).sum() >= 1
     hist = np.bincount(fl_h.astype(int).ravel(), minlength=7)
     assert list(hist) == list(cnt.c2p_flags)
     tol = 1e-12 if prime == "Noble" else 1e-9
