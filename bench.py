@@ -1,0 +1,334 @@
+#!/usr/bin/env python3
+"""Throughput of the GRMHD method-of-lines substep on one MI355X.
+
+Workload (default): the curved-spacetime torus, one 192^3 patch over
+[-40,40]^3 M with 3 ghost zones, FP64, Kerr-Schild a = 0.9375, Fishbone-
+Moncrief disk (r_in 6, r_max 12, Gamma 4/3, K 1e-3, 2% pressure
+perturbations, min beta 100), WENO-Z + HLLE, Noble 2D con2prim with
+Palenzuela backup, floors rho 1e-8 / p 1e-10.
+
+One "step" is one RK substep of the reference's schedule on the whole patch:
+  RHS (fluxes, sources, A/Psi RHS) and the RK combination  (asx_substep)
+  SYNC of the state                                         (asx_bc_fill)
+  B from A + con2prim                                       (asx_con2prim_group)
+  SYNC of the primitives the next RHS reads                 (asx_bc_fill)
+cycling the three SSPRK3 stages (stage 1 reads no previous-stage state).
+
+Prints one JSON line: zone-updates/s (interior zones x substeps / s), the
+roofline of the dominant kernel timed with HIP events on the library's
+stream, and the CPU oracle timed on the same inputs on the host cores.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6           # MI355X FP64, TFLOP/s
+
+# Grid functions the next substep reads in the ghost zones.
+SYNC_PRIMS = ["rho", "velx", "vely", "velz", "eps", "press", "entropy", "Ye",
+              "Bvecx", "Bvecy", "Bvecz"]
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--n", type=int, default=192, help="cells per axis")
+    ap.add_argument("--recon", default="wenoz")
+    ap.add_argument("--flux", default="HLLE")
+    ap.add_argument("--cpu", choices=["on", "off"], default="on",
+                    help="time the CPU oracle on the same inputs")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def log(args, *a):
+    if args.verbose:
+        print(*a, file=sys.stderr, flush=True)
+
+
+def state_struct(patch, arrays):
+    from asterx_amd import abi
+    s = abi.asx_state()
+    for q, name in enumerate(abi.STATE):
+        a = arrays[name]
+        ex, ey, _ = abi.extents(patch.n, patch.ng, abi.centering(name))
+        s.gf[q].data = patch._ptr(a)
+        s.gf[q].sj = ex
+        s.gf[q].sk = ex * ey
+    return s
+
+
+SSPRK3 = [(0.0, 1.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0)]
+
+
+class Stepper:
+    """Runs substeps on one patch with three rotating state buffers."""
+
+    def __init__(self, patch, par, backend, dt, stream_handle):
+        import torch
+        from asterx_amd import abi
+        self.p, self.par, self.b, self.dt = patch, par, backend, dt
+        self.stream = stream_handle
+        self.bufs = [{n: patch.arr[n] for n in abi.STATE}]
+        for _ in range(2):
+            self.bufs.append({n: torch.empty_like(patch.arr[n])
+                              for n in abi.STATE})
+        self.cur = 0
+        self.base = 0
+        self.stage = 0
+        self.L = backend.lib
+        self.bc = abi.BC["copy"]
+        self.slots_state = [abi.SLOT[n] for n in abi.STATE]
+        self.slots_prims = [abi.SLOT[n] for n in SYNC_PRIMS]
+
+    def _bc(self, ps, slots):
+        for s in slots:
+            r = self.L.asx_bc_fill(C.byref(ps), s, self.bc, self.stream)
+            if r:
+                raise RuntimeError("asx_bc_fill: %d" % r)
+
+    def substep(self, ev=None):
+        from asterx_amd import abi
+        c0, c1, cf = SSPRK3[self.stage]
+        if self.stage == 0:
+            self.base = self.cur
+        out = [i for i in range(3) if i not in (self.cur, self.base)][0]
+        p = self.p
+        ps = p.struct()
+        u0s = state_struct(p, self.bufs[self.base])
+        outs = state_struct(p, self.bufs[out])
+        if ev:
+            ev[0].record(ev[5])
+        r = self.L.asx_substep(C.byref(ps), C.byref(u0s), C.byref(outs), c0, c1,
+                               cf * self.dt, C.byref(self.par), self.stream)
+        if r:
+            raise RuntimeError("asx_substep: %d" % r)
+        if ev:
+            ev[1].record(ev[5])
+        for n in abi.STATE:
+            p.arr[n] = self.bufs[out][n]
+        self.cur = out
+        ps = p.struct()
+        self._bc(ps, self.slots_state)
+        if ev:
+            ev[2].record(ev[5])
+        r = self.L.asx_con2prim_group(C.byref(ps), C.byref(self.par), None,
+                                      self.stream)
+        if r:
+            raise RuntimeError("asx_con2prim_group: %d" % r)
+        if ev:
+            ev[3].record(ev[5])
+        self._bc(ps, self.slots_prims)
+        if ev:
+            ev[4].record(ev[5])
+        self.stage = (self.stage + 1) % 3
+
+
+def stencil_bytes_per_zone(par, first_stage):
+    """Algorithmic HBM bytes of the RHS + RK combination per interior zone:
+    each input grid function read once, each output written once.
+    Reads: 11 cell primitives (rho, v^i, eps, p, s, Ye, B^i) [+T], 4 EM
+    state (A_i, Psi), 16 vertex ADM, 7 hydro conserved, 11 previous-stage
+    state (not for the first stage); writes: 11 new state."""
+    reads = 11 + 4 + 16 + 7 + (0 if first_stage else 11)
+    if par.reconstruct_with_temperature:
+        reads += 1
+    if par.recon_type in (1, 2):
+        reads += 3
+    return 8 * (reads + 11)
+
+
+def c2p_bytes_per_zone(with_mask):
+    """Algorithmic bytes of B-from-A + con2prim per interior zone.
+    Reads: 7 conserved, 3 A_i, 7 vertex metric (alpha, gamma_ij), 6 saved
+    primitives, temperature; writes: 12 primitives, 7 conserved, 3 dB,
+    6 saved primitives, 6 z/s vectors, flag [+ cell mask]."""
+    return 8 * (7 + 3 + 7 + 6 + 1 + 12 + 7 + 3 + 6 + 6 + 1 + (1 if with_mask else 0))
+
+
+def read_traffic(kernel, workload):
+    """Measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated)
+    from profiles/traffic.txt, lines: kernel workload fetch_B write_B."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 4 and not line.startswith("#") and t[0] == kernel \
+                    and t[1] == workload:
+                return float(t[2]) + float(t[3])
+    return None
+
+
+def cpu_baseline(host_patch, par, dt, steps):
+    """The CPU oracle (OpenMP over k-planes) on the same patch and inputs:
+    stage-1 substep + SYNC + B from A + con2prim + SYNC, per substep."""
+    import numpy as np
+    import oracle
+    from asterx_amd import abi
+    from asterx_amd.patch import StateBuf
+    L = oracle.lib()
+    p = host_patch
+    out = StateBuf(p)
+    bc = abi.BC["copy"]
+    t_all = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ps = p.struct()
+        os_ = out.struct()
+        abi.check("orc_substep", L.orc_substep(C.byref(ps), None, C.byref(os_),
+                                              0.0, 1.0, dt, C.byref(par)))
+        for n in abi.STATE:
+            p.arr[n], out.arr[n] = out.arr[n], p.arr[n]
+        ps = p.struct()
+        for n in abi.STATE:
+            abi.check("orc_bc_fill", L.orc_bc_fill(C.byref(ps), abi.SLOT[n], bc))
+        abi.check("orc_bstag_from_A", L.orc_bstag_from_A(C.byref(ps)))
+        abi.check("orc_dB_from_bstag", L.orc_dB_from_bstag(C.byref(ps)))
+        abi.check("orc_con2prim", L.orc_con2prim(C.byref(ps), C.byref(par)))
+        for n in SYNC_PRIMS:
+            abi.check("orc_bc_fill", L.orc_bc_fill(C.byref(ps), abi.SLOT[n], bc))
+        t_all.append(time.perf_counter() - t0)
+        assert np.isfinite(p.arr["dens"]).all()
+    return t_all
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; --gpus %d is not "
+              "supported (run with --gpus 1)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from asterx_amd import abi, lib, torus
+    from asterx_amd.driver import all_slots
+    from asterx_amd.patch import Patch
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = "cuda:0"
+    torch.cuda.set_device(0)
+    stream = torch.cuda.Stream()
+    sh = C.c_void_p(stream.cuda_stream)
+    be = lib.backend(sh)
+    n, ng, x0, dx = torus.torus_patch_geometry(n=args.n)
+    spec = torus.TorusSpec()
+    par = torus.eos_params(spec, reconstruction_method=args.recon,
+                           flux_type=args.flux)
+    slots = abi.ADM + abi.PRIMS + abi.CONS + abi.DB + abi.SAVED + abi.ZS + \
+        ["con2prim_flag", "aster_mask_cc"] + abi.DBSTAG + abi.AVEC
+    t0 = time.perf_counter()
+    with torch.cuda.stream(stream):
+        p = Patch(n, ng, x0, dx, slots, device=dev)
+        torus.build_torus(p, be, par, spec)
+    torch.cuda.synchronize()
+    # the fused substep keeps its face data in library scratch; con2prim
+    # derives dB from A directly, so the staggered field is not kept.
+    for s in abi.DBSTAG:
+        del p.arr[s]
+    log(args, "initial data: %.1f s" % (time.perf_counter() - t0))
+    zones = n[0] * n[1] * n[2]
+    dt = 0.2 * dx[0]
+    host = None
+    if args.cpu == "on":
+        # same inputs for the CPU oracle, copied before the GPU evolves them
+        host = p.to(None)
+        for s in all_slots():
+            if s not in host:
+                host.alloc(s)
+
+    st = Stepper(p, par, be, dt, sh)
+    for _ in range(args.warmup):
+        st.substep()
+    torch.cuda.synchronize()
+    K = args.steps
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] + [stream]
+           for _ in range(K)]
+    t0 = time.perf_counter()
+    for k in range(K):
+        st.substep(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / K
+    ph = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]),
+                    e[2].elapsed_time(e[3]), e[3].elapsed_time(e[4])]
+                   for e in evs])
+    stages = [(k + args.warmup) % 3 for k in range(K)]
+    log(args, "phases ms (stencil, bc_state, c2p, bc_prims):",
+        ph.mean(0).round(4))
+    # flags histogram of one more substep, outside the timed region
+    cnt = abi.asx_counters()
+    ps = p.struct()
+    abi.check("asx_con2prim_group", be.lib.asx_con2prim_group(
+        C.byref(ps), C.byref(par), C.byref(cnt), sh))
+    torch.cuda.synchronize()
+    log(args, "c2p flags:", list(cnt.c2p_flags))
+    for name in abi.CONS:
+        a = p.arr[name][p.interior(name)]
+        if not bool(torch.isfinite(a).all()):
+            raise RuntimeError("non-finite %s after the timed substeps" % name)
+
+    # roofline of the dominant kernel (the phase with the largest average
+    # duration; both phases below are single kernels or documented groups)
+    c2p_ms = float(ph[:, 2].mean())
+    sten_ms = float(ph[:, 0].mean())
+    first = np.array([s == 0 for s in stages])
+    b_sten = np.array([stencil_bytes_per_zone(par, f) for f in first])
+    sten_gbs = float((b_sten * zones / (ph[:, 0] * 1e-3)).mean() / 1e9)
+    b_c2p = c2p_bytes_per_zone(True)
+    c2p_gbs = b_c2p * zones / (c2p_ms * 1e-3) / 1e9
+    workload = "torus%d_%s_%s" % (args.n, args.recon, args.flux)
+    if c2p_ms >= sten_ms:
+        kern, achieved, dur = "k_con2prim", c2p_gbs, c2p_ms
+        traffic = read_traffic("k_con2prim", workload)
+    else:
+        kern, achieved, dur = "substep", sten_gbs, sten_ms
+        traffic = read_traffic("substep", workload)
+    roof = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "avg_ms": round(dur, 4),
+            "stencil_ms": round(sten_ms, 4), "stencil_GBs": round(sten_gbs, 2),
+            "c2p_ms": round(c2p_ms, 4), "c2p_GBs": round(c2p_gbs, 2)}
+
+    cpu = None
+    if host is not None:
+        ts = cpu_baseline(host, par, dt, args.cpu_steps)
+        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
+            len(os.sched_getaffinity(0))
+        cpu = {"value": round(zones / min(ts), 1), "unit": "zone-updates/s",
+               "cores": cores, "kind": "port",
+               "sample": "CPU oracle (g++ -O2, OpenMP) on the same %d^3 torus "
+                         "inputs: %d stage-1 substep(s) incl. SYNC, B from A "
+                         "and con2prim; best of %d, %.1f s" %
+                         (args.n, len(ts), len(ts), sum(ts))}
+
+    out = {"metric": "zone-updates/s per RK substep",
+           "value": round(zones / (ms_step * 1e-3), 1),
+           "unit": "zone-updates/s", "n_gpus": 1, "steps": K,
+           "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "torus %d^3 +3 ghosts, Kerr-Schild a=0.9375, "
+                                  "FM disk, %s + %s, Noble/Palenzuela c2p" %
+                                  (args.n, args.recon, args.flux),
+                      "zones": zones, "substep": "SSPRK3 stages cycled"},
+           "roofline": roof, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
