@@ -27,7 +27,7 @@ ASX_DEV void hist_flush(HistLDS &h, unsigned long long *g) {
 // Densitized cell B from the vector potential: curl at the six faces, then
 // the face -> cell average (computeBfromA.cxx:32-55 and :71-84).
 ASX_DEV void dB_from_A(const KPatch &P, int i, int j, int k, double dB[3]) {
-  const double idx[3] = {1 / P.dx[0], 1 / P.dx[1], 1 / P.dx[2]};
+  const double *idx = P.idx;
   const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
   auto bx = [&](int ii, int jj, int kk) {
     const int oz = off(P, 3, ii, jj, kk), oy = off(P, 5, ii, jj, kk);

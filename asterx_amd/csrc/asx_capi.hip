@@ -103,6 +103,7 @@ KPatch kpatch(const asx_patch *pt) {
   for (int d = 0; d < 3; ++d) {
     P.n[d] = pt->n[d];
     P.dx[d] = pt->dx[d];
+    P.idx[d] = 1.0 / pt->dx[d];
     P.x0[d] = pt->x0[d];
   }
   P.ng = pt->ng;
@@ -141,6 +142,7 @@ IdealGas ideal_gas(const asx_params *p) {
   e.gamma = p->gl_gamma;
   e.gm1 = p->gl_gamma - 1;
   e.inv_gamma = 1 / p->gl_gamma;
+  e.igm1 = 1 / e.gm1;
   e.eps_min = p->eps_min;
   e.eps_max = p->eps_max;
   if (e.gamma > 2) {

@@ -5,7 +5,9 @@
 // expressions keep the reference's evaluation order term for term and the
 // library is compiled with -ffp-contract=off, so results agree bit for bit
 // with a strict-IEEE CPU evaluation of the same formulas wherever only
-// +,-,*,/,sqrt are involved (all correctly rounded on CDNA4).
+// +,-,*,/,sqrt are involved (all correctly rounded on CDNA4).  Deliberate
+// exceptions, which agree to rounding level: finite differences multiply by
+// a host-computed 1/dx, and the source terms divide by alpha and W once.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,6 +27,7 @@ struct KPatch {
   int sj[8], sk[8]; // dense strides per centering code (bit d: vertex on d)
   int n[3], ng;
   double dx[3], x0[3];
+  double idx[3]; // 1/dx
 };
 
 // Centering code of each slot (include/asterx.h enum order).
@@ -77,6 +80,16 @@ ASX_DEV double smin(double a, double b) { return (b < a) ? b : a; }
 ASX_DEV double smax(double a, double b) { return (a < b) ? b : a; }
 ASX_DEV double pow2(double x) { return x * x; }
 ASX_DEV int sgn(double v) { return (0.0 < v) - (v < 0.0); }
+
+// Workgroups are handed to the 8 XCDs of the chip round-robin (w % 8), and
+// each XCD has its own L2.  Map workgroup w to a tile index such that every
+// XCD receives one contiguous run of tiles, so neighbouring tiles (sharing
+// halo cells and metric vertices) hit the same L2.  A bijection of [0, nb).
+ASX_DEV int xcd_tile(int w, int nb) {
+  constexpr int NX = 8;
+  const int x = w % NX, l = w / NX, per = (nb + NX - 1) / NX, rem = nb % NX;
+  return (rem == 0 || x < rem) ? x * per + l : rem * per + (x - rem) * (per - 1) + l;
+}
 
 struct Sym3 {
   double xx, xy, xz, yy, yz, zz;
@@ -196,7 +209,12 @@ ASX_DEV void ppm_cell(double gImm, double gIm, double gI, double gIp,
   }
 }
 
-// WENO-Z face values of the centre cell (ReconX/src/wenoz.hxx:23-99)
+// WENO-Z face values of the centre cell (ReconX/src/wenoz.hxx:23-99).
+// Same smoothness indicators, weights and stencils; the nonlinear weights
+// a_k = 1 + tau5/(b_k + eps) are carried over the common denominator
+// prod_k (b_k + eps), so each face value costs one division instead of the
+// reference's seven (identical up to rounding).  The (b_k + eps) are first
+// rescaled by a power of two (exact) so the triple products cannot overflow.
 ASX_DEV void wenoz_cell(double gImm, double gIm, double gI, double gIp,
                         double gIpp, double weno_eps, double &m, double &p) {
   const double b0 = (13.0 / 12.0) * pow2(gImm - 2.0 * gIm + gI) +
@@ -205,27 +223,35 @@ ASX_DEV void wenoz_cell(double gImm, double gIm, double gI, double gIp,
                     (1.0 / 4.0) * pow2(gIm - gIp);
   const double b2 = (13.0 / 12.0) * pow2(gI - 2.0 * gIp + gIpp) +
                     (1.0 / 4.0) * pow2(3.0 * gI - 4.0 * gIp + gIpp);
-  const double tau5 = fabs(b0 - b2);
-  const double a0 = 1.0 + tau5 / (b0 + weno_eps);
-  const double a1 = 1.0 + tau5 / (b1 + weno_eps);
-  const double a2 = 1.0 + tau5 / (b2 + weno_eps);
+  double s0 = b0 + weno_eps, s1 = b1 + weno_eps, s2 = b2 + weno_eps;
+  double tau5 = fabs(b0 - b2);
+  const int e = -ilogb(smax(s0, smax(s1, s2)));
+  s0 = ldexp(s0, e);
+  s1 = ldexp(s1, e);
+  s2 = ldexp(s2, e);
+  tau5 = ldexp(tau5, e);
+  // a_k * s0 s1 s2
+  const double n0 = (s0 + tau5) * (s1 * s2);
+  const double n1 = (s1 + tau5) * (s0 * s2);
+  const double n2 = (s2 + tau5) * (s0 * s1);
   const double w0 = 3.0 / 10.0, w1 = 3.0 / 5.0, w2 = 1.0 / 10.0;
-  const double am0 = w0 * a0, am1 = w1 * a1, am2 = w2 * a2;
-  const double ap0 = w2 * a0, ap1 = w1 * a1, ap2 = w0 * a2;
-  const double dm = am0 + am1 + am2, dp = ap0 + ap1 + ap2;
-  m = ((am2 / dm) / 6.0) * (2.0 * gIpp - 7.0 * gIp + 11.0 * gI) +
-      ((am1 / dm) / 6.0) * (-1.0 * gIp + 5.0 * gI + 2.0 * gIm) +
-      ((am0 / dm) / 6.0) * (2.0 * gI + 5.0 * gIm - 1.0 * gImm);
-  p = ((ap0 / dp) / 6.0) * (2.0 * gImm - 7.0 * gIm + 11.0 * gI) +
-      ((ap1 / dp) / 6.0) * (-1.0 * gIm + 5.0 * gI + 2.0 * gIp) +
-      ((ap2 / dp) / 6.0) * (2.0 * gI + 5.0 * gIp - 1.0 * gIpp);
+  const double am0 = w0 * n0, am1 = w1 * n1, am2 = w2 * n2;
+  const double ap0 = w2 * n0, ap1 = w1 * n1, ap2 = w0 * n2;
+  m = (am2 * (2.0 * gIpp - 7.0 * gIp + 11.0 * gI) +
+       am1 * (-1.0 * gIp + 5.0 * gI + 2.0 * gIm) +
+       am0 * (2.0 * gI + 5.0 * gIm - 1.0 * gImm)) /
+      (6.0 * (am0 + am1 + am2));
+  p = (ap0 * (2.0 * gImm - 7.0 * gIm + 11.0 * gI) +
+       ap1 * (-1.0 * gIm + 5.0 * gI + 2.0 * gIp) +
+       ap2 * (2.0 * gI + 5.0 * gIp - 1.0 * gIpp)) /
+      (6.0 * (ap0 + ap1 + ap2));
 }
 
 // MP5 value on the gIp side of cell gI (ReconX/src/mp5.hxx:26-77)
 ASX_DEV double mp5_side(double gImm, double gIm, double gI, double gIp,
                         double gIpp, double alpha) {
   const double ul =
-      (2 * gImm - 13 * gIm + 47 * gI + 27 * gIp - 3 * gIpp) / 60.0;
+      (2 * gImm - 13 * gIm + 47 * gI + 27 * gIp - 3 * gIpp) * (1.0 / 60.0);
   const double deltam = gI - gIm;
   const double ump = gI + minmod(gIp - gI, alpha * deltam);
   if ((ul - gI) * (ul - ump) <= 1.0e-10) return ul;
@@ -303,7 +329,7 @@ ASX_DEV void recon_lo(int m, const double s[6], double &o0, double &o1) {
 // Riemann fluxes and characteristic speeds (AsterX)
 // ---------------------------------------------------------------------------
 // AsterX/src/eigenvalues.hxx:19-79 for one side: returns (lambda+, lambda-)
-ASX_DEV void eig_side(double alp, double beta, double u, double vel,
+ASX_DEV void eig_side(double alp, double ialp, double beta, double u, double vel,
                       double rho, double cs2, double w, double h, double bsq,
                       double &lp, double &lm) {
   const double a0 = (bsq + cs2 * h * rho) * (pow2(beta) - pow2(alp) * u) -
@@ -314,8 +340,9 @@ ASX_DEV void eig_side(double alp, double beta, double u, double vel,
   double det = pow2(a1) - 4 * a2 * a0;
   if (det < 0) det = 0;
   const double sd = sqrt(det);
-  lp = ((-a1 + sd) / (2 * a2)) / alp;
-  lm = ((-a1 - sd) / (2 * a2)) / alp;
+  const double f = ialp / (2 * a2); // 1 / (2 a2 alpha)
+  lp = (-a1 + sd) * f;
+  lm = (-a1 - sd) * f;
 }
 // max/min over {0, l0+, l0+, l0-, l0-, l1+, l1+, l1-, l1-} (fluxes.hxx:35-42)
 ASX_DEV void char_extrema(double l0p, double l0m, double l1p, double l1m,
@@ -323,9 +350,10 @@ ASX_DEV void char_extrema(double l0p, double l0m, double l1p, double l1m,
   cmax = smax(smax(smax(smax(0.0, l0p), l0m), l1p), l1m);
   cmin = smin(smin(smin(smin(0.0, l0p), l0m), l1p), l1m);
 }
-ASX_DEV double hlle(double cmax, double cmin, double U0, double U1, double F0,
-                    double F1) {
-  return (cmax * F0 - cmin * F1 + cmax * cmin * (U1 - U0)) / (cmax - cmin);
+// HLLE with the common 1/(cmax - cmin) precomputed (AsterX/src/fluxes.hxx:31-50)
+ASX_DEV double hlle(double cmax, double cmin, double idc, double U0, double U1,
+                    double F0, double F1) {
+  return (cmax * F0 - cmin * F1 + cmax * cmin * (U1 - U0)) * idc;
 }
 ASX_DEV double laxf(double cabs, double U0, double U1, double F0, double F1) {
   return 0.5 * ((F0 + F1) - cabs * (U1 - U0));
@@ -338,6 +366,7 @@ struct IdealGas {
   double gamma, gm1, inv_gamma, temp_over_eps;
   double eps_min, eps_max, rho_min, rho_max, ye_min, ye_max, temp_min,
       temp_max;
+  double igm1; // 1/(gamma-1)
   ASX_DEV double press_re(double rho, double eps) const { return gm1 * rho * eps; }
   ASX_DEV double eps_rp(double rho, double press) const {
     return press / (rho * gm1);

@@ -1,29 +1,140 @@
 // Face, vertex and curl kernels of AsterX_Fluxes and the B-field group on
-// gfx950: one work-item per face / vertex, x-fastest index order so a
-// wavefront reads 64 consecutive doubles of every stencil row.
+// gfx950.  Vertex and curl kernels: one work-item per point, x-fastest so a
+// wavefront reads 64 consecutive doubles of every stencil row.  Face
+// kernels: cell tiles with an LDS hand-over between neighbouring cells.
 #include "asx_kernels.hpp"
 
 namespace asx {
 
 // ---------------------------------------------------------------------------
 // CalcFlux<D> (AsterX/src/fluxes.cxx:216-795) for faces normal to D.
+//
+// Work decomposition: one work-item per cell of a tile that is one cell
+// longer along D than the faces it produces.  Each work-item reconstructs
+// its cell once, both sides (ReconX's per-cell parabola / WENO weights are
+// shared by the two faces of the cell), keeps the low side in registers as
+// the right state of its own face and hands the high side to the next cell
+// along D through LDS, where it is the left state of that face.  Tiles
+// overlap by one cell along D.
 // ---------------------------------------------------------------------------
+
+// The reconstructed primitives of one side of a cell: rho, entropy, Ye,
+// pressure (or temperature), the two transverse B components (D+1, D+2)
+// and the velocity-like vector (v, z or s).
+struct Side {
+  double rho, ent, ye, th, b1, b2, v[3];
+};
+constexpr int kSideVars = 9;
+
+// Face values (lo = the cell's face at -D, up = at +D) of one grid function
+// from its 5-cell stencil along D.  Equal, value for value, to the two
+// sides that ReconX's reconstruct() yields for the faces of this cell.
+template <int M>
+ASX_DEV void cell_recon(const double *p, int oc, int sc, const double ps[5],
+                        const double vs[5], bool is_rho, const RParams &rp,
+                        double &lo, double &up) {
+  double s[5];
+  constexpr int m0 = M == ASX_RECON_GODUNOV ? 2
+                     : (M == ASX_RECON_MINMOD || M == ASX_RECON_MONOCENTRAL) ? 1
+                                                                            : 0;
+#pragma unroll
+  for (int m = m0; m <= 4 - m0; ++m) s[m] = p[oc + (m - 2) * sc];
+  if (M == ASX_RECON_GODUNOV) {
+    lo = s[2];
+    up = s[2];
+  } else if (M == ASX_RECON_MINMOD) {
+    const double sl = minmod(s[3] - s[2], s[2] - s[1]);
+    up = s[2] + sl / 2;
+    lo = s[2] - sl / 2;
+  } else if (M == ASX_RECON_MONOCENTRAL) {
+    const double sl = monocentral(s[3] - s[2], s[2] - s[1]);
+    up = s[2] + sl / 2;
+    lo = s[2] - sl / 2;
+  } else if (M == ASX_RECON_PPM) {
+    ppm_cell(s[0], s[1], s[2], s[3], s[4], ps[0], ps[1], ps[3], ps[4], vs[1], vs[3],
+             is_rho, rp, lo, up);
+  } else if (M == ASX_RECON_WENOZ) {
+    wenoz_cell(s[0], s[1], s[2], s[3], s[4], rp.weno_eps, lo, up);
+  } else {
+    up = mp5_side(s[0], s[1], s[2], s[3], s[4], rp.mp5_alpha);
+    lo = mp5_side(s[4], s[3], s[2], s[1], s[0], rp.mp5_alpha);
+  }
+}
+ASX_DEV void cell_recon_lo(int m, const double *p, int oc, int sc, double &lo,
+                           double &up) {
+  const double z[5] = {0, 0, 0, 0, 0};
+  const RParams rp{};
+  if (m == ASX_RECON_MINMOD)
+    cell_recon<ASX_RECON_MINMOD>(p, oc, sc, z, z, false, rp, lo, up);
+  else if (m == ASX_RECON_MONOCENTRAL)
+    cell_recon<ASX_RECON_MONOCENTRAL>(p, oc, sc, z, z, false, rp, lo, up);
+  else
+    cell_recon<ASX_RECON_GODUNOV>(p, oc, sc, z, z, false, rp, lo, up);
+}
+
+// Both sides of cell oc with the low-order fallback of fluxes.cxx:258-330:
+// a side whose rho, entropy, Ye or pressure/temperature is not positive is
+// reconstructed again, every variable, with the low-order method.
 template <int D, int M>
-__global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nf = (int64_t)A.cnt[0] * A.cnt[1] * A.cnt[2];
-  if (t >= nf) return;
-  const int i = A.lo[0] + (int)(t % A.cnt[0]);
-  const int j = A.lo[1] + (int)((t / A.cnt[0]) % A.cnt[1]);
-  const int k = A.lo[2] + (int)(t / ((int64_t)A.cnt[0] * A.cnt[1]));
+ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Side &up,
+                        bool &flo, bool &fup) {
+  constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
+  const int sc = step(P, 0, D);
+  double ps[5] = {0, 0, 0, 0, 0}, vs[5] = {0, 0, 0, 0, 0};
+  if (M == ASX_RECON_PPM) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      ps[m] = P.g[ASX_PRESS][oc + (m - 2) * sc];
+      vs[m] = P.g[ASX_VELX + D][oc + (m - 2) * sc];
+    }
+  }
+  auto rc = [&](int slot, bool is_rho, double &l, double &u) {
+    cell_recon<M>(P.g[slot], oc, sc, ps, vs, is_rho, A.rp, l, u);
+  };
+  const int thermo = A.with_T ? ASX_TEMPERATURE : ASX_PRESS;
+  rc(ASX_RHO, true, lo.rho, up.rho);
+  rc(ASX_ENTROPY, false, lo.ent, up.ent);
+  rc(ASX_YE, false, lo.ye, up.ye);
+  rc(thermo, false, lo.th, up.th);
+  flo = (lo.rho <= 0.0) || (lo.ent <= 0.0) || (lo.ye <= 0.0) || (lo.th <= 0.0);
+  fup = (up.rho <= 0.0) || (up.ent <= 0.0) || (up.ye <= 0.0) || (up.th <= 0.0);
+  const bool any = flo || fup;
+  auto fix = [&](int slot, double &l, double &u) {
+    if (any) {
+      double a, b;
+      cell_recon_lo(A.lo_method, P.g[slot], oc, sc, a, b);
+      if (flo) l = a;
+      if (fup) u = b;
+    }
+  };
+  fix(ASX_RHO, lo.rho, up.rho);
+  fix(ASX_ENTROPY, lo.ent, up.ent);
+  fix(ASX_YE, lo.ye, up.ye);
+  fix(thermo, lo.th, up.th);
+  rc(ASX_BVECX + D1, false, lo.b1, up.b1);
+  fix(ASX_BVECX + D1, lo.b1, up.b1);
+  rc(ASX_BVECX + D2, false, lo.b2, up.b2);
+  fix(ASX_BVECX + D2, lo.b2, up.b2);
+  const int v0 = A.rec_var == ASX_RECVAR_V   ? ASX_VELX
+                 : A.rec_var == ASX_RECVAR_Z ? ASX_ZVEC_X
+                                             : ASX_SVEC_X;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    rc(v0 + c, false, lo.v[c], up.v[c]);
+    fix(v0 + c, lo.v[c], up.v[c]);
+  }
+}
+
+// Numerical flux of every conserved variable at face (i, j, k) from the
+// left (0) and right (1) states.
+template <int D>
+ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
+                       const Side &L, const Side &R, bool lo0, bool lo1) {
   constexpr int CF = 1 << D;      // face centering
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
-  const int oc = off(P, 0, i, j, k); // cell I (right of the face)
-  const int sc = step(P, 0, D);
   const int ofc = off(P, CF, i, j, k);
   const int ov = off(P, 7, i, j, k);
   const int svj = P.sj[7], svk = P.sk[7];
-  const RParams &rp = A.rp;
   const IdealGas &eos = A.eos;
 
   // metric at the face
@@ -41,68 +152,22 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
   g.zz = avg_v2f<D>(P.g[ASX_GZZ], ov, svj, svk);
   const double detg = det3(g);
   const double sqrtg = sqrt(detg);
+  const double isqrtg = 1.0 / sqrtg;
+  const double ialp = 1.0 / alp;
 
-  constexpr int m0 = stencil_lo<M>();
-  const int mlo = stencil_lo_rt(A.lo_method);
-  auto load = [&](int slot, int from, double s[6]) {
-    const double *p = P.g[slot];
-#pragma unroll
-    for (int m = 0; m < 6; ++m)
-      if (m >= from && m <= 5 - (from == 2 ? 2 : (from == 1 ? 1 : 0)))
-        s[m] = p[oc + (m - 3) * sc];
-  };
-  double ps[6] = {0, 0, 0, 0, 0, 0}, vs[6] = {0, 0, 0, 0, 0, 0};
-  if (M == ASX_RECON_PPM) {
-    load(ASX_PRESS, 0, ps);
-    load(ASX_VELX + D, 0, vs);
-  }
-  // high-order face states of one variable
-  auto rc = [&](int slot, bool is_rho, double &o0, double &o1) {
-    double s[6];
-    load(slot, m0, s);
-    recon<M>(s, ps, vs, is_rho, rp, o0, o1);
-  };
-  auto rc_lo = [&](int slot, double &o0, double &o1) {
-    double s[6];
-    load(slot, mlo, s);
-    recon_lo(A.lo_method, s, o0, o1);
-  };
-
-  double rho0, rho1, ent0, ent1, ye0, ye1, th0, th1;
-  rc(ASX_RHO, true, rho0, rho1);
-  rc(ASX_ENTROPY, false, ent0, ent1);
-  rc(ASX_YE, false, ye0, ye1);
-  const int thermo = A.with_T ? ASX_TEMPERATURE : ASX_PRESS;
-  rc(thermo, false, th0, th1);
-  const bool lo0 = (rho0 <= 0.0) || (ent0 <= 0.0) || (ye0 <= 0.0) || (th0 <= 0.0);
-  const bool lo1 = (rho1 <= 0.0) || (ent1 <= 0.0) || (ye1 <= 0.0) || (th1 <= 0.0);
-  if (lo0 || lo1) {
-    double a, b;
-    rc_lo(ASX_RHO, a, b);
-    if (lo0) rho0 = a;
-    if (lo1) rho1 = b;
-    rc_lo(ASX_ENTROPY, a, b);
-    if (lo0) ent0 = a;
-    if (lo1) ent1 = b;
-    rc_lo(ASX_YE, a, b);
-    if (lo0) ye0 = a;
-    if (lo1) ye1 = b;
-    rc_lo(thermo, a, b);
-    if (lo0) th0 = a;
-    if (lo1) th1 = b;
-  }
-  double rho[2] = {rho0, rho1}, ent[2] = {ent0, ent1}, ye[2] = {ye0, ye1};
+  const double rho[2] = {L.rho, R.rho}, ent[2] = {L.ent, R.ent}, ye[2] = {L.ye, R.ye};
+  const double th[2] = {L.th, R.th};
+  const double irho[2] = {1.0 / rho[0], 1.0 / rho[1]};
   double eps[2], press[2];
-  if (A.with_T) {
-    eps[0] = eos.eps_rt(th0);
-    press[0] = eos.press_rt(rho0, th0);
-    eps[1] = eos.eps_rt(th1);
-    press[1] = eos.press_rt(rho1, th1);
-  } else {
-    press[0] = th0;
-    eps[0] = eos.eps_rp(rho0, th0);
-    press[1] = th1;
-    eps[1] = eos.eps_rp(rho1, th1);
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    if (A.with_T) {
+      eps[f] = eos.eps_rt(th[f]);
+      press[f] = eos.press_rt(rho[f], th[f]);
+    } else {
+      press[f] = th[f];
+      eps[f] = th[f] * irho[f] * eos.igm1;
+    }
   }
   double rhoh[2];
   rhoh[0] = rho[0] + rho[0] * eps[0] + press[0];
@@ -111,35 +176,16 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
   // magnetic field: normal from the staggered field, transverse recon
   double Bs[3][2];
   {
-    const double val = P.g[ASX_DBX_STAG + D][ofc] / sqrtg;
+    const double val = P.g[ASX_DBX_STAG + D][ofc] * isqrtg;
     Bs[D][0] = val;
     Bs[D][1] = val;
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int d = q == 0 ? D1 : D2;
-      rc(ASX_BVECX + d, false, Bs[d][0], Bs[d][1]);
-      if (lo0 || lo1) {
-        double a, b;
-        rc_lo(ASX_BVECX + d, a, b);
-        if (lo0) Bs[d][0] = a;
-        if (lo1) Bs[d][1] = b;
-      }
-    }
+    Bs[D1][0] = L.b1;
+    Bs[D1][1] = R.b1;
+    Bs[D2][0] = L.b2;
+    Bs[D2][1] = R.b2;
   }
 
-  double vel[3][2], vlow[3][2], W[2];
-  auto recon_vec = [&](int s0, double o[3][2]) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      rc(s0 + c, false, o[c][0], o[c][1]);
-      if (lo0 || lo1) {
-        double a, b;
-        rc_lo(s0 + c, a, b);
-        if (lo0) o[c][0] = a;
-        if (lo1) o[c][1] = b;
-      }
-    }
-  };
+  double vel[3][2], vlow[3][2], W[2], iW[2];
   auto lower2 = [&](const double a[3][2], double o[3][2]) {
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
@@ -154,37 +200,54 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
   auto dot2 = [](const double a[3][2], const double b[3][2], int f) {
     return 0.0 + a[0][f] * b[0][f] + a[1][f] * b[1][f] + a[2][f] * b[2][f];
   };
+  double rv[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    rv[c][0] = L.v[c];
+    rv[c][1] = R.v[c];
+  }
   if (A.rec_var == ASX_RECVAR_V) {
-    recon_vec(ASX_VELX, vel);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      vel[c][0] = rv[c][0];
+      vel[c][1] = rv[c][1];
+    }
     lower2(vel, vlow);
-    W[0] = 1 / sqrt(1 - dot2(vlow, vel, 0));
-    W[1] = 1 / sqrt(1 - dot2(vlow, vel, 1));
+    iW[0] = sqrt(1 - dot2(vlow, vel, 0));
+    iW[1] = sqrt(1 - dot2(vlow, vel, 1));
+    W[0] = 1 / iW[0];
+    W[1] = 1 / iW[1];
   } else if (A.rec_var == ASX_RECVAR_Z) {
-    double z[3][2], zl[3][2];
-    recon_vec(ASX_ZVEC_X, z);
-    lower2(z, zl);
-    W[0] = sqrt(1 + dot2(zl, z, 0));
-    W[1] = sqrt(1 + dot2(zl, z, 1));
+    double zl[3][2];
+    lower2(rv, zl);
+    W[0] = sqrt(1 + dot2(zl, rv, 0));
+    W[1] = sqrt(1 + dot2(zl, rv, 1));
+    iW[0] = 1 / W[0];
+    iW[1] = 1 / W[1];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
-        vel[c][f] = z[c][f] / W[f];
-        vlow[c][f] = zl[c][f] / W[f];
+        vel[c][f] = rv[c][f] * iW[f];
+        vlow[c][f] = zl[c][f] * iW[f];
       }
   } else {
-    double sv[3][2], sl[3][2];
-    recon_vec(ASX_SVEC_X, sv);
-    lower2(sv, sl);
+    double sl[3][2];
+    lower2(rv, sl);
+    double fac[2];
 #pragma unroll
-    for (int f = 0; f < 2; ++f)
-      W[f] = sqrt(0.5 + sqrt(0.25 + dot2(sl, sv, f) / rhoh[f] / rhoh[f]));
+    for (int f = 0; f < 2; ++f) {
+      const double irh = 1.0 / rhoh[f];
+      W[f] = sqrt(0.5 + sqrt(0.25 + dot2(sl, rv, f) * irh * irh));
+      iW[f] = 1 / W[f];
+      fac[f] = iW[f] * iW[f] * irh;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int f = 0; f < 2; ++f) {
-        vel[c][f] = sv[c][f] / W[f] / W[f] / rhoh[f];
-        vlow[c][f] = sl[c][f] / W[f] / W[f] / rhoh[f];
+        vel[c][f] = rv[c][f] * fac[f];
+        vlow[c][f] = sl[c][f] * fac[f];
       }
   }
 
@@ -202,9 +265,9 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int f = 0; f < 2; ++f) bl[c][f] = Bl[c][f] / W[f] + ab0[f] * vlow[c][f];
+    for (int f = 0; f < 2; ++f) bl[c][f] = Bl[c][f] * iW[f] + ab0[f] * vlow[c][f];
 #pragma unroll
-  for (int f = 0; f < 2; ++f) bsq[f] = (B2[f] + pow2(ab0[f])) / pow2(W[f]);
+  for (int f = 0; f < 2; ++f) bsq[f] = (B2[f] + pow2(ab0[f])) * pow2(iW[f]);
 
   double cs2[2], h[2], dens[2], dent[2], dhW[2], dhWb[2], ppm[2], tau[2];
   double mom[3][2], Bt[3][2];
@@ -212,7 +275,7 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
   for (int f = 0; f < 2; ++f) {
     const double cs = eos.csnd(eps[f]);
     cs2[f] = cs * cs;
-    h[f] = 1 + eps[f] + press[f] / rho[f];
+    h[f] = 1 + eps[f] + press[f] * irho[f];
     dens[f] = sqrtg * rho[f] * W[f];
     dent[f] = sqrtg * rho[f] * W[f] * ent[f];
     dhW[f] = dens[f] * h[f] * W[f];
@@ -230,24 +293,27 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
   const double alp_sqrtg = alp * sqrtg;
 
   // characteristic speeds (u = (gamma^-1)^{DD})
-  const Sym3 ginv = inv3(g, detg);
-  const double u = D == 0 ? ginv.xx : (D == 1 ? ginv.yy : ginv.zz);
+  const double u = (D == 0   ? g.yy * g.zz - pow2(g.yz)
+                    : D == 1 ? g.xx * g.zz - pow2(g.xz)
+                             : g.xx * g.yy - pow2(g.xy)) /
+                   detg; // (gamma^-1)^{DD}
   double l0p, l0m, l1p, l1m;
-  eig_side(alp, betas[D], u, vel[D][0], rho[0], cs2[0], W[0], h[0], bsq[0], l0p, l0m);
-  eig_side(alp, betas[D], u, vel[D][1], rho[1], cs2[1], W[1], h[1], bsq[1], l1p, l1m);
+  eig_side(alp, ialp, betas[D], u, vel[D][0], rho[0], cs2[0], W[0], h[0], bsq[0], l0p, l0m);
+  eig_side(alp, ialp, betas[D], u, vel[D][1], rho[1], cs2[1], W[1], h[1], bsq[1], l1p, l1m);
   double cmax, cmin;
   char_extrema(l0p, l0m, l1p, l1m, cmax, cmin);
   const double cabs = smax(smax(smax(smax(0.0, fabs(l0p)), fabs(l0m)), fabs(l1p)),
                            fabs(l1m));
   const bool use_hlle = A.flux_type == ASX_FLUX_HLLE;
+  const double idc = 1.0 / (cmax - cmin);
   auto num = [&](double U0, double U1, double F0, double F1) {
-    return use_hlle ? hlle(cmax, cmin, U0, U1, F0, F1)
+    return use_hlle ? hlle(cmax, cmin, idc, U0, U1, F0, F1)
                     : laxf(cabs, U0, U1, F0, F1);
   };
 
   double BoW[2];
-  BoW[0] = Bs[D][0] / W[0];
-  BoW[1] = Bs[D][1] / W[1];
+  BoW[0] = Bs[D][0] * iW[0];
+  BoW[1] = Bs[D][1] * iW[1];
   double *const *F = P.g + ASX_FXDENS + 10 * D;
   F[0][ofc] = num(dens[0], dens[1], dens[0] * vt[D][0], dens[1] * vt[D][1]);
   F[6][ofc] = num(dent[0], dent[1], dent[0] * vt[D][0], dent[1] * vt[D][1]);
@@ -297,8 +363,9 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
     // components in increasing order (AsterX/interface.ccl:137-150)
     constexpr int s1 = ASX_VTILDE_Y_XFACE + 2 * D + (D1 < D2 ? 0 : 1);
     constexpr int s2 = ASX_VTILDE_Y_XFACE + 2 * D + (D1 < D2 ? 1 : 0);
-    P.g[s1][ofc] = (amx * vt[D1][0] + amn * vt[D1][1]) / (amx + amn);
-    P.g[s2][ofc] = (amx * vt[D2][0] + amn * vt[D2][1]) / (amx + amn);
+    const double ia = 1.0 / (amx + amn);
+    P.g[s1][ofc] = (amx * vt[D1][0] + amn * vt[D1][1]) * ia;
+    P.g[s2][ofc] = (amx * vt[D2][0] + amn * vt[D2][1]) * ia;
   }
   if (A.cnt_dev) {
     const bool lo = lo0 || lo1;
@@ -306,10 +373,74 @@ __global__ __launch_bounds__(256) void k_flux(KPatch P, FluxArgs A) {
                      !(tau[0] == tau[0]) || !(tau[1] == tau[1]) ||
                      rho[0] < 0.0 || rho[1] < 0.0 || press[0] < 0.0 || press[1] < 0.0;
     const uint64_t mb = __ballot(bad), ml = __ballot(lo);
-    if ((threadIdx.x & 63) == 0) {
+    if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) {
       if (mb) atomicAdd((unsigned long long *)&A.cnt_dev[0], (unsigned long long)__popcll(mb));
       if (ml) atomicAdd((unsigned long long *)&A.cnt_dev[1], (unsigned long long)__popcll(ml));
     }
+  }
+}
+
+// Face tile shapes (x, y, z work-items; 256 per block).  The extent along
+// D is one cell more than the faces the tile produces.
+template <int D> struct FaceTile;
+template <> struct FaceTile<0> { static constexpr int X = 64, Y = 4, Z = 1; };
+template <> struct FaceTile<1> { static constexpr int X = 16, Y = 16, Z = 1; };
+template <> struct FaceTile<2> { static constexpr int X = 16, Y = 1, Z = 16; };
+
+#ifndef ASX_FACE_WAVES
+#define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
+#endif
+template <int D, int M>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES))) void k_face(KPatch P, FluxArgs A, int ntx, int nty,
+                                              int ntz) {
+  using T = FaceTile<D>;
+  static_assert(T::X * T::Y * T::Z == 256, "face tile");
+  constexpr int TD = D == 0 ? T::X : (D == 1 ? T::Y : T::Z);
+  constexpr int SD = D == 0 ? 1 : (D == 1 ? T::X : T::X * T::Y); // tile stride along D
+  __shared__ double sh[kSideVars][256];
+  __shared__ int shf[256];
+  const int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
+  const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+  const int t = threadIdx.x;
+  const int tx = t % T::X, ty = (t / T::X) % T::Y, tz = t / (T::X * T::Y);
+  // cell (and face) coordinates relative to the face box; along D the tile
+  // starts one cell below its first face
+  const int fx = bx * (D == 0 ? TD - 1 : T::X) + tx - (D == 0);
+  const int fy = by * (D == 1 ? TD - 1 : T::Y) + ty - (D == 1);
+  const int fz = bz * (D == 2 ? TD - 1 : T::Z) + tz - (D == 2);
+  const int td = D == 0 ? tx : (D == 1 ? ty : tz);
+  const int fd = D == 0 ? fx : (D == 1 ? fy : fz);
+  const bool need = fx < A.cnt[0] && fy < A.cnt[1] && fz < A.cnt[2];
+  const int i = A.lo[0] + fx, j = A.lo[1] + fy, k = A.lo[2] + fz;
+  Side lo, up;
+  bool flo = false, fup = false;
+  if (need) {
+    recon_cell<D, M>(P, A, off(P, 0, i, j, k), lo, up, flo, fup);
+    sh[0][t] = up.rho;
+    sh[1][t] = up.ent;
+    sh[2][t] = up.ye;
+    sh[3][t] = up.th;
+    sh[4][t] = up.b1;
+    sh[5][t] = up.b2;
+    sh[6][t] = up.v[0];
+    sh[7][t] = up.v[1];
+    sh[8][t] = up.v[2];
+    shf[t] = fup;
+  }
+  __syncthreads();
+  if (need && td >= 1 && fd >= 0) {
+    const int tl = t - SD;
+    Side L;
+    L.rho = sh[0][tl];
+    L.ent = sh[1][tl];
+    L.ye = sh[2][tl];
+    L.th = sh[3][tl];
+    L.b1 = sh[4][tl];
+    L.b2 = sh[5][tl];
+    L.v[0] = sh[6][tl];
+    L.v[1] = sh[7][tl];
+    L.v[2] = sh[8][tl];
+    face_flux<D>(P, A, i, j, k, L, lo, shf[tl] != 0, flo);
   }
 }
 
@@ -358,7 +489,7 @@ __global__ __launch_bounds__(256) void k_bstag(KPatch P, Box B, double *out) {
   if (t >= B.total()) return;
   int i, j, k;
   B.ijk(t, i, j, k);
-  const double idx[3] = {1 / P.dx[0], 1 / P.dx[1], 1 / P.dx[2]};
+  const double *idx = P.idx;
   const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
   double r;
   if (D == 0) {
@@ -441,16 +572,22 @@ __global__ __launch_bounds__(256) void k_prim2con(KPatch P, Box B) {
 template <int D>
 static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method,
                                 hipStream_t s) {
-  const int64_t nf = (int64_t)A.cnt[0] * A.cnt[1] * A.cnt[2];
-  if (nf <= 0) return hipSuccess;
-  const dim3 grid((unsigned)((nf + 255) / 256)), block(256);
+  using T = FaceTile<D>;
+  if ((int64_t)A.cnt[0] * A.cnt[1] * A.cnt[2] <= 0) return hipSuccess;
+  const int ex[3] = {T::X, T::Y, T::Z};
+  int nt[3];
+  for (int q = 0; q < 3; ++q) {
+    const int per = (q == D) ? ex[q] - 1 : ex[q];
+    nt[q] = (A.cnt[q] + per - 1) / per;
+  }
+  const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2])), block(256);
   switch (method) {
-  case ASX_RECON_GODUNOV: hipLaunchKernelGGL((k_flux<D, ASX_RECON_GODUNOV>), grid, block, 0, s, P, A); break;
-  case ASX_RECON_MINMOD: hipLaunchKernelGGL((k_flux<D, ASX_RECON_MINMOD>), grid, block, 0, s, P, A); break;
-  case ASX_RECON_MONOCENTRAL: hipLaunchKernelGGL((k_flux<D, ASX_RECON_MONOCENTRAL>), grid, block, 0, s, P, A); break;
-  case ASX_RECON_PPM: hipLaunchKernelGGL((k_flux<D, ASX_RECON_PPM>), grid, block, 0, s, P, A); break;
-  case ASX_RECON_WENOZ: hipLaunchKernelGGL((k_flux<D, ASX_RECON_WENOZ>), grid, block, 0, s, P, A); break;
-  default: hipLaunchKernelGGL((k_flux<D, ASX_RECON_MP5>), grid, block, 0, s, P, A); break;
+  case ASX_RECON_GODUNOV: hipLaunchKernelGGL((k_face<D, ASX_RECON_GODUNOV>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_MINMOD: hipLaunchKernelGGL((k_face<D, ASX_RECON_MINMOD>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_MONOCENTRAL: hipLaunchKernelGGL((k_face<D, ASX_RECON_MONOCENTRAL>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_PPM: hipLaunchKernelGGL((k_face<D, ASX_RECON_PPM>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_WENOZ: hipLaunchKernelGGL((k_face<D, ASX_RECON_WENOZ>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  default: hipLaunchKernelGGL((k_face<D, ASX_RECON_MP5>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
   }
   return hipGetLastError();
 }

@@ -20,6 +20,26 @@ struct Box {
   __host__ dim3 grid() const { return dim3((unsigned)((htotal() + 255) / 256)); }
 };
 
+// A box covered by workgroup tiles of TX x TY x 1 points (TX*TY work-items),
+// tiles ordered x, y, z and spread over the XCDs in contiguous runs.
+template <int TX, int TY> struct TileGrid {
+  int ntx, nty, ntz;
+  __host__ static TileGrid make(const Box &B) {
+    return TileGrid{(B.cnt[0] + TX - 1) / TX, (B.cnt[1] + TY - 1) / TY, B.cnt[2]};
+  }
+  __host__ dim3 grid() const { return dim3((unsigned)(ntx * nty * ntz)); }
+  // point of this work-item; false past the box
+  ASX_DEV bool point(const Box &B, int &i, int &j, int &k) const {
+    const int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
+    const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+    const int x = bx * TX + (int)threadIdx.x % TX, y = by * TY + (int)threadIdx.x / TX;
+    i = B.lo[0] + x;
+    j = B.lo[1] + y;
+    k = B.lo[2] + bz;
+    return x < B.cnt[0] && y < B.cnt[1];
+  }
+};
+
 struct FluxArgs {
   RParams rp;
   IdealGas eos;

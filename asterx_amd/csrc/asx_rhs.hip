@@ -1,67 +1,129 @@
 // Source terms, RHS (flux divergence, CT EMF, gauge), RK combination,
 // boundary fill and the fused update of the substep, for gfx950.
+#include <type_traits>
+
 #include "asx_kernels.hpp"
 
 namespace asx {
 
 // calc_fd_v2c<2|4> of a vertex field at cell o (AsterUtils aster_fd.hxx:49-111)
 template <int DIR>
-ASX_DEV double fd_v2c(const double *g, int o, int sj, int sk, int order, double dx) {
+ASX_DEV double fd_v2c(const double *g, int o, int sj, int sk, int order, double idx) {
   constexpr int D1 = (DIR + 1) % 3, D2 = (DIR + 2) % 3;
   const int sd = DIR == 0 ? 1 : (DIR == 1 ? sj : sk);
   const int s1 = D1 == 0 ? 1 : (D1 == 1 ? sj : sk);
   const int s2 = D2 == 0 ? 1 : (D2 == 1 ? sj : sk);
   if (order == 2) {
-    const double g1 = (g[o + sd] - g[o]) / dx;
-    const double g2 = (g[o + s1 + sd] - g[o + s1]) / dx;
-    const double g3 = (g[o + s2 + sd] - g[o + s2]) / dx;
-    const double g4 = (g[o + s1 + s2 + sd] - g[o + s1 + s2]) / dx;
+    const double g1 = (g[o + sd] - g[o]) * idx;
+    const double g2 = (g[o + s1 + sd] - g[o + s1]) * idx;
+    const double g3 = (g[o + s2 + sd] - g[o + s2]) * idx;
+    const double g4 = (g[o + s1 + s2 + sd] - g[o + s1 + s2]) * idx;
     return 0.25 * (g1 + g2 + g3 + g4);
   }
   auto d4 = [&](int b) {
     return ((1.0 / 24.0) * g[b - sd] - (27.0 / 24.0) * g[b] + (27.0 / 24.0) * g[b + sd] -
-            (1.0 / 24.0) * g[b + 2 * sd]) /
-           dx;
+            (1.0 / 24.0) * g[b + 2 * sd]) *
+           idx;
   };
   return 0.25 * (d4(o) + d4(o + s1) + d4(o + s2) + d4(o + s1 + s2));
 }
-ASX_DEV double fdv(int d, const double *g, int o, int sj, int sk, int order, double dx) {
-  return d == 0 ? fd_v2c<0>(g, o, sj, sk, order, dx)
-                : (d == 1 ? fd_v2c<1>(g, o, sj, sk, order, dx) : fd_v2c<2>(g, o, sj, sk, order, dx));
+ASX_DEV double fdv(int d, const double *g, int o, int sj, int sk, int order, double idx) {
+  return d == 0 ? fd_v2c<0>(g, o, sj, sk, order, idx)
+                : (d == 1 ? fd_v2c<1>(g, o, sj, sk, order, idx) : fd_v2c<2>(g, o, sj, sk, order, idx));
+}
+
+// The 8 vertex values around cell (vertex base o), bit d of the index set
+// for the +1 vertex along axis d.
+ASX_DEV void corners(const double *g, int o, int sj, int sk, double c[8]) {
+  c[0] = g[o];
+  c[1] = g[o + 1];
+  c[2] = g[o + sj];
+  c[3] = g[o + sj + 1];
+  c[4] = g[o + sk];
+  c[5] = g[o + sk + 1];
+  c[6] = g[o + sk + sj];
+  c[7] = g[o + sk + sj + 1];
+}
+// calc_avg_v2c from corners, same summation order as avg_v2c
+ASX_DEV double corner_avg(const double c[8]) {
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) s += c[q];
+  return s / 8.0;
+}
+// calc_fd_v2c<2> along d from corners (same expression as fd_v2c)
+ASX_DEV double corner_d2(const double c[8], int d, double idx) {
+  const int bd = 1 << d, b1 = 1 << ((d + 1) % 3), b2 = 1 << ((d + 2) % 3);
+  const double g1 = (c[bd] - c[0]) * idx;
+  const double g2 = (c[b1 | bd] - c[b1]) * idx;
+  const double g3 = (c[b2 | bd] - c[b2]) * idx;
+  const double g4 = (c[b1 | b2 | bd] - c[b1 | b2]) * idx;
+  return 0.25 * (g1 + g2 + g3 + g4);
+}
+// Scheduling fence: no instruction moves across it, so the loads of the
+// next field are not hoisted above the arithmetic of the previous one.  It
+// bounds how many stencil loads a work-item keeps in flight (each in-flight
+// double holds two VGPRs) and with it the register footprint.
+ASX_DEV void load_fence() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// Gradient of vertex field g at cell (vertex base o), order 2 or 4.
+template <int ORD>
+ASX_DEV void cell_grad(const KPatch &P, const double *g, int o, double out[3]) {
+  const int sj = P.sj[7], sk = P.sk[7];
+  load_fence();
+  if (ORD == 2) {
+    double c[8];
+    corners(g, o, sj, sk, c);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] = corner_d2(c, d, P.idx[d]);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) out[d] = fdv(d, g, o, sj, sk, 4, P.idx[d]);
+  }
+}
+// Cell average of vertex field g, fenced like cell_grad.
+ASX_DEV double cell_avg(const double *g, int o, int sj, int sk) {
+  load_fence();
+  return avg_v2c(g, o, sj, sk);
 }
 
 // SourceTerms (AsterX/src/source.cxx:32-167).  r[] = densrhs, momx, momy,
 // momz, tau, DEnt, DYe source contributions.
+//
+// Two passes over the vertex metric: the cell averages (and d alpha) give
+// T^{mu nu}; the momentum source is then linear in the metric derivatives,
+//   S_d = sum_{a<=b} C_ab d_d g_ab + sum_c T^0_c d_d beta^c - T^00 alpha d_d alpha,
+//   C_ab = w_ab (T^00 beta^a beta^b + T^ab) / 2 + (t0^a beta^b + t0^b beta^a) / (1 + [a==b]),
+// (w_ab = 1 on, 2 off the diagonal), which is source.cxx:138-150 regrouped
+// by derivative, so that each derivative is consumed as it is loaded.
+template <int ORD>
 ASX_DEV void source_cell(const KPatch &P, int i, int j, int k, const RhsArgs &A,
                          double r[7]) {
   const int oc = off(P, 0, i, j, k), ov = off(P, 7, i, j, k);
   const int sj = P.sj[7], sk = P.sk[7];
-  const double alp = avg_v2c(P.g[ASX_ALP], ov, sj, sk);
-  const double beta[3] = {avg_v2c(P.g[ASX_BETAX], ov, sj, sk),
-                          avg_v2c(P.g[ASX_BETAY], ov, sj, sk),
-                          avg_v2c(P.g[ASX_BETAZ], ov, sj, sk)};
-  const Sym3 g{avg_v2c(P.g[ASX_GXX], ov, sj, sk), avg_v2c(P.g[ASX_GXY], ov, sj, sk),
-               avg_v2c(P.g[ASX_GXZ], ov, sj, sk), avg_v2c(P.g[ASX_GYY], ov, sj, sk),
-               avg_v2c(P.g[ASX_GYZ], ov, sj, sk), avg_v2c(P.g[ASX_GZZ], ov, sj, sk)};
-  const Sym3 kk{avg_v2c(P.g[ASX_KXX], ov, sj, sk), avg_v2c(P.g[ASX_KXY], ov, sj, sk),
-                avg_v2c(P.g[ASX_KXZ], ov, sj, sk), avg_v2c(P.g[ASX_KYY], ov, sj, sk),
-                avg_v2c(P.g[ASX_KYZ], ov, sj, sk), avg_v2c(P.g[ASX_KZZ], ov, sj, sk)};
+  double c[8];
+  corners(P.g[ASX_ALP], ov, sj, sk, c);
+  const double alp = corner_avg(c);
+  double d_alp[3];
+  if (ORD == 2) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) d_alp[d] = corner_d2(c, d, P.idx[d]);
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) d_alp[d] = fdv(d, P.g[ASX_ALP], ov, sj, sk, 4, P.idx[d]);
+  }
+  const double beta[3] = {cell_avg(P.g[ASX_BETAX], ov, sj, sk),
+                          cell_avg(P.g[ASX_BETAY], ov, sj, sk),
+                          cell_avg(P.g[ASX_BETAZ], ov, sj, sk)};
+  const Sym3 g{cell_avg(P.g[ASX_GXX], ov, sj, sk), cell_avg(P.g[ASX_GXY], ov, sj, sk),
+               cell_avg(P.g[ASX_GXZ], ov, sj, sk), cell_avg(P.g[ASX_GYY], ov, sj, sk),
+               cell_avg(P.g[ASX_GYZ], ov, sj, sk), cell_avg(P.g[ASX_GZZ], ov, sj, sk)};
   const double detg = det3(g);
   const double sqrt_detg = sqrt(detg);
   const Sym3 ug = inv3(g, detg);
-  const int ord = A.fd_order;
-  double d_alp[3], d_beta[3][3];
-  Sym3 d_g[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const double dx = P.dx[d];
-    d_alp[d] = fdv(d, P.g[ASX_ALP], ov, sj, sk, ord, dx);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) d_beta[d][c] = fdv(d, P.g[ASX_BETAX + c], ov, sj, sk, ord, dx);
-    d_g[d] = Sym3{fdv(d, P.g[ASX_GXX], ov, sj, sk, ord, dx), fdv(d, P.g[ASX_GXY], ov, sj, sk, ord, dx),
-                  fdv(d, P.g[ASX_GXZ], ov, sj, sk, ord, dx), fdv(d, P.g[ASX_GYY], ov, sj, sk, ord, dx),
-                  fdv(d, P.g[ASX_GYZ], ov, sj, sk, ord, dx), fdv(d, P.g[ASX_GZZ], ov, sj, sk, ord, dx)};
-  }
   double v_up[3], v_low[3], w;
   if (A.rec_var == ASX_RECVAR_V) {
     v_up[0] = P.g[ASX_VELX][oc];
@@ -75,69 +137,94 @@ ASX_DEV void source_cell(const KPatch &P, int i, int j, int k, const RhsArgs &A,
     lower(g, z_up, z_low);
     w = sqrt(1.0 + dot3(z_low, z_up));
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v_up[c] = z_up[c] / w;
+    for (int q = 0; q < 3; ++q) v_up[q] = z_up[q] / w;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v_low[c] = z_low[c] / w;
+    for (int q = 0; q < 3; ++q) v_low[q] = z_low[q] / w;
   }
   const double rho = P.g[ASX_RHO][oc], eps = P.g[ASX_EPS][oc], press = P.g[ASX_PRESS][oc];
   const double rhW2 = (rho * (1.0 + eps) + press) * w * w;
+  const double ialp = 1.0 / alp, ialp2 = ialp * ialp, iw = 1.0 / w;
   double vs[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) vs[c] = v_up[c] - beta[c] / alp;
+  for (int q = 0; q < 3; ++q) vs[q] = v_up[q] - beta[q] * ialp;
   const double B_up[3] = {P.g[ASX_BVECX][oc], P.g[ASX_BVECY][oc], P.g[ASX_BVECZ][oc]};
   double B_low[3];
   lower(g, B_up, B_low);
-  const double bs0 = w * dot3(B_up, v_low) / alp;
+  const double bs0 = w * dot3(B_up, v_low) * ialp;
   double bs[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) bs[c] = B_up[c] / w + alp * bs0 * vs[c];
-  const double bs2 = (dot3(B_up, B_low) + pow2(alp * bs0)) / pow2(w);
+  for (int q = 0; q < 3; ++q) bs[q] = B_up[q] * iw + alp * bs0 * vs[q];
+  const double bs2 = (dot3(B_up, B_low) + pow2(alp * bs0)) * (iw * iw);
   double gb[3], gbs[3], bs_low[3];
   lower(g, beta, gb);
   lower(g, bs, gbs);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) bs_low[c] = gb[c] * bs0 + gbs[c];
+  for (int q = 0; q < 3; ++q) bs_low[q] = gb[q] * bs0 + gbs[q];
   const double Eh = rhW2 + bs2 * pow2(w);
   const double Ph = press + 0.5 * bs2;
-  const double t00 = (Eh - Ph) / pow2(alp) - pow2(bs0);
+  const double t00 = (Eh - Ph) * ialp2 - pow2(bs0);
   double t0[3], t0low[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) t0[c] = Eh * vs[c] / alp + Ph * beta[c] / pow2(alp) - bs0 * bs[c];
+  for (int q = 0; q < 3; ++q) t0[q] = Eh * vs[q] * ialp + Ph * beta[q] * ialp2 - bs0 * bs[q];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) t0low[c] = Eh * v_low[c] / alp - bs_low[c] * bs0;
+  for (int q = 0; q < 3; ++q) t0low[q] = Eh * v_low[q] * ialp - bs_low[q] * bs0;
   auto t = [&](int a, int b) {
-    return Eh * vs[a] * vs[b] + Ph * (ug(a, b) - beta[a] * beta[b] / pow2(alp)) - bs[a] * bs[b];
+    return Eh * vs[a] * vs[b] + Ph * (ug(a, b) - beta[a] * beta[b] * ialp2) - bs[a] * bs[b];
   };
-  // upper-triangle contraction with off-diagonal weight 2 (sum_symm)
-  auto symm = [&](auto f) {
-    double s = 0.0;
+  // tau source (source.cxx:152-158), linear in K:
+  //   tau_S = -T^00 beta.d(alpha) - t0.d(alpha) + sum_{a<=b} K_ab D_ab,
+  //   D_ab = w_ab (T^00 beta^a beta^b + T^ab) + 2 (t0^a beta^b + t0^b beta^a) / (1 + [a==b])
+  double tau_source = -t00 * dot3(beta, d_alp) - dot3(t0, d_alp);
+  {
+    double dk[6];
 #pragma unroll
     for (int x = 0; x < 3; ++x)
 #pragma unroll
-      for (int y = x; y < 3; ++y) s += (x == y ? 1 : 2) * f(x, y);
-    return s;
-  };
-  double shiftk[3];
-  lower(kk, beta, shiftk);
-  const double shiftshiftk = dot3(shiftk, beta);
-  double tmp[3];
+      for (int y = x; y < 3; ++y) {
+        const double wab = x == y ? 1.0 : 2.0;
+        const double cross = x == y ? 2.0 * t0[x] * beta[x]
+                                    : 2.0 * (t0[x] * beta[y] + t0[y] * beta[x]);
+        dk[x == 0 ? y : (x == 1 ? 2 + y : 5)] =
+            wab * (t00 * beta[x] * beta[y] + t(x, y)) + cross;
+      }
+#pragma unroll 1
+    for (int f = 0; f < 6; ++f) {
+      double c = dk[0];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) tmp[c] = -d_alp[c] + 2.0 * shiftk[c];
-  const double tau_source = t00 * (shiftshiftk - dot3(beta, d_alp)) + dot3(t0, tmp) +
-                            symm([&](int a, int b) { return kk(a, b) * t(a, b); });
+      for (int q = 1; q < 6; ++q) c = (f == q) ? dk[q] : c;
+      tau_source += c * cell_avg(P.g[ASX_KXX + f], ov, sj, sk);
+    }
+  }
   double ms[3];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const Sym3 &dg = d_g[d];
-    double s2 = 0.0;
+  for (int d = 0; d < 3; ++d) ms[d] = -(t00 * alp) * d_alp[d];
+  // the derivative pass re-reads the metric (cache hits) rather than keeping
+  // 80 corner values live across the stress-energy computation
+  // coefficient of each derivative: beta^x, beta^y, beta^z, then g_ab
+  double cf[9];
 #pragma unroll
-    for (int x = 0; x < 3; ++x)
+  for (int q = 0; q < 3; ++q) cf[q] = t0low[q];
 #pragma unroll
-      for (int y = 0; y < 3; ++y) s2 += dg(x, y) * t0[x] * beta[y];
-    ms[d] = t00 * (0.5 * symm([&](int a, int b) { return dg(a, b) * beta[a] * beta[b]; }) -
-                   alp * d_alp[d]) +
-            s2 + dot3(t0low, d_beta[d]) +
-            0.5 * symm([&](int a, int b) { return t(a, b) * dg(a, b); });
+  for (int x = 0; x < 3; ++x)
+#pragma unroll
+    for (int y = x; y < 3; ++y) {
+      const double wab = x == y ? 1.0 : 2.0;
+      const double cross = x == y ? t0[x] * beta[x] : t0[x] * beta[y] + t0[y] * beta[x];
+      cf[3 + (x == 0 ? y : (x == 1 ? 2 + y : 5))] =
+          0.5 * wab * (t00 * beta[x] * beta[y] + t(x, y)) + cross;
+    }
+  // one field per iteration (not unrolled), so only that field's corner
+  // values are live
+#pragma unroll 1
+  for (int f = 0; f < 9; ++f) {
+    const int slot = f < 3 ? ASX_BETAX + f : ASX_GXX + (f - 3);
+    double dq[3];
+    cell_grad<ORD>(P, P.g[slot], ov, dq);
+    double c = cf[0];
+#pragma unroll
+    for (int q = 1; q < 9; ++q) c = (f == q) ? cf[q] : c;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ms[d] += c * dq[d];
   }
   r[0] = 0.0;
   r[1] = alp * sqrt_detg * ms[0];
@@ -149,26 +236,33 @@ ASX_DEV void source_cell(const KPatch &P, int i, int j, int k, const RhsArgs &A,
 }
 
 // rhs slots in the order source_cell returns them
-__constant__ int kRhsSlot[7] = {ASX_DENSRHS, ASX_MOMXRHS, ASX_MOMYRHS, ASX_MOMZRHS,
+ASX_DEV constexpr int kRhsSlot(int q) {
+  constexpr int t[7] = {ASX_DENSRHS, ASX_MOMXRHS, ASX_MOMYRHS, ASX_MOMZRHS,
                                 ASX_TAURHS,  ASX_DENTRHS, ASX_DYE_RHS};
+  return t[q];
+}
 // flux variable (offset in a flux group) of each of those
-__constant__ int kFluxVar[7] = {0, 1, 2, 3, 4, 6, 5};
+ASX_DEV constexpr int kFluxVar(int q) {
+  constexpr int t[7] = {0, 1, 2, 3, 4, 6, 5};
+  return t[q];
+}
 
+template <int ORD>
 __global__ __launch_bounds__(256) void k_sources(KPatch P, Box B, RhsArgs A) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.total()) return;
   int i, j, k;
   B.ijk(t, i, j, k);
   double r[7];
-  source_cell(P, i, j, k, A, r);
+  source_cell<ORD>(P, i, j, k, A, r);
   const int oc = off(P, 0, i, j, k);
 #pragma unroll
-  for (int q = 0; q < 7; ++q) P.g[kRhsSlot[q]][oc] = r[q];
+  for (int q = 0; q < 7; ++q) P.g[kRhsSlot(q)][oc] = r[q];
 }
 
 // -(idx . dF) of one flux variable at cell o (AsterX/src/rhs.cxx:107-114)
 ASX_DEV double flux_div(const KPatch &P, int i, int j, int k, int var) {
-  const double idx[3] = {1 / P.dx[0], 1 / P.dx[1], 1 / P.dx[2]};
+  const double *idx = P.idx;
   const int o1 = off(P, 1, i, j, k), o2 = off(P, 2, i, j, k), o4 = off(P, 4, i, j, k);
   const double *Fx = P.g[ASX_FXDENS + var], *Fy = P.g[ASX_FYDENS + var],
                *Fz = P.g[ASX_FZDENS + var];
@@ -184,18 +278,42 @@ __global__ __launch_bounds__(256) void k_rhs_hydro(KPatch P, Box B) {
   B.ijk(t, i, j, k);
   const int oc = off(P, 0, i, j, k);
 #pragma unroll
-  for (int q = 0; q < 7; ++q) P.g[kRhsSlot[q]][oc] += flux_div(P, i, j, k, kFluxVar[q]);
+  for (int q = 0; q < 7; ++q) P.g[kRhsSlot(q)][oc] += flux_div(P, i, j, k, kFluxVar(q));
 }
 
 // UCT face data of AsterX/src/rhs.cxx:87-105 by edge direction
-__constant__ int kDBone[3] = {ASX_DBY_STAG, ASX_DBZ_STAG, ASX_DBX_STAG};
-__constant__ int kDBtwo[3] = {ASX_DBZ_STAG, ASX_DBX_STAG, ASX_DBY_STAG};
-__constant__ int kVTone[3] = {ASX_VTILDE_Z_YFACE, ASX_VTILDE_X_ZFACE, ASX_VTILDE_Y_XFACE};
-__constant__ int kVTtwo[3] = {ASX_VTILDE_Y_ZFACE, ASX_VTILDE_Z_XFACE, ASX_VTILDE_X_YFACE};
-__constant__ int kAmaxOne[3] = {ASX_AMAX_ZFACE, ASX_AMAX_XFACE, ASX_AMAX_YFACE};
-__constant__ int kAmaxTwo[3] = {ASX_AMAX_YFACE, ASX_AMAX_ZFACE, ASX_AMAX_XFACE};
-__constant__ int kAminOne[3] = {ASX_AMIN_ZFACE, ASX_AMIN_XFACE, ASX_AMIN_YFACE};
-__constant__ int kAminTwo[3] = {ASX_AMIN_YFACE, ASX_AMIN_ZFACE, ASX_AMIN_XFACE};
+ASX_DEV constexpr int kDBone(int q) {
+  constexpr int t[3] = {ASX_DBY_STAG, ASX_DBZ_STAG, ASX_DBX_STAG};
+  return t[q];
+}
+ASX_DEV constexpr int kDBtwo(int q) {
+  constexpr int t[3] = {ASX_DBZ_STAG, ASX_DBX_STAG, ASX_DBY_STAG};
+  return t[q];
+}
+ASX_DEV constexpr int kVTone(int q) {
+  constexpr int t[3] = {ASX_VTILDE_Z_YFACE, ASX_VTILDE_X_ZFACE, ASX_VTILDE_Y_XFACE};
+  return t[q];
+}
+ASX_DEV constexpr int kVTtwo(int q) {
+  constexpr int t[3] = {ASX_VTILDE_Y_ZFACE, ASX_VTILDE_Z_XFACE, ASX_VTILDE_X_YFACE};
+  return t[q];
+}
+ASX_DEV constexpr int kAmaxOne(int q) {
+  constexpr int t[3] = {ASX_AMAX_ZFACE, ASX_AMAX_XFACE, ASX_AMAX_YFACE};
+  return t[q];
+}
+ASX_DEV constexpr int kAmaxTwo(int q) {
+  constexpr int t[3] = {ASX_AMAX_YFACE, ASX_AMAX_ZFACE, ASX_AMAX_XFACE};
+  return t[q];
+}
+ASX_DEV constexpr int kAminOne(int q) {
+  constexpr int t[3] = {ASX_AMIN_ZFACE, ASX_AMIN_XFACE, ASX_AMIN_YFACE};
+  return t[q];
+}
+ASX_DEV constexpr int kAminTwo(int q) {
+  constexpr int t[3] = {ASX_AMIN_YFACE, ASX_AMIN_ZFACE, ASX_AMIN_XFACE};
+  return t[q];
+}
 
 // Reconstruct a face field along `dir` at the edge index (as the reference
 // does, with the edge's I used in the face array's layout).
@@ -239,12 +357,12 @@ ASX_DEV double avec_rhs(const KPatch &P, int i, int j, int k, const RhsArgs &A) 
   double E;
   if (A.use_uct) {
     double dB1[2], dB2[2], vt1[2], vt2[2];
-    recon_rt(P, A.recon, kDBone[I], K, i, j, k, A.rp, dB1[0], dB1[1]);
-    recon_rt(P, A.recon, kDBtwo[I], J, i, j, k, A.rp, dB2[0], dB2[1]);
-    recon_rt(P, A.recon, kVTone[I], K, i, j, k, A.rp, vt1[0], vt1[1]);
-    recon_rt(P, A.recon, kVTtwo[I], J, i, j, k, A.rp, vt2[0], vt2[1]);
-    const double ax1 = at(P, kAmaxOne[I], i, j, k), an1 = at(P, kAminOne[I], i, j, k);
-    const double ax2 = at(P, kAmaxTwo[I], i, j, k), an2 = at(P, kAminTwo[I], i, j, k);
+    recon_rt(P, A.recon, kDBone(I), K, i, j, k, A.rp, dB1[0], dB1[1]);
+    recon_rt(P, A.recon, kDBtwo(I), J, i, j, k, A.rp, dB2[0], dB2[1]);
+    recon_rt(P, A.recon, kVTone(I), K, i, j, k, A.rp, vt1[0], vt1[1]);
+    recon_rt(P, A.recon, kVTtwo(I), J, i, j, k, A.rp, vt2[0], vt2[1]);
+    const double ax1 = at(P, kAmaxOne(I), i, j, k), an1 = at(P, kAminOne(I), i, j, k);
+    const double ax2 = at(P, kAmaxTwo(I), i, j, k), an2 = at(P, kAminTwo(I), i, j, k);
     const double den1 = ax1 + an1;
     E = (ax1 * vt1[0] * dB1[0] + an1 * vt1[1] * dB1[1] - ax1 * an1 * (dB1[1] - dB1[0])) / den1;
     const double den2 = ax2 + an2;
@@ -262,7 +380,7 @@ ASX_DEV double avec_rhs(const KPatch &P, int i, int j, int k, const RhsArgs &A) 
   const int ov = off(P, 7, i, j, k);
   const double *G = P.g[ASX_G];
   (void)CE;
-  return -E - (G[ov + step(P, 7, I)] - G[ov]) / P.dx[I];
+  return -E - (G[ov + step(P, 7, I)] - G[ov]) * P.idx[I];
 }
 
 template <int I>
@@ -282,13 +400,13 @@ ASX_DEV double psi_rhs(const KPatch &P, int i, int j, int k, const RhsArgs &A) {
   double dF = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    const double dx = P.dx[d];
+    const double hidx = 0.5 * P.idx[d]; // == 0.5 / dx exactly
     const int st = step(P, 7, d);
     const double *F = P.g[ASX_FX + d], *Fb = P.g[ASX_FBETAX + d];
-    const double c2c = (0.5 / dx) * (F[ov + st] - F[ov - st]);
+    const double c2c = hidx * (F[ov + st] - F[ov - st]);
     const int sign = P.g[ASX_BETAX + d][ov] < 0 ? -1 : 1;
     const double one =
-        -sign * (Fb[ov + 2 * sign * st] - 4.0 * Fb[ov + sign * st] + 3.0 * Fb[ov]) * (0.5 / dx);
+        -sign * (Fb[ov + 2 * sign * st] - 4.0 * Fb[ov + sign * st] + 3.0 * Fb[ov]) * hidx;
     dF += c2c - one;
   }
   return -dF - A.lorenz_damp_fac * P.g[ASX_ALP][ov] * P.g[ASX_PSI][ov];
@@ -361,57 +479,64 @@ struct Comb {
     return acc + c2 * r;
   }
 };
-__constant__ int kStateSlot[ASX_NUM_STATE] = {ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ,
+ASX_DEV constexpr int kStateSlot(int q) {
+  constexpr int t[ASX_NUM_STATE] = {ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ,
                                               ASX_TAU,  ASX_DENT, ASX_DYE,  ASX_AVEC_X,
                                               ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI};
+  return t[q];
+}
 // state index of each rhs component returned by source_cell
-__constant__ int kSrcState[7] = {0, 1, 2, 3, 4, 5, 6};
+ASX_DEV constexpr int kSrcState(int q) {
+  constexpr int t[7] = {0, 1, 2, 3, 4, 5, 6};
+  return t[q];
+}
 
-__global__ __launch_bounds__(256) void k_update_cells(KPatch P, Box B, RhsArgs A,
-                                                      StatePtrs u0, StatePtrs out,
-                                                      Comb cb) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B.total()) return;
+// One work-item per point (i, j, k) of the box [ng, ng+n] on every axis:
+// the interior cell (sources + flux divergence), the three interior edges
+// and the interior vertex that share the index, each combined with the
+// previous stages as out = c0*u0 + c1*u + c2*rhs.
+// waves per SIMD the register budget is sized for (order-4 derivatives
+// need 32 more doubles live than order 2)
+template <int ORD> constexpr int kUpdateWaves = ORD == 2 ? 4 : 3;
+using UpdateGrid = TileGrid<64, 4>;
+template <int ORD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<ORD>)))
+void k_update_all(KPatch P, Box B, UpdateGrid G, RhsArgs A, StatePtrs u0, StatePtrs out,
+                  Comb cb) {
   int i, j, k;
-  B.ijk(t, i, j, k);
-  double r[7];
-  source_cell(P, i, j, k, A, r);
-  const int oc = off(P, 0, i, j, k);
+  if (!G.point(B, i, j, k)) return;
+  const bool cx = i < P.ng + P.n[0], cy = j < P.ng + P.n[1], cz = k < P.ng + P.n[2];
+  if (cx && cy && cz) {
+    double r[7];
+    source_cell<ORD>(P, i, j, k, A, r);
+    const int oc = off(P, 0, i, j, k);
 #pragma unroll
-  for (int q = 0; q < 7; ++q) {
-    const double rhs = r[q] + flux_div(P, i, j, k, kFluxVar[q]);
-    const int s = kSrcState[q];
-    const double u = P.g[kStateSlot[s]][oc];
-    const double z = (cb.c0 != 0.0) ? u0.g[s][oc] : 0.0;
-    out.g[s][oc] = cb(z, u, rhs);
+    for (int q = 0; q < 7; ++q) {
+      const double rhs = r[q] + flux_div(P, i, j, k, kFluxVar(q));
+      const int s = kSrcState(q);
+      const double u = P.g[kStateSlot(s)][oc];
+      const double z = (cb.c0 != 0.0) ? u0.g[s][oc] : 0.0;
+      out.g[s][oc] = cb(z, u, rhs);
+    }
   }
-}
-template <int I>
-__global__ __launch_bounds__(256) void k_update_avec(KPatch P, Box B, RhsArgs A,
-                                                     StatePtrs u0, StatePtrs out,
-                                                     Comb cb) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B.total()) return;
-  int i, j, k;
-  B.ijk(t, i, j, k);
-  constexpr int CE = 7 ^ (1 << I);
-  const int o = off(P, CE, i, j, k);
-  const double rhs = avec_rhs<I>(P, i, j, k, A);
-  const double u = P.g[ASX_AVEC_X + I][o];
-  const double z = (cb.c0 != 0.0) ? u0.g[7 + I][o] : 0.0;
-  out.g[7 + I][o] = cb(z, u, rhs);
-}
-__global__ __launch_bounds__(256) void k_update_psi(KPatch P, Box B, RhsArgs A,
-                                                    StatePtrs u0, StatePtrs out,
-                                                    Comb cb) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B.total()) return;
-  int i, j, k;
-  B.ijk(t, i, j, k);
-  const int o = off(P, 7, i, j, k);
-  const double rhs = psi_rhs(P, i, j, k, A);
-  const double z = (cb.c0 != 0.0) ? u0.g[10][o] : 0.0;
-  out.g[10][o] = cb(z, P.g[ASX_PSI][o], rhs);
+  auto edge = [&](auto I_) {
+    constexpr int I = decltype(I_)::value;
+    constexpr int CE = 7 ^ (1 << I);
+    const int o = off(P, CE, i, j, k);
+    const double rhs = avec_rhs<I>(P, i, j, k, A);
+    const double u = P.g[ASX_AVEC_X + I][o];
+    const double z = (cb.c0 != 0.0) ? u0.g[7 + I][o] : 0.0;
+    out.g[7 + I][o] = cb(z, u, rhs);
+  };
+  if (cx) edge(std::integral_constant<int, 0>{});
+  if (cy) edge(std::integral_constant<int, 1>{});
+  if (cz) edge(std::integral_constant<int, 2>{});
+  {
+    const int o = off(P, 7, i, j, k);
+    const double rhs = psi_rhs(P, i, j, k, A);
+    const double z = (cb.c0 != 0.0) ? u0.g[10][o] : 0.0;
+    out.g[10][o] = cb(z, P.g[ASX_PSI][o], rhs);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -427,7 +552,10 @@ static Box interior_box(const KPatch &P, int c, int halo = 0) {
 }
 
 hipError_t launch_sources(const KPatch &P, const Box &B, const RhsArgs &A, hipStream_t s) {
-  hipLaunchKernelGGL(k_sources, B.grid(), dim3(256), 0, s, P, B, A);
+  if (A.fd_order == 4)
+    hipLaunchKernelGGL(k_sources<4>, B.grid(), dim3(256), 0, s, P, B, A);
+  else
+    hipLaunchKernelGGL(k_sources<2>, B.grid(), dim3(256), 0, s, P, B, A);
   return hipGetLastError();
 }
 
@@ -477,16 +605,12 @@ hipError_t launch_update(const KPatch &P, const RhsArgs &A, const StatePtrs &u0,
                          const StatePtrs &out, double c0, double c1, double c2,
                          hipStream_t s) {
   const Comb cb{c0, c1, c2};
-  Box b = interior_box(P, 0);
-  hipLaunchKernelGGL(k_update_cells, b.grid(), dim3(256), 0, s, P, b, A, u0, out, cb);
-  b = interior_box(P, 6);
-  hipLaunchKernelGGL(k_update_avec<0>, b.grid(), dim3(256), 0, s, P, b, A, u0, out, cb);
-  b = interior_box(P, 5);
-  hipLaunchKernelGGL(k_update_avec<1>, b.grid(), dim3(256), 0, s, P, b, A, u0, out, cb);
-  b = interior_box(P, 3);
-  hipLaunchKernelGGL(k_update_avec<2>, b.grid(), dim3(256), 0, s, P, b, A, u0, out, cb);
-  b = interior_box(P, 7);
-  hipLaunchKernelGGL(k_update_psi, b.grid(), dim3(256), 0, s, P, b, A, u0, out, cb);
+  const Box b = interior_box(P, 7);
+  const UpdateGrid G = UpdateGrid::make(b);
+  if (A.fd_order == 4)
+    hipLaunchKernelGGL(k_update_all<4>, G.grid(), dim3(256), 0, s, P, b, G, A, u0, out, cb);
+  else
+    hipLaunchKernelGGL(k_update_all<2>, G.grid(), dim3(256), 0, s, P, b, G, A, u0, out, cb);
   return hipGetLastError();
 }
 
