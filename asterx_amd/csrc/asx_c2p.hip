@@ -46,9 +46,96 @@ ASX_DEV void dB_from_A(const KPatch &P, int i, int j, int k, double dB[3]) {
   dB[2] = 0.5 * (bz(i, j, k) + bz(i, j, k + 1));
 }
 
+// Inputs of the zone at cell (i, j, k): cell-averaged metric, conserved
+// variables (dB from A or from the dB grid functions) and the seed.
 template <bool FROM_A>
-__global__ __launch_bounds__(256) void k_con2prim(KPatch P, Box B, IdealGas eos, C2PArgs a,
-                                                  unsigned long long *hist) {
+ASX_DEV void load_zone(const KPatch &P, int i, int j, int k, const C2PArgs &a, C2PIn &in) {
+  const int oc = off(P, 0, i, j, k), ov = off(P, 7, i, j, k);
+  const int sj = P.sj[7], sk = P.sk[7];
+  const double x = P.x0[0] + (i - P.ng + 0.5) * P.dx[0];
+  const double y = P.x0[1] + (j - P.ng + 0.5) * P.dx[1];
+  const double z = P.x0[2] + (k - P.ng + 0.5) * P.dx[2];
+  in.r = sqrt(x * x + y * y + z * z);
+  in.alp = avg_v2c(P.g[ASX_ALP], ov, sj, sk);
+  in.glo = Sym3{avg_v2c(P.g[ASX_GXX], ov, sj, sk), avg_v2c(P.g[ASX_GXY], ov, sj, sk),
+                avg_v2c(P.g[ASX_GXZ], ov, sj, sk), avg_v2c(P.g[ASX_GYY], ov, sj, sk),
+                avg_v2c(P.g[ASX_GYZ], ov, sj, sk), avg_v2c(P.g[ASX_GZZ], ov, sj, sk)};
+  in.mask = a.use_mask ? avg_v2c(P.g[ASX_MASK_VC], ov, sj, sk) : 1.0;
+  in.cv.dens = P.g[ASX_DENS][oc];
+  in.cv.mom[0] = P.g[ASX_MOMX][oc];
+  in.cv.mom[1] = P.g[ASX_MOMY][oc];
+  in.cv.mom[2] = P.g[ASX_MOMZ][oc];
+  in.cv.tau = P.g[ASX_TAU][oc];
+  in.cv.DYe = P.g[ASX_DYE][oc];
+  in.cv.DEnt = P.g[ASX_DENT][oc];
+  if (FROM_A) {
+    dB_from_A(P, i, j, k, in.cv.dB);
+  } else {
+    in.cv.dB[0] = P.g[ASX_DBX][oc];
+    in.cv.dB[1] = P.g[ASX_DBY][oc];
+    in.cv.dB[2] = P.g[ASX_DBZ][oc];
+  }
+  in.saved_rho = P.g[ASX_SAVED_RHO][oc];
+  in.saved_eps = P.g[ASX_SAVED_EPS][oc];
+  in.saved_Ye = P.g[ASX_SAVED_YE][oc];
+  in.saved_vel[0] = P.g[ASX_SAVED_VELX][oc];
+  in.saved_vel[1] = P.g[ASX_SAVED_VELY][oc];
+  in.saved_vel[2] = P.g[ASX_SAVED_VELZ][oc];
+  in.temperature = P.g[ASX_TEMPERATURE][oc];
+}
+
+// Results of the zone (con2prim.cxx:418-453).
+ASX_DEV void store_zone(const KPatch &P, int oc, const C2POut &o) {
+  const Prim &pv = o.pv;
+  const double w = pv.w_lor;
+  P.g[ASX_CON2PRIM_FLAG][oc] = o.flag;
+  if (P.g[ASX_MASK_CC]) P.g[ASX_MASK_CC][oc] = o.mask_cc;
+  P.g[ASX_RHO][oc] = pv.rho;
+  P.g[ASX_EPS][oc] = pv.eps;
+  P.g[ASX_YE][oc] = pv.Ye;
+  P.g[ASX_PRESS][oc] = pv.press;
+  P.g[ASX_TEMPERATURE][oc] = pv.temperature;
+  P.g[ASX_ENTROPY][oc] = pv.entropy;
+  P.g[ASX_VELX][oc] = pv.vel[0];
+  P.g[ASX_VELY][oc] = pv.vel[1];
+  P.g[ASX_VELZ][oc] = pv.vel[2];
+  P.g[ASX_BVECX][oc] = pv.B[0];
+  P.g[ASX_BVECY][oc] = pv.B[1];
+  P.g[ASX_BVECZ][oc] = pv.B[2];
+  P.g[ASX_ZVEC_X][oc] = w * pv.vel[0];
+  P.g[ASX_ZVEC_Y][oc] = w * pv.vel[1];
+  P.g[ASX_ZVEC_Z][oc] = w * pv.vel[2];
+  const double hw = (pv.rho + pv.rho * pv.eps + pv.press) * w * w;
+  P.g[ASX_SVEC_X][oc] = hw * pv.vel[0];
+  P.g[ASX_SVEC_Y][oc] = hw * pv.vel[1];
+  P.g[ASX_SVEC_Z][oc] = hw * pv.vel[2];
+  const Cons &cv = o.cv;
+  P.g[ASX_DENS][oc] = cv.dens;
+  P.g[ASX_MOMX][oc] = cv.mom[0];
+  P.g[ASX_MOMY][oc] = cv.mom[1];
+  P.g[ASX_MOMZ][oc] = cv.mom[2];
+  P.g[ASX_TAU][oc] = cv.tau;
+  P.g[ASX_DYE][oc] = cv.DYe;
+  P.g[ASX_DENT][oc] = cv.DEnt;
+  P.g[ASX_DBX][oc] = cv.dB[0];
+  P.g[ASX_DBY][oc] = cv.dB[1];
+  P.g[ASX_DBZ][oc] = cv.dB[2];
+  P.g[ASX_SAVED_RHO][oc] = pv.rho;
+  P.g[ASX_SAVED_VELX][oc] = pv.vel[0];
+  P.g[ASX_SAVED_VELY][oc] = pv.vel[1];
+  P.g[ASX_SAVED_VELZ][oc] = pv.vel[2];
+  P.g[ASX_SAVED_EPS][oc] = pv.eps;
+  P.g[ASX_SAVED_YE][oc] = pv.Ye;
+}
+
+// Pass 1: every zone through the primary solver (compiled for PRIME only,
+// so the backup solvers cost no registers here).  Zones whose primary solve
+// fails store nothing; a wavefront ballot packs their indices into `queue`
+// with one atomic per wavefront.
+template <bool FROM_A, int PRIME>
+__global__ __launch_bounds__(256) void k_c2p_main(KPatch P, Box B, IdealGas eos, C2PArgs a,
+                                                  unsigned long long *hist, int *queue,
+                                                  unsigned *qcount) {
   __shared__ HistLDS h;
   if (hist) {
     hist_init(h);
@@ -56,84 +143,59 @@ __global__ __launch_bounds__(256) void k_con2prim(KPatch P, Box B, IdealGas eos,
   }
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = t < B.total();
+  bool fail = false;
   if (live) {
     int i, j, k;
     B.ijk(t, i, j, k);
-    const int oc = off(P, 0, i, j, k), ov = off(P, 7, i, j, k);
-    const int sj = P.sj[7], sk = P.sk[7];
     C2PIn in;
-    const double x = P.x0[0] + (i - P.ng + 0.5) * P.dx[0];
-    const double y = P.x0[1] + (j - P.ng + 0.5) * P.dx[1];
-    const double z = P.x0[2] + (k - P.ng + 0.5) * P.dx[2];
-    in.r = sqrt(x * x + y * y + z * z);
-    in.alp = avg_v2c(P.g[ASX_ALP], ov, sj, sk);
-    in.glo = Sym3{avg_v2c(P.g[ASX_GXX], ov, sj, sk), avg_v2c(P.g[ASX_GXY], ov, sj, sk),
-                  avg_v2c(P.g[ASX_GXZ], ov, sj, sk), avg_v2c(P.g[ASX_GYY], ov, sj, sk),
-                  avg_v2c(P.g[ASX_GYZ], ov, sj, sk), avg_v2c(P.g[ASX_GZZ], ov, sj, sk)};
-    in.mask = a.use_mask ? avg_v2c(P.g[ASX_MASK_VC], ov, sj, sk) : 1.0;
-    in.cv.dens = P.g[ASX_DENS][oc];
-    in.cv.mom[0] = P.g[ASX_MOMX][oc];
-    in.cv.mom[1] = P.g[ASX_MOMY][oc];
-    in.cv.mom[2] = P.g[ASX_MOMZ][oc];
-    in.cv.tau = P.g[ASX_TAU][oc];
-    in.cv.DYe = P.g[ASX_DYE][oc];
-    in.cv.DEnt = P.g[ASX_DENT][oc];
-    if (FROM_A) {
-      dB_from_A(P, i, j, k, in.cv.dB);
+    load_zone<FROM_A>(P, i, j, k, a, in);
+    C2POut o;
+    if (con2prim_zone<PRIME>(eos, a, in, o)) {
+      store_zone(P, off(P, 0, i, j, k), o);
+      if (hist) {
+        atomicAdd(&h.flags[(int)o.flag], 1u);
+        atomicAdd(&h.iters[o.iters < 63 ? o.iters : 63], 1u);
+      }
     } else {
-      in.cv.dB[0] = P.g[ASX_DBX][oc];
-      in.cv.dB[1] = P.g[ASX_DBY][oc];
-      in.cv.dB[2] = P.g[ASX_DBZ][oc];
+      fail = true;
     }
-    in.saved_rho = P.g[ASX_SAVED_RHO][oc];
-    in.saved_eps = P.g[ASX_SAVED_EPS][oc];
-    in.saved_Ye = P.g[ASX_SAVED_YE][oc];
-    in.saved_vel[0] = P.g[ASX_SAVED_VELX][oc];
-    in.saved_vel[1] = P.g[ASX_SAVED_VELY][oc];
-    in.saved_vel[2] = P.g[ASX_SAVED_VELZ][oc];
-    in.temperature = P.g[ASX_TEMPERATURE][oc];
+  }
+  const uint64_t m = __ballot(fail);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (fail) queue[base + __popcll(m & ((1ull << lane) - 1))] = (int)t;
+  }
+  if (hist) {
+    __syncthreads();
+    hist_flush(h, hist);
+  }
+}
+
+// Pass 2: the queued zones through the reference's full chain (primary
+// again, then the backup), grid-stride over the device-side count.
+template <bool FROM_A>
+__global__ __launch_bounds__(256) void k_c2p_rest(KPatch P, Box B, IdealGas eos, C2PArgs a,
+                                                  unsigned long long *hist, const int *queue,
+                                                  const unsigned *qcount) {
+  __shared__ HistLDS h;
+  if (hist) {
+    hist_init(h);
+    __syncthreads();
+  }
+  const unsigned nq = *qcount;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += gridDim.x * blockDim.x) {
+    int i, j, k;
+    B.ijk(queue[q], i, j, k);
+    C2PIn in;
+    load_zone<FROM_A>(P, i, j, k, a, in);
     C2POut o;
     con2prim_cell(eos, a, in, o);
-    const Prim &pv = o.pv;
-    const double w = pv.w_lor;
-    P.g[ASX_CON2PRIM_FLAG][oc] = o.flag;
-    if (P.g[ASX_MASK_CC]) P.g[ASX_MASK_CC][oc] = o.mask_cc;
-    P.g[ASX_RHO][oc] = pv.rho;
-    P.g[ASX_EPS][oc] = pv.eps;
-    P.g[ASX_YE][oc] = pv.Ye;
-    P.g[ASX_PRESS][oc] = pv.press;
-    P.g[ASX_TEMPERATURE][oc] = pv.temperature;
-    P.g[ASX_ENTROPY][oc] = pv.entropy;
-    P.g[ASX_VELX][oc] = pv.vel[0];
-    P.g[ASX_VELY][oc] = pv.vel[1];
-    P.g[ASX_VELZ][oc] = pv.vel[2];
-    P.g[ASX_BVECX][oc] = pv.B[0];
-    P.g[ASX_BVECY][oc] = pv.B[1];
-    P.g[ASX_BVECZ][oc] = pv.B[2];
-    P.g[ASX_ZVEC_X][oc] = w * pv.vel[0];
-    P.g[ASX_ZVEC_Y][oc] = w * pv.vel[1];
-    P.g[ASX_ZVEC_Z][oc] = w * pv.vel[2];
-    const double hw = (pv.rho + pv.rho * pv.eps + pv.press) * w * w;
-    P.g[ASX_SVEC_X][oc] = hw * pv.vel[0];
-    P.g[ASX_SVEC_Y][oc] = hw * pv.vel[1];
-    P.g[ASX_SVEC_Z][oc] = hw * pv.vel[2];
-    const Cons &cv = o.cv;
-    P.g[ASX_DENS][oc] = cv.dens;
-    P.g[ASX_MOMX][oc] = cv.mom[0];
-    P.g[ASX_MOMY][oc] = cv.mom[1];
-    P.g[ASX_MOMZ][oc] = cv.mom[2];
-    P.g[ASX_TAU][oc] = cv.tau;
-    P.g[ASX_DYE][oc] = cv.DYe;
-    P.g[ASX_DENT][oc] = cv.DEnt;
-    P.g[ASX_DBX][oc] = cv.dB[0];
-    P.g[ASX_DBY][oc] = cv.dB[1];
-    P.g[ASX_DBZ][oc] = cv.dB[2];
-    P.g[ASX_SAVED_RHO][oc] = pv.rho;
-    P.g[ASX_SAVED_VELX][oc] = pv.vel[0];
-    P.g[ASX_SAVED_VELY][oc] = pv.vel[1];
-    P.g[ASX_SAVED_VELZ][oc] = pv.vel[2];
-    P.g[ASX_SAVED_EPS][oc] = pv.eps;
-    P.g[ASX_SAVED_YE][oc] = pv.Ye;
+    store_zone(P, off(P, 0, i, j, k), o);
     if (hist) {
       atomicAdd(&h.flags[(int)o.flag], 1u);
       atomicAdd(&h.iters[o.iters < 63 ? o.iters : 63], 1u);
@@ -144,6 +206,8 @@ __global__ __launch_bounds__(256) void k_con2prim(KPatch P, Box B, IdealGas eos,
     hist_flush(h, hist);
   }
 }
+
+__global__ void k_zero_u32(unsigned *p) { *p = 0u; }
 
 // Independent zones (layout of asx_c2p_batch in include/asterx.h).
 __global__ __launch_bounds__(256) void k_c2p_batch(int64_t nz, const double *cons,
@@ -189,17 +253,49 @@ __global__ __launch_bounds__(256) void k_c2p_batch(int64_t nz, const double *con
   iters[z] = o.iters;
 }
 
-hipError_t launch_con2prim(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
-                           bool from_A, unsigned long long *hist, hipStream_t s) {
+static Box c2p_box(const KPatch &P) {
   Box b;
   for (int d = 0; d < 3; ++d) {
     b.lo[d] = P.ng;
     b.cnt[d] = P.n[d];
   }
+  return b;
+}
+
+hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, int *queue,
+                                unsigned *qcount, hipStream_t s) {
+  const Box b = c2p_box(P);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, qcount);
+  if (b.htotal() <= 0) return hipGetLastError();
+  const dim3 grid = b.grid(), block(256);
+  const bool noble = a.prime == ASX_C2P_NOBLE;
+  if (from_A) {
+    if (noble)
+      hipLaunchKernelGGL((k_c2p_main<true, ASX_C2P_NOBLE>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
+    else
+      hipLaunchKernelGGL((k_c2p_main<true, ASX_C2P_PALENZUELA>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
+  } else {
+    if (noble)
+      hipLaunchKernelGGL((k_c2p_main<false, ASX_C2P_NOBLE>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
+    else
+      hipLaunchKernelGGL((k_c2p_main<false, ASX_C2P_PALENZUELA>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, const int *queue,
+                                const unsigned *qcount, hipStream_t s) {
+  const Box b = c2p_box(P);
+  const int64_t nz = b.htotal();
+  if (nz <= 0) return hipSuccess;
+  // enough work-items for a few percent of the zones; more stride
+  const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
   if (from_A)
-    hipLaunchKernelGGL(k_con2prim<true>, b.grid(), dim3(256), 0, s, P, b, eos, a, hist);
+    hipLaunchKernelGGL(k_c2p_rest<true>, dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
   else
-    hipLaunchKernelGGL(k_con2prim<false>, b.grid(), dim3(256), 0, s, P, b, eos, a, hist);
+    hipLaunchKernelGGL(k_c2p_rest<false>, dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
   return hipGetLastError();
 }
 

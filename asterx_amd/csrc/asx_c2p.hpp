@@ -667,8 +667,15 @@ struct C2POut {
   int iters;
 };
 
-// AsterX/src/con2prim.cxx:81-416
-ASX_DEV void con2prim_cell(const IdealGas &eos, const C2PArgs &a, const C2PIn &in,
+// AsterX/src/con2prim.cxx:81-416.
+// PRIME < 0: the full chain, primary and backup solver chosen at run time
+// from a.prime / a.second (the reference's behaviour); returns true.
+// PRIME = ASX_C2P_NOBLE or ASX_C2P_PALENZUELA: only the primary solver,
+// compiled for that method (a.prime must equal it); returns false, with
+// `out` not set, when it fails: the zone then needs the full chain, which
+// repeats the same primary solve before the backup.
+template <int PRIME>
+ASX_DEV bool con2prim_zone(const IdealGas &eos, const C2PArgs &a, const C2PIn &in,
                            C2POut &out, double zguess = -1.0, double vguess = -1.0) {
   const Atmo atmo = make_atmo(eos, a, in.r);
   const Sym3 &glo = in.glo;
@@ -725,10 +732,12 @@ ASX_DEV void con2prim_cell(const IdealGas &eos, const C2PArgs &a, const C2PIn &i
   out.iters = 0;
   if (call_c2p) {
     code = ASX_C2P_FLAG_PRIME;
-    if (a.prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
+    const int prime = PRIME < 0 ? a.prime : PRIME;
+    if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
     else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1);
     out.iters = rep1.iters;
-    if (rep1.failed()) {
+    if (PRIME >= 0 && rep1.failed()) return false;
+    if (PRIME < 0 && rep1.failed()) {
       code = ASX_C2P_FLAG_SECOND;
       if (a.second == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep2);
       else if (a.second == ASX_C2P_PALENZUELA) palenzuela_solve(eos, atmo, a, pv, cv, glo, rep2);
@@ -750,6 +759,11 @@ ASX_DEV void con2prim_cell(const IdealGas &eos, const C2PArgs &a, const C2PIn &i
   out.flag = code;
   out.pv = pv;
   out.cv = cv;
+  return true;
+}
+ASX_DEV void con2prim_cell(const IdealGas &eos, const C2PArgs &a, const C2PIn &in,
+                           C2POut &out, double zguess = -1.0, double vguess = -1.0) {
+  (void)con2prim_zone<-1>(eos, a, in, out, zguess, vguess);
 }
 
 } // namespace asx

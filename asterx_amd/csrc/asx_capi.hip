@@ -1,5 +1,6 @@
 // C ABI of the substep library (include/asterx.h): argument validation,
 // parameter resolution and kernel launches.  Host code only.
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -245,6 +246,71 @@ std::mutex g_mu;
 std::map<int, unsigned long long *> g_cnt;
 std::map<std::tuple<int, void *, int, int, int, int>, Scratch> g_scratch;
 
+// ---------------------------------------------------------------------------
+// Kernel timing (diagnostics): HIP events around launches, on the caller's
+// stream, per kernel class; events come from a per-device pool.
+// ---------------------------------------------------------------------------
+struct TimedLaunch {
+  int dev, cls;
+  hipEvent_t a, b;
+};
+std::atomic<bool> g_timing{false};
+std::mutex g_tmu;
+std::map<int, std::vector<hipEvent_t>> g_pool;
+std::vector<TimedLaunch> g_pending;
+double g_tms[ASX_KC_COUNT];
+int64_t g_tn[ASX_KC_COUNT];
+
+hipEvent_t take_event(int dev) {
+  std::vector<hipEvent_t> &v = g_pool[dev];
+  if (!v.empty()) {
+    hipEvent_t e = v.back();
+    v.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Drain finished (blocking) launches into the accumulators.
+asx_status drain_timing() {
+  for (const TimedLaunch &t : g_pending) {
+    if (hipEventSynchronize(t.b) != hipSuccess) return ASX_ERR_DEVICE;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.a, t.b) != hipSuccess) return ASX_ERR_DEVICE;
+    g_tms[t.cls] += ms;
+    g_tn[t.cls] += 1;
+    g_pool[t.dev].push_back(t.a);
+    g_pool[t.dev].push_back(t.b);
+  }
+  g_pending.clear();
+  return ASX_OK;
+}
+
+struct KTimer {
+  int dev = 0, cls;
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  KTimer(int c, hipStream_t st) : cls(c), s(st) {
+    if (!g_timing.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> l(g_tmu);
+    (void)hipGetDevice(&dev);
+    a = take_event(dev);
+    b = take_event(dev);
+    if (!a || !b || hipEventRecord(a, s) != hipSuccess) a = nullptr;
+  }
+  ~KTimer() {
+    if (!a) return;
+    std::lock_guard<std::mutex> l(g_tmu);
+    if (hipEventRecord(b, s) == hipSuccess) g_pending.push_back({dev, cls, a, b});
+  }
+};
+template <class F> hipError_t timed(int cls, hipStream_t s, F &&f) {
+  KTimer t(cls, s);
+  return f();
+}
+
 unsigned long long *counter_buf(int dev) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cnt.find(dev);
@@ -253,6 +319,34 @@ unsigned long long *counter_buf(int dev) {
   if (hipMalloc(&p, 80 * sizeof(unsigned long long)) != hipSuccess) return nullptr;
   g_cnt[dev] = p;
   return p;
+}
+
+// Queue of zones the primary con2prim solver failed, per (device, stream):
+// one int per interior zone and a count, grown on demand.
+struct C2PQueue {
+  int *q = nullptr;
+  unsigned *n = nullptr;
+  int64_t cap = 0;
+};
+std::map<std::pair<int, void *>, C2PQueue> g_c2pq;
+
+C2PQueue *c2p_queue(void *stream, int64_t nz) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  C2PQueue &Q = g_c2pq[{dev, stream}];
+  if (!Q.n && hipMalloc(&Q.n, sizeof(unsigned)) != hipSuccess) return nullptr;
+  if (Q.cap < nz) {
+    // the previous buffer may still be read by work queued on this stream
+    if (Q.q && (hipStreamSynchronize((hipStream_t)stream) != hipSuccess ||
+                hipFree(Q.q) != hipSuccess))
+      return nullptr;
+    Q.q = nullptr;
+    Q.cap = 0;
+    if (hipMalloc(&Q.q, (size_t)nz * sizeof(int)) != hipSuccess) return nullptr;
+    Q.cap = nz;
+  }
+  return &Q;
 }
 
 __global__ void k_zero_u64(unsigned long long *p, int n) {
@@ -419,12 +513,12 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
-    if ((st = dev_status(launch_flux(d, P, A, par->reconstruction_method, s)))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, s); })))) return st;
   }
   if (aux) {
     const int h = 2 * face_halo;
     const int halo[3] = {h, h, h};
-    if ((st = dev_status(launch_aux(P, box(pt, 7, halo), s)))) return st;
+    if ((st = dev_status(timed(ASX_KC_AUX, s, [&] { return launch_aux(P, box(pt, 7, halo), s); })))) return st;
   }
   if (cnt) {
     unsigned long long hbuf[2];
@@ -463,8 +557,8 @@ asx_status asx_source_terms(const asx_patch *pt, const asx_params *par, void *st
     return st;
   if ((st = need_range(pt, ASX_DENSRHS, ASX_DYE_RHS))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(launch_sources(kpatch(pt), box(pt, 0, halo), rhs_args(par),
-                                   (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_sources(kpatch(pt), box(pt, 0, halo), rhs_args(par),
+                                   (hipStream_t)stream); }));
 }
 
 asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {
@@ -485,7 +579,7 @@ asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {
     if (pt->ng < recon_ghosts(par->reconstruction_method)) return ASX_ERR_GHOSTS;
   }
   if (pt->ng < 2) return ASX_ERR_GHOSTS;
-  return dev_status(launch_rhs(kpatch(pt), rhs_args(par), (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_rhs(kpatch(pt), rhs_args(par), (hipStream_t)stream); }));
 }
 
 asx_status asx_bstag_from_A(const asx_patch *pt, void *stream) {
@@ -497,8 +591,8 @@ asx_status asx_bstag_from_A(const asx_patch *pt, void *stream) {
   const KPatch P = kpatch(pt);
   const int halo[3] = {0, 0, 0};
   for (int d = 0; d < 3; ++d)
-    if ((st = dev_status(launch_bstag(d, P, box(pt, 1 << d, halo), pt->gf[ASX_DBX_STAG + d].data,
-                                      (hipStream_t)stream))))
+    if ((st = dev_status(timed(ASX_KC_BSTAG, (hipStream_t)stream, [&] { return launch_bstag(d, P, box(pt, 1 << d, halo), pt->gf[ASX_DBX_STAG + d].data,
+                                      (hipStream_t)stream); }))))
       return st;
   return ASX_OK;
 }
@@ -509,7 +603,7 @@ asx_status asx_dB_from_bstag(const asx_patch *pt, void *stream) {
   if ((st = need(pt, {ASX_DBX_STAG, ASX_DBY_STAG, ASX_DBZ_STAG, ASX_DBX, ASX_DBY, ASX_DBZ})))
     return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(launch_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
 }
 
 asx_status asx_B_from_dB(const asx_patch *pt, void *stream) {
@@ -518,7 +612,7 @@ asx_status asx_B_from_dB(const asx_patch *pt, void *stream) {
   if ((st = need_range(pt, ASX_GXX, ASX_GZZ))) return st;
   if ((st = need(pt, {ASX_DBX, ASX_DBY, ASX_DBZ, ASX_BVECX, ASX_BVECY, ASX_BVECZ}))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(launch_B_from_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_B_from_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
 }
 
 asx_status asx_prim2con(const asx_patch *pt, void *stream) {
@@ -530,7 +624,7 @@ asx_status asx_prim2con(const asx_patch *pt, void *stream) {
     return st;
   if ((st = need_range(pt, ASX_DENS, ASX_DBZ))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(launch_prim2con(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_prim2con(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
 }
 
 static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool from_A,
@@ -551,8 +645,8 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
     if ((st = need(pt, {ASX_DBY_STAG, ASX_DBZ_STAG}))) return st;
     const int halo[3] = {0, 0, 0};
     for (int d = 0; d < 3; ++d)
-      if ((st = dev_status(launch_bstag(d, P, box(pt, 1 << d, halo),
-                                        pt->gf[ASX_DBX_STAG + d].data, s))))
+      if ((st = dev_status(timed(ASX_KC_BSTAG, s, [&] { return launch_bstag(d, P, box(pt, 1 << d, halo),
+                                        pt->gf[ASX_DBX_STAG + d].data, s); }))))
         return st;
   }
   unsigned long long *cd = nullptr;
@@ -563,8 +657,18 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
     if (!cd) return ASX_ERR_DEVICE;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, cd, 80);
   }
-  if ((st = dev_status(launch_con2prim(P, ideal_gas(par), c2p_args(par), from_A,
-                                       cd ? cd + 2 : nullptr, s))))
+  C2PQueue *Q = c2p_queue(stream, (int64_t)pt->n[0] * pt->n[1] * pt->n[2]);
+  if (!Q) return ASX_ERR_DEVICE;
+  const IdealGas eos = ideal_gas(par);
+  const C2PArgs ca = c2p_args(par);
+  unsigned long long *hist = cd ? cd + 2 : nullptr;
+  if ((st = dev_status(timed(ASX_KC_C2P, s, [&] {
+         return launch_con2prim_main(P, eos, ca, from_A, hist, Q->q, Q->n, s);
+       }))))
+    return st;
+  if ((st = dev_status(timed(ASX_KC_C2P_REST, s, [&] {
+         return launch_con2prim_rest(P, eos, ca, from_A, hist, Q->q, Q->n, s);
+       }))))
     return st;
   if (cnt) {
     unsigned long long h[71];
@@ -605,8 +709,8 @@ asx_status asx_state_lincomb(const asx_patch *pt, const asx_state *dst, double s
       if (src[k]->gf[q].sj != sj || src[k]->gf[q].sk != sk) return ASX_ERR_LAYOUT;
       srcp[k] = src[k]->gf[q].data;
     }
-    if ((st = dev_status(launch_lincomb(dst->gf[q].data, npoints(pt->n, pt->ng, c), scale,
-                                        nsrc, fac, srcp, (hipStream_t)stream))))
+    if ((st = dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_lincomb(dst->gf[q].data, npoints(pt->n, pt->ng, c), scale,
+                                        nsrc, fac, srcp, (hipStream_t)stream); }))))
       return st;
   }
   return ASX_OK;
@@ -623,7 +727,53 @@ asx_status asx_bc_fill(const asx_patch *pt, int32_t slot, int32_t kind, void *st
     if (kind == ASX_BC_LINEAR && pt->n[d] + c < 2) return ASX_ERR_SIZE;
     if (kind == ASX_BC_PERIODIC && pt->n[d] < pt->ng) return ASX_ERR_SIZE;
   }
-  return dev_status(launch_bc(kpatch(pt), slot, kind, (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_BC, (hipStream_t)stream, [&] { return launch_bc(kpatch(pt), slot, kind, (hipStream_t)stream); }));
+}
+
+asx_status asx_timing_enable(int32_t on) {
+  std::lock_guard<std::mutex> l(g_tmu);
+  asx_status st = drain_timing();
+  for (int q = 0; q < ASX_KC_COUNT; ++q) {
+    g_tms[q] = 0;
+    g_tn[q] = 0;
+  }
+  g_timing.store(on != 0);
+  return st;
+}
+
+asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms) {
+  if (kclass < 0 || kclass >= ASX_KC_COUNT) return ASX_ERR_PARAM;
+  if (!launches || !ms) return ASX_ERR_NULL;
+  std::lock_guard<std::mutex> l(g_tmu);
+  asx_status st = drain_timing();
+  *launches = g_tn[kclass];
+  *ms = g_tms[kclass];
+  return st;
+}
+
+asx_status asx_bc_fill_many(const asx_patch *pt, int32_t nslot, const int32_t *slots,
+                            int32_t kind, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (nslot < 0 || nslot > kMaxBC) return ASX_ERR_PARAM;
+  if (nslot > 0 && !slots) return ASX_ERR_NULL;
+  if (kind < ASX_BC_LINEAR || kind > ASX_BC_COPY) return ASX_ERR_PARAM;
+  BCList L;
+  L.nf = nslot;
+  for (int q = 0; q < nslot; ++q) {
+    const int slot = slots[q];
+    if (slot < 0 || slot >= ASX_NUM_GF) return ASX_ERR_PARAM;
+    if ((st = need(pt, {slot}))) return st;
+    const int c = centering(slot);
+    for (int d = 0; d < 3; ++d) {
+      const int cd = (c >> d) & 1;
+      if (kind == ASX_BC_LINEAR && pt->n[d] + cd < 2) return ASX_ERR_SIZE;
+      if (kind == ASX_BC_PERIODIC && pt->n[d] < pt->ng) return ASX_ERR_SIZE;
+    }
+    L.g[q] = pt->gf[slot].data;
+    L.cent[q] = c;
+  }
+  return dev_status(timed(ASX_KC_BC, (hipStream_t)stream, [&] { return launch_bc_many(kpatch(pt), L, kind, (hipStream_t)stream); }));
 }
 
 asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_state *out,
@@ -673,7 +823,7 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
       b.lo[q] = 0;
       b.cnt[q] = (q == d) ? stage->n[q] + 2 * stage->ng + 1 : stage->n[q] + 2 * stage->ng;
     }
-    if ((st = dev_status(launch_bstag(d, P, b, P.g[ASX_DBX_STAG + d], s)))) return st;
+    if ((st = dev_status(timed(ASX_KC_BSTAG, s, [&] { return launch_bstag(d, P, b, P.g[ASX_DBX_STAG + d], s); })))) return st;
   }
   FluxArgs A;
   A.rp = rparams(par);
@@ -692,18 +842,18 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
-    if ((st = dev_status(launch_flux(d, P, A, par->reconstruction_method, s)))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, s); })))) return st;
   }
   {
     const int halo[3] = {2, 2, 2};
-    if ((st = dev_status(launch_aux(P, box(stage, 7, halo), s)))) return st;
+    if ((st = dev_status(timed(ASX_KC_AUX, s, [&] { return launch_aux(P, box(stage, 7, halo), s); })))) return st;
   }
   StatePtrs pu0, pout;
   for (int q = 0; q < ASX_NUM_STATE; ++q) {
     pu0.g[q] = (c0 != 0.0) ? u0->gf[q].data : nullptr;
     pout.g[q] = out->gf[q].data;
   }
-  return dev_status(launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, s));
+  return dev_status(timed(ASX_KC_UPDATE, s, [&] { return launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, s); }));
 }
 
 asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
@@ -713,8 +863,8 @@ asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
   if (st) return st;
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !seed || !prim || !flag || !iters)) return ASX_ERR_NULL;
-  return dev_status(launch_c2p_batch(nz, cons, g, seed, prim, flag, iters, ideal_gas(par),
-                                     c2p_args(par), (hipStream_t)stream));
+  return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] { return launch_c2p_batch(nz, cons, g, seed, prim, flag, iters, ideal_gas(par),
+                                     c2p_args(par), (hipStream_t)stream); }));
 }
 
 } // extern "C"

@@ -74,13 +74,27 @@ hipError_t launch_rhs(const KPatch &P, const RhsArgs &A, hipStream_t s);
 hipError_t launch_lincomb(double *dst, int64_t np, double scale, int nsrc,
                           const double *fac, double *const *src, hipStream_t s);
 hipError_t launch_bc(const KPatch &P, int slot, int kind, hipStream_t s);
+// Grid functions filled by one multi-field boundary launch.
+constexpr int kMaxBC = 64;
+struct BCList {
+  double *g[kMaxBC];
+  int cent[kMaxBC];
+  int nf;
+};
+hipError_t launch_bc_many(const KPatch &P, const BCList &L, int kind, hipStream_t s);
 hipError_t launch_update(const KPatch &P, const RhsArgs &A, const StatePtrs &u0,
                          const StatePtrs &out, double c0, double c1, double c2,
                          hipStream_t s);
 
-hipError_t launch_con2prim(const KPatch &P, const IdealGas &eos,
-                           const C2PArgs &a, bool with_bstag,
-                           unsigned long long *hist, hipStream_t s);
+// con2prim of the patch interior in two passes: every zone through the
+// primary solver, the zones it failed queued (queue: one int per interior
+// zone, qcount: one unsigned) and run through the full chain.
+hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, int *queue,
+                                unsigned *qcount, hipStream_t s);
+hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, const int *queue,
+                                const unsigned *qcount, hipStream_t s);
 hipError_t launch_c2p_batch(int64_t nz, const double *cons, const double *g,
                             const double *seed, double *prim, double *flag,
                             int32_t *iters, const IdealGas &eos,

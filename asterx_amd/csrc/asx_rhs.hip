@@ -467,6 +467,45 @@ __global__ __launch_bounds__(256) void k_bc(double *g, int ex, int ey, int ez, i
   }
 }
 
+// The same fill for up to kMaxBC grid functions of any centering in one
+// launch per axis (blockIdx.y = grid function).
+template <int D>
+__global__ __launch_bounds__(256) void k_bc_many(BCList L, int n0, int n1, int n2, int ng,
+                                                 int kind) {
+  constexpr int A = (D + 1) % 3, Bx = (D + 2) % 3;
+  const int f = blockIdx.y;
+  const int c = L.cent[f];
+  const int n[3] = {n0, n1, n2};
+  const int e[3] = {n0 + 2 * ng + (c & 1), n1 + 2 * ng + ((c >> 1) & 1),
+                    n2 + 2 * ng + ((c >> 2) & 1)};
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int na = e[A], nb = e[Bx];
+  if (t >= na * nb) return;
+  const int ia = t % na, ib = t / na;
+  const int64_t ex = e[0], exy = (int64_t)e[0] * e[1];
+  const int64_t sd = D == 0 ? 1 : (D == 1 ? ex : exy);
+  const int64_t sa = A == 0 ? 1 : (A == 1 ? ex : exy);
+  const int64_t sb = Bx == 0 ? 1 : (Bx == 1 ? ex : exy);
+  double *l = L.g[f] + ia * sa + ib * sb;
+  const int lo = ng, hi = ng + n[D] - 1 + ((c >> D) & 1);
+  for (int q = lo - 1; q >= 0; --q) {
+    if (kind == ASX_BC_LINEAR)
+      l[q * sd] = l[lo * sd] + (lo - q) * (l[lo * sd] - l[(lo + 1) * sd]);
+    else if (kind == ASX_BC_PERIODIC)
+      l[q * sd] = l[(q + n[D]) * sd];
+    else
+      l[q * sd] = l[lo * sd];
+  }
+  for (int q = hi + 1; q < e[D]; ++q) {
+    if (kind == ASX_BC_LINEAR)
+      l[q * sd] = l[hi * sd] + (q - hi) * (l[hi * sd] - l[(hi - 1) * sd]);
+    else if (kind == ASX_BC_PERIODIC)
+      l[q * sd] = l[(q - n[D]) * sd];
+    else
+      l[q * sd] = l[hi * sd];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused update of the substep: out = c0*u0 + c1*u + c2*rhs on interior
 // points, rhs = sources + flux divergence (cells), CT EMF + gauge (edges),
@@ -598,6 +637,21 @@ hipError_t launch_bc(const KPatch &P, int slot, int kind, hipStream_t s) {
     if (d == 1) hipLaunchKernelGGL(k_bc<1>, grid, dim3(256), 0, s, P.g[slot], e[0], e[1], e[2], lo, hi, P.n[1], kind);
     if (d == 2) hipLaunchKernelGGL(k_bc<2>, grid, dim3(256), 0, s, P.g[slot], e[0], e[1], e[2], lo, hi, P.n[2], kind);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_many(const KPatch &P, const BCList &L, int kind, hipStream_t s) {
+  if (L.nf <= 0) return hipSuccess;
+  int lines = 0; // largest face plane over the listed centerings
+  for (int d = 0; d < 3; ++d) {
+    const int a = (d + 1) % 3, b = (d + 2) % 3;
+    const int m = (P.n[a] + 2 * P.ng + 1) * (P.n[b] + 2 * P.ng + 1);
+    lines = m > lines ? m : lines;
+  }
+  const dim3 grid((unsigned)((lines + 255) / 256), (unsigned)L.nf);
+  hipLaunchKernelGGL(k_bc_many<0>, grid, dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
+  hipLaunchKernelGGL(k_bc_many<1>, grid, dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
+  hipLaunchKernelGGL(k_bc_many<2>, grid, dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
   return hipGetLastError();
 }
 

@@ -49,9 +49,8 @@ class Evolution:
 
     def sync(self, names):
         ps = self.p.struct()
-        for name in names:
-            if name in self.p:
-                self.b.bc_fill(ps, abi.SLOT[name], self.bc)
+        self.b.bc_fill_many(ps, [abi.SLOT[n] for n in names if n in self.p],
+                            self.bc)
 
     # AsterX_InitialGroup (check_prims is given: it is initial-data only)
     def initial(self, check_prims):
@@ -185,6 +184,18 @@ class Backend:
 
     def bc_fill(self, ps, slot, kind):
         self._call("bc_fill", ps_ref(ps), slot, kind, *self._s())
+
+    def bc_fill_many(self, ps, slots, kind):
+        if self.prefix != "asx_":  # the oracle fills one grid function at a time
+            for s in slots:
+                self.bc_fill(ps, s, kind)
+            return
+        import ctypes as C
+        for i in range(0, len(slots), 64):
+            chunk = slots[i:i + 64]
+            arr = (C.c_int32 * len(chunk))(*chunk)
+            self._call("bc_fill_many", ps_ref(ps), len(chunk), arr, kind,
+                       *self._s())
 
 
 def ref(x):

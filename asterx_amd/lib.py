@@ -31,6 +31,10 @@ SIGNATURES = {
     "asx_state_lincomb": ([_P, _S, C.c_double, C.c_int32, C.POINTER(C.c_double),
                            C.POINTER(_S), _V], C.c_int),
     "asx_bc_fill": ([_P, C.c_int32, C.c_int32, _V], C.c_int),
+    "asx_bc_fill_many": ([_P, C.c_int32, C.POINTER(C.c_int32), C.c_int32, _V], C.c_int),
+    "asx_timing_enable": ([C.c_int32], C.c_int),
+    "asx_timing_read": ([C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
+                        C.c_int),
     "asx_substep": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, _V],
                     C.c_int),
     "asx_c2p_batch": ([C.c_int64, _V, _V, _V, _V, _V, _V, _PP, _V], C.c_int),

@@ -9,14 +9,15 @@ Palenzuela backup, floors rho 1e-8 / p 1e-10.
 
 One "step" is one RK substep of the reference's schedule on the whole patch:
   RHS (fluxes, sources, A/Psi RHS) and the RK combination  (asx_substep)
-  SYNC of the state                                         (asx_bc_fill)
+  SYNC of the state                                         (asx_bc_fill_many)
   B from A + con2prim                                       (asx_con2prim_group)
-  SYNC of the primitives the next RHS reads                 (asx_bc_fill)
+  SYNC of the primitives the next RHS reads                 (asx_bc_fill_many)
 cycling the three SSPRK3 stages (stage 1 reads no previous-stage state).
 
 Prints one JSON line: zone-updates/s (interior zones x substeps / s), the
-roofline of the dominant kernel timed with HIP events on the library's
-stream, and the CPU oracle timed on the same inputs on the host cores.
+roofline of the dominant kernel class (the library brackets each launch
+with HIP events on the caller's stream, asx_timing_*), and the CPU oracle
+timed on the same inputs on the host cores.
 """
 import argparse
 import ctypes as C
@@ -90,16 +91,18 @@ class Stepper:
         self.stage = 0
         self.L = backend.lib
         self.bc = abi.BC["copy"]
-        self.slots_state = [abi.SLOT[n] for n in abi.STATE]
-        self.slots_prims = [abi.SLOT[n] for n in SYNC_PRIMS]
+        st = [abi.SLOT[n] for n in abi.STATE]
+        pr = [abi.SLOT[n] for n in SYNC_PRIMS]
+        self.slots_state = (C.c_int32 * len(st))(*st)
+        self.slots_prims = (C.c_int32 * len(pr))(*pr)
 
     def _bc(self, ps, slots):
-        for s in slots:
-            r = self.L.asx_bc_fill(C.byref(ps), s, self.bc, self.stream)
-            if r:
-                raise RuntimeError("asx_bc_fill: %d" % r)
+        r = self.L.asx_bc_fill_many(C.byref(ps), len(slots), slots, self.bc,
+                                    self.stream)
+        if r:
+            raise RuntimeError("asx_bc_fill_many: %d" % r)
 
-    def substep(self, ev=None):
+    def substep(self):
         from asterx_amd import abi
         c0, c1, cf = SSPRK3[self.stage]
         if self.stage == 0:
@@ -109,31 +112,43 @@ class Stepper:
         ps = p.struct()
         u0s = state_struct(p, self.bufs[self.base])
         outs = state_struct(p, self.bufs[out])
-        if ev:
-            ev[0].record(ev[5])
         r = self.L.asx_substep(C.byref(ps), C.byref(u0s), C.byref(outs), c0, c1,
                                cf * self.dt, C.byref(self.par), self.stream)
         if r:
             raise RuntimeError("asx_substep: %d" % r)
-        if ev:
-            ev[1].record(ev[5])
         for n in abi.STATE:
             p.arr[n] = self.bufs[out][n]
         self.cur = out
         ps = p.struct()
         self._bc(ps, self.slots_state)
-        if ev:
-            ev[2].record(ev[5])
         r = self.L.asx_con2prim_group(C.byref(ps), C.byref(self.par), None,
                                       self.stream)
         if r:
             raise RuntimeError("asx_con2prim_group: %d" % r)
-        if ev:
-            ev[3].record(ev[5])
         self._bc(ps, self.slots_prims)
-        if ev:
-            ev[4].record(ev[5])
         self.stage = (self.stage + 1) % 3
+
+
+def face_bytes(par):
+    """Algorithmic HBM bytes per face of one face-flux launch: the 9
+    reconstructed cell fields (rho, s, Ye, p or T, two transverse B, the
+    velocity-like vector) and the normal staggered B read once, the 10
+    vertex metric fields (alpha, beta^i, gamma_ij) read once, the 10 face
+    fluxes written (4 more upwind-CT fields with use_uct)."""
+    return 8 * (9 + 1 + 10 + 10 + (4 if par.use_uct else 0))
+
+
+def update_bytes(par, first_stage):
+    """Algorithmic bytes per interior zone of the fused update: 21 hydro
+    face fluxes (7 variables x 3 directions), 6 induction fluxes for the
+    EMF, 7 vertex auxiliaries, 16 vertex ADM fields, 9 cell primitives
+    (rho, eps, p, v^i, B^i; 3 more with z/s reconstruction vectors), the 11
+    state fields and the 11 previous-stage fields (not in stage 1) read
+    once; 11 new state fields written."""
+    reads = 21 + 6 + 7 + 16 + 9 + 11 + (0 if first_stage else 11)
+    if par.recon_type in (1, 2):
+        reads += 3
+    return 8 * (reads + 11)
 
 
 def stencil_bytes_per_zone(par, first_stage):
@@ -255,54 +270,71 @@ def main():
         st.substep()
     torch.cuda.synchronize()
     K = args.steps
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] + [stream]
-           for _ in range(K)]
+    L = be.lib
+    # per-kernel HIP events on the library's stream, over the timed region
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    first_stages = sum(1 for k in range(K) if (k + args.warmup) % 3 == 0)
     t0 = time.perf_counter()
     for k in range(K):
-        st.substep(evs[k])
+        st.substep()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_step = 1e3 * wall / K
-    ph = np.array([[e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2]),
-                    e[2].elapsed_time(e[3]), e[3].elapsed_time(e[4])]
-                   for e in evs])
-    stages = [(k + args.warmup) % 3 for k in range(K)]
-    log(args, "phases ms (stencil, bc_state, c2p, bc_prims):",
-        ph.mean(0).round(4))
+    kt = {}
+    for name, kc in abi.KCLASS.items():
+        nl, ms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(kc, C.byref(nl), C.byref(ms)))
+        if nl.value:
+            kt[name] = (nl.value, ms.value)
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    log(args, "kernel classes (launches, total ms, avg ms):")
+    for name, (nl, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
+        log(args, "  %-8s %4d %9.3f %8.4f" % (name, nl, ms, ms / nl))
     # flags histogram of one more substep, outside the timed region
     cnt = abi.asx_counters()
     ps = p.struct()
-    abi.check("asx_con2prim_group", be.lib.asx_con2prim_group(
+    abi.check("asx_con2prim_group", L.asx_con2prim_group(
         C.byref(ps), C.byref(par), C.byref(cnt), sh))
     torch.cuda.synchronize()
     log(args, "c2p flags:", list(cnt.c2p_flags))
+    log(args, "c2p iterations:", {i: v for i, v in enumerate(cnt.c2p_iters) if v})
     for name in abi.CONS:
         a = p.arr[name][p.interior(name)]
         if not bool(torch.isfinite(a).all()):
             raise RuntimeError("non-finite %s after the timed substeps" % name)
 
-    # roofline of the dominant kernel (the phase with the largest average
-    # duration; both phases below are single kernels or documented groups)
-    c2p_ms = float(ph[:, 2].mean())
-    sten_ms = float(ph[:, 0].mean())
-    first = np.array([s == 0 for s in stages])
-    b_sten = np.array([stencil_bytes_per_zone(par, f) for f in first])
-    sten_gbs = float((b_sten * zones / (ph[:, 0] * 1e-3)).mean() / 1e9)
-    b_c2p = c2p_bytes_per_zone(True)
-    c2p_gbs = b_c2p * zones / (c2p_ms * 1e-3) / 1e9
+    # roofline of the dominant kernel class (largest summed time): its
+    # algorithmic bytes per launch over its average launch duration
+    nf = [(n[d] + 1) * (n[(d + 1) % 3] + 2) * (n[(d + 2) % 3] + 2) for d in range(3)]
+    upd_b = (first_stages * update_bytes(par, True) +
+             (K - first_stages) * update_bytes(par, False)) / K
+    per_launch = {"face_x": face_bytes(par) * nf[0],
+                  "face_y": face_bytes(par) * nf[1],
+                  "face_z": face_bytes(par) * nf[2],
+                  "update": upd_b * zones,
+                  "c2p": c2p_bytes_per_zone(True) * zones}
     workload = "torus%d_%s_%s" % (args.n, args.recon, args.flux)
-    if c2p_ms >= sten_ms:
-        kern, achieved, dur = "k_con2prim", c2p_gbs, c2p_ms
-        traffic = read_traffic("k_con2prim", workload)
-    else:
-        kern, achieved, dur = "substep", sten_gbs, sten_ms
-        traffic = read_traffic("substep", workload)
-    roof = {"bound": "hbm", "kernel": kern, "achieved": round(achieved, 2),
+    dom = max((k for k in kt if k in per_launch), key=lambda k: kt[k][1])
+    nl, ms = kt[dom]
+    avg = ms / nl
+    achieved = per_launch[dom] / (avg * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "avg_ms": round(dur, 4),
-            "stencil_ms": round(sten_ms, 4), "stencil_GBs": round(sten_gbs, 2),
-            "c2p_ms": round(c2p_ms, 4), "c2p_GBs": round(c2p_gbs, 2)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(dom, workload),
+            "avg_ms": round(avg, 4),
+            "bytes_per_launch": int(per_launch[dom]),
+            "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()}}
+    # the whole stencil half of the substep against the per-zone figure
+    # (480 B/zone, 392 B/zone in stage 1: each field of the substep read or
+    # written once), over the summed time of its kernels
+    sten_ms = sum(kt[k][1] for k in ("bstag", "face_x", "face_y", "face_z", "aux",
+                                     "update") if k in kt) / K
+    sten_b = (first_stages * stencil_bytes_per_zone(par, True) +
+              (K - first_stages) * stencil_bytes_per_zone(par, False)) / K
+    roof["substep_stencil_ms"] = round(sten_ms, 4)
+    roof["substep_stencil_frac"] = round(sten_b * zones / (sten_ms * 1e-3) / 1e9 /
+                                         HBM_PEAK_GBS, 4)
 
     cpu = None
     if host is not None:

@@ -13,8 +13,11 @@
  * (names in the comment of each declaration, file:line in the reference
  * tree), take plain pointers and sizes, return an asx_status and never abort.
  * Numerical failures are data (con2prim flags, counters), as in the reference.
- * All work is stream-ordered on the given HIP stream (NULL = default stream);
- * entry points are re-entrant per stream and hold no mutable global state.
+ * All work is stream-ordered on the given HIP stream (NULL = default stream).
+ * Device buffers the library keeps (face data of asx_substep, the queue of
+ * asx_con2prim*) are held per device and stream, so calls on different
+ * streams do not share them; the counters buffer is per device, so calls that
+ * pass asx_counters must not overlap on one device.
  */
 #ifndef ASTERX_H
 #define ASTERX_H
@@ -246,6 +249,29 @@ void asx_gf_strides(const int32_t n[3], int32_t ng, int slot, int64_t *sj,
 /* Library version string. */
 const char *asx_version(void);
 
+/* Kernel timing (diagnostics; no counterpart in the reference).  While
+ * enabled, every kernel launch of the library is bracketed by HIP events on
+ * the caller's stream.  asx_timing_read returns, for one kernel class, the
+ * number of launches and their summed durations in milliseconds since the
+ * last asx_timing_enable call, waiting for launches still in flight.
+ * Enabling or disabling resets the sums. */
+enum {
+  ASX_KC_BSTAG = 0,  /* dB*_stag from A */
+  ASX_KC_FACE_X = 1, /* face reconstruction + numerical flux, x faces */
+  ASX_KC_FACE_Y = 2,
+  ASX_KC_FACE_Z = 3,
+  ASX_KC_AUX = 4,    /* vertex auxiliaries of the A/Psi gauge */
+  ASX_KC_UPDATE = 5, /* fused RHS + RK combination */
+  ASX_KC_C2P = 6,    /* con2prim: every zone, primary solver (patch or batch) */
+  ASX_KC_BC = 7,     /* boundary fills */
+  ASX_KC_RHS = 8,    /* unfused source terms / RHS */
+  ASX_KC_OTHER = 9,
+  ASX_KC_C2P_REST = 10, /* con2prim: zones the primary solver failed */
+  ASX_KC_COUNT = 11
+};
+asx_status asx_timing_enable(int32_t on);
+asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
+
 /* ------------------------------------------------------------------------ */
 /* Scheduled functions (drop-in for the reference's extern "C" entry points) */
 /* ------------------------------------------------------------------------ */
@@ -300,6 +326,12 @@ asx_status asx_state_lincomb(const asx_patch *patch, const asx_state *dst,
 enum { ASX_BC_LINEAR = 0, ASX_BC_PERIODIC = 1, ASX_BC_COPY = 2 };
 asx_status asx_bc_fill(const asx_patch *patch, int32_t slot, int32_t kind,
                        void *stream);
+/* The same fill for a list of grid functions (any centerings, at most 64) in
+ * one launch per axis: the physical-boundary half of a SYNC statement with
+ * several groups (e.g. AsterX/schedule.ccl:78-81).  Same arguments and errors
+ * as asx_bc_fill for each listed slot; ASX_ERR_PARAM for nslot > 64. */
+asx_status asx_bc_fill_many(const asx_patch *patch, int32_t nslot,
+                            const int32_t *slots, int32_t kind, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused substep (the performance path; no reference counterpart as one    */

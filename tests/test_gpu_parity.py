@@ -256,3 +256,54 @@ def test_c2p_batch():
     assert np.array_equal(flag_d.cpu().numpy(), flag_h)
     assert np.array_equal(it_d.cpu().numpy(), it_h)
     assert rel_err(prim_d.cpu().numpy(), prim_h) <= 1e-12
+
+
+@pytest.mark.parametrize("kind", ["linear", "periodic", "copy"])
+def test_bc_fill_many(kind):
+    """One multi-field launch per axis fills every listed grid function
+    exactly as the per-field fill and as the oracle do."""
+    _torch()
+    names = ["rho", "Avec_x", "Avec_y", "Avec_z", "Psi", "dBy_stag", "alp"]
+    h, d = _pair(curved=True)
+    g = _gpu()
+    d2 = h.to("cuda")
+    ps, ps2 = d.struct(), d2.struct()
+    slots = [abi.SLOT[n] for n in names]
+    g.bc_fill_many(ps, slots, abi.BC[kind])
+    for s in slots:
+        g.bc_fill(ps2, s, abi.BC[kind])
+    be = oracle.backend()
+    for s in slots:
+        be.bc_fill(h.struct(), s, abi.BC[kind])
+    _sync()
+    for n in names:
+        assert np.array_equal(d.numpy(n), d2.numpy(n)), n
+        assert np.array_equal(d.numpy(n), h.numpy(n)), n
+
+
+def test_timing_api():
+    """Kernel timing counts each launch of a fused substep once."""
+    _torch()
+    from asterx_amd.patch import StateBuf
+    h, d = _pair(curved=True)
+    L = _gpu().lib
+    par = abi.params(reconstruction_method="wenoz", flux_type="HLLE")
+    out = StateBuf(d)
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    ps, os_ = d.struct(), out.struct()
+    abi.check("asx_substep", L.asx_substep(C.byref(ps), None, C.byref(os_), 0.0, 1.0,
+                                           1e-3, C.byref(par), None))
+    counts = {}
+    for name, kc in abi.KCLASS.items():
+        nl, ms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(kc, C.byref(nl), C.byref(ms)))
+        counts[name] = (nl.value, ms.value)
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    assert counts["bstag"][0] == 3
+    for k in ("face_x", "face_y", "face_z", "aux", "update"):
+        assert counts[k][0] == 1 and counts[k][1] > 0.0, (k, counts[k])
+    assert counts["c2p"][0] == 0
+    nl, ms = C.c_int64(), C.c_double()
+    abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["face_x"], C.byref(nl),
+                                                   C.byref(ms)))
+    assert nl.value == 0  # disabling reset the sums
