@@ -1,5 +1,6 @@
 // C ABI of the substep library (include/asterx.h): argument validation,
 // parameter resolution and kernel launches.  Host code only.
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -311,6 +312,33 @@ template <class F> hipError_t timed(int cls, hipStream_t s, F &&f) {
   return f();
 }
 
+// Launch-shape variant per device and kernel class (default 0).
+std::map<int, std::array<int, ASX_KC_COUNT>> g_var;
+
+int variant_count(int kc) {
+  if (kc == ASX_KC_FACE_X || kc == ASX_KC_FACE_Y || kc == ASX_KC_FACE_Z) return kFaceVariants;
+  if (kc == ASX_KC_UPDATE) return kUpdateVariants;
+  return 1;
+}
+int variant_of(int kc) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_var.find(dev);
+  return it == g_var.end() ? 0 : it->second[kc];
+}
+void set_variant(int kc, int v) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_var.find(dev);
+  if (it == g_var.end()) {
+    std::array<int, ASX_KC_COUNT> z{};
+    it = g_var.emplace(dev, z).first;
+  }
+  it->second[kc] = v;
+}
+
 unsigned long long *counter_buf(int dev) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cnt.find(dev);
@@ -513,7 +541,7 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
-    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, s); })))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s); })))) return st;
   }
   if (aux) {
     const int h = 2 * face_halo;
@@ -842,7 +870,7 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
-    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, s); })))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s); })))) return st;
   }
   {
     const int halo[3] = {2, 2, 2};
@@ -853,7 +881,7 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
     pu0.g[q] = (c0 != 0.0) ? u0->gf[q].data : nullptr;
     pout.g[q] = out->gf[q].data;
   }
-  return dev_status(timed(ASX_KC_UPDATE, s, [&] { return launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, s); }));
+  return dev_status(timed(ASX_KC_UPDATE, s, [&] { return launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, variant_of(ASX_KC_UPDATE), s); }));
 }
 
 asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
@@ -865,6 +893,85 @@ asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
   if (nz > 0 && (!cons || !g || !seed || !prim || !flag || !iters)) return ASX_ERR_NULL;
   return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] { return launch_c2p_batch(nz, cons, g, seed, prim, flag, iters, ideal_gas(par),
                                      c2p_args(par), (hipStream_t)stream); }));
+}
+
+int32_t asx_variant_count(int32_t kclass) {
+  if (kclass < 0 || kclass >= ASX_KC_COUNT) return 0;
+  return variant_count(kclass);
+}
+
+asx_status asx_set_variant(int32_t kclass, int32_t v) {
+  if (kclass < 0 || kclass >= ASX_KC_COUNT) return ASX_ERR_PARAM;
+  if (v < 0 || v >= variant_count(kclass)) return ASX_ERR_PARAM;
+  set_variant(kclass, v);
+  return ASX_OK;
+}
+
+asx_status asx_get_variant(int32_t kclass, int32_t *v) {
+  if (kclass < 0 || kclass >= ASX_KC_COUNT) return ASX_ERR_PARAM;
+  if (!v) return ASX_ERR_NULL;
+  *v = variant_of(kclass);
+  return ASX_OK;
+}
+
+asx_status asx_autotune_substep(const asx_patch *stage, const asx_state *u0,
+                                const asx_state *out, double c0, double c1, double c2,
+                                const asx_params *par, int32_t reps, void *stream) {
+  if (reps < 1) return ASX_ERR_PARAM;
+  const int tuned[4] = {ASX_KC_FACE_X, ASX_KC_FACE_Y, ASX_KC_FACE_Z, ASX_KC_UPDATE};
+  int nv = 1;
+  for (int kc : tuned) nv = variant_count(kc) > nv ? variant_count(kc) : nv;
+  // keep the caller's timing state: drain it, measure, put it back
+  bool was_on;
+  double tms[ASX_KC_COUNT];
+  int64_t tn[ASX_KC_COUNT];
+  {
+    std::lock_guard<std::mutex> l(g_tmu);
+    asx_status st = drain_timing();
+    if (st) return st;
+    was_on = g_timing.load();
+    for (int q = 0; q < ASX_KC_COUNT; ++q) {
+      tms[q] = g_tms[q];
+      tn[q] = g_tn[q];
+    }
+  }
+  double best[4];
+  int arg[4];
+  for (int q = 0; q < 4; ++q) {
+    best[q] = 1e300;
+    arg[q] = variant_of(tuned[q]);
+  }
+  asx_status st = ASX_OK;
+  for (int v = 0; v < nv && !st; ++v) {
+    for (int kc : tuned) set_variant(kc, v < variant_count(kc) ? v : 0);
+    // one untimed call (first launch of a variant loads its code object)
+    if ((st = asx_substep(stage, u0, out, c0, c1, c2, par, stream))) break;
+    if ((st = asx_timing_enable(1))) break;
+    for (int r = 0; r < reps && !st; ++r)
+      st = asx_substep(stage, u0, out, c0, c1, c2, par, stream);
+    for (int q = 0; q < 4 && !st; ++q) {
+      if (v >= variant_count(tuned[q])) continue;
+      int64_t n = 0;
+      double ms = 0;
+      if ((st = asx_timing_read(tuned[q], &n, &ms))) break;
+      if (n > 0 && ms / n < best[q]) {
+        best[q] = ms / n;
+        arg[q] = v;
+      }
+    }
+  }
+  for (int q = 0; q < 4; ++q) set_variant(tuned[q], arg[q]);
+  {
+    std::lock_guard<std::mutex> l(g_tmu);
+    asx_status st2 = drain_timing();
+    for (int q = 0; q < ASX_KC_COUNT; ++q) {
+      g_tms[q] = tms[q];
+      g_tn[q] = tn[q];
+    }
+    g_timing.store(was_on);
+    if (!st) st = st2;
+  }
+  return st;
 }
 
 } // extern "C"

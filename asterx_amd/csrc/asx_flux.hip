@@ -380,20 +380,31 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
   }
 }
 
-// Face tile shapes (x, y, z work-items; 256 per block).  The extent along
-// D is one cell more than the faces the tile produces.
-template <int D> struct FaceTile;
-template <> struct FaceTile<0> { static constexpr int X = 64, Y = 4, Z = 1; };
-template <> struct FaceTile<1> { static constexpr int X = 16, Y = 16, Z = 1; };
-template <> struct FaceTile<2> { static constexpr int X = 16, Y = 1, Z = 16; };
+// Face tile shapes (x, y, z work-items; 256 per block), variant V of the
+// faces normal to D.  The extent along D is one cell more than the faces the
+// tile produces.
+template <int X_, int Y_, int Z_> struct Tile3 {
+  static constexpr int X = X_, Y = Y_, Z = Z_;
+};
+template <int D, int V> struct FaceTileT;
+template <> struct FaceTileT<0, 0> { using T = Tile3<64, 4, 1>; };
+template <> struct FaceTileT<0, 1> { using T = Tile3<64, 2, 2>; };
+template <> struct FaceTileT<0, 2> { using T = Tile3<32, 4, 2>; };
+template <> struct FaceTileT<1, 0> { using T = Tile3<16, 16, 1>; };
+template <> struct FaceTileT<1, 1> { using T = Tile3<32, 8, 1>; };
+template <> struct FaceTileT<1, 2> { using T = Tile3<16, 8, 2>; };
+template <> struct FaceTileT<2, 0> { using T = Tile3<16, 1, 16>; };
+template <> struct FaceTileT<2, 1> { using T = Tile3<32, 1, 8>; };
+template <> struct FaceTileT<2, 2> { using T = Tile3<16, 2, 8>; };
+template <int D, int V> using FaceTile = typename FaceTileT<D, V>::T;
 
 #ifndef ASX_FACE_WAVES
 #define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
 #endif
-template <int D, int M>
+template <int D, int M, int V>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES))) void k_face(KPatch P, FluxArgs A, int ntx, int nty,
                                               int ntz) {
-  using T = FaceTile<D>;
+  using T = FaceTile<D, V>;
   static_assert(T::X * T::Y * T::Z == 256, "face tile");
   constexpr int TD = D == 0 ? T::X : (D == 1 ? T::Y : T::Z);
   constexpr int SD = D == 0 ? 1 : (D == 1 ? T::X : T::X * T::Y); // tile stride along D
@@ -569,10 +580,10 @@ __global__ __launch_bounds__(256) void k_prim2con(KPatch P, Box B) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-template <int D>
-static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method,
-                                hipStream_t s) {
-  using T = FaceTile<D>;
+template <int D, int V>
+static hipError_t launch_flux_dv(const KPatch &P, const FluxArgs &A, int method,
+                                 hipStream_t s) {
+  using T = FaceTile<D, V>;
   if ((int64_t)A.cnt[0] * A.cnt[1] * A.cnt[2] <= 0) return hipSuccess;
   const int ex[3] = {T::X, T::Y, T::Z};
   int nt[3];
@@ -582,21 +593,28 @@ static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method,
   }
   const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2])), block(256);
   switch (method) {
-  case ASX_RECON_GODUNOV: hipLaunchKernelGGL((k_face<D, ASX_RECON_GODUNOV>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
-  case ASX_RECON_MINMOD: hipLaunchKernelGGL((k_face<D, ASX_RECON_MINMOD>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
-  case ASX_RECON_MONOCENTRAL: hipLaunchKernelGGL((k_face<D, ASX_RECON_MONOCENTRAL>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
-  case ASX_RECON_PPM: hipLaunchKernelGGL((k_face<D, ASX_RECON_PPM>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
-  case ASX_RECON_WENOZ: hipLaunchKernelGGL((k_face<D, ASX_RECON_WENOZ>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
-  default: hipLaunchKernelGGL((k_face<D, ASX_RECON_MP5>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_GODUNOV: hipLaunchKernelGGL((k_face<D, ASX_RECON_GODUNOV, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_MINMOD: hipLaunchKernelGGL((k_face<D, ASX_RECON_MINMOD, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_MONOCENTRAL: hipLaunchKernelGGL((k_face<D, ASX_RECON_MONOCENTRAL, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_PPM: hipLaunchKernelGGL((k_face<D, ASX_RECON_PPM, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  case ASX_RECON_WENOZ: hipLaunchKernelGGL((k_face<D, ASX_RECON_WENOZ, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+  default: hipLaunchKernelGGL((k_face<D, ASX_RECON_MP5, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
   }
   return hipGetLastError();
 }
+template <int D>
+static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method, int variant,
+                                hipStream_t s) {
+  if (variant == 1) return launch_flux_dv<D, 1>(P, A, method, s);
+  if (variant == 2) return launch_flux_dv<D, 2>(P, A, method, s);
+  return launch_flux_dv<D, 0>(P, A, method, s);
+}
 
-hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method,
+hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, int variant,
                        hipStream_t s) {
-  if (d == 0) return launch_flux_d<0>(P, A, method, s);
-  if (d == 1) return launch_flux_d<1>(P, A, method, s);
-  return launch_flux_d<2>(P, A, method, s);
+  if (d == 0) return launch_flux_d<0>(P, A, method, variant, s);
+  if (d == 1) return launch_flux_d<1>(P, A, method, variant, s);
+  return launch_flux_d<2>(P, A, method, variant, s);
 }
 
 hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s) {

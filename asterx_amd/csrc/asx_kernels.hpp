@@ -20,25 +20,35 @@ struct Box {
   __host__ dim3 grid() const { return dim3((unsigned)((htotal() + 255) / 256)); }
 };
 
-// A box covered by workgroup tiles of TX x TY x 1 points (TX*TY work-items),
-// tiles ordered x, y, z and spread over the XCDs in contiguous runs.
-template <int TX, int TY> struct TileGrid {
+// A box covered by workgroup tiles of TX x TY x TZ points (one work-item
+// each), tiles ordered x, y, z and spread over the XCDs in contiguous runs.
+// Deeper tiles (TZ > 1) read each shared vertex / face plane once per tile
+// instead of once per plane.
+template <int TX, int TY, int TZ = 1> struct TileGrid {
+  static constexpr int kThreads = TX * TY * TZ;
   int ntx, nty, ntz;
   __host__ static TileGrid make(const Box &B) {
-    return TileGrid{(B.cnt[0] + TX - 1) / TX, (B.cnt[1] + TY - 1) / TY, B.cnt[2]};
+    return TileGrid{(B.cnt[0] + TX - 1) / TX, (B.cnt[1] + TY - 1) / TY,
+                    (B.cnt[2] + TZ - 1) / TZ};
   }
   __host__ dim3 grid() const { return dim3((unsigned)(ntx * nty * ntz)); }
   // point of this work-item; false past the box
   ASX_DEV bool point(const Box &B, int &i, int &j, int &k) const {
     const int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
     const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
-    const int x = bx * TX + (int)threadIdx.x % TX, y = by * TY + (int)threadIdx.x / TX;
+    const int t = (int)threadIdx.x;
+    const int x = bx * TX + t % TX, y = by * TY + (t / TX) % TY, z = bz * TZ + t / (TX * TY);
     i = B.lo[0] + x;
     j = B.lo[1] + y;
-    k = B.lo[2] + bz;
-    return x < B.cnt[0] && y < B.cnt[1];
+    k = B.lo[2] + z;
+    return x < B.cnt[0] && y < B.cnt[1] && z < B.cnt[2];
   }
 };
+
+// Launch-shape variants of the tunable kernels (selected per call; see
+// asx_set_variant / asx_autotune_substep in the C ABI).
+constexpr int kUpdateVariants = 5;
+constexpr int kFaceVariants = 3;
 
 struct FluxArgs {
   RParams rp;
@@ -59,7 +69,7 @@ struct StatePtrs {
   double *g[ASX_NUM_STATE];
 };
 
-hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method,
+hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, int variant,
                        hipStream_t s);
 hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s);
 hipError_t launch_bstag(int d, const KPatch &P, const Box &B, double *out,
@@ -84,7 +94,7 @@ struct BCList {
 hipError_t launch_bc_many(const KPatch &P, const BCList &L, int kind, hipStream_t s);
 hipError_t launch_update(const KPatch &P, const RhsArgs &A, const StatePtrs &u0,
                          const StatePtrs &out, double c0, double c1, double c2,
-                         hipStream_t s);
+                         int variant, hipStream_t s);
 
 // con2prim of the patch interior in two passes: every zone through the
 // primary solver, the zones it failed queued (queue: one int per interior

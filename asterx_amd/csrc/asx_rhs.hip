@@ -537,11 +537,9 @@ ASX_DEV constexpr int kSrcState(int q) {
 // waves per SIMD the register budget is sized for (order-4 derivatives
 // need 32 more doubles live than order 2)
 template <int ORD> constexpr int kUpdateWaves = ORD == 2 ? 4 : 3;
-using UpdateGrid = TileGrid<64, 4>;
-template <int ORD>
+template <int ORD, class G_>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<ORD>)))
-void k_update_all(KPatch P, Box B, UpdateGrid G, RhsArgs A, StatePtrs u0, StatePtrs out,
-                  Comb cb) {
+void k_update_all(KPatch P, Box B, G_ G, RhsArgs A, StatePtrs u0, StatePtrs out, Comb cb) {
   int i, j, k;
   if (!G.point(B, i, j, k)) return;
   const bool cx = i < P.ng + P.n[0], cy = j < P.ng + P.n[1], cz = k < P.ng + P.n[2];
@@ -655,16 +653,37 @@ hipError_t launch_bc_many(const KPatch &P, const BCList &L, int kind, hipStream_
   return hipGetLastError();
 }
 
+template <int ORD, int TX, int TY, int TZ>
+static void launch_update_t(const KPatch &P, const Box &b, const RhsArgs &A,
+                            const StatePtrs &u0, const StatePtrs &out, const Comb &cb,
+                            hipStream_t s) {
+  using G = TileGrid<TX, TY, TZ>;
+  static_assert(G::kThreads == 256, "update tile");
+  const G g = G::make(b);
+  hipLaunchKernelGGL((k_update_all<ORD, G>), g.grid(), dim3(256), 0, s, P, b, g, A, u0, out, cb);
+}
+template <int ORD>
+static void launch_update_v(int v, const KPatch &P, const Box &b, const RhsArgs &A,
+                            const StatePtrs &u0, const StatePtrs &out, const Comb &cb,
+                            hipStream_t s) {
+  switch (v) {
+  case 1: launch_update_t<ORD, 32, 4, 2>(P, b, A, u0, out, cb, s); break;
+  case 2: launch_update_t<ORD, 32, 2, 4>(P, b, A, u0, out, cb, s); break;
+  case 3: launch_update_t<ORD, 16, 4, 4>(P, b, A, u0, out, cb, s); break;
+  case 4: launch_update_t<ORD, 64, 2, 2>(P, b, A, u0, out, cb, s); break;
+  default: launch_update_t<ORD, 64, 4, 1>(P, b, A, u0, out, cb, s); break;
+  }
+}
+
 hipError_t launch_update(const KPatch &P, const RhsArgs &A, const StatePtrs &u0,
                          const StatePtrs &out, double c0, double c1, double c2,
-                         hipStream_t s) {
+                         int variant, hipStream_t s) {
   const Comb cb{c0, c1, c2};
   const Box b = interior_box(P, 7);
-  const UpdateGrid G = UpdateGrid::make(b);
   if (A.fd_order == 4)
-    hipLaunchKernelGGL(k_update_all<4>, G.grid(), dim3(256), 0, s, P, b, G, A, u0, out, cb);
+    launch_update_v<4>(variant, P, b, A, u0, out, cb, s);
   else
-    hipLaunchKernelGGL(k_update_all<2>, G.grid(), dim3(256), 0, s, P, b, G, A, u0, out, cb);
+    launch_update_v<2>(variant, P, b, A, u0, out, cb, s);
   return hipGetLastError();
 }
 

@@ -35,6 +35,11 @@ SIGNATURES = {
     "asx_timing_enable": ([C.c_int32], C.c_int),
     "asx_timing_read": ([C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
                         C.c_int),
+    "asx_variant_count": ([C.c_int32], C.c_int32),
+    "asx_set_variant": ([C.c_int32, C.c_int32], C.c_int),
+    "asx_get_variant": ([C.c_int32, C.POINTER(C.c_int32)], C.c_int),
+    "asx_autotune_substep": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP,
+                              C.c_int32, _V], C.c_int),
     "asx_substep": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, _V],
                     C.c_int),
     "asx_c2p_batch": ([C.c_int64, _V, _V, _V, _V, _V, _V, _PP, _V], C.c_int),

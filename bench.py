@@ -51,7 +51,32 @@ def parse():
                     help="time the CPU oracle on the same inputs")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--tune", choices=["on", "off"], default="on",
+                    help="pick launch-shape variants with asx_autotune_substep "
+                         "before the warm-up")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a device-to-device copy of 1 GiB five times "
+                         "(reference kernel for the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """Five copies of a 1 GiB FP64 array: each reads and writes exactly
+    2^30 bytes, so FETCH_SIZE / WRITE_SIZE per copy dispatch give the
+    counters' scale for known traffic."""
+    import torch
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 8,
+                      "bytes_written": n * 8, "ms": round(dt * 1e3, 4),
+                      "GBs": round(2 * n * 8 / dt / 1e9, 1)}), flush=True)
 
 
 def log(args, *a):
@@ -227,6 +252,9 @@ def main():
         print("bench.py: this build measures one GPU; --gpus %d is not "
               "supported (run with --gpus 1)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch
     from asterx_amd import abi, lib, torus
@@ -266,6 +294,23 @@ def main():
                 host.alloc(s)
 
     st = Stepper(p, par, be, dt, sh)
+    variants = {}
+    if args.tune == "on":
+        # SSPRK3 stage-2 coefficients into a scratch state (setup, untimed)
+        from asterx_amd.patch import StateBuf
+        scratch = StateBuf(p)
+        ps, u0s, outs = p.struct(), state_struct(p, st.bufs[0]), scratch.struct()
+        abi.check("asx_autotune_substep", be.lib.asx_autotune_substep(
+            C.byref(ps), C.byref(u0s), C.byref(outs), 0.75, 0.25, 0.25 * dt,
+            C.byref(par), 3, sh))
+        torch.cuda.synchronize()
+        del scratch
+        for name in ("face_x", "face_y", "face_z", "update"):
+            v = C.c_int32()
+            abi.check("asx_get_variant", be.lib.asx_get_variant(abi.KCLASS[name],
+                                                                C.byref(v)))
+            variants[name] = v.value
+        log(args, "tuned variants:", variants)
     for _ in range(args.warmup):
         st.substep()
     torch.cuda.synchronize()
@@ -324,7 +369,8 @@ def main():
             "traffic": read_traffic(dom, workload),
             "avg_ms": round(avg, 4),
             "bytes_per_launch": int(per_launch[dom]),
-            "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()}}
+            "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()},
+            "variants": variants}
     # the whole stencil half of the substep against the per-zone figure
     # (480 B/zone, 392 B/zone in stage 1: each field of the substep read or
     # written once), over the summed time of its kernels

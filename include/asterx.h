@@ -272,6 +272,21 @@ enum {
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
 
+/* Launch-shape variants and autotuning (no counterpart in the reference).
+ * The face kernels (ASX_KC_FACE_*) and the fused update (ASX_KC_UPDATE) are
+ * built in several workgroup tile shapes; results do not depend on the
+ * variant.  The choice is per device (default 0).  asx_autotune_substep
+ * runs asx_substep with the given arguments once untimed and `reps` times
+ * timed for every variant, keeps the fastest per kernel class, and leaves the
+ * caller's timing state as it was.  It overwrites `out`. */
+int32_t asx_variant_count(int32_t kclass);
+asx_status asx_set_variant(int32_t kclass, int32_t v);
+asx_status asx_get_variant(int32_t kclass, int32_t *v);
+asx_status asx_autotune_substep(const asx_patch *stage, const asx_state *u0,
+                                const asx_state *out, double c0, double c1,
+                                double c2, const asx_params *par, int32_t reps,
+                                void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Scheduled functions (drop-in for the reference's extern "C" entry points) */
 /* ------------------------------------------------------------------------ */

@@ -307,3 +307,47 @@ def test_timing_api():
     abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["face_x"], C.byref(nl),
                                                    C.byref(ms)))
     assert nl.value == 0  # disabling reset the sums
+
+
+def test_variants_bit_identical():
+    """Every launch-shape variant of the face and update kernels gives the
+    same bits as variant 0; the autotuner leaves a valid choice."""
+    _torch()
+    from asterx_amd.patch import StateBuf
+    h, d = _pair(curved=True, zero_entropy_block=True)
+    L = _gpu().lib
+    par = abi.params(reconstruction_method="wenoz", flux_type="HLLE",
+                     loworder_method="minmod")
+    u0 = StateBuf(d)
+    for name in abi.STATE:
+        u0.arr[name] = d[name].clone() * 1.01
+    kcs = [abi.KCLASS[n] for n in ("face_x", "face_y", "face_z", "update")]
+    nv = max(L.asx_variant_count(kc) for kc in kcs)
+    assert nv >= 2
+    ref = None
+    for v in range(nv):
+        for kc in kcs:
+            abi.check("asx_set_variant",
+                      L.asx_set_variant(kc, v if v < L.asx_variant_count(kc) else 0))
+        out = StateBuf(d, fill=0.0)
+        ps, u0s, os_ = d.struct(), u0.struct(), out.struct()
+        abi.check("asx_substep", L.asx_substep(C.byref(ps), C.byref(u0s), C.byref(os_),
+                                               0.75, 0.25, 0.0025, C.byref(par), None))
+        _sync()
+        got = {n: out.arr[n].cpu().numpy() for n in abi.STATE}
+        if ref is None:
+            ref = got
+        else:
+            for n in abi.STATE:
+                assert np.array_equal(got[n], ref[n]), (v, n)
+    out = StateBuf(d, fill=0.0)
+    ps, u0s, os_ = d.struct(), u0.struct(), out.struct()
+    abi.check("asx_autotune_substep", L.asx_autotune_substep(
+        C.byref(ps), C.byref(u0s), C.byref(os_), 0.75, 0.25, 0.0025, C.byref(par), 2,
+        None))
+    for kc in kcs:
+        v = C.c_int32()
+        abi.check("asx_get_variant", L.asx_get_variant(kc, C.byref(v)))
+        assert 0 <= v.value < L.asx_variant_count(kc)
+        abi.check("asx_set_variant", L.asx_set_variant(kc, 0))
+    assert L.asx_set_variant(kcs[0], 99) == 3  # ASX_ERR_PARAM
