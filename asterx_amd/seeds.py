@@ -105,3 +105,59 @@ def post_initial(patch, gamma, particle_mass=1.0, Ye_atmo=0.5):
     eps = patch.numpy("eps")
     patch["temperature"] = (gamma - 1) * particle_mass * eps
     patch["entropy"] = (gamma - 1.0) * eps * np.power(rho, 1.0 - gamma)
+
+
+def _set_hydro(patch, rho, vel, press, gamma):
+    shp = patch.shape("rho")
+    patch["rho"] = np.broadcast_to(rho, shp)
+    for name, v in zip(("velx", "vely", "velz"), vel):
+        patch[name] = np.broadcast_to(v, shp)
+    patch["press"] = np.broadcast_to(press, shp)
+    patch["eps"] = np.broadcast_to(press / (rho * (gamma - 1.0)), shp)
+    for name in ("Bvecx", "Bvecy", "Bvecz"):
+        patch[name] = 0.0
+
+
+def _set_avec(patch, fx, fy, fz):
+    for name, cent, f in (("Avec_x", 6, fx), ("Avec_y", 5, fy), ("Avec_z", 3, fz)):
+        z, y, x = patch.coords(cent)
+        patch[name] = np.broadcast_to(f(x, y, z), patch.shape(name)).astype(np.float64)
+
+
+def sound_wave(patch, gamma, amplitude=1e-3):
+    """AsterSeeds/src/1D_tests.cxx:66-92 ("sound wave"; amplitude default
+    AsterSeeds/param.ccl:78-81): rho = p = 1, v^x = amplitude sin(pi x), A = 0."""
+    z, y, x = patch.coords(0)
+    _set_hydro(patch, 1.0, (amplitude * np.sin(math.pi * x), 0.0, 0.0), 1.0, gamma)
+    zero = lambda x, y, z: 0.0 * (x + y + z)
+    _set_avec(patch, zero, zero, zero)
+
+
+def alfven_wave(patch, gamma):
+    """AsterSeeds/src/1D_tests.cxx:94-127 ("Alfven wave"): rho = 1, p = 0.5,
+    v = -va A0 (0, cos kx, sin kx), A = (z cos kx - y sin kx, -z/2, y/2),
+    A0 = 1, va = 0.5, k = 2 pi."""
+    A0, va, k = 1.0, 0.5, 2.0 * math.pi
+    z, y, x = patch.coords(0)
+    _set_hydro(patch, 1.0, (0.0, -va * A0 * np.cos(k * x), -va * A0 * np.sin(k * x)),
+               0.5, gamma)
+    _set_avec(patch, lambda x, y, z: z * np.cos(k * x) - y * np.sin(k * x),
+              lambda x, y, z: -z / 2.0 + 0.0 * (x + y),
+              lambda x, y, z: y / 2.0 + 0.0 * (x + z))
+
+
+def magnetic_rotor(patch, gamma):
+    """AsterSeeds/src/2D_tests.cxx:33-73 ("magnetic rotor"): inside
+    r_cyl < 0.1 rho = 10 rotating with omega = 9.95, outside rho = 1 at rest,
+    p = 1; A = (0, 0, y)."""
+    z, y, x = patch.coords(0)
+    shp = patch.shape("rho")
+    x, y = np.broadcast_to(x, shp), np.broadcast_to(y, shp)
+    inside = np.sqrt(x * x + y * y) < 0.1
+    omega = 9.95
+    rho = np.where(inside, 10.0, 1.0)
+    vx = np.where(inside, -omega * y, 0.0)
+    vy = np.where(inside, omega * x, 0.0)
+    _set_hydro(patch, rho, (vx, vy, 0.0), 1.0, gamma)
+    zero = lambda x, y, z: 0.0 * (x + y + z)
+    _set_avec(patch, zero, zero, lambda x, y, z: y + 0.0 * (x + z))

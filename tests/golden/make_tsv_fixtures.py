@@ -32,6 +32,9 @@ CASES = [
     "Balsara3_shocktube_xdir_minmod_LxF",
     "Balsara4_shocktube_xdir_ppm",
     "Balsara5_shocktube_xdir_ppm",
+    "Alfven_wave",
+    "Sound_wave",
+    "Magnetic_rotor",
 ]
 VARS = ["rho", "vel", "eps", "press", "bvec"]
 
