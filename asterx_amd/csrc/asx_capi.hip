@@ -974,4 +974,86 @@ asx_status asx_autotune_substep(const asx_patch *stage, const asx_state *u0,
   return st;
 }
 
+// ---------------------------------------------------------------------------
+// Independent zones and the learned con2prim guess
+// ---------------------------------------------------------------------------
+asx_status asx_zone_sample(int64_t nz, uint64_t seed, double gamma, double seed_rel,
+                           double *prim, double *g, double *seed_out, double *target,
+                           void *stream) {
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!prim || !g)) return ASX_ERR_NULL;
+  if (!(gamma > 1.0) || !(seed_rel >= 0.0)) return ASX_ERR_PARAM;
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] {
+    return launch_zone_sample(nz, seed, gamma, seed_rel, prim, g, seed_out, target,
+                              (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_prim2con_batch(int64_t nz, const double *prim, const double *g, double *cons,
+                              double *cold_seed, const asx_params *par, void *stream) {
+  asx_status st = check_par(par);
+  if (st) return st;
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!prim || !g || !cons)) return ASX_ERR_NULL;
+  const IdealGas eos = ideal_gas(par);
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] {
+    return launch_prim2con_batch(nz, prim, g, cons, cold_seed, eos, par->Ye_atmo,
+                                 (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
+                            void *stream) {
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !x)) return ASX_ERR_NULL;
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] {
+    return launch_mlp_features(nz, cons, g, x, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_mlp_forward(int64_t nz, const double *cons, const double *g, const float *blob,
+                           double *out, void *stream) {
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !blob || !out)) return ASX_ERR_NULL;
+  return dev_status(timed(ASX_KC_MLP, (hipStream_t)stream, [&] {
+    return launch_mlp_forward(nz, cons, g, blob, out, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_mlp_train_step(int64_t nz, const double *cons, const double *g,
+                              const double *target, float *blob, float *m, float *v, float lr,
+                              int64_t step, float *partial, float *loss, float *grad,
+                              void *stream) {
+  if (nz < 1 || nz > (int64_t)1 << 30) return ASX_ERR_SIZE;
+  if (!cons || !g || !target || !blob || !partial) return ASX_ERR_NULL;
+  if (lr != 0.0f && (!m || !v || step < 1)) return ASX_ERR_PARAM;
+  hipStream_t s = (hipStream_t)stream;
+  const int nw = (int)((nz + 63) / 64);
+  asx_status st = dev_status(timed(ASX_KC_MLP, s, [&] {
+    return launch_mlp_grad(nz, cons, g, target, blob, partial, s);
+  }));
+  if (st) return st;
+  return dev_status(timed(ASX_KC_MLP, s, [&] {
+    return launch_mlp_adam(nw, partial, blob, m, v, lr, 0.9f, 0.999f, 1e-8f,
+                           step < 1 ? 1 : step, loss, grad, s);
+  }));
+}
+
+asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
+                             const double *seed, const float *blob, int32_t kguess,
+                             double *prim, double *flag, int32_t *iters, int32_t *fallback,
+                             const asx_params *par, void *stream) {
+  asx_status st = check_c2p_par(par);
+  if (st) return st;
+  if (par->c2p_prime != ASX_C2P_NOBLE) return ASX_ERR_UNSUPPORTED;
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (kguess < 1) return ASX_ERR_PARAM;
+  if (nz > 0 && (!cons || !g || !seed || !blob || !prim || !flag || !iters))
+    return ASX_ERR_NULL;
+  return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] {
+    return launch_c2p_batch_mlp(nz, cons, g, seed, blob, kguess, prim, flag, iters, fallback,
+                                ideal_gas(par), c2p_args(par), (hipStream_t)stream);
+  }));
+}
+
 } // extern "C"

@@ -105,6 +105,43 @@ hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PA
 hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
                                 bool from_A, unsigned long long *hist, const int *queue,
                                 const unsigned *qcount, hipStream_t s);
+// ---------------------------------------------------------------------------
+// Independent zones: sampler, batched prim2con, learned con2prim guess
+// ---------------------------------------------------------------------------
+// Network: 5 inputs -> 64 tanh -> 64 tanh -> 2 outputs.  One FP32 blob:
+//   W1[64][5] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]   (4674 parameters)
+//   in_mean[5] in_istd[5] out_mean[2] out_std[2]         (normalisation)
+constexpr int kMlpIn = 5, kMlpH = 64, kMlpOut = 2;
+constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpH * kMlpIn, kMlpW2 = kMlpB1 + kMlpH,
+              kMlpB2 = kMlpW2 + kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH,
+              kMlpB3 = kMlpW3 + kMlpOut * kMlpH, kMlpNParam = kMlpB3 + kMlpOut;
+constexpr int kMlpInMean = kMlpNParam, kMlpInIstd = kMlpInMean + kMlpIn,
+              kMlpOutMean = kMlpInIstd + kMlpIn, kMlpOutStd = kMlpOutMean + kMlpOut,
+              kMlpNBlob = kMlpOutStd + kMlpOut;
+constexpr int kMlpPartial = kMlpNParam + 1; // per-wavefront gradient + loss
+
+hipError_t launch_zone_sample(int64_t nz, uint64_t seed, double gamma, double seed_rel,
+                              double *prim, double *g, double *sd, double *target,
+                              hipStream_t s);
+hipError_t launch_prim2con_batch(int64_t nz, const double *prim, const double *g,
+                                 double *cons, double *cold, const IdealGas &eos, double Ye,
+                                 hipStream_t s);
+hipError_t launch_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
+                               hipStream_t s);
+hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
+                              const float *blob, double *out, hipStream_t s);
+hipError_t launch_mlp_grad(int64_t nz, const double *cons, const double *g,
+                           const double *target, const float *blob, float *partial,
+                           hipStream_t s);
+hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float *m,
+                           float *v, float lr, float b1, float b2, float eps, int64_t step,
+                           float *loss, float *grad, hipStream_t s);
+hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
+                                const double *seed, const float *blob, int kguess,
+                                double *prim, double *flag, int32_t *iters,
+                                int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
+                                hipStream_t s);
+
 hipError_t launch_c2p_batch(int64_t nz, const double *cons, const double *g,
                             const double *seed, double *prim, double *flag,
                             int32_t *iters, const IdealGas &eos,

@@ -1,0 +1,627 @@
+// Independent-zone kernels for gfx950: the sampler of the learned-guess
+// workload, batched prim2con, and the learned con2prim initial guess --
+// inference, training (forward + backward + Adam) and con2prim from the
+// guess with the plain guess as fallback.
+//
+// The network is 5 -> 64 tanh -> 64 tanh -> 2.  Its 64x64 layer, and both
+// 64x64 products of its backward pass, run on the matrix cores
+// (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32 accumulation); the first
+// and last layers (5 and 2 wide) run on the vector ALU with bf16-rounded
+// operands and fp32 accumulation.  Weights are kept in fp32 (the blob) and
+// rounded to bf16 where they enter a product.
+#include "asx_kernels.hpp"
+
+namespace asx {
+
+// ---------------------------------------------------------------------------
+// Counter-based random numbers: splitmix64 (Steele, Lea & Flood, OOPSLA 2014)
+// of (seed, zone, draw), so a zone's draws do not depend on the launch shape.
+// ---------------------------------------------------------------------------
+ASX_DEV uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// uniform in (0, 1)
+ASX_DEV double urand(uint64_t seed, int64_t z, int draw) {
+  const uint64_t h = mix64(seed ^ mix64((uint64_t)z * 64u + (uint64_t)draw));
+  return ((double)(h >> 11) + 0.5) * 0x1.0p-53;
+}
+ASX_DEV double loguni(uint64_t seed, int64_t z, int d, double a, double b) {
+  return exp(log(a) + (log(b) - log(a)) * urand(seed, z, d));
+}
+// three standard normals (Box-Muller), draws d .. d+3
+ASX_DEV void normal3(uint64_t seed, int64_t z, int d, double n[3]) {
+  const double u1 = urand(seed, z, d), u2 = urand(seed, z, d + 1);
+  const double u3 = urand(seed, z, d + 2), u4 = urand(seed, z, d + 3);
+  const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
+  const double tp = 6.283185307179586;
+  n[0] = r1 * cos(tp * u2);
+  n[1] = r1 * sin(tp * u2);
+  n[2] = r2 * cos(tp * u4);
+}
+ASX_DEV void unit_in(const Sym3 &g, double v[3]) {
+  double l[3];
+  lower(g, v, l);
+  const double n = sqrt(dot3(v, l));
+  v[0] /= n;
+  v[1] /= n;
+  v[2] /= n;
+}
+
+// One zone of the learned-guess workload: rho log-uniform in [1e-10, 1e-2],
+// p/rho log-uniform in [1e-6, 1e2], W uniform in [1, 10] along a random
+// direction, b^2/rho log-uniform in [1e-4, 1e2] along a random direction, a
+// symmetric 3-metric within 10% of the identity.  prim[8] = rho, eps, v^i,
+// B^i; sd[5] (optional) = the true rho, eps, v^i each times (1 + s u), u
+// uniform in [-1, 1]; target[2] (optional) = log W, log(Z / D) with
+// Z = rho h W^2 and D = rho W.
+__global__ __launch_bounds__(256) void k_zone_sample(int64_t nz, uint64_t seed, double gamma,
+                                                     double seed_rel, double *prim, double *g,
+                                                     double *sd, double *target) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  const double rho = loguni(seed, z, 0, 1e-10, 1e-2);
+  const double press = rho * loguni(seed, z, 1, 1e-6, 1e2);
+  const double eps = press / ((gamma - 1.0) * rho);
+  const double W = 1.0 + 9.0 * urand(seed, z, 2);
+  double M[6];
+  for (int q = 0; q < 6; ++q) M[q] = (2.0 * urand(seed, z, 3 + q) - 1.0) * (0.1 / 3.0);
+  const Sym3 gl{1.0 + M[0], M[1], M[2], 1.0 + M[3], M[4], 1.0 + M[5]};
+  double nv[3], nb[3];
+  normal3(seed, z, 9, nv);
+  unit_in(gl, nv);
+  normal3(seed, z, 13, nb);
+  unit_in(gl, nb);
+  const double vabs = sqrt(1.0 - 1.0 / (W * W));
+  const double vel[3] = {nv[0] * vabs, nv[1] * vabs, nv[2] * vabs};
+  const double b2 = rho * loguni(seed, z, 17, 1e-4, 1e2);
+  double vl[3];
+  lower(gl, vel, vl);
+  const double ndv = dot3(nb, vl);
+  const double beta = sqrt(b2 / (1.0 / (W * W) + ndv * ndv));
+  prim[z] = rho;
+  prim[nz + z] = eps;
+  for (int i = 0; i < 3; ++i) prim[(2 + i) * nz + z] = vel[i];
+  for (int i = 0; i < 3; ++i) prim[(5 + i) * nz + z] = nb[i] * beta;
+  g[z] = gl.xx;
+  g[nz + z] = gl.xy;
+  g[2 * nz + z] = gl.xz;
+  g[3 * nz + z] = gl.yy;
+  g[4 * nz + z] = gl.yz;
+  g[5 * nz + z] = gl.zz;
+  if (sd) {
+    sd[z] = rho * (1.0 + seed_rel * (2.0 * urand(seed, z, 18) - 1.0));
+    sd[nz + z] = eps * (1.0 + seed_rel * (2.0 * urand(seed, z, 19) - 1.0));
+    for (int i = 0; i < 3; ++i)
+      sd[(2 + i) * nz + z] = vel[i] * (1.0 + seed_rel * (2.0 * urand(seed, z, 20 + i) - 1.0));
+  }
+  if (target) {
+    target[z] = log(W);
+    target[nz + z] = log((1.0 + eps + press / rho) * W);
+  }
+}
+
+// Batched Valencia prim -> cons (Con2PrimFactory/src/cons.hxx:49-91 with W
+// from v); cons[8] = dens, S_i, tau, dB^i (densitized).  cold (optional)
+// receives the cold-start seed rho = D, eps = tau/D, v = 0.
+__global__ __launch_bounds__(256) void k_prim2con_batch(int64_t nz, const double *prim,
+                                                        const double *g, double *cons,
+                                                        double *cold, IdealGas eos,
+                                                        double Ye) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  const double rho = prim[z], eps = prim[nz + z];
+  const double vel[3] = {prim[2 * nz + z], prim[3 * nz + z], prim[4 * nz + z]};
+  const double Bv[3] = {prim[5 * nz + z], prim[6 * nz + z], prim[7 * nz + z]};
+  double vl[3];
+  lower(gl, vel, vl);
+  const double w = 1.0 / sqrt(1.0 - dot3(vl, vel));
+  Cons cv;
+  cons_from_prim(gl, rho, eps, eos.press_re(rho, eps), vel, w, Bv, Ye, 0.0, cv);
+  cons[z] = cv.dens;
+  for (int i = 0; i < 3; ++i) cons[(1 + i) * nz + z] = cv.mom[i];
+  cons[4 * nz + z] = cv.tau;
+  for (int i = 0; i < 3; ++i) cons[(5 + i) * nz + z] = cv.dB[i];
+  if (cold) {
+    cold[z] = cv.dens / sqrt(det3(gl));
+    cold[nz + z] = cv.tau / cv.dens;
+    cold[2 * nz + z] = 0.0;
+    cold[3 * nz + z] = 0.0;
+    cold[4 * nz + z] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Network
+// ---------------------------------------------------------------------------
+// Inputs from the densitized conserved variables: log D, log S^2, log tau,
+// log B^2 (undensitized, S^2 = gamma^ij S_i S_j, B^2 = gamma_ij B^i B^j) and
+// the cosine of S and B; small floors keep the logs finite.
+ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, double x[5]) {
+  const double detg = det3(gl), sq = sqrt(detg);
+  const Sym3 gu = inv3(gl, detg);
+  const double is = 1.0 / sq;
+  const double D = c[0] * is, tau = c[4] * is;
+  const double S[3] = {c[1] * is, c[2] * is, c[3] * is};
+  const double B[3] = {c[5] * is, c[6] * is, c[7] * is};
+  double Su[3], Bl[3];
+  lower(gu, S, Su);
+  lower(gl, B, Bl);
+  const double S2 = dot3(S, Su), B2 = dot3(B, Bl), SB = dot3(S, B);
+  const double floor2 = 1e-30 * D * D;
+  x[0] = log(D);
+  x[1] = log(S2 + floor2);
+  x[2] = log(tau > 1e-15 * D ? tau : 1e-15 * D);
+  x[3] = log(B2 + floor2);
+  const double den = sqrt(S2 * B2);
+  x[4] = den > 0.0 ? SB / den : 0.0;
+}
+
+// bf16 round-to-nearest-even and back
+ASX_DEV uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+ASX_DEV float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+ASX_DEV float bfr(float f) { return bf2f(f2bf(f)); }
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+// C[16x16] += A[16x16] B[16x16] on one wavefront.  Fragments, for lane l:
+//   A: A[l%16][4(l/16) .. 4(l/16)+3]      B: B[4(l/16) .. 4(l/16)+3][l%16]
+//   C: C[4(l/16) .. 4(l/16)+3][l%16]
+ASX_DEV f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// 64x64 bf16 tiles in LDS, row-major with a padded row of kTS elements
+// (136 bytes: fragment reads stay 8-byte aligned and spread over banks)
+constexpr int kTS = 68;
+ASX_DEV bf16x4 frag(const uint16_t *t, int row, int col) {
+  return *reinterpret_cast<const bf16x4 *>(t + row * kTS + col);
+}
+
+// Normalised inputs of zone z (zeros for lanes past the batch).
+ASX_DEV void zone_inputs(bool live, int64_t z, int64_t nz, const double *cons,
+                         const double *g, const float *blob, float xn[5]) {
+  double x[5] = {0, 0, 0, 0, 0};
+  if (live) {
+    double c[8];
+    for (int q = 0; q < 8; ++q) c[q] = cons[q * nz + z];
+    const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+    mlp_features(c, gl, x);
+  }
+  for (int i = 0; i < 5; ++i)
+    xn[i] = live ? ((float)x[i] - blob[kMlpInMean + i]) * blob[kMlpInIstd + i] : 0.0f;
+}
+
+// Stage W2 (bf16, [j][k]) of the blob into LDS; every work-item of the block.
+ASX_DEV void stage_w2(const float *blob, uint16_t *w2s, uint16_t *w2t) {
+  for (int q = threadIdx.x; q < kMlpH * kMlpH; q += blockDim.x) {
+    const int j = q / kMlpH, k = q % kMlpH;
+    const uint16_t b = f2bf(blob[kMlpW2 + q]);
+    w2s[j * kTS + k] = b;
+    if (w2t) w2t[k * kTS + j] = b;
+  }
+}
+
+// Layer 1 for this lane's zone: h1 = tanh(W1 x + b1), written as bf16 to
+// the zone's row of h1 (and column of h1t when given).
+ASX_DEV void layer1(const float *blob, const float xb[5], uint16_t *h1, uint16_t *h1t, int l) {
+  for (int j = 0; j < kMlpH; ++j) {
+    float zj = blob[kMlpB1 + j];
+    for (int i = 0; i < kMlpIn; ++i) zj += bfr(blob[kMlpW1 + j * kMlpIn + i]) * xb[i];
+    const uint16_t hb = f2bf(tanhf(zj));
+    h1[l * kTS + j] = hb;
+    if (h1t) h1t[j * kTS + l] = hb;
+  }
+}
+
+// Layer 2 products Z2[z][j] = sum_k H1[z][k] W2[j][k] for the wavefront's
+// 64 zones: 4 zone tiles x 4 output tiles x 4 k-steps of 16.
+ASX_DEV void layer2(const uint16_t *h1, const uint16_t *w2s, f32x4 acc[4][4], int l) {
+  const int r16 = l & 15, k4 = 4 * (l >> 4);
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ob = 0; ob < 4; ++ob) acc[rt][ob] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    bf16x4 b[4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ++ob) b[ob] = frag(w2s, ob * 16 + r16, kb * 16 + k4);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x4 a = frag(h1, rt * 16 + r16, kb * 16 + k4);
+#pragma unroll
+      for (int ob = 0; ob < 4; ++ob) acc[rt][ob] = mfma16(a, b[ob], acc[rt][ob]);
+    }
+  }
+}
+
+// Forward pass of one wavefront (64 zones, lane = zone).  act is the
+// wavefront's 64x64 scratch tile.  Every work-item of the block calls it
+// (it synchronises the block).
+ASX_DEV void mlp_wave(const float *blob, const uint16_t *w2s, uint16_t *act, const float xn[5],
+                      float y[2]) {
+  const int l = threadIdx.x & 63;
+  float xb[5];
+  for (int i = 0; i < kMlpIn; ++i) xb[i] = bfr(xn[i]);
+  layer1(blob, xb, act, nullptr, l);
+  __syncthreads();
+  f32x4 acc[4][4];
+  layer2(act, w2s, acc, l);
+  __syncthreads();
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ob = 0; ob < 4; ++ob)
+      for (int r = 0; r < 4; ++r) {
+        const int zr = rt * 16 + 4 * (l >> 4) + r, o = ob * 16 + (l & 15);
+        act[zr * kTS + o] = f2bf(tanhf(acc[rt][ob][r] + blob[kMlpB2 + o]));
+      }
+  __syncthreads();
+  for (int o = 0; o < kMlpOut; ++o) {
+    float s = blob[kMlpB3 + o];
+    for (int k = 0; k < kMlpH; ++k) s += bfr(blob[kMlpW3 + o * kMlpH + k]) * bf2f(act[l * kTS + k]);
+    y[o] = s;
+  }
+}
+
+// Network outputs in target units (log W, log(Z/D)) for every zone.
+__global__ __launch_bounds__(256) void k_mlp_forward(int64_t nz, const double *cons,
+                                                     const double *g, const float *blob,
+                                                     double *out) {
+  __shared__ uint16_t w2s[kMlpH * kTS];
+  __shared__ uint16_t act[4][kMlpH * kTS];
+  stage_w2(blob, w2s, nullptr);
+  __syncthreads();
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = z < nz;
+  float xn[5], y[2];
+  zone_inputs(live, z, nz, cons, g, blob, xn);
+  mlp_wave(blob, w2s, act[threadIdx.x >> 6], xn, y);
+  if (live) {
+    out[z] = (double)(y[0] * blob[kMlpOutStd] + blob[kMlpOutMean]);
+    out[nz + z] = (double)(y[1] * blob[kMlpOutStd + 1] + blob[kMlpOutMean + 1]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_features(int64_t nz, const double *cons,
+                                                      const double *g, float *xo) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  double c[8], x[5];
+  for (int q = 0; q < 8; ++q) c[q] = cons[q * nz + z];
+  const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  mlp_features(c, gl, x);
+  for (int i = 0; i < 5; ++i) xo[i * nz + z] = (float)x[i];
+}
+
+// ---------------------------------------------------------------------------
+// Training: gradient of the mean squared error of the normalised outputs
+// over the batch, one wavefront (64 zones) per workgroup; every workgroup
+// writes its partial gradient (and loss) to partial[blockIdx.x].  bf16
+// rounding is taken as the identity in the derivatives; the products that
+// run on the matrix cores use the bf16-rounded operands, as in the forward
+// pass.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons, const double *g,
+                                                 const double *target, const float *blob,
+                                                 float *partial) {
+  __shared__ uint16_t w2s[kMlpH * kTS], w2t[kMlpH * kTS];
+  __shared__ uint16_t h1[kMlpH * kTS], h1t[kMlpH * kTS];
+  __shared__ uint16_t a2[kMlpH * kTS], dz2t[kMlpH * kTS];
+  __shared__ float xs[64][kMlpIn];
+  __shared__ float dys[64][kMlpOut];
+  __shared__ float red[4][kMlpH][kMlpIn + 1];
+  __shared__ float lsum[64];
+  const int l = threadIdx.x;
+  const int r16 = l & 15, grp = l >> 4, k4 = 4 * grp;
+  stage_w2(blob, w2s, w2t);
+  const int64_t z = (int64_t)blockIdx.x * 64 + l;
+  const bool live = z < nz;
+  float xn[5];
+  zone_inputs(live, z, nz, cons, g, blob, xn);
+  float xb[5];
+  for (int i = 0; i < kMlpIn; ++i) {
+    xb[i] = bfr(xn[i]);
+    xs[l][i] = xb[i];
+  }
+  float tn[2] = {0.f, 0.f};
+  if (live)
+    for (int o = 0; o < kMlpOut; ++o)
+      tn[o] = ((float)target[o * nz + z] - blob[kMlpOutMean + o]) / blob[kMlpOutStd + o];
+  __syncthreads();
+  layer1(blob, xb, h1, h1t, l);
+  __syncthreads();
+  f32x4 acc[4][4];
+  layer2(h1, w2s, acc, l);
+  // h2 = tanh(z2 + b2), in place of acc, and as bf16 into a2
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ob = 0; ob < 4; ++ob)
+      for (int r = 0; r < 4; ++r) {
+        const int zr = rt * 16 + k4 + r, o = ob * 16 + r16;
+        const float h = tanhf(acc[rt][ob][r] + blob[kMlpB2 + o]);
+        acc[rt][ob][r] = h;
+        a2[zr * kTS + o] = f2bf(h);
+      }
+  __syncthreads();
+  // layer 3 and the loss
+  const float inv_n = 1.0f / (float)nz;
+  float loss = 0.f;
+  for (int o = 0; o < kMlpOut; ++o) {
+    float s = blob[kMlpB3 + o];
+    for (int k = 0; k < kMlpH; ++k) s += bfr(blob[kMlpW3 + o * kMlpH + k]) * bf2f(a2[l * kTS + k]);
+    const float e = live ? s - tn[o] : 0.f;
+    loss += e * e * inv_n;
+    dys[l][o] = 2.f * e * inv_n;
+  }
+  lsum[l] = loss;
+  __syncthreads();
+  float *out = partial + (int64_t)blockIdx.x * kMlpPartial;
+  // dW3[o][k] (lane = k), db3, loss
+  for (int o = 0; o < kMlpOut; ++o) {
+    float s = 0.f;
+    for (int q = 0; q < 64; ++q) s += dys[q][o] * bf2f(a2[q * kTS + l]);
+    out[kMlpW3 + o * kMlpH + l] = s;
+  }
+  if (l < kMlpOut) {
+    float s = 0.f;
+    for (int q = 0; q < 64; ++q) s += dys[q][l];
+    out[kMlpB3 + l] = s;
+  }
+  if (l == 0) {
+    float s = 0.f;
+    for (int q = 0; q < 64; ++q) s += lsum[q];
+    out[kMlpNParam] = s;
+  }
+  // dz2 = (W3^T dy) (1 - h2^2), in place of acc; db2 partials per lane group
+  float db2p[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ob = 0; ob < 4; ++ob)
+      for (int r = 0; r < 4; ++r) {
+        const int zr = rt * 16 + k4 + r, j = ob * 16 + r16;
+        float dh = 0.f;
+        for (int o = 0; o < kMlpOut; ++o) dh += dys[zr][o] * bfr(blob[kMlpW3 + o * kMlpH + j]);
+        const float h = acc[rt][ob][r];
+        const float dz = dh * (1.f - h * h);
+        acc[rt][ob][r] = dz;
+        db2p[ob] += dz;
+      }
+  __syncthreads(); // a2 (h2) no longer read: reuse it for dz2
+  for (int ob = 0; ob < 4; ++ob) red[grp][ob * 16 + r16][0] = db2p[ob];
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ob = 0; ob < 4; ++ob)
+      for (int r = 0; r < 4; ++r) {
+        const int zr = rt * 16 + k4 + r, j = ob * 16 + r16;
+        const uint16_t b = f2bf(acc[rt][ob][r]);
+        a2[zr * kTS + j] = b;
+        dz2t[j * kTS + zr] = b;
+      }
+  __syncthreads();
+  out[kMlpB2 + l] = red[0][l][0] + red[1][l][0] + red[2][l][0] + red[3][l][0];
+  // dW2[j][k] = sum_z dz2[z][j] H1[z][k]: A = dz2^T (rows j), B = H1 (cols k)
+  {
+    f32x4 gw[4][4];
+    for (int jt = 0; jt < 4; ++jt)
+      for (int kt = 0; kt < 4; ++kt) gw[jt][kt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kz = 0; kz < 4; ++kz) {
+      bf16x4 b[4];
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt) b[kt] = frag(h1t, kt * 16 + r16, kz * 16 + k4);
+#pragma unroll
+      for (int jt = 0; jt < 4; ++jt) {
+        const bf16x4 a = frag(dz2t, jt * 16 + r16, kz * 16 + k4);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) gw[jt][kt] = mfma16(a, b[kt], gw[jt][kt]);
+      }
+    }
+    for (int jt = 0; jt < 4; ++jt)
+      for (int kt = 0; kt < 4; ++kt)
+        for (int r = 0; r < 4; ++r)
+          out[kMlpW2 + (jt * 16 + k4 + r) * kMlpH + kt * 16 + r16] = gw[jt][kt][r];
+  }
+  // dH1[z][k] = sum_j dz2[z][j] W2[j][k]: A = dz2 (rows z), B = W2 (cols k)
+  f32x4 dh1[4][4];
+  for (int rt = 0; rt < 4; ++rt)
+    for (int kt = 0; kt < 4; ++kt) dh1[rt][kt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int kj = 0; kj < 4; ++kj) {
+    bf16x4 b[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) b[kt] = frag(w2t, kt * 16 + r16, kj * 16 + k4);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x4 a = frag(a2, rt * 16 + r16, kj * 16 + k4);
+#pragma unroll
+      for (int kt = 0; kt < 4; ++kt) dh1[rt][kt] = mfma16(a, b[kt], dh1[rt][kt]);
+    }
+  }
+  // dz1 = dH1 (1 - h1^2) with h1 recomputed in fp32; dW1 and db1 partials
+  // per (lane group, column), reduced over the four groups
+  __syncthreads();
+  for (int kt = 0; kt < 4; ++kt) {
+    const int k = kt * 16 + r16;
+    float sd = 0.f, sw[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f};
+    float wk[kMlpIn];
+    for (int i = 0; i < kMlpIn; ++i) wk[i] = bfr(blob[kMlpW1 + k * kMlpIn + i]);
+    for (int rt = 0; rt < 4; ++rt)
+      for (int r = 0; r < 4; ++r) {
+        const int zr = rt * 16 + k4 + r;
+        float zk = blob[kMlpB1 + k];
+        for (int i = 0; i < kMlpIn; ++i) zk += wk[i] * xs[zr][i];
+        const float h = tanhf(zk);
+        const float dz = dh1[rt][kt][r] * (1.f - h * h);
+        sd += dz;
+        for (int i = 0; i < kMlpIn; ++i) sw[i] += dz * xs[zr][i];
+      }
+    for (int i = 0; i < kMlpIn; ++i) red[grp][k][i] = sw[i];
+    red[grp][k][kMlpIn] = sd;
+  }
+  __syncthreads();
+  for (int i = 0; i < kMlpIn; ++i)
+    out[kMlpW1 + l * kMlpIn + i] = red[0][l][i] + red[1][l][i] + red[2][l][i] + red[3][l][i];
+  out[kMlpB1 + l] = red[0][l][kMlpIn] + red[1][l][kMlpIn] + red[2][l][kMlpIn] + red[3][l][kMlpIn];
+}
+
+// Sum the per-workgroup partials in a fixed order (deterministic) and take
+// one Adam step (Kingma & Ba 2015) on the fp32 parameters.
+__global__ __launch_bounds__(256) void k_mlp_adam(int nw, const float *partial, float *blob,
+                                                  float *m, float *v, float lr, float b1,
+                                                  float b2, float eps, float c1, float c2,
+                                                  float *loss, float *grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > kMlpNParam) return;
+  float s = 0.f;
+  for (int w = 0; w < nw; ++w) s += partial[(int64_t)w * kMlpPartial + p];
+  if (p == kMlpNParam) {
+    if (loss) *loss = s;
+    return;
+  }
+  if (grad) grad[p] = s;
+  if (lr == 0.f) return;
+  const float mp = b1 * m[p] + (1.f - b1) * s;
+  const float vp = b2 * v[p] + (1.f - b2) * s * s;
+  m[p] = mp;
+  v[p] = vp;
+  blob[p] -= lr * (mp / c1) / (sqrtf(vp / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// Con2prim from the network's guess: Noble from (Z, v^2) of the network for
+// at most kguess iterations; zones it does not converge run the full chain
+// from the plain seed (fallback[z] = 1), with iters counting both.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_c2p_batch_mlp(int64_t nz, const double *cons,
+                                                       const double *g, const double *seed,
+                                                       const float *blob, int kguess,
+                                                       double *prim, double *flag,
+                                                       int32_t *iters, int32_t *fallback,
+                                                       IdealGas eos, C2PArgs a) {
+  __shared__ uint16_t w2s[kMlpH * kTS];
+  __shared__ uint16_t act[4][kMlpH * kTS];
+  stage_w2(blob, w2s, nullptr);
+  __syncthreads();
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = z < nz;
+  float xn[5], y[2];
+  zone_inputs(live, z, nz, cons, g, blob, xn);
+  mlp_wave(blob, w2s, act[threadIdx.x >> 6], xn, y);
+  if (!live) return;
+  C2PIn in;
+  in.r = 0.0;
+  in.alp = 1.0;
+  in.mask = 1.0;
+  in.glo = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  in.cv.dens = cons[z];
+  in.cv.mom[0] = cons[nz + z];
+  in.cv.mom[1] = cons[2 * nz + z];
+  in.cv.mom[2] = cons[3 * nz + z];
+  in.cv.tau = cons[4 * nz + z];
+  in.cv.dB[0] = cons[5 * nz + z];
+  in.cv.dB[1] = cons[6 * nz + z];
+  in.cv.dB[2] = cons[7 * nz + z];
+  in.cv.DYe = in.cv.dens * a.Ye_atmo;
+  in.cv.DEnt = 0.0;
+  in.saved_rho = seed[z];
+  in.saved_eps = seed[nz + z];
+  in.saved_Ye = a.Ye_atmo;
+  in.saved_vel[0] = seed[2 * nz + z];
+  in.saved_vel[1] = seed[3 * nz + z];
+  in.saved_vel[2] = seed[4 * nz + z];
+  in.temperature = 0.0;
+  const double lw = (double)(y[0] * blob[kMlpOutStd] + blob[kMlpOutMean]);
+  const double lz = (double)(y[1] * blob[kMlpOutStd + 1] + blob[kMlpOutMean + 1]);
+  const double W = smax(1.0, exp(lw));
+  const double vsq = 1.0 - 1.0 / (W * W);
+  const double D = in.cv.dens / sqrt(det3(in.glo));
+  double Zg = D * exp(lz);
+  if (!(Zg > 0.0) || !isfinite(Zg) || !isfinite(vsq)) Zg = -1.0; // plain seed instead
+  C2PArgs ak = a;
+  ak.max_iter = kguess;
+  C2POut o;
+  int fb = 0, it;
+  if (con2prim_zone<ASX_C2P_NOBLE>(eos, ak, in, o, Zg, vsq)) {
+    it = o.iters;
+  } else {
+    const int spent = o.iters;
+    con2prim_cell(eos, a, in, o);
+    it = spent + o.iters;
+    fb = 1;
+  }
+  prim[z] = o.pv.rho;
+  prim[nz + z] = o.pv.eps;
+  prim[2 * nz + z] = o.pv.press;
+  prim[3 * nz + z] = o.pv.vel[0];
+  prim[4 * nz + z] = o.pv.vel[1];
+  prim[5 * nz + z] = o.pv.vel[2];
+  prim[6 * nz + z] = o.pv.w_lor;
+  prim[7 * nz + z] = o.pv.temperature;
+  flag[z] = o.flag;
+  iters[z] = it;
+  if (fallback) fallback[z] = fb;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static dim3 grid256(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+hipError_t launch_zone_sample(int64_t nz, uint64_t seed, double gamma, double seed_rel,
+                              double *prim, double *g, double *sd, double *target,
+                              hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zone_sample, grid256(nz), dim3(256), 0, s, nz, seed, gamma, seed_rel,
+                     prim, g, sd, target);
+  return hipGetLastError();
+}
+hipError_t launch_prim2con_batch(int64_t nz, const double *prim, const double *g,
+                                 double *cons, double *cold, const IdealGas &eos, double Ye,
+                                 hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prim2con_batch, grid256(nz), dim3(256), 0, s, nz, prim, g, cons, cold,
+                     eos, Ye);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
+                               hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_features, grid256(nz), dim3(256), 0, s, nz, cons, g, x);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
+                              const float *blob, double *out, hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_forward, grid256(nz), dim3(256), 0, s, nz, cons, g, blob, out);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_grad(int64_t nz, const double *cons, const double *g,
+                           const double *target, const float *blob, float *partial,
+                           hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)((nz + 63) / 64)), dim3(64), 0, s, nz, cons, g,
+                     target, blob, partial);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float *m,
+                           float *v, float lr, float b1, float b2, float eps, int64_t step,
+                           float *loss, float *grad, hipStream_t s) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_mlp_adam, grid256(kMlpNParam + 1), dim3(256), 0, s, nwaves, partial, blob,
+                     m, v, lr, b1, b2, eps, c1, c2, loss, grad);
+  return hipGetLastError();
+}
+hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
+                                const double *seed, const float *blob, int kguess,
+                                double *prim, double *flag, int32_t *iters,
+                                int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
+                                hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_c2p_batch_mlp, grid256(nz), dim3(256), 0, s, nz, cons, g, seed, blob,
+                     kguess, prim, flag, iters, fallback, eos, a);
+  return hipGetLastError();
+}
+
+} // namespace asx

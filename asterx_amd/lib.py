@@ -43,6 +43,15 @@ SIGNATURES = {
     "asx_substep": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, _V],
                     C.c_int),
     "asx_c2p_batch": ([C.c_int64, _V, _V, _V, _V, _V, _V, _PP, _V], C.c_int),
+    "asx_zone_sample": ([C.c_int64, C.c_uint64, C.c_double, C.c_double, _V, _V, _V, _V,
+                         _V], C.c_int),
+    "asx_prim2con_batch": ([C.c_int64, _V, _V, _V, _V, _PP, _V], C.c_int),
+    "asx_mlp_features": ([C.c_int64, _V, _V, _V, _V], C.c_int),
+    "asx_mlp_forward": ([C.c_int64, _V, _V, _V, _V, _V], C.c_int),
+    "asx_mlp_train_step": ([C.c_int64, _V, _V, _V, _V, _V, _V, C.c_float, C.c_int64, _V,
+                            _V, _V, _V], C.c_int),
+    "asx_c2p_batch_mlp": ([C.c_int64, _V, _V, _V, _V, C.c_int32, _V, _V, _V, _V, _PP, _V],
+                          C.c_int),
     "asx_params_default": ([_PP], None),
     "asx_gf_centering": ([C.c_int], C.c_int),
     "asx_gf_strides": ([C.POINTER(C.c_int32), C.c_int32, C.c_int,

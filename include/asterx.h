@@ -267,7 +267,8 @@ enum {
   ASX_KC_RHS = 8,    /* unfused source terms / RHS */
   ASX_KC_OTHER = 9,
   ASX_KC_C2P_REST = 10, /* con2prim: zones the primary solver failed */
-  ASX_KC_COUNT = 11
+  ASX_KC_MLP = 11,      /* learned con2prim guess: inference and training */
+  ASX_KC_COUNT = 12
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
@@ -383,6 +384,63 @@ asx_status asx_con2prim_group(const asx_patch *patch, const asx_params *par,
 asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
                          const double *seed, double *prim, double *flag,
                          int32_t *iters, const asx_params *par, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Independent zones and the learned con2prim guess (no counterpart in the  */
+/* reference: workload generation and the network around the Noble solver)  */
+/* ------------------------------------------------------------------------ */
+/* Zones of the learned-guess workload: rho log-uniform in [1e-10, 1e-2],
+ * p/rho log-uniform in [1e-6, 1e2], W uniform in [1, 10] along a random
+ * direction, b^2/rho log-uniform in [1e-4, 1e2] along a random direction, a
+ * symmetric 3-metric within 10% of the identity; ideal gas `gamma`.
+ * prim[8][nz] = rho, eps, v^i, B^i; g[6][nz]; seed_out[5][nz] (optional) =
+ * rho, eps, v^i of the truth times (1 + seed_rel u), u ~ U[-1, 1];
+ * target[2][nz] (optional) = log W, log(rho h W^2 / (rho W)).  Zone z
+ * depends only on (seed, z). */
+asx_status asx_zone_sample(int64_t nz, uint64_t seed, double gamma, double seed_rel,
+                           double *prim, double *g, double *seed_out, double *target,
+                           void *stream);
+/* Valencia prim -> cons of independent zones (Con2PrimFactory/src/cons.hxx:
+ * 49-91, W from v): prim[8][nz] as above, cons[8][nz] = dens, S_i, tau, dB^i
+ * (densitized, the asx_c2p_batch layout); cold_seed[5][nz] (optional) = the
+ * cold-start seed rho = D, eps = tau/D, v = 0. */
+asx_status asx_prim2con_batch(int64_t nz, const double *prim, const double *g, double *cons,
+                              double *cold_seed, const asx_params *par, void *stream);
+
+/* The network: 5 inputs (log D, log S^2, log tau, log B^2, cos(S,B)) ->
+ * 64 tanh -> 64 tanh -> 2 outputs (log W, log(Z/D), Z = rho h W^2), on a
+ * float32 blob of ASX_MLP_NBLOB values: W1[64][5] b1[64] W2[64][64] b2[64]
+ * W3[2][64] b3[2] (ASX_MLP_NPARAM parameters), then the input means and
+ * inverse standard deviations (5 + 5) and output means and standard
+ * deviations (2 + 2).  Products take bf16-rounded operands with float32
+ * accumulation; the 64x64 layer runs on the matrix cores. */
+#define ASX_MLP_NPARAM 4674
+#define ASX_MLP_NBLOB 4688
+/* x[5][nz] (float32): the raw inputs (before normalisation). */
+asx_status asx_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
+                            void *stream);
+/* out[2][nz]: the network's outputs in target units (log W, log(Z/D)). */
+asx_status asx_mlp_forward(int64_t nz, const double *cons, const double *g, const float *blob,
+                           double *out, void *stream);
+/* One training step on a batch of nz zones: gradient of the mean squared
+ * error of the normalised outputs against target[2][nz], then an Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8, bias-corrected with `step` >= 1) on the
+ * parameters in blob with moments m, v (ASX_MLP_NPARAM floats each).
+ * lr = 0: gradient only (m, v untouched).  partial: caller scratch of
+ * ceil(nz/64) * (ASX_MLP_NPARAM + 1) floats.  loss (1 float) and grad
+ * (ASX_MLP_NPARAM floats) are optional device outputs.  Deterministic. */
+asx_status asx_mlp_train_step(int64_t nz, const double *cons, const double *g,
+                              const double *target, float *blob, float *m, float *v, float lr,
+                              int64_t step, float *partial, float *loss, float *grad,
+                              void *stream);
+/* asx_c2p_batch with the network's guess: Noble (c2p_prime must be Noble)
+ * from the network's (Z, v^2) for at most kguess iterations; a zone not
+ * converged by then runs the full chain from the plain seed, as
+ * asx_c2p_batch would (fallback[z] = 1, optional).  iters counts both. */
+asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
+                             const double *seed, const float *blob, int32_t kguess,
+                             double *prim, double *flag, int32_t *iters, int32_t *fallback,
+                             const asx_params *par, void *stream);
 
 #ifdef __cplusplus
 }

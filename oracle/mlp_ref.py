@@ -1,0 +1,130 @@
+"""CPU reference of the learned con2prim-guess network.
+
+Test infrastructure only: the HIP kernels' inference and gradient are
+checked against it.  Same network, parameter layout and arithmetic as the
+kernels -- inputs computed in double precision from the densitized conserved
+variables, every product with bf16-rounded operands and float32
+accumulation, tanh activations -- and the gradient of the mean squared error
+of the normalised outputs with bf16 rounding taken as the identity in the
+derivatives (bf16=True).  With bf16=False every rounding is dropped and the
+arithmetic is float64, so the backpropagation itself can be checked against
+finite differences.
+"""
+import numpy as np
+
+NIN, NH, NOUT = 5, 64, 2
+W1 = 0
+B1 = W1 + NH * NIN
+W2 = B1 + NH
+B2 = W2 + NH * NH
+W3 = B2 + NH
+B3 = W3 + NOUT * NH
+NPARAM = B3 + NOUT
+IN_MEAN = NPARAM
+IN_ISTD = IN_MEAN + NIN
+OUT_MEAN = IN_ISTD + NIN
+OUT_STD = OUT_MEAN + NOUT
+NBLOB = OUT_STD + NOUT
+
+
+def bf16(x):
+    """float32 -> nearest bf16 (ties to even), returned as float32."""
+    a = np.asarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    out = r.astype(np.uint32).view(np.float32)
+    return np.where(np.isnan(a), np.float32(np.nan), out)
+
+
+def features(cons, g):
+    """x[nz, 5]: log D, log S^2, log tau, log B^2, cos(S, B) (float64)."""
+    gxx, gxy, gxz, gyy, gyz, gzz = g
+    det = 2 * gxy * gxz * gyz + gxx * gyy * gzz - gyy * gxz ** 2 - \
+        gxx * gyz ** 2 - gzz * gxy ** 2
+    sq = np.sqrt(det)
+    inv = [(gyy * gzz - gyz ** 2) / det, (gxz * gyz - gxy * gzz) / det,
+           (gxy * gyz - gxz * gyy) / det, (gxx * gzz - gxz ** 2) / det,
+           (gxy * gxz - gxx * gyz) / det, (gxx * gyy - gxy ** 2) / det]
+    D = cons[0] / sq
+    S = cons[1:4] / sq
+    tau = cons[4] / sq
+    B = cons[5:8] / sq
+
+    def quad(m, a, b):
+        xx, xy, xz, yy, yz, zz = m
+        return (a[0] * (xx * b[0] + xy * b[1] + xz * b[2]) +
+                a[1] * (xy * b[0] + yy * b[1] + yz * b[2]) +
+                a[2] * (xz * b[0] + yz * b[1] + zz * b[2]))
+    S2 = quad(inv, S, S)
+    B2 = quad(g, B, B)
+    SB = S[0] * B[0] + S[1] * B[1] + S[2] * B[2]
+    fl = 1e-30 * D * D
+    den = np.sqrt(S2 * B2)
+    cosv = np.where(den > 0, SB / np.where(den > 0, den, 1.0), 0.0)
+    return np.stack([np.log(D), np.log(S2 + fl),
+                     np.log(np.maximum(tau, 1e-15 * D)), np.log(B2 + fl), cosv],
+                    axis=1)
+
+
+def _params(blob, bf):
+    f = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
+    dt = np.float32 if bf else np.float64
+    p = dict(W1=f(blob[W1:B1].reshape(NH, NIN)), b1=blob[B1:W2].astype(dt),
+             W2=f(blob[W2:B2].reshape(NH, NH)), b2=blob[B2:W3].astype(dt),
+             W3=f(blob[W3:B3].reshape(NOUT, NH)), b3=blob[B3:NPARAM].astype(dt))
+    return p
+
+
+def normalized_inputs(blob, cons, g, bf=True):
+    x = features(cons, g)
+    dt = np.float32 if bf else np.float64
+    x = x.astype(np.float32) if bf else x
+    xn = (x - blob[IN_MEAN:IN_ISTD].astype(dt)) * blob[IN_ISTD:OUT_MEAN].astype(dt)
+    return xn.astype(dt)
+
+
+def forward(blob, cons, g, bf=True):
+    """Normalised outputs y[nz, 2] and the cache for backward."""
+    r = bf16 if bf else (lambda a: a)
+    p = _params(blob, bf)
+    xb = r(normalized_inputs(blob, cons, g, bf))
+    z1 = xb @ p["W1"].T + p["b1"]
+    h1 = np.tanh(z1)
+    h1b = r(h1)
+    z2 = h1b @ p["W2"].T + p["b2"]
+    h2 = np.tanh(z2)
+    h2b = r(h2)
+    y = h2b @ p["W3"].T + p["b3"]
+    return y, dict(p=p, xb=xb, h1=h1, h1b=h1b, h2=h2, h2b=h2b)
+
+
+def normalized_targets(blob, target, bf=True):
+    dt = np.float32 if bf else np.float64
+    t = np.asarray(target).T.astype(dt)
+    return (t - blob[OUT_MEAN:OUT_STD].astype(dt)) / blob[OUT_STD:NBLOB].astype(dt)
+
+
+def loss_and_grad(blob, cons, g, target, bf=True):
+    """Mean squared error of the normalised outputs and its gradient with
+    respect to the NPARAM parameters (flat, blob order)."""
+    r = bf16 if bf else (lambda a: a)
+    y, c = forward(blob, cons, g, bf)
+    tn = normalized_targets(blob, target, bf)
+    n = y.shape[0]
+    e = y - tn
+    loss = float((e * e).sum() / n)
+    dy = 2.0 * e / n
+    p = c["p"]
+    gW3 = dy.T @ c["h2b"]
+    gb3 = dy.sum(0)
+    dh2 = dy @ p["W3"]
+    dz2 = dh2 * (1 - c["h2"] ** 2)
+    dz2b = r(dz2)
+    gW2 = dz2b.T @ c["h1b"]
+    gb2 = dz2.sum(0)
+    dh1 = dz2b @ p["W2"]
+    dz1 = dh1 * (1 - c["h1"] ** 2)
+    gW1 = dz1.T @ c["xb"]
+    gb1 = dz1.sum(0)
+    grad = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+    return loss, grad

@@ -1,0 +1,182 @@
+"""Learned con2prim guess on the GPU: the workload sampler and batched
+prim2con against the oracle, the matrix-core forward pass and training
+gradient against the CPU reference (oracle/mlp_ref.py), training reducing
+the loss, and con2prim from the guess against plain con2prim."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp_ref as M
+from asterx_amd import abi
+
+pytestmark = pytest.mark.gpu
+
+GAMMA = 5.0 / 3.0
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _L():
+    from asterx_amd import lib
+    return lib.lib()
+
+
+def _par(**kw):
+    base = dict(gl_gamma=GAMMA, eps_max=1e8, rho_max=1e8, rho_abs_min=1e-12,
+                vw_lim=1e3, B_lim=1e3, c2p_tol=1e-10)
+    base.update(kw)
+    return abi.params(**base)
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _zones(nz, seed, par):
+    from asterx_amd.mlp import Zones
+    z = Zones(_L(), nz, seed, par, GAMMA)
+    _torch().cuda.synchronize()
+    return z
+
+
+def test_sampler_and_prim2con():
+    torch = _torch()
+    par = _par()
+    nz = 20000
+    z = _zones(nz, 11, par)
+    z2 = _zones(nz, 11, par)
+    assert torch.equal(z.cons, z2.cons)  # zone z depends only on (seed, z)
+    prim = z.prim.cpu().numpy()
+    g = np.ascontiguousarray(z.g.cpu().numpy())
+    rho, eps = prim[0], prim[1]
+    press = (GAMMA - 1) * rho * eps
+    assert rho.min() >= 1e-10 and rho.max() <= 1e-2
+    assert (press / rho).min() >= 1e-6 * 0.999 and (press / rho).max() <= 1e2 * 1.001
+    W = np.exp(z.target.cpu().numpy()[0])
+    assert W.min() >= 1.0 and W.max() <= 10.0
+    out = np.zeros((8, nz))
+    abi.check("orc_prim2con_batch", oracle.lib().orc_prim2con_batch(
+        nz, np.ascontiguousarray(prim).ctypes.data, g.ctypes.data, out.ctypes.data,
+        C.byref(par)))
+    cons = z.cons.cpu().numpy()
+    scale = np.abs(out).max(1, keepdims=True)
+    assert np.max(np.abs(cons - out) / scale) < 1e-14
+
+
+def _rand_blob(seed=0):
+    rng = np.random.default_rng(seed)
+    b = np.zeros(M.NBLOB, dtype=np.float32)
+    b[:M.NPARAM] = rng.normal(0, 0.25, M.NPARAM)
+    b[M.IN_MEAN:M.IN_ISTD] = [-12.0, -18.0, -10.0, -16.0, 0.0]
+    b[M.IN_ISTD:M.OUT_MEAN] = [0.25, 0.1, 0.2, 0.1, 1.5]
+    b[M.OUT_MEAN:M.OUT_STD] = [1.5, 2.5]
+    b[M.OUT_STD:M.NBLOB] = [0.5, 1.2]
+    return b
+
+
+def test_forward_matches_cpu_reference():
+    """Pins the matrix-core fragment layout: a wrong lane mapping scrambles
+    the 64x64 layer and the outputs."""
+    torch = _torch()
+    par = _par()
+    nz = 3000  # not a multiple of 64: partial last wavefront
+    z = _zones(nz, 5, par)
+    blob = _rand_blob()
+    out = torch.empty((2, nz), dtype=torch.float64, device="cuda")
+    bt = torch.from_numpy(blob).cuda()
+    abi.check("asx_mlp_forward", _L().asx_mlp_forward(nz, _p(z.cons), _p(z.g), _p(bt),
+                                                      _p(out), None))
+    torch.cuda.synchronize()
+    y, _ = M.forward(blob, z.cons.cpu().numpy(), z.g.cpu().numpy())
+    ref = y * blob[M.OUT_STD:M.NBLOB] + blob[M.OUT_MEAN:M.OUT_STD]
+    d = np.abs(out.cpu().numpy().T - ref)
+    # fp32 accumulation order and tanh ulps may flip a bf16 rounding
+    assert np.percentile(d, 99.5) < 2e-3 and d.max() < 5e-2, (np.percentile(d, 99.5), d.max())
+
+
+def test_gradient_matches_cpu_reference():
+    torch = _torch()
+    par = _par()
+    nz = 1000
+    z = _zones(nz, 6, par)
+    blob = _rand_blob(1)
+    bt = torch.from_numpy(blob).cuda()
+    nw = (nz + 63) // 64
+    partial = torch.empty(nw * (M.NPARAM + 1), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    grad = torch.zeros(M.NPARAM, dtype=torch.float32, device="cuda")
+    abi.check("asx_mlp_train_step", _L().asx_mlp_train_step(
+        nz, _p(z.cons), _p(z.g), _p(z.target), _p(bt), None, None, C.c_float(0.0), 1,
+        _p(partial), _p(loss), _p(grad), None))
+    torch.cuda.synchronize()
+    lref, gref = M.loss_and_grad(blob, z.cons.cpu().numpy(), z.g.cpu().numpy(),
+                                 z.target.cpu().numpy())
+    gd = grad.cpu().numpy()
+    assert abs(loss.item() - lref) <= 1e-3 * lref
+    for lo, hi, name in ((M.W1, M.B1, "W1"), (M.B1, M.W2, "b1"), (M.W2, M.B2, "W2"),
+                         (M.B2, M.W3, "b2"), (M.W3, M.B3, "W3"), (M.B3, M.NPARAM, "b3")):
+        e = np.linalg.norm(gd[lo:hi] - gref[lo:hi]) / np.linalg.norm(gref[lo:hi])
+        assert e < 2e-2, (name, e)
+    # gradient only: the parameters are untouched
+    assert np.array_equal(bt.cpu().numpy(), blob)
+
+
+def test_training_reduces_loss():
+    _torch()
+    from asterx_amd import mlp
+    par = _par()
+    blob, hist = mlp.train(_L(), par, GAMMA, samples=300 * 4096, batch=4096, lr=3e-3,
+                           lr_end=3e-4, seed=7, log_every=50, log=None)
+    first, last = hist[0][1], hist[-1][1]
+    assert last < 0.2 * first, hist
+    assert last < 0.05, hist
+
+
+def test_c2p_from_guess_matches_plain():
+    """Zones that fall back run the reference chain from the plain seed and
+    agree bit for bit with asx_c2p_batch; zones converged from the guess
+    agree with it to the solver tolerance, with the same flags."""
+    torch = _torch()
+    par = _par()
+    nz = 20000
+    z = _zones(nz, 9, par)
+    L = _L()
+    blob = torch.from_numpy(_rand_blob(2)).cuda()
+    f64 = dict(dtype=torch.float64, device="cuda")
+    i32 = dict(dtype=torch.int32, device="cuda")
+    res = {}
+    for mode in ("plain", "guess"):
+        prim = torch.zeros((8, nz), **f64)
+        flag = torch.zeros(nz, **f64)
+        it = torch.zeros(nz, **i32)
+        fb = torch.zeros(nz, **i32)
+        if mode == "plain":
+            abi.check("asx_c2p_batch", L.asx_c2p_batch(
+                nz, _p(z.cons), _p(z.g), _p(z.seed), _p(prim), _p(flag), _p(it),
+                C.byref(par), None))
+        else:
+            abi.check("asx_c2p_batch_mlp", L.asx_c2p_batch_mlp(
+                nz, _p(z.cons), _p(z.g), _p(z.seed), _p(blob), 6, _p(prim), _p(flag),
+                _p(it), _p(fb), C.byref(par), None))
+        torch.cuda.synchronize()
+        res[mode] = (prim.cpu().numpy(), flag.cpu().numpy(), it.cpu().numpy(),
+                     fb.cpu().numpy())
+    pp, fp, ip, _ = res["plain"]
+    pg, fg, ig, fbg = res["guess"]
+    f = fbg == 1
+    assert f.any() and (~f).any()
+    assert np.array_equal(pg[:, f], pp[:, f]) and np.array_equal(fg[f], fp[f])
+    ok = (~f) & (fp == 1)
+    assert np.array_equal(fg[ok], fp[ok])
+    rel = np.abs(pg[:, ok] - pp[:, ok]) / (np.abs(pp[:, ok]) + 1e-300)
+    assert np.nanmax(rel[[0, 1, 2, 6]]) < 1e-6
+    # velocities: compare against the Lorentz factor scale
+    assert np.nanmax(np.abs(pg[3:6, ok] - pp[3:6, ok])) < 1e-6

@@ -1,0 +1,88 @@
+"""CPU reference of the learned con2prim guess, pinned by mathematics:
+its backpropagation agrees with central finite differences of its own loss
+(float64, rounding off), its inputs are scale-covariant, and the oracle's
+prim2con of the workload's zones reproduces the numpy generator's cons."""
+import ctypes as C
+
+import numpy as np
+
+import oracle
+from oracle import mlp_ref as M
+from asterx_amd import abi, zones
+
+
+def _blob(rng):
+    b = np.zeros(M.NBLOB)
+    b[:M.NPARAM] = rng.normal(0, 0.3, M.NPARAM)
+    b[M.IN_MEAN:M.IN_ISTD] = [-12.0, -20.0, -12.0, -15.0, 0.0]
+    b[M.IN_ISTD:M.OUT_MEAN] = [0.2, 0.1, 0.2, 0.1, 1.0]
+    b[M.OUT_MEAN:M.OUT_STD] = [1.5, 2.0]
+    b[M.OUT_STD:M.NBLOB] = [0.5, 1.0]
+    return b
+
+
+def test_backprop_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    cons, g, _, true = zones.make_batch(40, seed=1)
+    target = np.stack([np.log(true["W"]),
+                       np.log((1 + true["eps"] + true["press"] / true["rho"]) * true["W"])])
+    blob = _blob(rng)
+    loss, grad = M.loss_and_grad(blob, cons, g, target, bf=False)
+    idx = np.concatenate([rng.choice(np.arange(M.W1, M.B1), 8),
+                          rng.choice(np.arange(M.B1, M.W2), 4),
+                          rng.choice(np.arange(M.W2, M.B2), 16),
+                          rng.choice(np.arange(M.B2, M.W3), 4),
+                          rng.choice(np.arange(M.W3, M.B3), 6), [M.B3, M.B3 + 1]])
+    h = 1e-6
+    for p in idx:
+        bp, bm = blob.copy(), blob.copy()
+        bp[p] += h
+        bm[p] -= h
+        lp, _ = M.loss_and_grad(bp, cons, g, target, bf=False)
+        lm, _ = M.loss_and_grad(bm, cons, g, target, bf=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - grad[p]) <= 1e-6 * (abs(fd) + 1e-3), (p, fd, grad[p])
+
+
+def test_features_scale_covariant():
+    """Scaling every conserved variable by s shifts log D, log tau by log s
+    and log S^2, log B^2 by 2 log s; the cosine is unchanged."""
+    cons, g, _, _ = zones.make_batch(100, seed=2)
+    x = M.features(cons, g)
+    s = 3.7
+    cons2 = cons.copy()
+    cons2[:5] *= s
+    cons2[5:] *= np.sqrt(s)  # B^2 scales with energy density
+    x2 = M.features(cons2, g)
+    ls = np.log(s)
+    assert np.allclose(x2[:, 0] - x[:, 0], ls, atol=1e-12)
+    assert np.allclose(x2[:, 1] - x[:, 1], 2 * ls, atol=1e-9)
+    assert np.allclose(x2[:, 2] - x[:, 2], ls, atol=1e-12)
+    assert np.allclose(x2[:, 3] - x[:, 3], ls, atol=1e-9)
+    assert np.allclose(x2[:, 4], x[:, 4], atol=1e-12)
+
+
+def test_bf16_rounding():
+    """Nearest bf16, ties to even: 1 + 2^-8 (half-way between 1 and
+    1 + 2^-7) rounds down to the even 1.0, 1 + 3*2^-8 rounds up to the even
+    1 + 2^-6; below half-way rounds down."""
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -3.14159],
+                 np.float32)
+    r = M.bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == 1.0
+    assert abs(r[4] + 3.140625) < 1e-7
+    assert np.all((r.view(np.uint32) & 0xFFFF) == 0)
+
+
+def test_oracle_prim2con_matches_generator():
+    """The workload generator's cons (numpy restatement) equal the oracle's
+    batched prim2con of the same primitives."""
+    nz = 500
+    cons, g, _, true = zones.make_batch(nz, seed=3)
+    prim = np.ascontiguousarray(np.concatenate(
+        [true["rho"][None], true["eps"][None], true["vel"], true["B"]]))
+    out = np.zeros((8, nz))
+    par = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8)
+    abi.check("orc_prim2con_batch", oracle.lib().orc_prim2con_batch(
+        nz, prim.ctypes.data, g.ctypes.data, out.ctypes.data, C.byref(par)))
+    assert np.max(np.abs(out - cons) / (np.abs(cons).max(1, keepdims=True))) < 1e-13
