@@ -114,3 +114,99 @@ def load(path=DEFAULT_WEIGHTS):
 def save(blob, path=DEFAULT_WEIGHTS):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.save(path, np.asarray(blob, dtype=np.float32), allow_pickle=False)
+
+
+def evaluate(L, par, gamma, blob, nz=1 << 20, seed=99, kguess=8, reps=3, stream=None):
+    """Plain and learned-guess con2prim on nz fresh zones, for the perturbed
+    seed (previous-step primitives) and the cold start: zones/s (median of
+    reps timed runs), iteration histograms, fallback rate, accuracy."""
+    import torch
+    z = Zones(L, nz, seed, par, gamma, stream=stream)
+    bt = torch.from_numpy(np.asarray(blob, dtype=np.float32)).cuda()
+    f64 = dict(dtype=torch.float64, device="cuda")
+    prim = torch.empty((8, nz), **f64)
+    flag = torch.empty(nz, **f64)
+    it = torch.empty(nz, dtype=torch.int32, device="cuda")
+    fb = torch.empty(nz, dtype=torch.int32, device="cuda")
+    out = {}
+    for sname, sd in (("perturbed", z.seed), ("cold", z.cold)):
+        for mode in ("plain", "mlp"):
+            def run():
+                if mode == "plain":
+                    return L.asx_c2p_batch(nz, _p(z.cons), _p(z.g), _p(sd), _p(prim),
+                                           _p(flag), _p(it), C.byref(par), stream)
+                return L.asx_c2p_batch_mlp(nz, _p(z.cons), _p(z.g), _p(sd), _p(bt), kguess,
+                                           _p(prim), _p(flag), _p(it), _p(fb), C.byref(par),
+                                           stream)
+            abi.check("c2p", run())
+            ts = []
+            for _ in range(reps):
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                abi.check("c2p", run())
+                e1.record()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            ms = sorted(ts)[len(ts) // 2]
+            iters = it.cpu().numpy()
+            fl = flag.cpu().numpy()
+            rho = prim[0].cpu().numpy()
+            rho_true = z.prim[0].cpu().numpy()
+            r = dict(zones_per_s=nz / (ms * 1e-3), ms=ms,
+                     iters_mean=float(iters.mean()),
+                     iters_hist={int(k): int(v) for k, v in
+                                 zip(*np.unique(iters, return_counts=True))},
+                     flags={int(k): int(v) for k, v in
+                            zip(*np.unique(fl, return_counts=True))},
+                     rho_rel_err_max=float(np.max(np.abs(rho - rho_true) / rho_true)))
+            if mode == "mlp":
+                r["fallback_rate"] = float(fb.cpu().numpy().mean())
+            out["%s_%s" % (sname, mode)] = r
+    # accuracy of the guess itself
+    g2 = torch.empty((2, nz), **f64)
+    abi.check("asx_mlp_forward", L.asx_mlp_forward(nz, _p(z.cons), _p(z.g), _p(bt), _p(g2),
+                                                   stream))
+    torch.cuda.synchronize()
+    t = z.target.cpu().numpy()
+    gh = g2.cpu().numpy()
+    out["guess_abs_err"] = dict(logW_median=float(np.median(np.abs(gh[0] - t[0]))),
+                                logZD_median=float(np.median(np.abs(gh[1] - t[1]))),
+                                logW_p99=float(np.percentile(np.abs(gh[0] - t[0]), 99)),
+                                logZD_p99=float(np.percentile(np.abs(gh[1] - t[1]), 99)))
+    return out
+
+
+def main():
+    import argparse
+    import json
+    import sys
+    import time
+    from . import lib
+    ap = argparse.ArgumentParser(description="train / evaluate the con2prim guess")
+    ap.add_argument("--samples", type=float, default=1e7)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--eval-only", action="store_true")
+    ap.add_argument("--eval-zones", type=int, default=1 << 20)
+    ap.add_argument("--kguess", type=int, default=8)
+    args = ap.parse_args()
+    L = lib.lib()
+    par = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8, rho_abs_min=1e-12,
+                     vw_lim=1e3, B_lim=1e3, c2p_tol=1e-10)
+    if args.eval_only:
+        blob = load(args.out)
+    else:
+        t0 = time.time()
+        blob, hist = train(L, par, 5.0 / 3.0, samples=int(args.samples), batch=args.batch,
+                           lr=args.lr, log=lambda s: print(s, file=sys.stderr, flush=True))
+        save(blob, args.out)
+        print(json.dumps({"trained": args.out, "seconds": round(time.time() - t0, 1),
+                          "loss": hist}), flush=True)
+    print(json.dumps(evaluate(L, par, 5.0 / 3.0, blob, nz=args.eval_zones,
+                              kguess=args.kguess)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
