@@ -128,6 +128,20 @@ ASX_DEV void store_zone(const KPatch &P, int oc, const C2POut &o) {
   P.g[ASX_SAVED_YE][oc] = pv.Ye;
 }
 
+// Append `value` of every work-item with `fail` set to the queue: a
+// wavefront ballot, one atomic per wavefront, lanes in order.
+ASX_DEV void queue_push(bool fail, int value, int *queue, unsigned *qcount) {
+  const uint64_t m = __ballot(fail);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (fail) queue[base + __popcll(m & ((1ull << lane) - 1))] = value;
+  }
+}
+
 // Pass 1: every zone through the primary solver (compiled for PRIME only,
 // so the backup solvers cost no registers here).  Zones whose primary solve
 // fails store nothing; a wavefront ballot packs their indices into `queue`
@@ -160,15 +174,7 @@ __global__ __launch_bounds__(256) void k_c2p_main(KPatch P, Box B, IdealGas eos,
       fail = true;
     }
   }
-  const uint64_t m = __ballot(fail);
-  if (m) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __builtin_ctzll(m);
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
-    if (fail) queue[base + __popcll(m & ((1ull << lane) - 1))] = (int)t;
-  }
+  queue_push(fail, (int)t, queue, qcount);
   if (hist) {
     __syncthreads();
     hist_flush(h, hist);
@@ -208,6 +214,68 @@ __global__ __launch_bounds__(256) void k_c2p_rest(KPatch P, Box B, IdealGas eos,
 }
 
 __global__ void k_zero_u32(unsigned *p) { *p = 0u; }
+
+// Batched forms over many patches.  Zone t of patch p (x-fastest over the
+// interior) has the global index zbase[p] + t.
+ASX_DEV void interior_ijk(const KPatch &P, int t, int &i, int &j, int &k) {
+  i = P.ng + t % P.n[0];
+  j = P.ng + (t / P.n[0]) % P.n[1];
+  k = P.ng + t / (P.n[0] * P.n[1]);
+}
+// patch of global zone z: zbase[p] <= z < zbase[p + 1]
+ASX_DEV int patch_of(const int *zbase, int np, int z) {
+  int lo = 0, hi = np;
+  while (hi - lo > 1) {
+    const int m = (lo + hi) >> 1;
+    if (zbase[m] <= z) lo = m;
+    else hi = m;
+  }
+  return lo;
+}
+
+template <int PRIME>
+__global__ __launch_bounds__(256) void k_c2p_main_batch(const KPatch *__restrict__ tab,
+                                                        const int *__restrict__ zbase,
+                                                        IdealGas eos, C2PArgs a,
+                                                        const BatchRef *__restrict__ map,
+                                                        int nmap, int *queue,
+                                                        unsigned *qcount) {
+  const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
+  const KPatch &P = tab[r.pid];
+  const int t = r.chunk * 256 + (int)threadIdx.x;
+  bool fail = false;
+  if (t < P.n[0] * P.n[1] * P.n[2]) {
+    int i, j, k;
+    interior_ijk(P, t, i, j, k);
+    C2PIn in;
+    load_zone<true>(P, i, j, k, a, in);
+    C2POut o;
+    if (con2prim_zone<PRIME>(eos, a, in, o)) store_zone(P, off(P, 0, i, j, k), o);
+    else fail = true;
+  }
+  queue_push(fail, zbase[r.pid] + t, queue, qcount);
+}
+
+__global__ __launch_bounds__(256) void k_c2p_rest_batch(const KPatch *__restrict__ tab,
+                                                        const int *__restrict__ zbase, int np,
+                                                        IdealGas eos, C2PArgs a,
+                                                        const int *queue,
+                                                        const unsigned *qcount) {
+  const unsigned nq = *qcount;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += gridDim.x * blockDim.x) {
+    const int z = queue[q];
+    const int p = patch_of(zbase, np, z);
+    const KPatch &P = tab[p];
+    int i, j, k;
+    interior_ijk(P, z - zbase[p], i, j, k);
+    C2PIn in;
+    load_zone<true>(P, i, j, k, a, in);
+    C2POut o;
+    con2prim_cell(eos, a, in, o);
+    store_zone(P, off(P, 0, i, j, k), o);
+  }
+}
 
 // Independent zones (layout of asx_c2p_batch in include/asterx.h).
 __global__ __launch_bounds__(256) void k_c2p_batch(int64_t nz, const double *cons,
@@ -296,6 +364,31 @@ hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PA
     hipLaunchKernelGGL(k_c2p_rest<true>, dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
   else
     hipLaunchKernelGGL(k_c2p_rest<false>, dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
+  return hipGetLastError();
+}
+
+hipError_t launch_con2prim_main_batch(const KPatch *tab, const int *zbase, int np,
+                                      const IdealGas &eos, const C2PArgs &a,
+                                      const BatchRef *map, int nmap, int *queue,
+                                      unsigned *qcount, hipStream_t s) {
+  (void)np;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, qcount);
+  if (nmap <= 0) return hipGetLastError();
+  const dim3 grid((unsigned)nmap), block(256);
+  if (a.prime == ASX_C2P_NOBLE)
+    hipLaunchKernelGGL(k_c2p_main_batch<ASX_C2P_NOBLE>, grid, block, 0, s, tab, zbase, eos, a, map, nmap, queue, qcount);
+  else
+    hipLaunchKernelGGL(k_c2p_main_batch<ASX_C2P_PALENZUELA>, grid, block, 0, s, tab, zbase, eos, a, map, nmap, queue, qcount);
+  return hipGetLastError();
+}
+
+hipError_t launch_con2prim_rest_batch(const KPatch *tab, const int *zbase, int np,
+                                      const IdealGas &eos, const C2PArgs &a, int64_t nzones,
+                                      const int *queue, const unsigned *qcount,
+                                      hipStream_t s) {
+  if (nzones <= 0 || np <= 0) return hipSuccess;
+  const unsigned g2 = (unsigned)((nzones + 255) / 256 < 2048 ? (nzones + 255) / 256 : 2048);
+  hipLaunchKernelGGL(k_c2p_rest_batch, dim3(g2), dim3(256), 0, s, tab, zbase, np, eos, a, queue, qcount);
   return hipGetLastError();
 }
 

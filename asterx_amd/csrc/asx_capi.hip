@@ -289,12 +289,15 @@ asx_status drain_timing() {
   return ASX_OK;
 }
 
+// Set while this thread records a graph: launches are then not timed.
+thread_local bool g_capturing = false;
+
 struct KTimer {
   int dev = 0, cls;
   hipStream_t s;
   hipEvent_t a = nullptr, b = nullptr;
   KTimer(int c, hipStream_t st) : cls(c), s(st) {
-    if (!g_timing.load(std::memory_order_relaxed)) return;
+    if (g_capturing || !g_timing.load(std::memory_order_relaxed)) return;
     std::lock_guard<std::mutex> l(g_tmu);
     (void)hipGetDevice(&dev);
     a = take_event(dev);
@@ -1054,6 +1057,566 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
     return launch_c2p_batch_mlp(nz, cons, g, seed, blob, kguess, prim, flag, iters, fallback,
                                 ideal_gas(par), c2p_args(par), (hipStream_t)stream);
   }));
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched patches
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kSyncPrims[11] = {ASX_RHO,     ASX_VELX, ASX_VELY,  ASX_VELZ,
+                                ASX_EPS,     ASX_PRESS, ASX_ENTROPY, ASX_YE,
+                                ASX_BVECX,   ASX_BVECY, ASX_BVECZ};
+// SSPRK3 stages: out = c0 u0 + c1 u + c2 dt rhs(u)
+constexpr double kRK3[3][3] = {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25},
+                               {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+constexpr int kStageIn[3] = {0, 1, 2}, kStageOut[3] = {1, 2, 0};
+
+// Launch kinds of a plan, each with its per-patch boxes and workgroup list.
+enum BatchKind {
+  BK_BSTAG = 0,  // + d: dB_stag scratch over the whole array
+  BK_FACE = 3,   // + d: faces, tiles of the face variant
+  BK_AUX = 6,    // vertices, two layers into the ghosts
+  BK_UPDATE = 7, // interior vertices, tiles of the update variant
+  BK_BC = 8,     // + d: ghost layers normal to d
+  BK_UBSTAG = 11, // + d: interior faces of the caller's dB_stag
+  BK_C2P = 14,   // interior cells
+  BK_COUNT = 15
+};
+
+struct BatchLaunch {
+  BatchBox *box = nullptr;
+  BatchRef *map = nullptr;
+  int nmap = 0;
+};
+
+} // namespace
+
+struct asx_batch_s {
+  int dev = 0;
+  int np = 0;
+  asx_params par;
+  double dt = 0;
+  int bc_kind = 0;
+  bool user_bstag = false;
+  std::vector<asx_patch> pt; // caller's descriptors (state of buffer 0)
+  std::vector<void *> allocs;
+  int64_t bytes = 0;
+  int64_t nzones = 0;
+  KPatch *stab[3] = {}; // stage views: scratch slots substituted, state of buffer k
+  KPatch *utab[3] = {}; // caller views, state of buffer k
+  StatePtrs *st[3] = {};
+  int *zbase = nullptr;
+  int *queue = nullptr;
+  unsigned *qcount = nullptr;
+  std::map<std::pair<int, int>, BatchLaunch> launches; // (kind, variant)
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_var[4] = {-1, -1, -1, -1};
+};
+
+namespace {
+
+hipError_t plan_alloc(asx_batch *b, size_t bytes, void **p) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) {
+    b->allocs.push_back(*p);
+    b->bytes += (int64_t)bytes;
+  }
+  return e;
+}
+
+template <class T>
+asx_status plan_upload(asx_batch *b, const std::vector<T> &v, T **dst) {
+  void *p = nullptr;
+  if (plan_alloc(b, v.size() * sizeof(T), &p) != hipSuccess) return ASX_ERR_DEVICE;
+  if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) !=
+                        hipSuccess)
+    return ASX_ERR_DEVICE;
+  *dst = (T *)p;
+  return ASX_OK;
+}
+
+// Boxes and workgroup list of launch kind k (variant v for faces / update).
+asx_status plan_launch(asx_batch *b, int k, int v, BatchLaunch **out) {
+  const auto key = std::make_pair(k, v);
+  auto it = b->launches.find(key);
+  if (it != b->launches.end()) {
+    *out = &it->second;
+    return ASX_OK;
+  }
+  std::vector<BatchBox> boxes(b->np);
+  std::vector<BatchRef> refs;
+  for (int p = 0; p < b->np; ++p) {
+    const asx_patch &pt = b->pt[p];
+    BatchBox &bb = boxes[p];
+    int tile[3] = {0, 0, 0}; // 0: 256-point chunks of the flattened box
+    if (k >= BK_BSTAG && k < BK_BSTAG + 3) {
+      const int d = k - BK_BSTAG;
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = 0;
+        bb.cnt[q] = pt.n[q] + 2 * pt.ng + (q == d);
+      }
+    } else if (k >= BK_FACE && k < BK_FACE + 3) {
+      const int d = k - BK_FACE;
+      int halo[3] = {1, 1, 1};
+      halo[d] = 0;
+      const Box bx = box(&pt, 1 << d, halo);
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = bx.lo[q];
+        bb.cnt[q] = bx.cnt[q];
+      }
+      face_tile_extent(d, v, tile);
+      tile[d] -= 1; // tiles along d overlap by one cell
+    } else if (k == BK_AUX) {
+      const int halo[3] = {2, 2, 2};
+      const Box bx = box(&pt, 7, halo);
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = bx.lo[q];
+        bb.cnt[q] = bx.cnt[q];
+      }
+    } else if (k == BK_UPDATE) {
+      const int halo[3] = {0, 0, 0};
+      const Box bx = box(&pt, 7, halo);
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = bx.lo[q];
+        bb.cnt[q] = bx.cnt[q];
+      }
+      update_tile_extent(v, tile);
+    } else if (k >= BK_BC && k < BK_BC + 3) {
+      const int d = k - BK_BC; // ghost points normal to d, widest centering
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = 0;
+        bb.cnt[q] = pt.n[q] + 2 * pt.ng + 1;
+      }
+      bb.cnt[d] = 2 * pt.ng;
+    } else if (k >= BK_UBSTAG && k < BK_UBSTAG + 3) {
+      const int d = k - BK_UBSTAG;
+      const int halo[3] = {0, 0, 0};
+      const Box bx = box(&pt, 1 << d, halo);
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = bx.lo[q];
+        bb.cnt[q] = bx.cnt[q];
+      }
+    } else { // BK_C2P
+      for (int q = 0; q < 3; ++q) {
+        bb.lo[q] = pt.ng;
+        bb.cnt[q] = pt.n[q];
+      }
+    }
+    int64_t chunks;
+    if (tile[0] > 0) {
+      for (int q = 0; q < 3; ++q) bb.nt[q] = (bb.cnt[q] + tile[q] - 1) / tile[q];
+      chunks = (int64_t)bb.nt[0] * bb.nt[1] * bb.nt[2];
+    } else {
+      const int64_t tot = (int64_t)bb.cnt[0] * bb.cnt[1] * bb.cnt[2];
+      chunks = (tot + 255) / 256;
+      bb.nt[0] = (int)chunks;
+      bb.nt[1] = bb.nt[2] = 1;
+    }
+    for (int64_t c = 0; c < chunks; ++c) refs.push_back(BatchRef{p, (int)c});
+  }
+  if (refs.size() >= (size_t(1) << 31)) return ASX_ERR_SIZE;
+  BatchLaunch L;
+  L.nmap = (int)refs.size();
+  asx_status st;
+  if ((st = plan_upload(b, boxes, &L.box)) || (st = plan_upload(b, refs, &L.map))) return st;
+  *out = &(b->launches[key] = L);
+  return ASX_OK;
+}
+
+FluxArgs plan_flux_args(const asx_params *par) {
+  FluxArgs A;
+  A.rp = rparams(par);
+  A.eos = ideal_gas(par);
+  A.lo_method = par->loworder_method;
+  A.flux_type = par->flux_type;
+  A.rec_var = par->recon_type;
+  A.with_T = par->reconstruct_with_temperature;
+  A.write_uct = 0;
+  A.cnt_dev = nullptr;
+  for (int q = 0; q < 3; ++q) A.lo[q] = A.cnt[q] = 0;
+  return A;
+}
+
+// Stage-s stencil work: dB_stag, faces, aux, update into buffer (1,2,0)[s].
+asx_status plan_stencil(asx_batch *b, int s, hipStream_t str) {
+  const int bi = kStageIn[s], bo = kStageOut[s];
+  const KPatch *T = b->stab[bi];
+  BatchLaunch *L;
+  asx_status st;
+  for (int d = 0; d < 3; ++d) {
+    if ((st = plan_launch(b, BK_BSTAG + d, 0, &L))) return st;
+    if ((st = dev_status(timed(ASX_KC_BSTAG, str, [&] {
+           return launch_bstag_batch(d, T, L->box, L->map, L->nmap, str);
+         }))))
+      return st;
+  }
+  const FluxArgs A = plan_flux_args(&b->par);
+  for (int d = 0; d < 3; ++d) {
+    const int v = variant_of(ASX_KC_FACE_X + d);
+    if ((st = plan_launch(b, BK_FACE + d, v, &L))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, str, [&] {
+           return launch_face_batch(d, T, A, L->box, L->map, L->nmap,
+                                    b->par.reconstruction_method, v, str);
+         }))))
+      return st;
+  }
+  if ((st = plan_launch(b, BK_AUX, 0, &L))) return st;
+  if ((st = dev_status(timed(ASX_KC_AUX, str, [&] {
+         return launch_aux_batch(T, L->box, L->map, L->nmap, str);
+       }))))
+    return st;
+  const int v = variant_of(ASX_KC_UPDATE);
+  if ((st = plan_launch(b, BK_UPDATE, v, &L))) return st;
+  const RhsArgs R = rhs_args(&b->par);
+  return dev_status(timed(ASX_KC_UPDATE, str, [&] {
+    return launch_update_batch(T, b->st[0], b->st[bo], R, kRK3[s][0], kRK3[s][1],
+                               kRK3[s][2] * b->dt, L->box, L->map, L->nmap, v, str);
+  }));
+}
+
+asx_status plan_bc(asx_batch *b, const KPatch *T, const int *slots, int n, hipStream_t str) {
+  BCSlots S;
+  S.nf = n;
+  for (int q = 0; q < n; ++q) {
+    S.slot[q] = slots[q];
+    S.cent[q] = centering(slots[q]);
+  }
+  for (int d = 0; d < 3; ++d) {
+    BatchLaunch *L;
+    asx_status st;
+    if ((st = plan_launch(b, BK_BC + d, 0, &L))) return st;
+    if ((st = dev_status(timed(ASX_KC_BC, str, [&] {
+           return launch_bc_batch(d, T, S, L->map, L->nmap, b->bc_kind, str);
+         }))))
+      return st;
+  }
+  return ASX_OK;
+}
+
+asx_status plan_stage(asx_batch *b, int s, hipStream_t str) {
+  asx_status st;
+  if ((st = plan_stencil(b, s, str))) return st;
+  const KPatch *U = b->utab[kStageOut[s]];
+  if ((st = plan_bc(b, U, kStateSlots, ASX_NUM_STATE, str))) return st;
+  BatchLaunch *L;
+  if (b->user_bstag)
+    for (int d = 0; d < 3; ++d) {
+      if ((st = plan_launch(b, BK_UBSTAG + d, 0, &L))) return st;
+      if ((st = dev_status(timed(ASX_KC_BSTAG, str, [&] {
+             return launch_bstag_batch(d, U, L->box, L->map, L->nmap, str);
+           }))))
+        return st;
+    }
+  if ((st = plan_launch(b, BK_C2P, 0, &L))) return st;
+  const IdealGas eos = ideal_gas(&b->par);
+  const C2PArgs ca = c2p_args(&b->par);
+  if ((st = dev_status(timed(ASX_KC_C2P, str, [&] {
+         return launch_con2prim_main_batch(U, b->zbase, b->np, eos, ca, L->map, L->nmap,
+                                           b->queue, b->qcount, str);
+       }))))
+    return st;
+  if ((st = dev_status(timed(ASX_KC_C2P_REST, str, [&] {
+         return launch_con2prim_rest_batch(U, b->zbase, b->np, eos, ca, b->nzones, b->queue,
+                                           b->qcount, str);
+       }))))
+    return st;
+  return plan_bc(b, U, kSyncPrims, 11, str);
+}
+
+// Build every launch list the stages use with the current variants (no
+// allocation may happen while a graph is recorded).
+asx_status plan_prepare(asx_batch *b) {
+  BatchLaunch *L;
+  asx_status st;
+  for (int k = 0; k < BK_COUNT; ++k) {
+    int v = 0;
+    if (k >= BK_FACE && k < BK_FACE + 3) v = variant_of(ASX_KC_FACE_X + (k - BK_FACE));
+    if (k == BK_UPDATE) v = variant_of(ASX_KC_UPDATE);
+    if ((st = plan_launch(b, k, v, &L))) return st;
+  }
+  return ASX_OK;
+}
+
+void plan_drop_graph(asx_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+asx_status plan_capture(asx_batch *b) {
+  int var[4];
+  for (int q = 0; q < 3; ++q) var[q] = variant_of(ASX_KC_FACE_X + q);
+  var[3] = variant_of(ASX_KC_UPDATE);
+  if (b->exec && std::equal(var, var + 4, b->graph_var)) return ASX_OK;
+  plan_drop_graph(b);
+  asx_status st = plan_prepare(b);
+  if (st) return st;
+  if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return ASX_ERR_DEVICE;
+  g_capturing = true;
+  for (int s = 0; s < 3 && !st; ++s) st = plan_stage(b, s, b->cap);
+  g_capturing = false;
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(b->cap, &g);
+  if (st || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st ? st : ASX_ERR_DEVICE;
+  }
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    b->exec = nullptr;
+    plan_drop_graph(b);
+    return ASX_ERR_DEVICE;
+  }
+  std::copy(var, var + 4, b->graph_var);
+  return ASX_OK;
+}
+
+// a plan runs on the device it was created on
+asx_status plan_device(const asx_batch *b) {
+  int d = -1;
+  if (hipGetDevice(&d) != hipSuccess) return ASX_ERR_DEVICE;
+  return d == b->dev ? ASX_OK : ASX_ERR_PARAM;
+}
+
+void plan_free(asx_batch *b) {
+  plan_drop_graph(b);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (void *p : b->allocs) (void)hipFree(p);
+  delete b;
+}
+
+// asx_substep + asx_con2prim_group requirements of one patch
+asx_status plan_check_patch(const asx_patch *pt, const asx_params *par) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (pt->ng < 3 || pt->ng < recon_ghosts(par->reconstruction_method)) return ASX_ERR_GHOSTS;
+  if ((st = need_range(pt, ASX_ALP, ASX_KZZ))) return st;
+  if ((st = need_range(pt, ASX_RHO, ASX_DBZ))) return st;
+  if ((st = need_range(pt, ASX_SAVED_RHO, ASX_CON2PRIM_FLAG))) return st;
+  if ((st = opt(pt, ASX_MASK_CC))) return st;
+  if (par->use_mask && (st = need(pt, {ASX_MASK_VC}))) return st;
+  if ((st = need(pt, {ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI}))) return st;
+  return ASX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_state *bufs,
+                            const asx_params *par, double dt, int32_t bc_kind,
+                            asx_batch **plan) {
+  if (!plan) return ASX_ERR_NULL;
+  *plan = nullptr;
+  if (!patches || !bufs || !par) return ASX_ERR_NULL;
+  if (npatch < 1) return ASX_ERR_SIZE;
+  asx_status st;
+  if ((st = check_c2p_par(par))) return st;
+  if (par->use_uct) return ASX_ERR_UNSUPPORTED;
+  if (bc_kind < ASX_BC_LINEAR || bc_kind > ASX_BC_COPY) return ASX_ERR_PARAM;
+  if (!std::isfinite(dt)) return ASX_ERR_PARAM;
+  bool ubst = patches[0].gf[ASX_DBX_STAG].data != nullptr;
+  int64_t zones = 0;
+  for (int p = 0; p < npatch; ++p) {
+    const asx_patch *pt = &patches[p];
+    if ((st = plan_check_patch(pt, par))) return st;
+    if (par->reconstruct_with_temperature && (st = need(pt, {ASX_TEMPERATURE}))) return st;
+    if ((pt->gf[ASX_DBX_STAG].data != nullptr) != ubst) return ASX_ERR_NULL;
+    if (ubst && (st = need_range(pt, ASX_DBX_STAG, ASX_DBZ_STAG))) return st;
+    for (int d = 0; d < 3; ++d) {
+      if (bc_kind == ASX_BC_LINEAR && pt->n[d] < 2) return ASX_ERR_SIZE;
+      if (bc_kind == ASX_BC_PERIODIC && pt->n[d] < pt->ng) return ASX_ERR_SIZE;
+    }
+    for (int k = 0; k < 2; ++k) {
+      const asx_state &S = bufs[2 * p + k];
+      for (int q = 0; q < ASX_NUM_STATE; ++q) {
+        int64_t sj, sk;
+        strides(pt->n, pt->ng, centering(kStateSlots[q]), sj, sk);
+        if (!S.gf[q].data) return ASX_ERR_NULL;
+        if (S.gf[q].sj != sj || S.gf[q].sk != sk) return ASX_ERR_LAYOUT;
+      }
+    }
+    zones += (int64_t)pt->n[0] * pt->n[1] * pt->n[2];
+  }
+  if (zones >= (int64_t(1) << 31)) return ASX_ERR_SIZE;
+  asx_batch *b = new asx_batch;
+  b->np = npatch;
+  b->par = *par;
+  b->dt = dt;
+  b->bc_kind = bc_kind;
+  b->user_bstag = ubst;
+  b->nzones = zones;
+  b->pt.assign(patches, patches + npatch);
+  st = ASX_OK;
+  auto fail = [&](asx_status e) {
+    plan_free(b);
+    return e;
+  };
+  if (hipGetDevice(&b->dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
+    return fail(ASX_ERR_DEVICE);
+  // one scratch pool: face fluxes, aux and dB_stag of every patch
+  std::vector<int> sslots;
+  for (int q = ASX_FXDENS; q <= ASX_FZBZ; ++q) sslots.push_back(q);
+  for (int q = ASX_FX; q <= ASX_G; ++q) sslots.push_back(q);
+  for (int q = ASX_DBX_STAG; q <= ASX_DBZ_STAG; ++q) sslots.push_back(q);
+  size_t pool = 0;
+  for (int p = 0; p < npatch; ++p)
+    for (int q : sslots)
+      pool += ((size_t)npoints(patches[p].n, patches[p].ng, centering(q)) * 8 + 255) & ~size_t(255);
+  void *base = nullptr;
+  if (plan_alloc(b, pool, &base) != hipSuccess) return fail(ASX_ERR_DEVICE);
+  std::vector<KPatch> stab[3], utab[3];
+  std::vector<StatePtrs> sp[3];
+  char *cur = (char *)base;
+  for (int p = 0; p < npatch; ++p) {
+    KPatch U = kpatch(&patches[p]);
+    KPatch S = U;
+    for (int q : sslots) {
+      S.g[q] = (double *)cur;
+      cur += ((size_t)npoints(patches[p].n, patches[p].ng, centering(q)) * 8 + 255) &
+             ~size_t(255);
+    }
+    for (int k = 0; k < 3; ++k) {
+      StatePtrs P;
+      for (int q = 0; q < ASX_NUM_STATE; ++q)
+        P.g[q] = k == 0 ? patches[p].gf[kStateSlots[q]].data : bufs[2 * p + k - 1].gf[q].data;
+      KPatch Sk = S, Uk = U;
+      for (int q = 0; q < ASX_NUM_STATE; ++q) Sk.g[kStateSlots[q]] = Uk.g[kStateSlots[q]] = P.g[q];
+      stab[k].push_back(Sk);
+      utab[k].push_back(Uk);
+      sp[k].push_back(P);
+    }
+  }
+  std::vector<int> zb(npatch + 1, 0);
+  for (int p = 0; p < npatch; ++p)
+    zb[p + 1] = zb[p] + patches[p].n[0] * patches[p].n[1] * patches[p].n[2];
+  for (int k = 0; k < 3 && !st; ++k)
+    if (!(st = plan_upload(b, stab[k], &b->stab[k])) &&
+        !(st = plan_upload(b, utab[k], &b->utab[k])))
+      st = plan_upload(b, sp[k], &b->st[k]);
+  if (!st) st = plan_upload(b, zb, &b->zbase);
+  if (st) return fail(st);
+  void *q = nullptr, *qc = nullptr;
+  if (plan_alloc(b, (size_t)zones * sizeof(int), &q) != hipSuccess ||
+      plan_alloc(b, sizeof(unsigned), &qc) != hipSuccess)
+    return fail(ASX_ERR_DEVICE);
+  b->queue = (int *)q;
+  b->qcount = (unsigned *)qc;
+  if ((st = plan_prepare(b))) return fail(st);
+  *plan = b;
+  return ASX_OK;
+}
+
+asx_status asx_batch_stage(asx_batch *plan, int32_t stage, void *stream) {
+  if (!plan) return ASX_ERR_NULL;
+  if (stage < 0 || stage > 2) return ASX_ERR_PARAM;
+  if (asx_status st = plan_device(plan)) return st;
+  return plan_stage(plan, stage, (hipStream_t)stream);
+}
+
+asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, void *stream) {
+  if (!plan) return ASX_ERR_NULL;
+  if (nsteps < 0) return ASX_ERR_PARAM;
+  hipStream_t s = (hipStream_t)stream;
+  asx_status st = plan_device(plan);
+  if (st) return st;
+  if (!use_graph) {
+    for (int n = 0; n < nsteps && !st; ++n)
+      for (int k = 0; k < 3 && !st; ++k) st = plan_stage(plan, k, s);
+    return st;
+  }
+  if ((st = plan_capture(plan))) return st;
+  for (int n = 0; n < nsteps && !st; ++n)
+    st = dev_status(timed(ASX_KC_GRAPH, s, [&] { return hipGraphLaunch(plan->exec, s); }));
+  return st;
+}
+
+asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream) {
+  if (!plan) return ASX_ERR_NULL;
+  if (reps < 1) return ASX_ERR_PARAM;
+  if (asx_status st = plan_device(plan)) return st;
+  hipStream_t s = (hipStream_t)stream;
+  const int tuned[4] = {ASX_KC_FACE_X, ASX_KC_FACE_Y, ASX_KC_FACE_Z, ASX_KC_UPDATE};
+  int nv = 1;
+  for (int kc : tuned) nv = variant_count(kc) > nv ? variant_count(kc) : nv;
+  bool was_on;
+  double tms[ASX_KC_COUNT];
+  int64_t tn[ASX_KC_COUNT];
+  {
+    std::lock_guard<std::mutex> l(g_tmu);
+    asx_status st = drain_timing();
+    if (st) return st;
+    was_on = g_timing.load();
+    for (int q = 0; q < ASX_KC_COUNT; ++q) {
+      tms[q] = g_tms[q];
+      tn[q] = g_tn[q];
+    }
+  }
+  double best[4];
+  int arg[4];
+  for (int q = 0; q < 4; ++q) {
+    best[q] = 1e300;
+    arg[q] = variant_of(tuned[q]);
+  }
+  asx_status st = ASX_OK;
+  for (int v = 0; v < nv && !st; ++v) {
+    for (int kc : tuned) set_variant(kc, v < variant_count(kc) ? v : 0);
+    if ((st = plan_stencil(plan, 0, s))) break;
+    if ((st = asx_timing_enable(1))) break;
+    for (int r = 0; r < reps && !st; ++r) st = plan_stencil(plan, 0, s);
+    for (int q = 0; q < 4 && !st; ++q) {
+      if (v >= variant_count(tuned[q])) continue;
+      int64_t n = 0;
+      double ms = 0;
+      if ((st = asx_timing_read(tuned[q], &n, &ms))) break;
+      if (n > 0 && ms / n < best[q]) {
+        best[q] = ms / n;
+        arg[q] = v;
+      }
+    }
+  }
+  for (int q = 0; q < 4; ++q) set_variant(tuned[q], arg[q]);
+  {
+    std::lock_guard<std::mutex> l(g_tmu);
+    asx_status st2 = drain_timing();
+    for (int q = 0; q < ASX_KC_COUNT; ++q) {
+      g_tms[q] = tms[q];
+      g_tn[q] = tn[q];
+    }
+    g_timing.store(was_on);
+    if (!st) st = st2;
+  }
+  if (!st) st = plan_prepare(plan);
+  return st;
+}
+
+asx_status asx_batch_info(const asx_batch *plan, int64_t *zones, int64_t *device_bytes) {
+  if (!plan) return ASX_ERR_NULL;
+  if (zones) *zones = plan->nzones;
+  if (device_bytes) *device_bytes = plan->bytes;
+  return ASX_OK;
+}
+
+asx_status asx_batch_destroy(asx_batch *plan) {
+  if (!plan) return ASX_OK;
+  int cur = 0;
+  const int dev = plan->dev;
+  (void)hipGetDevice(&cur);
+  if (cur != dev) (void)hipSetDevice(dev);
+  const hipError_t e = hipDeviceSynchronize();
+  plan_free(plan);
+  if (cur != dev) (void)hipSetDevice(cur);
+  return e == hipSuccess ? ASX_OK : ASX_ERR_DEVICE;
 }
 
 } // extern "C"

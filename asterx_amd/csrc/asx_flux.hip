@@ -398,19 +398,17 @@ template <> struct FaceTileT<2, 1> { using T = Tile3<32, 1, 8>; };
 template <> struct FaceTileT<2, 2> { using T = Tile3<16, 2, 8>; };
 template <int D, int V> using FaceTile = typename FaceTileT<D, V>::T;
 
-#ifndef ASX_FACE_WAVES
-#define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
-#endif
+// One tile of a face box (lo, cnt), tile index b of a grid of ntx x nty x ntz
+// tiles (ordered x, y, z).  Every work-item of the workgroup calls it.
 template <int D, int M, int V>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES))) void k_face(KPatch P, FluxArgs A, int ntx, int nty,
-                                              int ntz) {
+ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], const int cnt[3],
+                       int b, int ntx, int nty) {
   using T = FaceTile<D, V>;
   static_assert(T::X * T::Y * T::Z == 256, "face tile");
   constexpr int TD = D == 0 ? T::X : (D == 1 ? T::Y : T::Z);
   constexpr int SD = D == 0 ? 1 : (D == 1 ? T::X : T::X * T::Y); // tile stride along D
   __shared__ double sh[kSideVars][256];
   __shared__ int shf[256];
-  const int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
   const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
   const int t = threadIdx.x;
   const int tx = t % T::X, ty = (t / T::X) % T::Y, tz = t / (T::X * T::Y);
@@ -421,8 +419,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WA
   const int fz = bz * (D == 2 ? TD - 1 : T::Z) + tz - (D == 2);
   const int td = D == 0 ? tx : (D == 1 ? ty : tz);
   const int fd = D == 0 ? fx : (D == 1 ? fy : fz);
-  const bool need = fx < A.cnt[0] && fy < A.cnt[1] && fz < A.cnt[2];
-  const int i = A.lo[0] + fx, j = A.lo[1] + fy, k = A.lo[2] + fz;
+  const bool need = fx < cnt[0] && fy < cnt[1] && fz < cnt[2];
+  const int i = lo3[0] + fx, j = lo3[1] + fy, k = lo3[2] + fz;
   Side lo, up;
   bool flo = false, fup = false;
   if (need) {
@@ -455,12 +453,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WA
   }
 }
 
-// CalcAuxForAvecPsi (AsterX/src/fluxes.cxx:803-825)
-__global__ __launch_bounds__(256) void k_aux(KPatch P, Box B) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B.total()) return;
-  int i, j, k;
-  B.ijk(t, i, j, k);
+#ifndef ASX_FACE_WAVES
+#define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
+#endif
+template <int D, int M, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES)))
+void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
+  face_tile<D, M, V>(P, A, A.lo, A.cnt, xcd_tile((int)blockIdx.x, ntx * nty * ntz), ntx, nty);
+}
+
+// CalcAuxForAvecPsi (AsterX/src/fluxes.cxx:803-825) at vertex (i, j, k)
+ASX_DEV void aux_point(const KPatch &P, int i, int j, int k) {
   const int ov = off(P, 7, i, j, k);
   double A[3];
   {
@@ -492,14 +495,17 @@ __global__ __launch_bounds__(256) void k_aux(KPatch P, Box B) {
   P.g[ASX_FBETAZ][ov] = beta[2] * psi;
   P.g[ASX_G][ov] = alp * psi / sqrtg - dot3(beta, A);
 }
-
-// ComputeStaggeredB<D> (AsterX/src/computeBfromA.cxx:32-55)
-template <int D>
-__global__ __launch_bounds__(256) void k_bstag(KPatch P, Box B, double *out) {
+__global__ __launch_bounds__(256) void k_aux(KPatch P, Box B) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= B.total()) return;
   int i, j, k;
   B.ijk(t, i, j, k);
+  aux_point(P, i, j, k);
+}
+
+// ComputeStaggeredB<D> (AsterX/src/computeBfromA.cxx:32-55)
+template <int D>
+ASX_DEV void bstag_point(const KPatch &P, int i, int j, int k, double *out) {
   const double *idx = P.idx;
   const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
   double r;
@@ -514,6 +520,14 @@ __global__ __launch_bounds__(256) void k_bstag(KPatch P, Box B, double *out) {
     r = idx[0] * (Ay[oy + 1] - Ay[oy]) - idx[1] * (Ax[ox + P.sj[6]] - Ax[ox]);
   }
   out[off(P, 1 << D, i, j, k)] = r;
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_bstag(KPatch P, Box B, double *out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= B.total()) return;
+  int i, j, k;
+  B.ijk(t, i, j, k);
+  bstag_point<D>(P, i, j, k, out);
 }
 
 // AsterX_ComputedBFromdBstag (computeBfromA.cxx:71-84)
@@ -642,6 +656,113 @@ hipError_t launch_B_from_dB(const KPatch &P, const Box &B, hipStream_t s) {
 }
 hipError_t launch_prim2con(const KPatch &P, const Box &B, hipStream_t s) {
   hipLaunchKernelGGL(k_prim2con, B.grid(), dim3(256), 0, s, P, B);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched forms: one launch over the listed (patch, chunk) workgroups of
+// many patches, per-patch views and boxes from device tables.
+// ---------------------------------------------------------------------------
+template <int D, int M, int V>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES)))
+void k_face_batch(const KPatch *__restrict__ tab, FluxArgs A, const BatchBox *__restrict__ box,
+                  const BatchRef *__restrict__ map, int nmap) {
+  const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
+  const BatchBox &bb = box[r.pid];
+  face_tile<D, M, V>(tab[r.pid], A, bb.lo, bb.cnt, r.chunk, bb.nt[0], bb.nt[1]);
+}
+
+ASX_DEV bool batch_point(const BatchBox &bb, int chunk, int &i, int &j, int &k) {
+  const int t = chunk * 256 + (int)threadIdx.x;
+  const int nxy = bb.cnt[0] * bb.cnt[1];
+  if (t >= nxy * bb.cnt[2]) return false;
+  i = bb.lo[0] + t % bb.cnt[0];
+  j = bb.lo[1] + (t / bb.cnt[0]) % bb.cnt[1];
+  k = bb.lo[2] + t / nxy;
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_aux_batch(const KPatch *__restrict__ tab,
+                                                   const BatchBox *__restrict__ box,
+                                                   const BatchRef *__restrict__ map, int nmap) {
+  const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
+  int i, j, k;
+  if (batch_point(box[r.pid], r.chunk, i, j, k)) aux_point(tab[r.pid], i, j, k);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_bstag_batch(const KPatch *__restrict__ tab,
+                                                     const BatchBox *__restrict__ box,
+                                                     const BatchRef *__restrict__ map, int nmap) {
+  const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
+  int i, j, k;
+  if (batch_point(box[r.pid], r.chunk, i, j, k)) {
+    const KPatch &P = tab[r.pid];
+    bstag_point<D>(P, i, j, k, P.g[ASX_DBX_STAG + D]);
+  }
+}
+
+template <int D, int V>
+static void launch_face_batch_dv(const KPatch *tab, const FluxArgs &A, const BatchBox *box,
+                                 const BatchRef *map, int nmap, int method, hipStream_t s) {
+  const dim3 grid((unsigned)nmap), block(256);
+  switch (method) {
+  case ASX_RECON_GODUNOV: hipLaunchKernelGGL((k_face_batch<D, ASX_RECON_GODUNOV, V>), grid, block, 0, s, tab, A, box, map, nmap); break;
+  case ASX_RECON_MINMOD: hipLaunchKernelGGL((k_face_batch<D, ASX_RECON_MINMOD, V>), grid, block, 0, s, tab, A, box, map, nmap); break;
+  case ASX_RECON_MONOCENTRAL: hipLaunchKernelGGL((k_face_batch<D, ASX_RECON_MONOCENTRAL, V>), grid, block, 0, s, tab, A, box, map, nmap); break;
+  case ASX_RECON_PPM: hipLaunchKernelGGL((k_face_batch<D, ASX_RECON_PPM, V>), grid, block, 0, s, tab, A, box, map, nmap); break;
+  case ASX_RECON_WENOZ: hipLaunchKernelGGL((k_face_batch<D, ASX_RECON_WENOZ, V>), grid, block, 0, s, tab, A, box, map, nmap); break;
+  default: hipLaunchKernelGGL((k_face_batch<D, ASX_RECON_MP5, V>), grid, block, 0, s, tab, A, box, map, nmap); break;
+  }
+}
+template <int D>
+static void launch_face_batch_d(const KPatch *tab, const FluxArgs &A, const BatchBox *box,
+                                const BatchRef *map, int nmap, int method, int variant,
+                                hipStream_t s) {
+  if (variant == 1) launch_face_batch_dv<D, 1>(tab, A, box, map, nmap, method, s);
+  else if (variant == 2) launch_face_batch_dv<D, 2>(tab, A, box, map, nmap, method, s);
+  else launch_face_batch_dv<D, 0>(tab, A, box, map, nmap, method, s);
+}
+
+template <int D, int V> static void face_ext(int ex[3]) {
+  using T = FaceTile<D, V>;
+  ex[0] = T::X;
+  ex[1] = T::Y;
+  ex[2] = T::Z;
+}
+template <int D> static void face_ext_d(int variant, int ex[3]) {
+  if (variant == 1) face_ext<D, 1>(ex);
+  else if (variant == 2) face_ext<D, 2>(ex);
+  else face_ext<D, 0>(ex);
+}
+void face_tile_extent(int d, int variant, int ex[3]) {
+  if (d == 0) face_ext_d<0>(variant, ex);
+  else if (d == 1) face_ext_d<1>(variant, ex);
+  else face_ext_d<2>(variant, ex);
+}
+
+hipError_t launch_face_batch(int d, const KPatch *tab, const FluxArgs &A, const BatchBox *box,
+                             const BatchRef *map, int nmap, int method, int variant,
+                             hipStream_t s) {
+  if (nmap <= 0) return hipSuccess;
+  if (d == 0) launch_face_batch_d<0>(tab, A, box, map, nmap, method, variant, s);
+  else if (d == 1) launch_face_batch_d<1>(tab, A, box, map, nmap, method, variant, s);
+  else launch_face_batch_d<2>(tab, A, box, map, nmap, method, variant, s);
+  return hipGetLastError();
+}
+hipError_t launch_aux_batch(const KPatch *tab, const BatchBox *box, const BatchRef *map,
+                            int nmap, hipStream_t s) {
+  if (nmap <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_aux_batch, dim3((unsigned)nmap), dim3(256), 0, s, tab, box, map, nmap);
+  return hipGetLastError();
+}
+hipError_t launch_bstag_batch(int d, const KPatch *tab, const BatchBox *box,
+                              const BatchRef *map, int nmap, hipStream_t s) {
+  if (nmap <= 0) return hipSuccess;
+  const dim3 grid((unsigned)nmap), block(256);
+  if (d == 0) hipLaunchKernelGGL(k_bstag_batch<0>, grid, block, 0, s, tab, box, map, nmap);
+  else if (d == 1) hipLaunchKernelGGL(k_bstag_batch<1>, grid, block, 0, s, tab, box, map, nmap);
+  else hipLaunchKernelGGL(k_bstag_batch<2>, grid, block, 0, s, tab, box, map, nmap);
   return hipGetLastError();
 }
 

@@ -20,6 +20,20 @@ struct Box {
   __host__ dim3 grid() const { return dim3((unsigned)((htotal() + 255) / 256)); }
 };
 
+// Point of this work-item in tile b of a grid of ntx x nty tiles of
+// TX x TY x TZ points over the box (lo, cnt); false past the box.
+template <int TX, int TY, int TZ>
+ASX_DEV bool tile_point(const int lo[3], const int cnt[3], int b, int ntx, int nty, int &i,
+                        int &j, int &k) {
+  const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+  const int t = (int)threadIdx.x;
+  const int x = bx * TX + t % TX, y = by * TY + (t / TX) % TY, z = bz * TZ + t / (TX * TY);
+  i = lo[0] + x;
+  j = lo[1] + y;
+  k = lo[2] + z;
+  return x < cnt[0] && y < cnt[1] && z < cnt[2];
+}
+
 // A box covered by workgroup tiles of TX x TY x TZ points (one work-item
 // each), tiles ordered x, y, z and spread over the XCDs in contiguous runs.
 // Deeper tiles (TZ > 1) read each shared vertex / face plane once per tile
@@ -35,13 +49,7 @@ template <int TX, int TY, int TZ = 1> struct TileGrid {
   // point of this work-item; false past the box
   ASX_DEV bool point(const Box &B, int &i, int &j, int &k) const {
     const int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
-    const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
-    const int t = (int)threadIdx.x;
-    const int x = bx * TX + t % TX, y = by * TY + (t / TX) % TY, z = bz * TZ + t / (TX * TY);
-    i = B.lo[0] + x;
-    j = B.lo[1] + y;
-    k = B.lo[2] + z;
-    return x < B.cnt[0] && y < B.cnt[1] && z < B.cnt[2];
+    return tile_point<TX, TY, TZ>(B.lo, B.cnt, b, ntx, nty, i, j, k);
   }
 };
 
@@ -49,6 +57,16 @@ template <int TX, int TY, int TZ = 1> struct TileGrid {
 // asx_set_variant / asx_autotune_substep in the C ABI).
 constexpr int kUpdateVariants = 5;
 constexpr int kFaceVariants = 3;
+
+// Batched launches over many patches: the workgroups of one launch are
+// listed as (patch, chunk) pairs, chunk = tile index or 256-point block of
+// that patch's box; the per-patch data come from device tables.
+struct BatchRef {
+  int pid, chunk;
+};
+struct BatchBox {
+  int lo[3], cnt[3], nt[3]; // box of points and its tile grid
+};
 
 struct FluxArgs {
   RParams rp;
@@ -105,6 +123,46 @@ hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PA
 hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
                                 bool from_A, unsigned long long *hist, const int *queue,
                                 const unsigned *qcount, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Batched launches over many patches (device tables, one entry per patch;
+// map = the (patch, chunk) list of the launch's workgroups)
+// ---------------------------------------------------------------------------
+// Tile extents (x, y, z work-items) of the launch-shape variants.
+void face_tile_extent(int d, int variant, int ex[3]);
+void update_tile_extent(int variant, int ex[3]);
+// Slots of a multi-field boundary fill shared by every patch.
+struct BCSlots {
+  int slot[kMaxBC], cent[kMaxBC];
+  int nf;
+};
+
+hipError_t launch_face_batch(int d, const KPatch *tab, const FluxArgs &A, const BatchBox *box,
+                             const BatchRef *map, int nmap, int method, int variant,
+                             hipStream_t s);
+hipError_t launch_aux_batch(const KPatch *tab, const BatchBox *box, const BatchRef *map,
+                            int nmap, hipStream_t s);
+// face B from the vector potential into each patch's dB_stag slot d
+hipError_t launch_bstag_batch(int d, const KPatch *tab, const BatchBox *box,
+                              const BatchRef *map, int nmap, hipStream_t s);
+hipError_t launch_update_batch(const KPatch *tab, const StatePtrs *u0, const StatePtrs *out,
+                               const RhsArgs &A, double c0, double c1, double c2,
+                               const BatchBox *box, const BatchRef *map, int nmap,
+                               int variant, hipStream_t s);
+// boundary fill along axis d; map chunks are 256-point blocks of the 2 ng
+// ghost layers normal to d, sized for the widest centering
+hipError_t launch_bc_batch(int d, const KPatch *tab, const BCSlots &L, const BatchRef *map,
+                           int nmap, int kind, hipStream_t s);
+// con2prim of the patch interiors; zbase[p] = first global zone of patch p
+// (np + 1 entries), the queue holds global zone indices
+hipError_t launch_con2prim_main_batch(const KPatch *tab, const int *zbase, int np,
+                                      const IdealGas &eos, const C2PArgs &a,
+                                      const BatchRef *map, int nmap, int *queue,
+                                      unsigned *qcount, hipStream_t s);
+hipError_t launch_con2prim_rest_batch(const KPatch *tab, const int *zbase, int np,
+                                      const IdealGas &eos, const C2PArgs &a, int64_t nzones,
+                                      const int *queue, const unsigned *qcount,
+                                      hipStream_t s);
 // ---------------------------------------------------------------------------
 // Independent zones: sampler, batched prim2con, learned con2prim guess
 // ---------------------------------------------------------------------------

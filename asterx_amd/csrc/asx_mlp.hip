@@ -138,8 +138,9 @@ __global__ __launch_bounds__(256) void k_prim2con_batch(int64_t nz, const double
 // Network
 // ---------------------------------------------------------------------------
 // Inputs from the densitized conserved variables: log D, log S^2, log tau,
-// log B^2 (undensitized, S^2 = gamma^ij S_i S_j, B^2 = gamma_ij B^i B^j) and
-// the cosine of S and B; small floors keep the logs finite.
+// log B^2 (undensitized, S^2 = gamma^ij S_i S_j, B^2 = gamma_ij B^i B^j) in
+// units of E = D + tau (so scale-free), and the cosine of S and B; small
+// floors keep the logs finite.
 ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, double x[5]) {
   const double detg = det3(gl), sq = sqrt(detg);
   const Sym3 gu = inv3(gl, detg);
@@ -152,10 +153,14 @@ ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, double x[5]) {
   lower(gl, B, Bl);
   const double S2 = dot3(S, Su), B2 = dot3(B, Bl), SB = dot3(S, B);
   const double floor2 = 1e-30 * D * D;
-  x[0] = log(D);
-  x[1] = log(S2 + floor2);
-  x[2] = log(tau > 1e-15 * D ? tau : 1e-15 * D);
-  x[3] = log(B2 + floor2);
+  const double taup = tau > 1e-15 * D ? tau : 1e-15 * D;
+  // in units of E = D + tau: the ideal-gas con2prim is invariant under
+  // D, S, tau -> l D, l S, l tau and B -> sqrt(l) B
+  const double lE = log(D + taup);
+  x[0] = log(D) - lE;
+  x[1] = log(S2 + floor2) - 2.0 * lE;
+  x[2] = log(taup) - lE;
+  x[3] = log(B2 + floor2) - lE;
   const double den = sqrt(S2 * B2);
   x[4] = den > 0.0 ? SB / den : 0.0;
 }

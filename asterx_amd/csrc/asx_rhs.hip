@@ -468,42 +468,67 @@ __global__ __launch_bounds__(256) void k_bc(double *g, int ex, int ey, int ez, i
 }
 
 // The same fill for up to kMaxBC grid functions of any centering in one
-// launch per axis (blockIdx.y = grid function).
+// launch per axis (blockIdx.y = grid function): one work-item per ghost point,
+// point t of the 2 ng ghost layers normal to D (x fastest, so that the layers
+// normal to y and z are written in full rows).  Each ghost value depends
+// only on interior points along D, so the points of one axis are
+// independent; the axes run in order x, y, z as in k_bc.
 template <int D>
-__global__ __launch_bounds__(256) void k_bc_many(BCList L, int n0, int n1, int n2, int ng,
-                                                 int kind) {
-  constexpr int A = (D + 1) % 3, Bx = (D + 2) % 3;
-  const int f = blockIdx.y;
-  const int c = L.cent[f];
+ASX_DEV void bc_point(double *g, int c, int n0, int n1, int n2, int ng, int kind, int t) {
   const int n[3] = {n0, n1, n2};
   const int e[3] = {n0 + 2 * ng + (c & 1), n1 + 2 * ng + ((c >> 1) & 1),
                     n2 + 2 * ng + ((c >> 2) & 1)};
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int na = e[A], nb = e[Bx];
-  if (t >= na * nb) return;
-  const int ia = t % na, ib = t / na;
+  int w[3] = {e[0], e[1], e[2]};
+  w[D] = 2 * ng;
+  if (t >= w[0] * w[1] * w[2]) return;
+  int x[3] = {t % w[0], (t / w[0]) % w[1], t / (w[0] * w[1])};
+  const int lo = ng, hi = ng + n[D] - 1 + ((c >> D) & 1);
+  const int q = x[D] < ng ? x[D] : hi + 1 + (x[D] - ng);
   const int64_t ex = e[0], exy = (int64_t)e[0] * e[1];
   const int64_t sd = D == 0 ? 1 : (D == 1 ? ex : exy);
-  const int64_t sa = A == 0 ? 1 : (A == 1 ? ex : exy);
-  const int64_t sb = Bx == 0 ? 1 : (Bx == 1 ? ex : exy);
-  double *l = L.g[f] + ia * sa + ib * sb;
-  const int lo = ng, hi = ng + n[D] - 1 + ((c >> D) & 1);
-  for (int q = lo - 1; q >= 0; --q) {
+  x[D] = 0;
+  double *l = g + x[0] + x[1] * ex + x[2] * exy;
+  double v;
+  if (q < lo) {
     if (kind == ASX_BC_LINEAR)
-      l[q * sd] = l[lo * sd] + (lo - q) * (l[lo * sd] - l[(lo + 1) * sd]);
+      v = l[lo * sd] + (lo - q) * (l[lo * sd] - l[(lo + 1) * sd]);
     else if (kind == ASX_BC_PERIODIC)
-      l[q * sd] = l[(q + n[D]) * sd];
+      v = l[(q + n[D]) * sd];
     else
-      l[q * sd] = l[lo * sd];
-  }
-  for (int q = hi + 1; q < e[D]; ++q) {
+      v = l[lo * sd];
+  } else {
     if (kind == ASX_BC_LINEAR)
-      l[q * sd] = l[hi * sd] + (q - hi) * (l[hi * sd] - l[(hi - 1) * sd]);
+      v = l[hi * sd] + (q - hi) * (l[hi * sd] - l[(hi - 1) * sd]);
     else if (kind == ASX_BC_PERIODIC)
-      l[q * sd] = l[(q - n[D]) * sd];
+      v = l[(q - n[D]) * sd];
     else
-      l[q * sd] = l[hi * sd];
+      v = l[hi * sd];
   }
+  l[q * sd] = v;
+}
+// ghost points normal to d of the widest centering (vertex on every axis)
+__host__ inline int bc_points(const int n[3], int ng, int d) {
+  int w[3] = {n[0] + 2 * ng + 1, n[1] + 2 * ng + 1, n[2] + 2 * ng + 1};
+  w[d] = 2 * ng;
+  return w[0] * w[1] * w[2];
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_bc_many(BCList L, int n0, int n1, int n2, int ng,
+                                                 int kind) {
+  const int f = blockIdx.y;
+  bc_point<D>(L.g[f], L.cent[f], n0, n1, n2, ng, kind, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_bc_batch(const KPatch *__restrict__ tab, BCSlots L,
+                                                  const BatchRef *__restrict__ map, int nmap,
+                                                  int kind) {
+  const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
+  const KPatch &P = tab[r.pid];
+  const int f = blockIdx.y;
+  bc_point<D>(P.g[L.slot[f]], L.cent[f], P.n[0], P.n[1], P.n[2], P.ng, kind,
+              r.chunk * 256 + (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -537,11 +562,9 @@ ASX_DEV constexpr int kSrcState(int q) {
 // waves per SIMD the register budget is sized for (order-4 derivatives
 // need 32 more doubles live than order 2)
 template <int ORD> constexpr int kUpdateWaves = ORD == 2 ? 4 : 3;
-template <int ORD, class G_>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<ORD>)))
-void k_update_all(KPatch P, Box B, G_ G, RhsArgs A, StatePtrs u0, StatePtrs out, Comb cb) {
-  int i, j, k;
-  if (!G.point(B, i, j, k)) return;
+template <int ORD>
+ASX_DEV void update_point(const KPatch &P, int i, int j, int k, const RhsArgs &A,
+                          const StatePtrs &u0, const StatePtrs &out, const Comb &cb) {
   const bool cx = i < P.ng + P.n[0], cy = j < P.ng + P.n[1], cz = k < P.ng + P.n[2];
   if (cx && cy && cz) {
     double r[7];
@@ -574,6 +597,27 @@ void k_update_all(KPatch P, Box B, G_ G, RhsArgs A, StatePtrs u0, StatePtrs out,
     const double z = (cb.c0 != 0.0) ? u0.g[10][o] : 0.0;
     out.g[10][o] = cb(z, P.g[ASX_PSI][o], rhs);
   }
+}
+
+template <int ORD, class G_>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<ORD>)))
+void k_update_all(KPatch P, Box B, G_ G, RhsArgs A, StatePtrs u0, StatePtrs out, Comb cb) {
+  int i, j, k;
+  if (!G.point(B, i, j, k)) return;
+  update_point<ORD>(P, i, j, k, A, u0, out, cb);
+}
+
+template <int ORD, int TX, int TY, int TZ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kUpdateWaves<ORD>)))
+void k_update_batch(const KPatch *__restrict__ tab, const StatePtrs *__restrict__ u0,
+                    const StatePtrs *__restrict__ out, RhsArgs A, Comb cb,
+                    const BatchBox *__restrict__ box, const BatchRef *__restrict__ map,
+                    int nmap) {
+  const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
+  const BatchBox &bb = box[r.pid];
+  int i, j, k;
+  if (!tile_point<TX, TY, TZ>(bb.lo, bb.cnt, r.chunk, bb.nt[0], bb.nt[1], i, j, k)) return;
+  update_point<ORD>(tab[r.pid], i, j, k, A, u0[r.pid], out[r.pid], cb);
 }
 
 // ---------------------------------------------------------------------------
@@ -640,16 +684,12 @@ hipError_t launch_bc(const KPatch &P, int slot, int kind, hipStream_t s) {
 
 hipError_t launch_bc_many(const KPatch &P, const BCList &L, int kind, hipStream_t s) {
   if (L.nf <= 0) return hipSuccess;
-  int lines = 0; // largest face plane over the listed centerings
-  for (int d = 0; d < 3; ++d) {
-    const int a = (d + 1) % 3, b = (d + 2) % 3;
-    const int m = (P.n[a] + 2 * P.ng + 1) * (P.n[b] + 2 * P.ng + 1);
-    lines = m > lines ? m : lines;
-  }
-  const dim3 grid((unsigned)((lines + 255) / 256), (unsigned)L.nf);
-  hipLaunchKernelGGL(k_bc_many<0>, grid, dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
-  hipLaunchKernelGGL(k_bc_many<1>, grid, dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
-  hipLaunchKernelGGL(k_bc_many<2>, grid, dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
+  auto grid = [&](int d) {
+    return dim3((unsigned)((bc_points(P.n, P.ng, d) + 255) / 256), (unsigned)L.nf);
+  };
+  hipLaunchKernelGGL(k_bc_many<0>, grid(0), dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
+  hipLaunchKernelGGL(k_bc_many<1>, grid(1), dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
+  hipLaunchKernelGGL(k_bc_many<2>, grid(2), dim3(256), 0, s, L, P.n[0], P.n[1], P.n[2], P.ng, kind);
   return hipGetLastError();
 }
 
@@ -673,6 +713,66 @@ static void launch_update_v(int v, const KPatch &P, const Box &b, const RhsArgs 
   case 4: launch_update_t<ORD, 64, 2, 2>(P, b, A, u0, out, cb, s); break;
   default: launch_update_t<ORD, 64, 4, 1>(P, b, A, u0, out, cb, s); break;
   }
+}
+
+template <int TX, int TY, int TZ> static void ext3(int ex[3]) {
+  ex[0] = TX;
+  ex[1] = TY;
+  ex[2] = TZ;
+}
+void update_tile_extent(int variant, int ex[3]) {
+  switch (variant) {
+  case 1: ext3<32, 4, 2>(ex); break;
+  case 2: ext3<32, 2, 4>(ex); break;
+  case 3: ext3<16, 4, 4>(ex); break;
+  case 4: ext3<64, 2, 2>(ex); break;
+  default: ext3<64, 4, 1>(ex); break;
+  }
+}
+
+template <int ORD, int TX, int TY, int TZ>
+static void launch_update_batch_t(const KPatch *tab, const StatePtrs *u0, const StatePtrs *out,
+                                  const RhsArgs &A, const Comb &cb, const BatchBox *box,
+                                  const BatchRef *map, int nmap, hipStream_t s) {
+  static_assert(TX * TY * TZ == 256, "update tile");
+  hipLaunchKernelGGL((k_update_batch<ORD, TX, TY, TZ>), dim3((unsigned)nmap), dim3(256), 0, s,
+                     tab, u0, out, A, cb, box, map, nmap);
+}
+template <int ORD>
+static void launch_update_batch_v(int v, const KPatch *tab, const StatePtrs *u0,
+                                  const StatePtrs *out, const RhsArgs &A, const Comb &cb,
+                                  const BatchBox *box, const BatchRef *map, int nmap,
+                                  hipStream_t s) {
+  switch (v) {
+  case 1: launch_update_batch_t<ORD, 32, 4, 2>(tab, u0, out, A, cb, box, map, nmap, s); break;
+  case 2: launch_update_batch_t<ORD, 32, 2, 4>(tab, u0, out, A, cb, box, map, nmap, s); break;
+  case 3: launch_update_batch_t<ORD, 16, 4, 4>(tab, u0, out, A, cb, box, map, nmap, s); break;
+  case 4: launch_update_batch_t<ORD, 64, 2, 2>(tab, u0, out, A, cb, box, map, nmap, s); break;
+  default: launch_update_batch_t<ORD, 64, 4, 1>(tab, u0, out, A, cb, box, map, nmap, s); break;
+  }
+}
+
+hipError_t launch_update_batch(const KPatch *tab, const StatePtrs *u0, const StatePtrs *out,
+                               const RhsArgs &A, double c0, double c1, double c2,
+                               const BatchBox *box, const BatchRef *map, int nmap,
+                               int variant, hipStream_t s) {
+  if (nmap <= 0) return hipSuccess;
+  const Comb cb{c0, c1, c2};
+  if (A.fd_order == 4)
+    launch_update_batch_v<4>(variant, tab, u0, out, A, cb, box, map, nmap, s);
+  else
+    launch_update_batch_v<2>(variant, tab, u0, out, A, cb, box, map, nmap, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_bc_batch(int d, const KPatch *tab, const BCSlots &L, const BatchRef *map,
+                           int nmap, int kind, hipStream_t s) {
+  if (nmap <= 0 || L.nf <= 0) return hipSuccess;
+  const dim3 grid((unsigned)nmap, (unsigned)L.nf);
+  if (d == 0) hipLaunchKernelGGL(k_bc_batch<0>, grid, dim3(256), 0, s, tab, L, map, nmap, kind);
+  else if (d == 1) hipLaunchKernelGGL(k_bc_batch<1>, grid, dim3(256), 0, s, tab, L, map, nmap, kind);
+  else hipLaunchKernelGGL(k_bc_batch<2>, grid, dim3(256), 0, s, tab, L, map, nmap, kind);
+  return hipGetLastError();
 }
 
 hipError_t launch_update(const KPatch &P, const RhsArgs &A, const StatePtrs &u0,

@@ -52,6 +52,13 @@ SIGNATURES = {
                             _V, _V, _V], C.c_int),
     "asx_c2p_batch_mlp": ([C.c_int64, _V, _V, _V, _V, C.c_int32, _V, _V, _V, _V, _PP, _V],
                           C.c_int),
+    "asx_batch_create": ([C.c_int32, _P, _S, _PP, C.c_double, C.c_int32,
+                          C.POINTER(C.c_void_p)], C.c_int),
+    "asx_batch_stage": ([_V, C.c_int32, _V], C.c_int),
+    "asx_batch_step": ([_V, C.c_int32, C.c_int32, _V], C.c_int),
+    "asx_batch_autotune": ([_V, C.c_int32, _V], C.c_int),
+    "asx_batch_info": ([_V, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
+    "asx_batch_destroy": ([_V], C.c_int),
     "asx_params_default": ([_PP], None),
     "asx_gf_centering": ([C.c_int], C.c_int),
     "asx_gf_strides": ([C.POINTER(C.c_int32), C.c_int32, C.c_int,
@@ -75,6 +82,15 @@ def lib():
             raise RuntimeError(
                 "libasterx_hip.so is not built (%s); run __graft_entry__.build()"
                 % LIBPATH)
+        # torch ships its own HIP runtime under the same soname and loads it
+        # by path; loaded first, it is the one this library binds to, so that
+        # device pointers from torch tensors and from this library share one
+        # runtime.  Loaded the other way round a process holds two runtimes
+        # and the library's launches fail on torch's allocations.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIBPATH)
         for name, (args, res) in SIGNATURES.items():
             f = getattr(L, name)

@@ -95,7 +95,7 @@ def train(L, par, gamma, samples=int(1e7), batch=8192, lr=2e-3, lr_end=2e-5,
         abi.check("asx_mlp_train_step", L.asx_mlp_train_step(
             batch, _p(z.cons), _p(z.g), _p(z.target), _p(b), _p(m), _p(v),
             C.c_float(lr_s), s, _p(partial), _p(loss), None, stream))
-        if s % log_every == 0 or s == steps:
+        if s == 1 or s % log_every == 0 or s == steps:
             torch.cuda.synchronize()
             hist.append((s, float(loss.item())))
             if log:

@@ -54,6 +54,16 @@ def parse():
     ap.add_argument("--tune", choices=["on", "off"], default="on",
                     help="pick launch-shape variants with asx_autotune_substep "
                          "before the warm-up")
+    ap.add_argument("--batch", choices=["on", "off"], default="on",
+                    help="also time the batched-patch plan: 400 patches with "
+                         "edges from {16,24,32,48,64} cut from the torus")
+    ap.add_argument("--batch-patches", type=int, default=400)
+    ap.add_argument("--batch-steps", type=int, default=10,
+                    help="timed SSPRK3 steps (3 substeps each) of the batch")
+    ap.add_argument("--c2p-guess", choices=["on", "off"], default="on",
+                    help="also time con2prim with the learned guess against "
+                         "the plain seed on 2^24 independent zones")
+    ap.add_argument("--c2p-zones", type=int, default=1 << 24)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -211,6 +221,95 @@ def read_traffic(kernel, workload):
                     and t[1] == workload:
                 return float(t[2]) + float(t[3])
     return None
+
+
+def bench_batched(args, src, par, dt, L, sh, stream):
+    """Batched uneven patches: `--batch-patches` patches with per-axis edges
+    drawn from {16,24,32,48,64}, cut at random offsets from the torus patch
+    (3 ghost zones each), whole SSPRK3 steps of the plan replayed as one HIP
+    graph, launch shapes from asx_batch_autotune.  zone-updates/s = interior
+    zones x 3 substeps x steps / wall time."""
+    import torch
+    from asterx_amd import abi, batch
+    boxes = batch.sample_boxes(src.n, args.batch_patches, seed=2024)
+    with torch.cuda.stream(stream):
+        patches = [batch.cut_patch(src, n, o) for n, o in boxes]
+    torch.cuda.synchronize()
+    plan = batch.Plan(L, patches, par, dt)
+    zones, dbytes = plan.info()
+    plan.autotune(reps=2, stream=sh)
+    variants = {}
+    for name in ("face_x", "face_y", "face_z", "update"):
+        v = C.c_int32()
+        abi.check("asx_get_variant", L.asx_get_variant(abi.KCLASS[name], C.byref(v)))
+        variants[name] = v.value
+    plan.step(1, graph=True, stream=sh)  # records the graph
+    torch.cuda.synchronize()
+    K = args.batch_steps
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    t0 = time.perf_counter()
+    plan.step(K, graph=True, stream=sh)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nl, gms = C.c_int64(), C.c_double()
+    abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["graph"], C.byref(nl),
+                                                   C.byref(gms)))
+    # per-kernel split from one step of plain launches
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    plan.step(1, graph=False, stream=sh)
+    torch.cuda.synchronize()
+    kt = {}
+    for name, kc in abi.KCLASS.items():
+        n_, ms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(kc, C.byref(n_), C.byref(ms)))
+        if n_.value:
+            kt[name] = round(ms.value, 4)
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    for p in patches[:8]:
+        for name in abi.CONS:
+            if not bool(torch.isfinite(p.arr[name]).all()):
+                raise RuntimeError("non-finite %s in the batch" % name)
+    edges = sorted({e for n, _ in boxes for e in n})
+    out = {"workload": "%d patches, edges per axis from %s, 3 ghosts, cut from the "
+                       "%d^3 torus; SSPRK3 steps as one HIP graph" %
+                       (len(patches), edges, src.n[0]),
+           "zones": zones, "steps": K,
+           "value": round(3 * zones * K / wall, 1), "unit": "zone-updates/s",
+           "ms_per_rk3_step": round(1e3 * wall / K, 4),
+           "graph_ms_per_step": round(gms.value / max(1, nl.value), 4),
+           "plain_launch_kernels_ms_per_step": kt,
+           "variants": variants, "plan_device_bytes": dbytes}
+    plan.close()
+    del patches
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_c2p_guess(args, L):
+    """Con2prim on 2^24 independent zones of the learned-guess workload
+    (ideal gas Gamma 5/3, tolerance 1e-10): plain seed against the trained
+    network's guess (8 guided iterations, then the plain chain), for the
+    cold-start seed and the previous-step seed (truth x (1 + 1e-2 u))."""
+    from asterx_amd import abi, mlp
+    if not os.path.exists(mlp.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights at %s" % mlp.DEFAULT_WEIGHTS}
+    blob = mlp.load()
+    par = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8, rho_abs_min=1e-12,
+                     vw_lim=1e3, B_lim=1e3, c2p_tol=1e-10)
+    ev = mlp.evaluate(L, par, 5.0 / 3.0, blob, nz=args.c2p_zones, reps=3)
+    out = {"zones": args.c2p_zones, "kguess": 8, "tolerance": 1e-10}
+    for seed in ("cold", "perturbed"):
+        a, b = ev["%s_plain" % seed], ev["%s_mlp" % seed]
+        out[seed] = {"plain_zones_per_s": round(a["zones_per_s"], 1),
+                     "mlp_zones_per_s": round(b["zones_per_s"], 1),
+                     "speedup": round(b["zones_per_s"] / a["zones_per_s"], 3),
+                     "iters_mean_plain": round(a["iters_mean"], 3),
+                     "iters_mean_mlp": round(b["iters_mean"], 3),
+                     "fallback_rate": round(b["fallback_rate"], 5),
+                     "flags_plain": a["flags"], "flags_mlp": b["flags"]}
+    out["guess_abs_err"] = ev["guess_abs_err"]
+    return out
 
 
 def cpu_baseline(host_patch, par, dt, steps):
@@ -382,6 +481,17 @@ def main():
     roof["substep_stencil_frac"] = round(sten_b * zones / (sten_ms * 1e-3) / 1e9 /
                                          HBM_PEAK_GBS, 4)
 
+    batched = None
+    if args.batch == "on":
+        t0 = time.perf_counter()
+        batched = bench_batched(args, p, par, dt, L, sh, stream)
+        log(args, "batched: %.1f s" % (time.perf_counter() - t0), batched)
+    guess = None
+    if args.c2p_guess == "on":
+        t0 = time.perf_counter()
+        guess = bench_c2p_guess(args, L)
+        log(args, "c2p guess: %.1f s" % (time.perf_counter() - t0), guess)
+
     cpu = None
     if host is not None:
         ts = cpu_baseline(host, par, dt, args.cpu_steps)
@@ -404,7 +514,8 @@ def main():
                                   "FM disk, %s + %s, Noble/Palenzuela c2p" %
                                   (args.n, args.recon, args.flux),
                       "zones": zones, "substep": "SSPRK3 stages cycled"},
-           "roofline": roof, "cpu_baseline": cpu}
+           "roofline": roof, "cpu_baseline": cpu,
+           "batched": batched, "c2p_guess": guess}
     print(json.dumps(out), flush=True)
 
 

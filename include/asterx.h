@@ -268,7 +268,8 @@ enum {
   ASX_KC_OTHER = 9,
   ASX_KC_C2P_REST = 10, /* con2prim: zones the primary solver failed */
   ASX_KC_MLP = 11,      /* learned con2prim guess: inference and training */
-  ASX_KC_COUNT = 12
+  ASX_KC_GRAPH = 12,    /* replays of a batch plan's captured step */
+  ASX_KC_COUNT = 13
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
@@ -407,8 +408,11 @@ asx_status asx_zone_sample(int64_t nz, uint64_t seed, double gamma, double seed_
 asx_status asx_prim2con_batch(int64_t nz, const double *prim, const double *g, double *cons,
                               double *cold_seed, const asx_params *par, void *stream);
 
-/* The network: 5 inputs (log D, log S^2, log tau, log B^2, cos(S,B)) ->
- * 64 tanh -> 64 tanh -> 2 outputs (log W, log(Z/D), Z = rho h W^2), on a
+/* The network: 5 inputs (log D, log S^2, log tau, log B^2 in units of
+ * E = D + tau, and cos(S,B); the ideal-gas con2prim is invariant under
+ * D, S, tau -> l D, l S, l tau, B -> sqrt(l) B, so the guess needs only these
+ * ratios) -> 64 tanh -> 64 tanh -> 2 outputs (log W, log(Z/D),
+ * Z = rho h W^2), on a
  * float32 blob of ASX_MLP_NBLOB values: W1[64][5] b1[64] W2[64][64] b2[64]
  * W3[2][64] b3[2] (ASX_MLP_NPARAM parameters), then the input means and
  * inverse standard deviations (5 + 5) and output means and standard
@@ -441,6 +445,43 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                              const double *seed, const float *blob, int32_t kguess,
                              double *prim, double *flag, int32_t *iters, int32_t *fallback,
                              const asx_params *par, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched patches (replaces the driver loop that calls the scheduled        */
+/* functions once per local box and RK stage: AsterX/schedule.ccl:119-232)  */
+/* ------------------------------------------------------------------------ */
+/* A plan runs whole SSPRK3 steps over npatch independent patches of any
+ * sizes: every kernel class is one launch over all patches, driven by
+ * device-side descriptor tables built once here.  Per stage s = 0, 1, 2 (with
+ * (c0, c1, c2) = (0, 1, dt), (3/4, 1/4, dt/4), (1/3, 2/3, 2dt/3)):
+ *   asx_substep from state buffer b_in = s into b_out = (1, 2, 0)[s] with
+ *   u0 = buffer 0, boundary fill of the 11 state fields of b_out,
+ *   asx_con2prim_group on b_out, boundary fill of the 11 synced primitives
+ *   (rho, vel^i, eps, press, entropy, Ye, Bvec^i),
+ * so a step leaves the new state in buffer 0.  Buffer 0 is the state slots
+ * of patches[p]; bufs[2p] and bufs[2p+1] are buffers 1 and 2 of patch p.
+ * Requirements per patch are those of asx_substep and asx_con2prim_group;
+ * ghost zones are filled with `bc_kind` on every face (there is no
+ * inter-patch exchange).  The plan owns the face/aux/dB_stag scratch of all
+ * patches and the con2prim queue; it holds the descriptors, not the data. */
+typedef struct asx_batch_s asx_batch;
+asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_state *bufs,
+                            const asx_params *par, double dt, int32_t bc_kind,
+                            asx_batch **plan);
+/* One stage (0, 1 or 2) with plain launches on `stream`. */
+asx_status asx_batch_stage(asx_batch *plan, int32_t stage, void *stream);
+/* nsteps whole steps.  use_graph: the three stages are captured once into a
+ * HIP graph (re-captured when a launch-shape variant changes) and the graph
+ * is replayed on `stream`; otherwise plain launches. */
+asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, void *stream);
+/* Launch-shape autotuning over the batch: for every face and update variant
+ * the stage-0 stencil work (dB_stag, faces, aux, update into buffer 1) runs
+ * once untimed and `reps` times timed; the fastest variant per class is
+ * kept for this device.  Writes only buffer 1 and the plan's scratch. */
+asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream);
+/* Interior zones of all patches and device bytes the plan allocated. */
+asx_status asx_batch_info(const asx_batch *plan, int64_t *zones, int64_t *device_bytes);
+asx_status asx_batch_destroy(asx_batch *plan);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,8 @@ def bf16(x):
 
 
 def features(cons, g):
-    """x[nz, 5]: log D, log S^2, log tau, log B^2, cos(S, B) (float64)."""
+    """x[nz, 5]: log D, log S^2, log tau, log B^2 in units of E = D + tau
+    (tau floored at 1e-15 D), and cos(S, B) (float64)."""
     gxx, gxy, gxz, gyy, gyz, gzz = g
     det = 2 * gxy * gxz * gyz + gxx * gyy * gzz - gyy * gxz ** 2 - \
         gxx * gyz ** 2 - gzz * gxy ** 2
@@ -61,9 +62,10 @@ def features(cons, g):
     fl = 1e-30 * D * D
     den = np.sqrt(S2 * B2)
     cosv = np.where(den > 0, SB / np.where(den > 0, den, 1.0), 0.0)
-    return np.stack([np.log(D), np.log(S2 + fl),
-                     np.log(np.maximum(tau, 1e-15 * D)), np.log(B2 + fl), cosv],
-                    axis=1)
+    taup = np.maximum(tau, 1e-15 * D)
+    lE = np.log(D + taup)
+    return np.stack([np.log(D) - lE, np.log(S2 + fl) - 2 * lE,
+                     np.log(taup) - lE, np.log(B2 + fl) - lE, cosv], axis=1)
 
 
 def _params(blob, bf):

@@ -83,3 +83,17 @@ def test_errors_are_status_codes():
     assert L.asx_fluxes(C.byref(ps), C.byref(ok), 0, None, None) == 1  # slots missing
     ps.gf[abi.SLOT["rho"]].sj += 1
     assert L.asx_bc_fill(C.byref(ps), abi.SLOT["rho"], 0, None) == 2  # layout
+
+
+def test_one_hip_runtime_when_library_loads_first():
+    """Loading the library before torch must still leave one HIP runtime in
+    the process (torch's), so torch's device pointers are valid for it."""
+    import subprocess
+    import sys
+    code = ("import asterx_amd.lib as l; l.lib(); import torch; "
+            "m = open('/proc/self/maps').read(); "
+            "print(len({x.split()[-1] for x in m.splitlines() if 'libamdhip64' in x}))")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True,
+                         text=True, timeout=600)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert out.stdout.strip().splitlines()[-1] == "1", out.stdout

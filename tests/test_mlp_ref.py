@@ -44,22 +44,22 @@ def test_backprop_matches_finite_differences():
         assert abs(fd - grad[p]) <= 1e-6 * (abs(fd) + 1e-3), (p, fd, grad[p])
 
 
-def test_features_scale_covariant():
-    """Scaling every conserved variable by s shifts log D, log tau by log s
-    and log S^2, log B^2 by 2 log s; the cosine is unchanged."""
-    cons, g, _, _ = zones.make_batch(100, seed=2)
+def test_features_scale_free():
+    """The inputs are in units of E = D + tau, so the scaling under which the
+    ideal-gas con2prim is invariant (D, S, tau -> s D, s S, s tau and
+    B -> sqrt(s) B) leaves them unchanged; D/E + tau/E = 1; and S^2 < E^2
+    for every physical state."""
+    cons, g, _, _ = zones.make_batch(2000, seed=2)
     x = M.features(cons, g)
-    s = 3.7
-    cons2 = cons.copy()
-    cons2[:5] *= s
-    cons2[5:] *= np.sqrt(s)  # B^2 scales with energy density
-    x2 = M.features(cons2, g)
-    ls = np.log(s)
-    assert np.allclose(x2[:, 0] - x[:, 0], ls, atol=1e-12)
-    assert np.allclose(x2[:, 1] - x[:, 1], 2 * ls, atol=1e-9)
-    assert np.allclose(x2[:, 2] - x[:, 2], ls, atol=1e-12)
-    assert np.allclose(x2[:, 3] - x[:, 3], ls, atol=1e-9)
-    assert np.allclose(x2[:, 4], x[:, 4], atol=1e-12)
+    for s in (3.7, 1e-6, 2e5):
+        cons2 = cons.copy()
+        cons2[:5] *= s
+        cons2[5:] *= np.sqrt(s)  # B^2 scales with energy density
+        x2 = M.features(cons2, g)
+        assert np.allclose(x2, x, rtol=0, atol=1e-9)
+    assert np.allclose(np.exp(x[:, 0]) + np.exp(x[:, 2]), 1.0, atol=1e-12)
+    assert (x[:, 1] < 0).all()
+    assert (np.abs(x[:, 4]) <= 1 + 1e-12).all()
 
 
 def test_bf16_rounding():
