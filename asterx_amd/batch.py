@@ -106,6 +106,16 @@ class Plan:
     def autotune(self, reps=3, stream=None):
         abi.check("asx_batch_autotune", self.L.asx_batch_autotune(self.h, reps, stream))
 
+    def variants(self):
+        """The plan's launch shapes: {face_x, face_y, face_z, update}."""
+        out = {}
+        for name in ("face_x", "face_y", "face_z", "update"):
+            v = C.c_int32()
+            abi.check("asx_batch_get_variant", self.L.asx_batch_get_variant(
+                self.h, abi.KCLASS[name], C.byref(v)))
+            out[name] = v.value
+        return out
+
     def info(self):
         z, b = C.c_int64(), C.c_int64()
         abi.check("asx_batch_info", self.L.asx_batch_info(self.h, C.byref(z), C.byref(b)))

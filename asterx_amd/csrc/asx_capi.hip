@@ -1112,6 +1112,7 @@ struct asx_batch_s {
   int *queue = nullptr;
   unsigned *qcount = nullptr;
   std::map<std::pair<int, int>, BatchLaunch> launches; // (kind, variant)
+  int var[4] = {0, 0, 0, 0}; // launch shapes: faces x, y, z, update
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1259,7 +1260,7 @@ asx_status plan_stencil(asx_batch *b, int s, hipStream_t str) {
   }
   const FluxArgs A = plan_flux_args(&b->par);
   for (int d = 0; d < 3; ++d) {
-    const int v = variant_of(ASX_KC_FACE_X + d);
+    const int v = b->var[d];
     if ((st = plan_launch(b, BK_FACE + d, v, &L))) return st;
     if ((st = dev_status(timed(ASX_KC_FACE_X + d, str, [&] {
            return launch_face_batch(d, T, A, L->box, L->map, L->nmap,
@@ -1272,7 +1273,7 @@ asx_status plan_stencil(asx_batch *b, int s, hipStream_t str) {
          return launch_aux_batch(T, L->box, L->map, L->nmap, str);
        }))))
     return st;
-  const int v = variant_of(ASX_KC_UPDATE);
+  const int v = b->var[3];
   if ((st = plan_launch(b, BK_UPDATE, v, &L))) return st;
   const RhsArgs R = rhs_args(&b->par);
   return dev_status(timed(ASX_KC_UPDATE, str, [&] {
@@ -1337,8 +1338,8 @@ asx_status plan_prepare(asx_batch *b) {
   asx_status st;
   for (int k = 0; k < BK_COUNT; ++k) {
     int v = 0;
-    if (k >= BK_FACE && k < BK_FACE + 3) v = variant_of(ASX_KC_FACE_X + (k - BK_FACE));
-    if (k == BK_UPDATE) v = variant_of(ASX_KC_UPDATE);
+    if (k >= BK_FACE && k < BK_FACE + 3) v = b->var[k - BK_FACE];
+    if (k == BK_UPDATE) v = b->var[3];
     if ((st = plan_launch(b, k, v, &L))) return st;
   }
   return ASX_OK;
@@ -1352,9 +1353,7 @@ void plan_drop_graph(asx_batch *b) {
 }
 
 asx_status plan_capture(asx_batch *b) {
-  int var[4];
-  for (int q = 0; q < 3; ++q) var[q] = variant_of(ASX_KC_FACE_X + q);
-  var[3] = variant_of(ASX_KC_UPDATE);
+  const int *var = b->var;
   if (b->exec && std::equal(var, var + 4, b->graph_var)) return ASX_OK;
   plan_drop_graph(b);
   asx_status st = plan_prepare(b);
@@ -1456,6 +1455,8 @@ asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_
   b->user_bstag = ubst;
   b->nzones = zones;
   b->pt.assign(patches, patches + npatch);
+  for (int q = 0; q < 3; ++q) b->var[q] = variant_of(ASX_KC_FACE_X + q);
+  b->var[3] = variant_of(ASX_KC_UPDATE);
   st = ASX_OK;
   auto fail = [&](asx_status e) {
     plan_free(b);
@@ -1566,11 +1567,12 @@ asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream) {
   int arg[4];
   for (int q = 0; q < 4; ++q) {
     best[q] = 1e300;
-    arg[q] = variant_of(tuned[q]);
+    arg[q] = plan->var[q];
   }
   asx_status st = ASX_OK;
   for (int v = 0; v < nv && !st; ++v) {
-    for (int kc : tuned) set_variant(kc, v < variant_count(kc) ? v : 0);
+    for (int q = 0; q < 4; ++q) plan->var[q] = v < variant_count(tuned[q]) ? v : 0;
+    // one untimed pass (first launch of a variant loads its code object)
     if ((st = plan_stencil(plan, 0, s))) break;
     if ((st = asx_timing_enable(1))) break;
     for (int r = 0; r < reps && !st; ++r) st = plan_stencil(plan, 0, s);
@@ -1585,7 +1587,7 @@ asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream) {
       }
     }
   }
-  for (int q = 0; q < 4; ++q) set_variant(tuned[q], arg[q]);
+  for (int q = 0; q < 4; ++q) plan->var[q] = arg[q];
   {
     std::lock_guard<std::mutex> l(g_tmu);
     asx_status st2 = drain_timing();
@@ -1598,6 +1600,15 @@ asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream) {
   }
   if (!st) st = plan_prepare(plan);
   return st;
+}
+
+asx_status asx_batch_get_variant(const asx_batch *plan, int32_t kclass, int32_t *v) {
+  if (!plan || !v) return ASX_ERR_NULL;
+  if (kclass >= ASX_KC_FACE_X && kclass <= ASX_KC_FACE_Z) *v = plan->var[kclass - ASX_KC_FACE_X];
+  else if (kclass == ASX_KC_UPDATE) *v = plan->var[3];
+  else if (kclass >= 0 && kclass < ASX_KC_COUNT) *v = 0;
+  else return ASX_ERR_PARAM;
+  return ASX_OK;
 }
 
 asx_status asx_batch_info(const asx_batch *plan, int64_t *zones, int64_t *device_bytes) {

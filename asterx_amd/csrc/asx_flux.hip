@@ -396,6 +396,13 @@ template <> struct FaceTileT<1, 2> { using T = Tile3<16, 8, 2>; };
 template <> struct FaceTileT<2, 0> { using T = Tile3<16, 1, 16>; };
 template <> struct FaceTileT<2, 1> { using T = Tile3<32, 1, 8>; };
 template <> struct FaceTileT<2, 2> { using T = Tile3<16, 2, 8>; };
+// narrower rows: fewer idle work-items on extents such as n + 2
+template <> struct FaceTileT<0, 3> { using T = Tile3<16, 4, 4>; };
+template <> struct FaceTileT<0, 4> { using T = Tile3<16, 8, 2>; };
+template <> struct FaceTileT<1, 3> { using T = Tile3<8, 16, 2>; };
+template <> struct FaceTileT<1, 4> { using T = Tile3<8, 32, 1>; };
+template <> struct FaceTileT<2, 3> { using T = Tile3<8, 2, 16>; };
+template <> struct FaceTileT<2, 4> { using T = Tile3<8, 4, 8>; };
 template <int D, int V> using FaceTile = typename FaceTileT<D, V>::T;
 
 // One tile of a face box (lo, cnt), tile index b of a grid of ntx x nty x ntz
@@ -619,9 +626,13 @@ static hipError_t launch_flux_dv(const KPatch &P, const FluxArgs &A, int method,
 template <int D>
 static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method, int variant,
                                 hipStream_t s) {
-  if (variant == 1) return launch_flux_dv<D, 1>(P, A, method, s);
-  if (variant == 2) return launch_flux_dv<D, 2>(P, A, method, s);
-  return launch_flux_dv<D, 0>(P, A, method, s);
+  switch (variant) {
+  case 1: return launch_flux_dv<D, 1>(P, A, method, s);
+  case 2: return launch_flux_dv<D, 2>(P, A, method, s);
+  case 3: return launch_flux_dv<D, 3>(P, A, method, s);
+  case 4: return launch_flux_dv<D, 4>(P, A, method, s);
+  default: return launch_flux_dv<D, 0>(P, A, method, s);
+  }
 }
 
 hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, int variant,
@@ -719,9 +730,13 @@ template <int D>
 static void launch_face_batch_d(const KPatch *tab, const FluxArgs &A, const BatchBox *box,
                                 const BatchRef *map, int nmap, int method, int variant,
                                 hipStream_t s) {
-  if (variant == 1) launch_face_batch_dv<D, 1>(tab, A, box, map, nmap, method, s);
-  else if (variant == 2) launch_face_batch_dv<D, 2>(tab, A, box, map, nmap, method, s);
-  else launch_face_batch_dv<D, 0>(tab, A, box, map, nmap, method, s);
+  switch (variant) {
+  case 1: launch_face_batch_dv<D, 1>(tab, A, box, map, nmap, method, s); break;
+  case 2: launch_face_batch_dv<D, 2>(tab, A, box, map, nmap, method, s); break;
+  case 3: launch_face_batch_dv<D, 3>(tab, A, box, map, nmap, method, s); break;
+  case 4: launch_face_batch_dv<D, 4>(tab, A, box, map, nmap, method, s); break;
+  default: launch_face_batch_dv<D, 0>(tab, A, box, map, nmap, method, s); break;
+  }
 }
 
 template <int D, int V> static void face_ext(int ex[3]) {
@@ -731,9 +746,13 @@ template <int D, int V> static void face_ext(int ex[3]) {
   ex[2] = T::Z;
 }
 template <int D> static void face_ext_d(int variant, int ex[3]) {
-  if (variant == 1) face_ext<D, 1>(ex);
-  else if (variant == 2) face_ext<D, 2>(ex);
-  else face_ext<D, 0>(ex);
+  switch (variant) {
+  case 1: face_ext<D, 1>(ex); break;
+  case 2: face_ext<D, 2>(ex); break;
+  case 3: face_ext<D, 3>(ex); break;
+  case 4: face_ext<D, 4>(ex); break;
+  default: face_ext<D, 0>(ex); break;
+  }
 }
 void face_tile_extent(int d, int variant, int ex[3]) {
   if (d == 0) face_ext_d<0>(variant, ex);

@@ -56,7 +56,7 @@ template <int TX, int TY, int TZ = 1> struct TileGrid {
 // Launch-shape variants of the tunable kernels (selected per call; see
 // asx_set_variant / asx_autotune_substep in the C ABI).
 constexpr int kUpdateVariants = 5;
-constexpr int kFaceVariants = 3;
+constexpr int kFaceVariants = 5;
 
 // Batched launches over many patches: the workgroups of one launch are
 // listed as (patch, chunk) pairs, chunk = tile index or 256-point block of

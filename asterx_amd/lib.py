@@ -57,6 +57,7 @@ SIGNATURES = {
     "asx_batch_stage": ([_V, C.c_int32, _V], C.c_int),
     "asx_batch_step": ([_V, C.c_int32, C.c_int32, _V], C.c_int),
     "asx_batch_autotune": ([_V, C.c_int32, _V], C.c_int),
+    "asx_batch_get_variant": ([_V, C.c_int32, C.POINTER(C.c_int32)], C.c_int),
     "asx_batch_info": ([_V, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int),
     "asx_batch_destroy": ([_V], C.c_int),
     "asx_params_default": ([_PP], None),

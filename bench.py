@@ -238,11 +238,7 @@ def bench_batched(args, src, par, dt, L, sh, stream):
     plan = batch.Plan(L, patches, par, dt)
     zones, dbytes = plan.info()
     plan.autotune(reps=2, stream=sh)
-    variants = {}
-    for name in ("face_x", "face_y", "face_z", "update"):
-        v = C.c_int32()
-        abi.check("asx_get_variant", L.asx_get_variant(abi.KCLASS[name], C.byref(v)))
-        variants[name] = v.value
+    variants = plan.variants()
     plan.step(1, graph=True, stream=sh)  # records the graph
     torch.cuda.synchronize()
     K = args.batch_steps

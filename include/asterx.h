@@ -474,11 +474,14 @@ asx_status asx_batch_stage(asx_batch *plan, int32_t stage, void *stream);
  * HIP graph (re-captured when a launch-shape variant changes) and the graph
  * is replayed on `stream`; otherwise plain launches. */
 asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, void *stream);
-/* Launch-shape autotuning over the batch: for every face and update variant
- * the stage-0 stencil work (dB_stag, faces, aux, update into buffer 1) runs
- * once untimed and `reps` times timed; the fastest variant per class is
- * kept for this device.  Writes only buffer 1 and the plan's scratch. */
+/* Launch shapes: a plan starts with the device's current variants
+ * (asx_set_variant) and keeps its own from then on.  asx_batch_autotune
+ * runs, for every face and update variant, the stage-0 stencil work
+ * (dB_stag, faces, aux, update into buffer 1) once untimed and `reps` times
+ * timed, and keeps the fastest variant per class in the plan.  It writes
+ * only buffer 1 and the plan's scratch. */
 asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream);
+asx_status asx_batch_get_variant(const asx_batch *plan, int32_t kclass, int32_t *v);
 /* Interior zones of all patches and device bytes the plan allocated. */
 asx_status asx_batch_info(const asx_batch *plan, int64_t *zones, int64_t *device_bytes);
 asx_status asx_batch_destroy(asx_batch *plan);

@@ -114,3 +114,5 @@ def test_plan_argument_checks():
     assert L.asx_batch_step(None, 1, 1, None) == 1
     assert L.asx_batch_autotune(None, 1, None) == 1
     assert L.asx_batch_info(None, None, None) == 1
+    v = C.c_int32()
+    assert L.asx_batch_get_variant(None, abi.KCLASS["update"], C.byref(v)) == 1

@@ -163,10 +163,8 @@ def test_autotune_leaves_state_alone():
     for p, s in zip(pa, snap):
         for name, a in p.arr.items():
             assert _eq(a, s[name]), name
-    for k in ("face_x", "face_y", "face_z", "update"):
-        v = C.c_int32()
-        abi.check("asx_get_variant", L.asx_get_variant(abi.KCLASS[k], C.byref(v)))
-        assert 0 <= v.value < L.asx_variant_count(abi.KCLASS[k])
+    for k, v in plan.variants().items():
+        assert 0 <= v < L.asx_variant_count(abi.KCLASS[k])
     plan.step(1, graph=True)
     torch.cuda.synchronize()
     plan.close()
