@@ -128,20 +128,6 @@ ASX_DEV void store_zone(const KPatch &P, int oc, const C2POut &o) {
   P.g[ASX_SAVED_YE][oc] = pv.Ye;
 }
 
-// Append `value` of every work-item with `fail` set to the queue: a
-// wavefront ballot, one atomic per wavefront, lanes in order.
-ASX_DEV void queue_push(bool fail, int value, int *queue, unsigned *qcount) {
-  const uint64_t m = __ballot(fail);
-  if (m) {
-    const int lane = threadIdx.x & 63;
-    const int leader = __builtin_ctzll(m);
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(m));
-    base = __shfl(base, leader);
-    if (fail) queue[base + __popcll(m & ((1ull << lane) - 1))] = value;
-  }
-}
-
 // Pass 1: every zone through the primary solver (compiled for PRIME only,
 // so the backup solvers cost no registers here).  Zones whose primary solve
 // fails store nothing; a wavefront ballot packs their indices into `queue`
@@ -286,38 +272,10 @@ __global__ __launch_bounds__(256) void k_c2p_batch(int64_t nz, const double *con
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= nz) return;
   C2PIn in;
-  in.r = 0.0;
-  in.alp = 1.0;
-  in.mask = 1.0;
-  in.glo = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
-  in.cv.dens = cons[z];
-  in.cv.mom[0] = cons[nz + z];
-  in.cv.mom[1] = cons[2 * nz + z];
-  in.cv.mom[2] = cons[3 * nz + z];
-  in.cv.tau = cons[4 * nz + z];
-  in.cv.dB[0] = cons[5 * nz + z];
-  in.cv.dB[1] = cons[6 * nz + z];
-  in.cv.dB[2] = cons[7 * nz + z];
-  in.cv.DYe = in.cv.dens * a.Ye_atmo;
-  in.cv.DEnt = 0.0;
-  in.saved_rho = seed[z];
-  in.saved_eps = seed[nz + z];
-  in.saved_Ye = a.Ye_atmo;
-  in.saved_vel[0] = seed[2 * nz + z];
-  in.saved_vel[1] = seed[3 * nz + z];
-  in.saved_vel[2] = seed[4 * nz + z];
-  in.temperature = 0.0;
+  batch_zone_in(z, nz, cons, g, seed, a, in);
   C2POut o;
   con2prim_cell(eos, a, in, o);
-  prim[z] = o.pv.rho;
-  prim[nz + z] = o.pv.eps;
-  prim[2 * nz + z] = o.pv.press;
-  prim[3 * nz + z] = o.pv.vel[0];
-  prim[4 * nz + z] = o.pv.vel[1];
-  prim[5 * nz + z] = o.pv.vel[2];
-  prim[6 * nz + z] = o.pv.w_lor;
-  prim[7 * nz + z] = o.pv.temperature;
-  flag[z] = o.flag;
+  batch_zone_out(z, nz, o, prim, flag);
   iters[z] = o.iters;
 }
 

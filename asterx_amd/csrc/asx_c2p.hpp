@@ -8,7 +8,10 @@
 // reference's call site documents (c2p_1DPalenzuela.hxx:380-393).
 #pragma once
 
+#include <type_traits>
+
 #include "asx_device.hpp"
+#include "asx_tab.hpp"
 
 namespace asx {
 
@@ -120,7 +123,7 @@ struct C2PArgs {
   double tol0;   // 2^-minbits
 };
 
-ASX_DEV Atmo make_atmo(const IdealGas &eos, const C2PArgs &a, double r) {
+template <class E> ASX_DEV Atmo make_atmo(const E &eos, const C2PArgs &a, double r) {
   double rho_atm = (r > a.r_atmo) ? (a.rho_abs_min * pow((a.r_atmo / r), a.n_rho_atmo))
                                   : a.rho_abs_min;
   rho_atm = smax(eos.rho_min, rho_atm);
@@ -128,28 +131,29 @@ ASX_DEV Atmo make_atmo(const IdealGas &eos, const C2PArgs &a, double r) {
   if (a.thermal_eos_atmo) {
     if (a.use_press_atmo) {
       press_atm = (r > a.r_atmo) ? (a.p_atmo * pow(a.r_atmo / r, a.n_press_atmo)) : a.p_atmo;
-      press_atm = smax(eos.press_rt(rho_atm, eos.temp_min), press_atm);
-      eps_atm = eos.eps_rp(rho_atm, press_atm);
-      temp_atm = eos.temp_re(eps_atm);
+      press_atm = smax(eos.press_rt(rho_atm, eos.temp_min, a.Ye_atmo), press_atm);
+      eps_atm = eos.eps_rp(rho_atm, press_atm, a.Ye_atmo);
+      temp_atm = eos.temp_re(rho_atm, eps_atm, a.Ye_atmo);
     } else {
       temp_atm = (r > a.r_atmo) ? (a.t_atmo * pow(a.r_atmo / r, a.n_temp_atmo)) : a.t_atmo;
       temp_atm = smax(eos.temp_min, temp_atm);
-      press_atm = eos.press_rt(rho_atm, temp_atm);
-      eps_atm = eos.eps_rt(temp_atm);
+      press_atm = eos.press_rt(rho_atm, temp_atm, a.Ye_atmo);
+      eps_atm = eos.eps_rt(rho_atm, temp_atm, a.Ye_atmo);
     }
   } else {
     const double gm1 = a.poly_np1 * pow(rho_atm / a.poly_rho_p, a.poly_invn);
     eps_atm = gm1 / a.poly_gamma;
     eps_atm = smin(smax(eos.eps_min, eps_atm), eos.eps_max);
-    press_atm = eos.press_re(rho_atm, eps_atm);
+    press_atm = eos.press_re(rho_atm, eps_atm, a.Ye_atmo);
   }
-  const double entropy_atm = eos.kappa(rho_atm, eps_atm);
+  const double entropy_atm = eos.kappa_re(rho_atm, eps_atm, a.Ye_atmo);
   return Atmo{rho_atm, eps_atm, a.Ye_atmo, press_atm, temp_atm, entropy_atm,
               rho_atm * (1 + a.atmo_tol)};
 }
 
 // c2p::prims_floors_and_ceilings (Con2PrimFactory/src/c2p.hxx:86-216)
-ASX_DEV void prims_floors(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a,
+template <class E>
+ASX_DEV void prims_floors(const E &eos, const Atmo &atmo, const C2PArgs &a,
                           Prim &pv, const Cons &cv, const Sym3 &glo, Report &rep) {
   bool recomp = false;
   if (pv.Ye < eos.ye_min) { pv.Ye = eos.ye_min; rep.adjust_cons = true; recomp = true; }
@@ -167,9 +171,9 @@ ASX_DEV void prims_floors(const IdealGas &eos, const Atmo &atmo, const C2PArgs &
       recomp = true;
     } else {
       recomp = false;
-      pv.eps = eos.eps_rp(pv.rho, pv.press);
-      pv.temperature = eos.temp_re(pv.eps);
-      pv.entropy = eos.kappa(pv.rho, pv.eps);
+      pv.eps = eos.eps_rp(pv.rho, pv.press, pv.Ye);
+      pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
     }
   }
   if (pv.rho > eos.rho_max) {
@@ -179,9 +183,9 @@ ASX_DEV void prims_floors(const IdealGas &eos, const Atmo &atmo, const C2PArgs &
       recomp = true;
     } else {
       recomp = false;
-      pv.eps = eos.eps_rp(pv.rho, pv.press);
-      pv.temperature = eos.temp_re(pv.eps);
-      pv.entropy = eos.kappa(pv.rho, pv.eps);
+      pv.eps = eos.eps_rp(pv.rho, pv.press, pv.Ye);
+      pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
     }
   }
   if (pv.temperature > eos.temp_max) {
@@ -192,9 +196,9 @@ ASX_DEV void prims_floors(const IdealGas &eos, const Atmo &atmo, const C2PArgs &
   if (a.use_press_atmo) {
     if (pv.press < atmo.press) {
       pv.press = atmo.press;
-      pv.eps = eos.eps_rp(pv.rho, pv.press);
-      pv.temperature = eos.temp_re(pv.eps);
-      pv.entropy = eos.kappa(pv.rho, pv.eps);
+      pv.eps = eos.eps_rp(pv.rho, pv.press, pv.Ye);
+      pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
       recomp = false;
       rep.adjust_cons = true;
     }
@@ -204,15 +208,15 @@ ASX_DEV void prims_floors(const IdealGas &eos, const Atmo &atmo, const C2PArgs &
     rep.adjust_cons = true;
   }
   if (recomp) {
-    pv.eps = eos.eps_rt(pv.temperature);
-    pv.press = eos.press_rt(pv.rho, pv.temperature);
-    pv.entropy = eos.kappa(pv.rho, pv.eps);
+    pv.eps = eos.eps_rt(pv.rho, pv.temperature, pv.Ye);
+    pv.press = eos.press_rt(pv.rho, pv.temperature, pv.Ye);
+    pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
   }
 }
 
 // c2p::bh_interior (Con2PrimFactory/src/c2p.hxx:218-328)
-template <bool limiting>
-ASX_DEV void bh_interior(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a,
+template <bool limiting, class E>
+ASX_DEV void bh_interior(const E &eos, const Atmo &atmo, const C2PArgs &a,
                          Prim &pv, Cons &cv, const Sym3 &glo) {
   const double wlim_BH = sqrt(1.0 + a.vwlim_BH * a.vwlim_BH);
   const double vlim_BH = a.vwlim_BH / wlim_BH;
@@ -228,18 +232,18 @@ ASX_DEV void bh_interior(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a
       recomp = true;
     }
     if (recomp) {
-      pv.temperature = eos.temp_re(pv.eps);
-      pv.press = eos.press_re(pv.rho, pv.eps);
-      pv.entropy = eos.kappa(pv.rho, pv.eps);
+      pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
+      pv.press = eos.press_re(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
       cons_from_prim(glo, pv, cv);
     }
   } else {
     pv.rho = a.rho_BH;
     pv.eps = a.eps_BH;
     pv.Ye = atmo.ye;
-    pv.temperature = eos.temp_re(pv.eps);
-    pv.press = eos.press_re(pv.rho, pv.eps);
-    pv.entropy = eos.kappa(pv.rho, pv.eps);
+    pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
+    pv.press = eos.press_re(pv.rho, pv.eps, pv.Ye);
+    pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
     const double detg = det3(glo);
     const Sym3 gup = inv3(glo, detg);
     const double Z = (pv.rho * (1.0 + pv.eps) + pv.press) * wlim_BH * wlim_BH;
@@ -465,11 +469,11 @@ ASX_DEV void noble_solve(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a
 }
 
 // Palenzuela residual f(x), c2p_1DPalenzuela.hxx:260-304
-struct PalF {
-  const IdealGas &eos;
+template <class E> struct PalF {
+  const E &eos;
   const Atmo &atmo;
   const C2PArgs &a;
-  double q, r, s, t, dens;
+  double q, r, s, t, dens, ye;
   ASX_DEV double operator()(double x) const {
     double Wm2 = 1.0 - (x * x * r + (2.0 * x + s) * t * t) / (x * x * (x + s) * (x + s));
     Wm2 = fmin(fmax(Wm2, 1e-10), 1);
@@ -481,7 +485,7 @@ struct PalF {
       eps = smax(eps, atmo.eps);
       eps = smin(eps, eos.eps_max);
     }
-    const double P = eos.press_re(rho, eps);
+    const double P = eos.press_re(rho, eps, ye);
     return (x - (1.0 + eps + P / rho) * W);
   }
 };
@@ -553,7 +557,8 @@ ASX_DEV void brent(const F &fn, double a, double b, double rel, int maxiters,
 }
 
 // Palenzuela 1D, c2p_1DPalenzuela.hxx:306-556
-ASX_DEV void palenzuela_solve(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a,
+template <class E>
+ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
                               Prim &pv, Cons &cv, const Sym3 &glo, Report &rep) {
   rep.iters = 0;
   rep.adjust_cons = false;
@@ -591,7 +596,7 @@ ASX_DEV void palenzuela_solve(const IdealGas &eos, const Atmo &atmo, const C2PAr
   const double s = Bsq / cv.dens;
   const double r = Ssq / pow2(cv.dens);
   const double t = BiSi / pow(cv.dens, 3. / 2.);
-  const PalF fn{eos, atmo, a, q, r, s, t, cv.dens};
+  const PalF<E> fn{eos, atmo, a, q, r, s, t, cv.dens, cv.DYe / cv.dens};
   const double xa = smax(1e-10, 1.0 + q - s);
   double xb = 2.0 + 2.0 * q - s;
   if (fn(xa) * fn(xb) > 0) xb = 3.0 + 3.0 * q - 1.5 * s;
@@ -611,9 +616,9 @@ ASX_DEV void palenzuela_solve(const IdealGas &eos, const Atmo &atmo, const C2PAr
     }
     vel_from_Z(x * pv.rho * W, W, Bsq, BiSi, cv, gup, glo, a.use_z, pv);
     pv.Ye = cv.DYe / cv.dens;
-    pv.press = eos.press_re(pv.rho, pv.eps);
-    pv.temperature = eos.temp_re(pv.eps);
-    pv.entropy = eos.kappa(pv.rho, pv.eps);
+    pv.press = eos.press_re(pv.rho, pv.eps, pv.Ye);
+    pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
+    pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
     pv.B[0] = cv.dB[0]; pv.B[1] = cv.dB[1]; pv.B[2] = cv.dB[2];
   }
   if (pv.rho <= 0.0) {
@@ -674,8 +679,8 @@ struct C2POut {
 // compiled for that method (a.prime must equal it); returns false, with
 // `out` not set, when it fails: the zone then needs the full chain, which
 // repeats the same primary solve before the backup.
-template <int PRIME>
-ASX_DEV bool con2prim_zone(const IdealGas &eos, const C2PArgs &a, const C2PIn &in,
+template <int PRIME, class E>
+ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
                            C2POut &out, double zguess = -1.0, double vguess = -1.0) {
   const Atmo atmo = make_atmo(eos, a, in.r);
   const Sym3 &glo = in.glo;
@@ -704,9 +709,14 @@ ASX_DEV bool con2prim_zone(const IdealGas &eos, const C2PArgs &a, const C2PIn &i
   seed.rho = in.saved_rho;
   seed.eps = in.saved_eps;
   seed.Ye = in.saved_Ye;
-  seed.press = eos.press_re(in.saved_rho, in.saved_eps);
+  {
+    // both calls take the saved eps by reference (a table may clamp it for
+    // the second); the seed keeps the value as loaded
+    double e = in.saved_eps;
+    seed.press = eos.press_re(in.saved_rho, e, in.saved_Ye);
+    seed.entropy = eos.kappa_re(in.saved_rho, e, in.saved_Ye);
+  }
   seed.temperature = in.temperature;
-  seed.entropy = eos.kappa(in.saved_rho, in.saved_eps);
   seed.vel[0] = v_up[0]; seed.vel[1] = v_up[1]; seed.vel[2] = v_up[2];
   seed.w_lor = sqrt(1.0 + zsq);
   seed.B[0] = Bup[0]; seed.B[1] = Bup[1]; seed.B[2] = Bup[2];
@@ -733,14 +743,23 @@ ASX_DEV bool con2prim_zone(const IdealGas &eos, const C2PArgs &a, const C2PIn &i
   if (call_c2p) {
     code = ASX_C2P_FLAG_PRIME;
     const int prime = PRIME < 0 ? a.prime : PRIME;
-    if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
-    else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1);
+    if constexpr (std::is_same<E, IdealGas>::value) {
+      if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
+      else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1);
+    } else {
+      (void)prime; // Noble needs the ideal-fluid Gamma; refused for tables at the C ABI
+      palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1);
+    }
     out.iters = rep1.iters;
     if (PRIME >= 0 && rep1.failed()) return false;
     if (PRIME < 0 && rep1.failed()) {
       code = ASX_C2P_FLAG_SECOND;
-      if (a.second == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep2);
-      else if (a.second == ASX_C2P_PALENZUELA) palenzuela_solve(eos, atmo, a, pv, cv, glo, rep2);
+      if constexpr (std::is_same<E, IdealGas>::value) {
+        if (a.second == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep2);
+        else if (a.second == ASX_C2P_PALENZUELA) palenzuela_solve(eos, atmo, a, pv, cv, glo, rep2);
+      } else {
+        if (a.second == ASX_C2P_PALENZUELA) palenzuela_solve(eos, atmo, a, pv, cv, glo, rep2);
+      }
     }
     if (rep1.failed() && rep2.failed()) {
       flag_local = false;
@@ -761,9 +780,64 @@ ASX_DEV bool con2prim_zone(const IdealGas &eos, const C2PArgs &a, const C2PIn &i
   out.cv = cv;
   return true;
 }
-ASX_DEV void con2prim_cell(const IdealGas &eos, const C2PArgs &a, const C2PIn &in,
-                           C2POut &out, double zguess = -1.0, double vguess = -1.0) {
+template <class E>
+ASX_DEV void con2prim_cell(const E &eos, const C2PArgs &a, const C2PIn &in, C2POut &out,
+                           double zguess = -1.0, double vguess = -1.0) {
   (void)con2prim_zone<-1>(eos, a, in, out, zguess, vguess);
+}
+
+// Append `value` of every work-item with `fail` set to the queue: a
+// wavefront ballot, one atomic per wavefront, lanes in order.
+ASX_DEV void queue_push(bool fail, int value, int *queue, unsigned *qcount) {
+  const uint64_t m = __ballot(fail);
+  if (m) {
+    const int lane = threadIdx.x & 63;
+    const int leader = __builtin_ctzll(m);
+    unsigned base = 0;
+    if (lane == leader) base = atomicAdd(qcount, (unsigned)__popcll(m));
+    base = __shfl(base, leader);
+    if (fail) queue[base + __popcll(m & ((1ull << lane) - 1))] = value;
+  }
+}
+
+// Independent zone z of a batch (layout of asx_c2p_batch: cons[8][nz],
+// g[6][nz], seed[5][nz]); no grid position, lapse 1, no mask.
+ASX_DEV void batch_zone_in(int64_t z, int64_t nz, const double *cons, const double *g,
+                           const double *seed, const C2PArgs &a, C2PIn &in) {
+  in.r = 0.0;
+  in.alp = 1.0;
+  in.mask = 1.0;
+  in.glo = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  in.cv.dens = cons[z];
+  in.cv.mom[0] = cons[nz + z];
+  in.cv.mom[1] = cons[2 * nz + z];
+  in.cv.mom[2] = cons[3 * nz + z];
+  in.cv.tau = cons[4 * nz + z];
+  in.cv.dB[0] = cons[5 * nz + z];
+  in.cv.dB[1] = cons[6 * nz + z];
+  in.cv.dB[2] = cons[7 * nz + z];
+  in.cv.DYe = in.cv.dens * a.Ye_atmo;
+  in.cv.DEnt = 0.0;
+  in.saved_rho = seed[z];
+  in.saved_eps = seed[nz + z];
+  in.saved_Ye = a.Ye_atmo;
+  in.saved_vel[0] = seed[2 * nz + z];
+  in.saved_vel[1] = seed[3 * nz + z];
+  in.saved_vel[2] = seed[4 * nz + z];
+  in.temperature = 0.0;
+}
+// prim[8][nz] = rho, eps, press, vel^i, w_lor, temperature; flag[nz]
+ASX_DEV void batch_zone_out(int64_t z, int64_t nz, const C2POut &o, double *prim,
+                            double *flag) {
+  prim[z] = o.pv.rho;
+  prim[nz + z] = o.pv.eps;
+  prim[2 * nz + z] = o.pv.press;
+  prim[3 * nz + z] = o.pv.vel[0];
+  prim[4 * nz + z] = o.pv.vel[1];
+  prim[5 * nz + z] = o.pv.vel[2];
+  prim[6 * nz + z] = o.pv.w_lor;
+  prim[7 * nz + z] = o.pv.temperature;
+  flag[z] = o.flag;
 }
 
 } // namespace asx

@@ -380,6 +380,32 @@ C2PQueue *c2p_queue(void *stream, int64_t nz) {
   return &Q;
 }
 
+// Device scratch per (device, stream, tag), grown on demand.
+enum { SCR_MLP_GUESS = 1 };
+struct DevScratch {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+std::map<std::tuple<int, void *, int>, DevScratch> g_dscratch;
+
+void *dev_scratch(int tag, void *stream, size_t bytes) {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  std::lock_guard<std::mutex> lk(g_mu);
+  DevScratch &S = g_dscratch[std::make_tuple(dev, stream, tag)];
+  if (S.bytes < bytes) {
+    // the previous buffer may still be read by work queued on this stream
+    if (S.p && (hipStreamSynchronize((hipStream_t)stream) != hipSuccess ||
+                hipFree(S.p) != hipSuccess))
+      return nullptr;
+    S.p = nullptr;
+    S.bytes = 0;
+    if (hipMalloc(&S.p, bytes) != hipSuccess) return nullptr;
+    S.bytes = bytes;
+  }
+  return S.p;
+}
+
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -1051,11 +1077,17 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
   if (par->c2p_prime != ASX_C2P_NOBLE) return ASX_ERR_UNSUPPORTED;
   if (nz < 0) return ASX_ERR_SIZE;
   if (kguess < 1) return ASX_ERR_PARAM;
+  if (nz >= (int64_t(1) << 31)) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !seed || !blob || !prim || !flag || !iters))
     return ASX_ERR_NULL;
+  if (nz == 0) return ASX_OK;
+  double *guess = (double *)dev_scratch(SCR_MLP_GUESS, stream, (size_t)nz * 2 * sizeof(double));
+  C2PQueue *Q = c2p_queue(stream, nz);
+  if (!guess || !Q) return ASX_ERR_DEVICE;
   return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] {
     return launch_c2p_batch_mlp(nz, cons, g, seed, blob, kguess, prim, flag, iters, fallback,
-                                ideal_gas(par), c2p_args(par), (hipStream_t)stream);
+                                ideal_gas(par), c2p_args(par), guess, Q->q, Q->n,
+                                (hipStream_t)stream);
   }));
 }
 

@@ -382,6 +382,15 @@ struct IdealGas {
   ASX_DEV double kappa(double rho, double eps) const {
     return (gamma - 1.0) * eps * pow(rho, 1.0 - gamma);
   }
+  // The 3-parameter interface shared with TabEOS (EOSX eos_3p.hxx): Ye is
+  // not used by the ideal gas, and eps is never changed.
+  ASX_DEV double press_re(double rho, double &eps, double) const { return press_re(rho, eps); }
+  ASX_DEV double temp_re(double, double &eps, double) const { return temp_re(eps); }
+  ASX_DEV double kappa_re(double rho, double &eps, double) const { return kappa(rho, eps); }
+  ASX_DEV double csnd_re(double, double &eps, double) const { return csnd(eps); }
+  ASX_DEV double eps_rt(double, double temp, double) const { return eps_rt(temp); }
+  ASX_DEV double press_rt(double rho, double temp, double) const { return press_rt(rho, temp); }
+  ASX_DEV double eps_rp(double rho, double press, double) const { return eps_rp(rho, press); }
 };
 
 // v2f metric average at a face normal to D (AsterUtils aster_interp.hxx:237-249)

@@ -194,11 +194,12 @@ hipError_t launch_mlp_grad(int64_t nz, const double *cons, const double *g,
 hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float *m,
                            float *v, float lr, float b1, float b2, float eps, int64_t step,
                            float *loss, float *grad, hipStream_t s);
+// guess: scratch of 2 nz doubles; queue: nz ints; qcount: one unsigned
 hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
-                                hipStream_t s);
+                                double *guess, int *queue, unsigned *qcount, hipStream_t s);
 
 hipError_t launch_c2p_batch(int64_t nz, const double *cons, const double *g,
                             const double *seed, double *prim, double *flag,

@@ -139,9 +139,12 @@ __global__ __launch_bounds__(256) void k_prim2con_batch(int64_t nz, const double
 // ---------------------------------------------------------------------------
 // Inputs from the densitized conserved variables: log D, log S^2, log tau,
 // log B^2 (undensitized, S^2 = gamma^ij S_i S_j, B^2 = gamma_ij B^i B^j) in
-// units of E = D + tau (so scale-free), and the cosine of S and B; small
-// floors keep the logs finite.
-ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, double x[5]) {
+// units of E = D + tau (so scale-free: the ideal-gas con2prim is invariant
+// under D, S, tau -> l D, l S, l tau and B -> sqrt(l) B), and the cosine of
+// S and B.  tau is floored at 1e-15 D and the ratios S^2/E^2, B^2/E at
+// 1e-30; the ratios are formed in double precision and their logarithms
+// taken in single precision (the network's own precision).
+ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, float x[5]) {
   const double detg = det3(gl), sq = sqrt(detg);
   const Sym3 gu = inv3(gl, detg);
   const double is = 1.0 / sq;
@@ -152,17 +155,23 @@ ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, double x[5]) {
   lower(gu, S, Su);
   lower(gl, B, Bl);
   const double S2 = dot3(S, Su), B2 = dot3(B, Bl), SB = dot3(S, B);
-  const double floor2 = 1e-30 * D * D;
   const double taup = tau > 1e-15 * D ? tau : 1e-15 * D;
-  // in units of E = D + tau: the ideal-gas con2prim is invariant under
-  // D, S, tau -> l D, l S, l tau and B -> sqrt(l) B
-  const double lE = log(D + taup);
-  x[0] = log(D) - lE;
-  x[1] = log(S2 + floor2) - 2.0 * lE;
-  x[2] = log(taup) - lE;
-  x[3] = log(B2 + floor2) - lE;
+  const double iE = 1.0 / (D + taup);
+  const double rS = S2 * iE * iE, rB = B2 * iE;
+  x[0] = __logf((float)(D * iE));
+  x[1] = __logf((float)(rS > 1e-30 ? rS : 1e-30));
+  x[2] = __logf((float)(taup * iE));
+  x[3] = __logf((float)(rB > 1e-30 ? rB : 1e-30));
   const double den = sqrt(S2 * B2);
-  x[4] = den > 0.0 ? SB / den : 0.0;
+  x[4] = den > 0.0 ? (float)(SB / den) : 0.0f;
+}
+
+// tanh from the hardware exp2 and reciprocal, 1 - 2 / (2^(2 x / ln 2) + 1):
+// within about 1e-7 of tanh, saturating to +-1 (the network's activation in
+// both inference and training)
+ASX_DEV float fast_tanh(float x) {
+  const float t = __builtin_amdgcn_exp2f(2.8853900817779268f * x);
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
 }
 
 // bf16 round-to-nearest-even and back
@@ -193,7 +202,7 @@ ASX_DEV bf16x4 frag(const uint16_t *t, int row, int col) {
 // Normalised inputs of zone z (zeros for lanes past the batch).
 ASX_DEV void zone_inputs(bool live, int64_t z, int64_t nz, const double *cons,
                          const double *g, const float *blob, float xn[5]) {
-  double x[5] = {0, 0, 0, 0, 0};
+  float x[5] = {0, 0, 0, 0, 0};
   if (live) {
     double c[8];
     for (int q = 0; q < 8; ++q) c[q] = cons[q * nz + z];
@@ -201,7 +210,7 @@ ASX_DEV void zone_inputs(bool live, int64_t z, int64_t nz, const double *cons,
     mlp_features(c, gl, x);
   }
   for (int i = 0; i < 5; ++i)
-    xn[i] = live ? ((float)x[i] - blob[kMlpInMean + i]) * blob[kMlpInIstd + i] : 0.0f;
+    xn[i] = live ? (x[i] - blob[kMlpInMean + i]) * blob[kMlpInIstd + i] : 0.0f;
 }
 
 // Stage W2 (bf16, [j][k]) of the blob into LDS; every work-item of the block.
@@ -220,7 +229,7 @@ ASX_DEV void layer1(const float *blob, const float xb[5], uint16_t *h1, uint16_t
   for (int j = 0; j < kMlpH; ++j) {
     float zj = blob[kMlpB1 + j];
     for (int i = 0; i < kMlpIn; ++i) zj += bfr(blob[kMlpW1 + j * kMlpIn + i]) * xb[i];
-    const uint16_t hb = f2bf(tanhf(zj));
+    const uint16_t hb = f2bf(fast_tanh(zj));
     h1[l * kTS + j] = hb;
     if (h1t) h1t[j * kTS + l] = hb;
   }
@@ -263,7 +272,7 @@ ASX_DEV void mlp_wave(const float *blob, const uint16_t *w2s, uint16_t *act, con
     for (int ob = 0; ob < 4; ++ob)
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + 4 * (l >> 4) + r, o = ob * 16 + (l & 15);
-        act[zr * kTS + o] = f2bf(tanhf(acc[rt][ob][r] + blob[kMlpB2 + o]));
+        act[zr * kTS + o] = f2bf(fast_tanh(acc[rt][ob][r] + blob[kMlpB2 + o]));
       }
   __syncthreads();
   for (int o = 0; o < kMlpOut; ++o) {
@@ -296,11 +305,12 @@ __global__ __launch_bounds__(256) void k_mlp_features(int64_t nz, const double *
                                                       const double *g, float *xo) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= nz) return;
-  double c[8], x[5];
+  double c[8];
+  float x[5];
   for (int q = 0; q < 8; ++q) c[q] = cons[q * nz + z];
   const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
   mlp_features(c, gl, x);
-  for (int i = 0; i < 5; ++i) xo[i * nz + z] = (float)x[i];
+  for (int i = 0; i < 5; ++i) xo[i * nz + z] = x[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -347,7 +357,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
     for (int ob = 0; ob < 4; ++ob)
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + k4 + r, o = ob * 16 + r16;
-        const float h = tanhf(acc[rt][ob][r] + blob[kMlpB2 + o]);
+        const float h = fast_tanh(acc[rt][ob][r] + blob[kMlpB2 + o]);
         acc[rt][ob][r] = h;
         a2[zr * kTS + o] = f2bf(h);
       }
@@ -457,7 +467,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
         const int zr = rt * 16 + k4 + r;
         float zk = blob[kMlpB1 + k];
         for (int i = 0; i < kMlpIn; ++i) zk += wk[i] * xs[zr][i];
-        const float h = tanhf(zk);
+        const float h = fast_tanh(zk);
         const float dz = dh1[rt][kt][r] * (1.f - h * h);
         sd += dz;
         for (int i = 0; i < kMlpIn; ++i) sw[i] += dz * xs[zr][i];
@@ -495,79 +505,75 @@ __global__ __launch_bounds__(256) void k_mlp_adam(int nw, const float *partial, 
 }
 
 // ---------------------------------------------------------------------------
-// Con2prim from the network's guess: Noble from (Z, v^2) of the network for
-// at most kguess iterations; zones it does not converge run the full chain
-// from the plain seed (fallback[z] = 1), with iters counting both.
+// Con2prim from the network's guess, after the forward pass has written
+// guess[2][nz] (log W, log(Z/D)), in two passes:
+//   1. Noble from (Z, v^2) of the guess for at most kguess iterations;
+//      converged zones are stored, the others queued by a wavefront ballot
+//      (the iterations they spent are kept in iters);
+//   2. the queued zones run the full chain from the plain seed, as
+//      asx_c2p_batch would (fallback[z] = 1), iters counting both.
+// The primary-only pass carries neither the backup solver nor the network,
+// so it runs at the occupancy of the plain solver.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_c2p_batch_mlp(int64_t nz, const double *cons,
-                                                       const double *g, const double *seed,
-                                                       const float *blob, int kguess,
-                                                       double *prim, double *flag,
-                                                       int32_t *iters, int32_t *fallback,
-                                                       IdealGas eos, C2PArgs a) {
-  __shared__ uint16_t w2s[kMlpH * kTS];
-  __shared__ uint16_t act[4][kMlpH * kTS];
-  stage_w2(blob, w2s, nullptr);
-  __syncthreads();
-  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = z < nz;
-  float xn[5], y[2];
-  zone_inputs(live, z, nz, cons, g, blob, xn);
-  mlp_wave(blob, w2s, act[threadIdx.x >> 6], xn, y);
-  if (!live) return;
-  C2PIn in;
-  in.r = 0.0;
-  in.alp = 1.0;
-  in.mask = 1.0;
-  in.glo = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
-  in.cv.dens = cons[z];
-  in.cv.mom[0] = cons[nz + z];
-  in.cv.mom[1] = cons[2 * nz + z];
-  in.cv.mom[2] = cons[3 * nz + z];
-  in.cv.tau = cons[4 * nz + z];
-  in.cv.dB[0] = cons[5 * nz + z];
-  in.cv.dB[1] = cons[6 * nz + z];
-  in.cv.dB[2] = cons[7 * nz + z];
-  in.cv.DYe = in.cv.dens * a.Ye_atmo;
-  in.cv.DEnt = 0.0;
-  in.saved_rho = seed[z];
-  in.saved_eps = seed[nz + z];
-  in.saved_Ye = a.Ye_atmo;
-  in.saved_vel[0] = seed[2 * nz + z];
-  in.saved_vel[1] = seed[3 * nz + z];
-  in.saved_vel[2] = seed[4 * nz + z];
-  in.temperature = 0.0;
-  const double lw = (double)(y[0] * blob[kMlpOutStd] + blob[kMlpOutMean]);
-  const double lz = (double)(y[1] * blob[kMlpOutStd + 1] + blob[kMlpOutMean + 1]);
-  const double W = smax(1.0, exp(lw));
-  const double vsq = 1.0 - 1.0 / (W * W);
+ASX_DEV void guess_zv(const double *guess, int64_t z, int64_t nz, const C2PIn &in, double &Zg,
+                      double &vsq) {
+  const double W = smax(1.0, exp(guess[z]));
+  vsq = 1.0 - 1.0 / (W * W);
   const double D = in.cv.dens / sqrt(det3(in.glo));
-  double Zg = D * exp(lz);
+  Zg = D * exp(guess[nz + z]);
   if (!(Zg > 0.0) || !isfinite(Zg) || !isfinite(vsq)) Zg = -1.0; // plain seed instead
-  C2PArgs ak = a;
-  ak.max_iter = kguess;
-  C2POut o;
-  int fb = 0, it;
-  if (con2prim_zone<ASX_C2P_NOBLE>(eos, ak, in, o, Zg, vsq)) {
-    it = o.iters;
-  } else {
-    const int spent = o.iters;
-    con2prim_cell(eos, a, in, o);
-    it = spent + o.iters;
-    fb = 1;
-  }
-  prim[z] = o.pv.rho;
-  prim[nz + z] = o.pv.eps;
-  prim[2 * nz + z] = o.pv.press;
-  prim[3 * nz + z] = o.pv.vel[0];
-  prim[4 * nz + z] = o.pv.vel[1];
-  prim[5 * nz + z] = o.pv.vel[2];
-  prim[6 * nz + z] = o.pv.w_lor;
-  prim[7 * nz + z] = o.pv.temperature;
-  flag[z] = o.flag;
-  iters[z] = it;
-  if (fallback) fallback[z] = fb;
 }
+
+__global__ __launch_bounds__(256) void k_c2p_guess_main(int64_t nz, const double *cons,
+                                                        const double *g, const double *seed,
+                                                        const double *guess, int kguess,
+                                                        double *prim, double *flag,
+                                                        int32_t *iters, int32_t *fallback,
+                                                        IdealGas eos, C2PArgs a, int *queue,
+                                                        unsigned *qcount) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool fail = false;
+  if (z < nz) {
+    C2PIn in;
+    batch_zone_in(z, nz, cons, g, seed, a, in);
+    double Zg, vsq;
+    guess_zv(guess, z, nz, in, Zg, vsq);
+    C2PArgs ak = a;
+    ak.max_iter = kguess;
+    C2POut o;
+    if (con2prim_zone<ASX_C2P_NOBLE>(eos, ak, in, o, Zg, vsq)) {
+      batch_zone_out(z, nz, o, prim, flag);
+      if (fallback) fallback[z] = 0;
+    } else {
+      fail = true;
+    }
+    iters[z] = o.iters;
+  }
+  queue_push(fail, (int)z, queue, qcount);
+}
+
+__global__ __launch_bounds__(256) void k_c2p_guess_rest(int64_t nz, const double *cons,
+                                                        const double *g, const double *seed,
+                                                        double *prim, double *flag,
+                                                        int32_t *iters, int32_t *fallback,
+                                                        IdealGas eos, C2PArgs a,
+                                                        const int *queue,
+                                                        const unsigned *qcount) {
+  const unsigned nq = *qcount;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += gridDim.x * blockDim.x) {
+    const int64_t z = queue[q];
+    C2PIn in;
+    batch_zone_in(z, nz, cons, g, seed, a, in);
+    C2POut o;
+    con2prim_cell(eos, a, in, o);
+    batch_zone_out(z, nz, o, prim, flag);
+    iters[z] += o.iters;
+    if (fallback) fallback[z] = 1;
+  }
+}
+
+__global__ void k_zero_count(unsigned *p) { *p = 0u; }
 
 // ---------------------------------------------------------------------------
 // Launchers
@@ -622,10 +628,16 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
-                                hipStream_t s) {
+                                double *guess, int *queue, unsigned *qcount, hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_c2p_batch_mlp, grid256(nz), dim3(256), 0, s, nz, cons, g, seed, blob,
-                     kguess, prim, flag, iters, fallback, eos, a);
+  hipError_t e = launch_mlp_forward(nz, cons, g, blob, guess, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
+  hipLaunchKernelGGL(k_c2p_guess_main, grid256(nz), dim3(256), 0, s, nz, cons, g, seed, guess,
+                     kguess, prim, flag, iters, fallback, eos, a, queue, qcount);
+  const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
+  hipLaunchKernelGGL(k_c2p_guess_rest, dim3(g2), dim3(256), 0, s, nz, cons, g, seed, prim,
+                     flag, iters, fallback, eos, a, queue, qcount);
   return hipGetLastError();
 }
 

@@ -440,7 +440,9 @@ asx_status asx_mlp_train_step(int64_t nz, const double *cons, const double *g,
 /* asx_c2p_batch with the network's guess: Noble (c2p_prime must be Noble)
  * from the network's (Z, v^2) for at most kguess iterations; a zone not
  * converged by then runs the full chain from the plain seed, as
- * asx_c2p_batch would (fallback[z] = 1, optional).  iters counts both. */
+ * asx_c2p_batch would (fallback[z] = 1, optional).  iters counts both.
+ * nz < 2^31.  The guesses and the queue of unconverged zones live in
+ * library scratch per device and stream, grown on demand. */
 asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                              const double *seed, const float *blob, int32_t kguess,
                              double *prim, double *flag, int32_t *iters, int32_t *fallback,

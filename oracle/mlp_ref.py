@@ -38,7 +38,8 @@ def bf16(x):
 
 def features(cons, g):
     """x[nz, 5]: log D, log S^2, log tau, log B^2 in units of E = D + tau
-    (tau floored at 1e-15 D), and cos(S, B) (float64)."""
+    (tau floored at 1e-15 D, S^2/E^2 and B^2/E at 1e-30), and cos(S, B)
+    (float64)."""
     gxx, gxy, gxz, gyy, gyz, gzz = g
     det = 2 * gxy * gxz * gyz + gxx * gyy * gzz - gyy * gxz ** 2 - \
         gxx * gyz ** 2 - gzz * gxy ** 2
@@ -59,13 +60,12 @@ def features(cons, g):
     S2 = quad(inv, S, S)
     B2 = quad(g, B, B)
     SB = S[0] * B[0] + S[1] * B[1] + S[2] * B[2]
-    fl = 1e-30 * D * D
     den = np.sqrt(S2 * B2)
     cosv = np.where(den > 0, SB / np.where(den > 0, den, 1.0), 0.0)
     taup = np.maximum(tau, 1e-15 * D)
-    lE = np.log(D + taup)
-    return np.stack([np.log(D) - lE, np.log(S2 + fl) - 2 * lE,
-                     np.log(taup) - lE, np.log(B2 + fl) - lE, cosv], axis=1)
+    E = D + taup
+    return np.stack([np.log(D / E), np.log(np.maximum(S2 / E ** 2, 1e-30)),
+                     np.log(taup / E), np.log(np.maximum(B2 / E, 1e-30)), cosv], axis=1)
 
 
 def _params(blob, bf):

@@ -14,6 +14,7 @@
 #include "orc_physics.hpp"
 
 #include <cstdio>
+#include <type_traits>
 #include <utility>
 
 namespace orc {
@@ -120,7 +121,8 @@ struct C2PParams {
 };
 
 // Con2PrimFactory/src/c2p.hxx:86-216
-inline void prims_floors(const IdealGas &eos, const Atmo &atmo,
+template <class E>
+inline void prims_floors(const E &eos, const Atmo &atmo,
                          const C2PParams &cp, Prim &pv, const Cons &cv,
                          const Sym3 &glo, Report &rep) {
   bool recomp = false;
@@ -140,9 +142,9 @@ inline void prims_floors(const IdealGas &eos, const Atmo &atmo,
       recomp = true;
     } else {
       recomp = false;
-      pv.eps = eos.eps_from_rho_press(pv.rho, pv.press);
-      pv.temperature = eos.temp_from_rho_eps(pv.rho, pv.eps);
-      pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+      pv.eps = eos.eps_from_rho_press_ye(pv.rho, pv.press, pv.Ye);
+      pv.temperature = eos.temp_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
     }
   }
   if (pv.rho > eos.rho_max) {
@@ -152,9 +154,9 @@ inline void prims_floors(const IdealGas &eos, const Atmo &atmo,
       recomp = true;
     } else {
       recomp = false;
-      pv.eps = eos.eps_from_rho_press(pv.rho, pv.press);
-      pv.temperature = eos.temp_from_rho_eps(pv.rho, pv.eps);
-      pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+      pv.eps = eos.eps_from_rho_press_ye(pv.rho, pv.press, pv.Ye);
+      pv.temperature = eos.temp_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
     }
   }
   if (pv.temperature > eos.temp_max) {
@@ -165,9 +167,9 @@ inline void prims_floors(const IdealGas &eos, const Atmo &atmo,
   if (cp.use_press_atmo) {
     if (pv.press < atmo.press) {
       pv.press = atmo.press;
-      pv.eps = eos.eps_from_rho_press(pv.rho, pv.press);
-      pv.temperature = eos.temp_from_rho_eps(pv.rho, pv.eps);
-      pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+      pv.eps = eos.eps_from_rho_press_ye(pv.rho, pv.press, pv.Ye);
+      pv.temperature = eos.temp_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
       recomp = false;
       rep.adjust_cons = true;
     }
@@ -179,15 +181,15 @@ inline void prims_floors(const IdealGas &eos, const Atmo &atmo,
     }
   }
   if (recomp) {
-    pv.eps = eos.eps_from_rho_temp(pv.rho, pv.temperature);
-    pv.press = eos.press_from_rho_temp(pv.rho, pv.temperature);
-    pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+    pv.eps = eos.eps_from_rho_temp_ye(pv.rho, pv.temperature, pv.Ye);
+    pv.press = eos.press_from_rho_temp_ye(pv.rho, pv.temperature, pv.Ye);
+    pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
   }
 }
 
 // Con2PrimFactory/src/c2p.hxx:218-328
-template <bool limiting>
-inline void bh_interior(const IdealGas &eos, const Atmo &atmo,
+template <bool limiting, class E>
+inline void bh_interior(const E &eos, const Atmo &atmo,
                         const C2PParams &cp, Prim &pv, Cons &cv,
                         const Sym3 &glo) {
   const double wlim_BH = std::sqrt(1.0 + cp.vwlim_BH * cp.vwlim_BH);
@@ -204,18 +206,18 @@ inline void bh_interior(const IdealGas &eos, const Atmo &atmo,
       recomp = true;
     }
     if (recomp) {
-      pv.temperature = eos.temp_from_rho_eps(pv.rho, pv.eps);
-      pv.press = eos.press_from_rho_eps(pv.rho, pv.eps);
-      pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+      pv.temperature = eos.temp_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+      pv.press = eos.press_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+      pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
       cv.from_prim(pv, glo);
     }
   } else {
     pv.rho = cp.rho_BH;
     pv.eps = cp.eps_BH;
     pv.Ye = atmo.ye;
-    pv.temperature = eos.temp_from_rho_eps(pv.rho, pv.eps);
-    pv.press = eos.press_from_rho_eps(pv.rho, pv.eps);
-    pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+    pv.temperature = eos.temp_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+    pv.press = eos.press_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+    pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
     const double detg = calc_det(glo);
     const Sym3 gup = calc_inv(glo, detg);
     const double Z = (pv.rho * (1.0 + pv.eps) + pv.press) * wlim_BH * wlim_BH;
@@ -536,7 +538,8 @@ inline std::pair<double, double> brent(const F &fn, double a, double b,
 }
 
 // Con2PrimFactory/src/c2p_1DPalenzuela.hxx:260-304
-inline double palenzuela_f(const IdealGas &eos, const Atmo &atmo,
+template <class E>
+inline double palenzuela_f(const E &eos, const Atmo &atmo,
                            const C2PParams &cp, double Ssq, double Bsq,
                            double BiSi, double x, const Cons &cv) {
   const double q = cv.tau / cv.dens;
@@ -554,12 +557,13 @@ inline double palenzuela_f(const IdealGas &eos, const Atmo &atmo,
     eps = smax(eps, atmo.eps);
     eps = smin(eps, eos.eps_max);
   }
-  const double P = eos.press_from_rho_eps(rho, eps);
+  const double P = eos.press_from_rho_eps_ye(rho, eps, cv.DYe / cv.dens);
   return (x - (1.0 + eps + P / rho) * W);
 }
 
 // Con2PrimFactory/src/c2p_1DPalenzuela.hxx:306-556
-inline void palenzuela_solve(const IdealGas &eos, const Atmo &atmo,
+template <class E>
+inline void palenzuela_solve(const E &eos, const Atmo &atmo,
                              const C2PParams &cp, Prim &pv, Cons &cv,
                              const Sym3 &glo, Report &rep) {
   rep.iters = 0;
@@ -631,9 +635,9 @@ inline void palenzuela_solve(const IdealGas &eos, const Atmo &atmo,
     const double Z_Sol = x_sol * pv.rho * W;
     vel_from_Z(Z_Sol, W, Bsq, BiSi, cv, gup, glo, cp.use_z, pv);
     pv.Ye = cv.DYe / cv.dens;
-    pv.press = eos.press_from_rho_eps(pv.rho, pv.eps);
-    pv.temperature = eos.temp_from_rho_eps(pv.rho, pv.eps);
-    pv.entropy = eos.kappa_from_rho_eps(pv.rho, pv.eps);
+    pv.press = eos.press_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+    pv.temperature = eos.temp_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
+    pv.entropy = eos.kappa_from_rho_eps_ye(pv.rho, pv.eps, pv.Ye);
     for (int i = 0; i < 3; ++i) pv.B[i] = cv.dB[i];
   }
   if (pv.rho <= 0.0) {
@@ -684,7 +688,8 @@ struct AtmoParams {
       n_press_atmo, n_temp_atmo;
   bool thermal_eos_atmo, use_press_atmo;
 };
-inline Atmo make_atmo(const IdealGas &eos, const Polytrope &poly,
+template <class E>
+inline Atmo make_atmo(const E &eos, const Polytrope &poly,
                       const AtmoParams &ap, double r) {
   double rho_atm = (r > ap.r_atmo)
                        ? (ap.rho_abs_min * std::pow((ap.r_atmo / r), ap.n_rho_atmo))
@@ -696,24 +701,24 @@ inline Atmo make_atmo(const IdealGas &eos, const Polytrope &poly,
       press_atm = (r > ap.r_atmo)
                       ? (ap.p_atmo * std::pow(ap.r_atmo / r, ap.n_press_atmo))
                       : ap.p_atmo;
-      press_atm = smax(eos.press_from_rho_temp(rho_atm, eos.temp_min), press_atm);
-      eps_atm = eos.eps_from_rho_press(rho_atm, press_atm);
-      temp_atm = eos.temp_from_rho_eps(rho_atm, eps_atm);
+      press_atm = smax(eos.press_from_rho_temp_ye(rho_atm, eos.temp_min, ap.Ye_atmo), press_atm);
+      eps_atm = eos.eps_from_rho_press_ye(rho_atm, press_atm, ap.Ye_atmo);
+      temp_atm = eos.temp_from_rho_eps_ye(rho_atm, eps_atm, ap.Ye_atmo);
     } else {
       temp_atm = (r > ap.r_atmo)
                      ? (ap.t_atmo * std::pow(ap.r_atmo / r, ap.n_temp_atmo))
                      : ap.t_atmo;
       temp_atm = smax(eos.temp_min, temp_atm);
-      press_atm = eos.press_from_rho_temp(rho_atm, temp_atm);
-      eps_atm = eos.eps_from_rho_temp(rho_atm, temp_atm);
+      press_atm = eos.press_from_rho_temp_ye(rho_atm, temp_atm, ap.Ye_atmo);
+      eps_atm = eos.eps_from_rho_temp_ye(rho_atm, temp_atm, ap.Ye_atmo);
     }
   } else {
     const double gm1 = poly.gm1_from_rho(rho_atm);
     eps_atm = poly.sed_from_gm1(gm1);
     eps_atm = smin(smax(eos.eps_min, eps_atm), eos.eps_max);
-    press_atm = eos.press_from_rho_eps(rho_atm, eps_atm);
+    press_atm = eos.press_from_rho_eps_ye(rho_atm, eps_atm, ap.Ye_atmo);
   }
-  const double entropy_atm = eos.kappa_from_rho_eps(rho_atm, eps_atm);
+  const double entropy_atm = eos.kappa_from_rho_eps_ye(rho_atm, eps_atm, ap.Ye_atmo);
   const double rho_cut = rho_atm * (1 + ap.atmo_tol);
   return Atmo{rho_atm, eps_atm, ap.Ye_atmo, press_atm, temp_atm, entropy_atm,
               rho_cut};
@@ -741,7 +746,8 @@ struct C2PDriver {
   C2PParams cp;
 };
 
-inline void con2prim_cell(const IdealGas &eos, const Polytrope &poly,
+template <class E>
+inline void con2prim_cell(const E &eos, const Polytrope &poly,
                           const C2PDriver &dp, const C2PCellIn &in,
                           C2PCellOut &out) {
   const Atmo atmo = make_atmo(eos, poly, dp.ap, in.r);
@@ -774,9 +780,15 @@ inline void con2prim_cell(const IdealGas &eos, const Polytrope &poly,
   seed.rho = in.saved_rho;
   seed.eps = in.saved_eps;
   seed.Ye = in.saved_Ye;
-  seed.press = eos.press_from_rho_eps(in.saved_rho, in.saved_eps);
+  {
+    // con2prim.cxx:199-209 passes the saved eps grid function by reference
+    // to both calls (a table may clamp it for the second); the seed keeps
+    // the value read first
+    double e = in.saved_eps;
+    seed.press = eos.press_from_rho_eps_ye(in.saved_rho, e, in.saved_Ye);
+    seed.entropy = eos.kappa_from_rho_eps_ye(in.saved_rho, e, in.saved_Ye);
+  }
   seed.temperature = in.temperature;
-  seed.entropy = eos.kappa_from_rho_eps(in.saved_rho, in.saved_eps);
   for (int i = 0; i < 3; ++i) seed.vel[i] = v_up[i];
   seed.w_lor = wlor;
   for (int i = 0; i < 3; ++i) seed.B[i] = Bup[i];
@@ -806,7 +818,8 @@ inline void con2prim_cell(const IdealGas &eos, const Polytrope &poly,
   if (call_c2p) {
     code = 1; // C2P_PRIME
     if (dp.prime == 1) {
-      noble_solve(eos, atmo, dp.cp, pv, seed, cv, glo, rep1);
+      if constexpr (std::is_same<E, IdealGas>::value)
+        noble_solve(eos, atmo, dp.cp, pv, seed, cv, glo, rep1);
     } else if (dp.prime == 2) {
       palenzuela_solve(eos, atmo, dp.cp, pv, cv, glo, rep1);
     }
@@ -814,7 +827,8 @@ inline void con2prim_cell(const IdealGas &eos, const Polytrope &poly,
     if (rep1.failed()) {
       code = 2; // C2P_SECOND
       if (dp.second == 1) {
-        noble_solve(eos, atmo, dp.cp, pv, seed, cv, glo, rep2);
+        if constexpr (std::is_same<E, IdealGas>::value)
+          noble_solve(eos, atmo, dp.cp, pv, seed, cv, glo, rep2);
       } else if (dp.second == 2) {
         palenzuela_solve(eos, atmo, dp.cp, pv, cv, glo, rep2);
       }

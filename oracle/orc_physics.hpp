@@ -403,7 +403,30 @@ struct IdealGas {
   // kappa = (gamma-1) eps rho^(1-gamma)  (eos_3p_idealgas.hxx:225-229)
   double kappa_from_rho_eps(double rho, double eps) const {
     return (gamma - 1.0) * eps * std::pow(rho, 1.0 - gamma);
+  }  // The 3-parameter interface (EOSX eos_3p.hxx) shared with the tabulated
+  // EOS: Ye unused, eps never changed by the ideal gas.
+  double press_from_rho_eps_ye(double rho, double &eps, double) const {
+    return press_from_rho_eps(rho, eps);
   }
+  double temp_from_rho_eps_ye(double rho, double &eps, double) const {
+    return temp_from_rho_eps(rho, eps);
+  }
+  double kappa_from_rho_eps_ye(double rho, double &eps, double) const {
+    return kappa_from_rho_eps(rho, eps);
+  }
+  double csnd_from_rho_eps_ye(double rho, double &eps, double) const {
+    return csnd_from_rho_eps(rho, eps);
+  }
+  double eps_from_rho_temp_ye(double rho, double temp, double) const {
+    return eps_from_rho_temp(rho, temp);
+  }
+  double press_from_rho_temp_ye(double rho, double temp, double) const {
+    return press_from_rho_temp(rho, temp);
+  }
+  double eps_from_rho_press_ye(double rho, double press, double) const {
+    return eps_from_rho_press(rho, press);
+  }
+
 };
 
 // EOSX/src/eos_1p_polytropic/eos_1p_polytropic.hxx:57-89 (atmosphere with
