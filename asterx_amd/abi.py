@@ -113,6 +113,14 @@ class asx_state(C.Structure):
     _fields_ = [("gf", asx_gf * len(STATE))]
 
 
+class asx_eos_table(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("axis", C.c_void_p * 3), ("n", C.c_int32 * 3),
+                ("nvars", C.c_int32), ("energy_shift", C.c_double),
+                ("eps_min", C.c_double), ("eps_max", C.c_double),
+                ("x_lo", C.c_double * 3), ("x_step", C.c_double * 3),
+                ("x_hi", C.c_double * 3)]
+
+
 class asx_counters(C.Structure):
     _fields_ = [("nan_faces", C.c_int64), ("lo_faces", C.c_int64),
                 ("c2p_flags", C.c_int64 * 7), ("c2p_iters", C.c_int64 * 64)]
@@ -157,6 +165,7 @@ _PARAM_FIELDS = [
     ("n_temp_atmo", C.c_double, 0.0), ("alp_thresh", C.c_double, 0.0),
     ("rho_BH", C.c_double, 1e100), ("eps_BH", C.c_double, 1e100),
     ("vwlim_BH", C.c_double, 1e100), ("cons_error_limit", C.c_double, -1.0),
+    ("eos_table", C.c_void_p, None),
 ]
 
 _KEYWORDS = {"reconstruction_method": RECON, "loworder_method": RECON,
@@ -187,5 +196,8 @@ def params(**kw):
             v = _KEYWORDS[n][v]
         if isinstance(v, str):
             v = {"yes": 1, "no": 0}[v.lower()]
-        setattr(p, n, t(v).value if t is not C.c_double else float(v))
+        if t is C.c_void_p:
+            setattr(p, n, v)
+        else:
+            setattr(p, n, t(v).value if t is not C.c_double else float(v))
     return p

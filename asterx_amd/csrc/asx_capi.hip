@@ -216,6 +216,47 @@ asx_status check_par(const asx_params *p) {
   return ASX_OK;
 }
 
+// A table described by asx_eos_table, as EOSX sets it up
+// (eos_3p_tabulated3d.hxx:165-183): idx from the first spacing, the ranges
+// from the axis ends, T inverted to 1e-14 in log T (:217).
+asx_status check_table(const asx_params *p) {
+  const asx_eos_table *t = p->eos_table;
+  if (!t || !t->data || !t->axis[0] || !t->axis[1] || !t->axis[2]) return ASX_ERR_NULL;
+  if (t->nvars != 3) return ASX_ERR_PARAM;
+  for (int d = 0; d < 3; ++d)
+    if (t->n[d] < 2 || !(t->x_step[d] > 0.0)) return ASX_ERR_PARAM;
+  if (!(t->eps_max >= t->eps_min)) return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+TabEOS tab_eos(const asx_params *p) {
+  const asx_eos_table *t = p->eos_table;
+  TabEOS e;
+  e.tab = t->data;
+  for (int d = 0; d < 3; ++d) {
+    e.ax[d] = t->axis[d];
+    e.n[d] = t->n[d];
+    e.idx[d] = 1. / t->x_step[d];
+  }
+  e.shift = t->energy_shift;
+  e.rho_min = std::exp(t->x_lo[0]);
+  e.rho_max = std::exp(t->x_hi[0]);
+  e.temp_min = std::exp(t->x_lo[1]);
+  e.temp_max = std::exp(t->x_hi[1]);
+  e.ye_min = t->x_lo[2];
+  e.ye_max = t->x_hi[2];
+  e.eps_min = t->eps_min;
+  e.eps_max = t->eps_max;
+  e.tol = 1.e-14;
+  return e;
+}
+// An ideal gas or a table (entry points with composition)
+asx_status check_eos(const asx_params *p) {
+  if (!p) return ASX_ERR_NULL;
+  if (p->evolution_eos == ASX_EOS_TABULATED) return check_table(p);
+  if (p->evolution_eos != ASX_EOS_IDEALGAS) return ASX_ERR_UNSUPPORTED;
+  return ASX_OK;
+}
+
 asx_status check_c2p_par(const asx_params *p) {
   asx_status st = check_par(p);
   if (st) return st;
@@ -1088,6 +1129,102 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
     return launch_c2p_batch_mlp(nz, cons, g, seed, blob, kguess, prim, flag, iters, fallback,
                                 ideal_gas(par), c2p_args(par), guess, Q->q, Q->n,
                                 (hipStream_t)stream);
+  }));
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Independent zones with composition (any 3-parameter EOS)
+// ---------------------------------------------------------------------------
+namespace {
+// con2prim parameters with a table: Palenzuela on temperature, no
+// pressure atmosphere (the paths the reference's table supports)
+asx_status check_c2p_eos(const asx_params *p) {
+  asx_status st = check_eos(p);
+  if (st) return st;
+  if (p->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
+  if (p->max_iter < 1 || !(p->c2p_tol > 0.0)) return ASX_ERR_PARAM;
+  if (p->evolution_eos == ASX_EOS_TABULATED) {
+    if (p->c2p_prime != ASX_C2P_PALENZUELA) return ASX_ERR_UNSUPPORTED;
+    if (p->c2p_second != ASX_C2P_PALENZUELA && p->c2p_second != ASX_C2P_NONE)
+      return ASX_ERR_UNSUPPORTED;
+    if (!p->use_temperature || p->use_press_atmo) return ASX_ERR_UNSUPPORTED;
+    return ASX_OK;
+  }
+  if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA)
+    return ASX_ERR_UNSUPPORTED;
+  if (p->c2p_second != ASX_C2P_NOBLE && p->c2p_second != ASX_C2P_PALENZUELA &&
+      p->c2p_second != ASX_C2P_NONE)
+    return ASX_ERR_UNSUPPORTED;
+  return ASX_OK;
+}
+} // namespace
+
+extern "C" {
+
+asx_status asx_eos_from_rho_temp(int64_t n, const double *rho, const double *temp,
+                                 const double *ye, double *out, const asx_params *par,
+                                 void *stream) {
+  asx_status st = check_eos(par);
+  if (st) return st;
+  if (n < 0) return ASX_ERR_SIZE;
+  if (n > 0 && (!rho || !temp || !ye || !out)) return ASX_ERR_NULL;
+  hipStream_t s = (hipStream_t)stream;
+  return dev_status(timed(ASX_KC_OTHER, s, [&] {
+    return par->evolution_eos == ASX_EOS_TABULATED
+               ? launch_eos_points(0, n, rho, temp, ye, out, tab_eos(par), s)
+               : launch_eos_points(0, n, rho, temp, ye, out, ideal_gas(par), s);
+  }));
+}
+
+asx_status asx_eos_from_rho_eps(int64_t n, const double *rho, const double *eps,
+                                const double *ye, double *out, const asx_params *par,
+                                void *stream) {
+  asx_status st = check_eos(par);
+  if (st) return st;
+  if (n < 0) return ASX_ERR_SIZE;
+  if (n > 0 && (!rho || !eps || !ye || !out)) return ASX_ERR_NULL;
+  hipStream_t s = (hipStream_t)stream;
+  return dev_status(timed(ASX_KC_OTHER, s, [&] {
+    return par->evolution_eos == ASX_EOS_TABULATED
+               ? launch_eos_points(1, n, rho, eps, ye, out, tab_eos(par), s)
+               : launch_eos_points(1, n, rho, eps, ye, out, ideal_gas(par), s);
+  }));
+}
+
+asx_status asx_prim2con_batch_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                                 double *thermo, const asx_params *par, void *stream) {
+  asx_status st = check_eos(par);
+  if (st) return st;
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!prim || !g || !cons)) return ASX_ERR_NULL;
+  hipStream_t s = (hipStream_t)stream;
+  return dev_status(timed(ASX_KC_OTHER, s, [&] {
+    return par->evolution_eos == ASX_EOS_TABULATED
+               ? launch_prim2con_ye(nz, prim, g, cons, thermo, tab_eos(par), s)
+               : launch_prim2con_ye(nz, prim, g, cons, thermo, ideal_gas(par), s);
+  }));
+}
+
+asx_status asx_c2p_batch_ye(int64_t nz, const double *cons, const double *g,
+                            const double *seed, double *prim, double *flag, int32_t *iters,
+                            const asx_params *par, void *stream) {
+  asx_status st = check_c2p_eos(par);
+  if (st) return st;
+  if (nz < 0 || nz >= (int64_t(1) << 31)) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !seed || !prim || !flag || !iters)) return ASX_ERR_NULL;
+  if (nz == 0) return ASX_OK;
+  C2PQueue *Q = c2p_queue(stream, nz);
+  if (!Q) return ASX_ERR_DEVICE;
+  hipStream_t s = (hipStream_t)stream;
+  const C2PArgs ca = c2p_args(par);
+  return dev_status(timed(ASX_KC_C2P, s, [&] {
+    return par->evolution_eos == ASX_EOS_TABULATED
+               ? launch_c2p_ye(nz, cons, g, seed, prim, flag, iters, tab_eos(par), ca, Q->q,
+                               Q->n, s)
+               : launch_c2p_ye(nz, cons, g, seed, prim, flag, iters, ideal_gas(par), ca, Q->q,
+                               Q->n, s);
   }));
 }
 

@@ -1,0 +1,208 @@
+// Independent zones with composition for any 3-parameter EOS (ideal gas or
+// table): the EOSX interface on points, prim -> cons, and con2prim in two
+// passes (primary solver with a wavefront-ballot queue of the zones it
+// failed, then the reference's full chain on the queue).
+#include "asx_kernels.hpp"
+
+namespace asx {
+
+// EOSX on points: from (rho, T, Ye) -> press, eps, c_s
+// (press_from_valid_rho_temp_ye, eps_from_..., csnd_from_valid_rho_temp_ye)
+template <class E>
+__global__ __launch_bounds__(256) void k_eos_rt(int64_t n, const double *rho,
+                                                const double *temp, const double *ye,
+                                                double *out, E eos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = eos.press_rt(rho[i], temp[i], ye[i]);
+  out[n + i] = eos.eps_rt(rho[i], temp[i], ye[i]);
+  out[2 * n + i] = eos.csnd_rt(rho[i], temp[i], ye[i]);
+}
+// from (rho, eps, Ye) -> T, press, eps after the EOS's clamping
+template <class E>
+__global__ __launch_bounds__(256) void k_eos_re(int64_t n, const double *rho,
+                                                const double *eps, const double *ye,
+                                                double *out, E eos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double e = eps[i];
+  out[i] = eos.temp_re(rho[i], e, ye[i]);
+  out[n + i] = eos.press_re(rho[i], e, ye[i]);
+  out[2 * n + i] = e;
+}
+
+// prim[9][nz] = rho, T, Ye, v^i, B^i -> cons[9][nz] (dens, S_i, tau, dB^i,
+// DYe); thermo[2][nz] (optional) = eps, press.  Con2PrimFactory cons.hxx:49-91
+template <class E>
+__global__ __launch_bounds__(256) void k_prim2con_ye(int64_t nz, const double *prim,
+                                                     const double *g, double *cons,
+                                                     double *thermo, E eos) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  const double rho = prim[z], temp = prim[nz + z], ye = prim[2 * nz + z];
+  const double vel[3] = {prim[3 * nz + z], prim[4 * nz + z], prim[5 * nz + z]};
+  const double B[3] = {prim[6 * nz + z], prim[7 * nz + z], prim[8 * nz + z]};
+  const double eps = eos.eps_rt(rho, temp, ye);
+  const double press = eos.press_rt(rho, temp, ye);
+  double vl[3];
+  lower(gl, vel, vl);
+  const double w = 1.0 / sqrt(1.0 - dot3(vl, vel));
+  Cons cv;
+  cons_from_prim(gl, rho, eps, press, vel, w, B, ye, 0.0, cv);
+  cons[z] = cv.dens;
+  cons[nz + z] = cv.mom[0];
+  cons[2 * nz + z] = cv.mom[1];
+  cons[3 * nz + z] = cv.mom[2];
+  cons[4 * nz + z] = cv.tau;
+  cons[5 * nz + z] = cv.dB[0];
+  cons[6 * nz + z] = cv.dB[1];
+  cons[7 * nz + z] = cv.dB[2];
+  cons[8 * nz + z] = cv.DYe;
+  if (thermo) {
+    thermo[z] = eps;
+    thermo[nz + z] = press;
+  }
+}
+
+// Zone z: cons[9][nz] (DYe last), seed[7][nz] = rho, eps, v^i, Ye, T
+ASX_DEV void zone_in_ye(int64_t z, int64_t nz, const double *cons, const double *g,
+                        const double *seed, C2PIn &in) {
+  in.r = 0.0;
+  in.alp = 1.0;
+  in.mask = 1.0;
+  in.glo = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+  in.cv.dens = cons[z];
+  in.cv.mom[0] = cons[nz + z];
+  in.cv.mom[1] = cons[2 * nz + z];
+  in.cv.mom[2] = cons[3 * nz + z];
+  in.cv.tau = cons[4 * nz + z];
+  in.cv.dB[0] = cons[5 * nz + z];
+  in.cv.dB[1] = cons[6 * nz + z];
+  in.cv.dB[2] = cons[7 * nz + z];
+  in.cv.DYe = cons[8 * nz + z];
+  in.cv.DEnt = 0.0;
+  in.saved_rho = seed[z];
+  in.saved_eps = seed[nz + z];
+  in.saved_vel[0] = seed[2 * nz + z];
+  in.saved_vel[1] = seed[3 * nz + z];
+  in.saved_vel[2] = seed[4 * nz + z];
+  in.saved_Ye = seed[5 * nz + z];
+  in.temperature = seed[6 * nz + z];
+}
+// prim[9][nz]: the asx_c2p_batch rows, then Ye
+ASX_DEV void zone_out_ye(int64_t z, int64_t nz, const C2POut &o, double *prim, double *flag,
+                         int32_t *iters, int spent) {
+  batch_zone_out(z, nz, o, prim, flag);
+  prim[8 * nz + z] = o.pv.Ye;
+  iters[z] = spent + o.iters;
+}
+
+template <int PRIME, class E>
+__global__ __launch_bounds__(256) void k_c2p_ye_main(int64_t nz, const double *cons,
+                                                     const double *g, const double *seed,
+                                                     double *prim, double *flag,
+                                                     int32_t *iters, E eos, C2PArgs a,
+                                                     int *queue, unsigned *qcount) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool fail = false;
+  if (z < nz) {
+    C2PIn in;
+    zone_in_ye(z, nz, cons, g, seed, in);
+    C2POut o;
+    if (con2prim_zone<PRIME>(eos, a, in, o)) zone_out_ye(z, nz, o, prim, flag, iters, 0);
+    else fail = true;
+  }
+  queue_push(fail, (int)z, queue, qcount);
+}
+
+template <class E>
+__global__ __launch_bounds__(256) void k_c2p_ye_rest(int64_t nz, const double *cons,
+                                                     const double *g, const double *seed,
+                                                     double *prim, double *flag,
+                                                     int32_t *iters, E eos, C2PArgs a,
+                                                     const int *queue,
+                                                     const unsigned *qcount) {
+  const unsigned nq = *qcount;
+  for (unsigned q = blockIdx.x * blockDim.x + threadIdx.x; q < nq;
+       q += gridDim.x * blockDim.x) {
+    const int64_t z = queue[q];
+    C2PIn in;
+    zone_in_ye(z, nz, cons, g, seed, in);
+    C2POut o;
+    con2prim_cell(eos, a, in, o);
+    zone_out_ye(z, nz, o, prim, flag, iters, 0);
+  }
+}
+
+__global__ void k_zero_qcount(unsigned *p) { *p = 0u; }
+
+static dim3 g256(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+template <class E>
+hipError_t launch_eos_points_t(int mode, int64_t n, const double *rho, const double *x,
+                               const double *ye, double *out, const E &eos, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (mode == 0) hipLaunchKernelGGL(k_eos_rt<E>, g256(n), dim3(256), 0, s, n, rho, x, ye, out, eos);
+  else hipLaunchKernelGGL(k_eos_re<E>, g256(n), dim3(256), 0, s, n, rho, x, ye, out, eos);
+  return hipGetLastError();
+}
+hipError_t launch_eos_points(int mode, int64_t n, const double *rho, const double *x,
+                             const double *ye, double *out, const IdealGas &eos,
+                             hipStream_t s) {
+  return launch_eos_points_t(mode, n, rho, x, ye, out, eos, s);
+}
+hipError_t launch_eos_points(int mode, int64_t n, const double *rho, const double *x,
+                             const double *ye, double *out, const TabEOS &eos,
+                             hipStream_t s) {
+  return launch_eos_points_t(mode, n, rho, x, ye, out, eos, s);
+}
+
+template <class E>
+static hipError_t prim2con_ye_t(int64_t nz, const double *prim, const double *g,
+                                double *cons, double *thermo, const E &eos, hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_prim2con_ye<E>, g256(nz), dim3(256), 0, s, nz, prim, g, cons, thermo,
+                     eos);
+  return hipGetLastError();
+}
+hipError_t launch_prim2con_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                              double *thermo, const IdealGas &eos, hipStream_t s) {
+  return prim2con_ye_t(nz, prim, g, cons, thermo, eos, s);
+}
+hipError_t launch_prim2con_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                              double *thermo, const TabEOS &eos, hipStream_t s) {
+  return prim2con_ye_t(nz, prim, g, cons, thermo, eos, s);
+}
+
+template <int PRIME, class E>
+static hipError_t c2p_ye_t(int64_t nz, const double *cons, const double *g,
+                           const double *seed, double *prim, double *flag, int32_t *iters,
+                           const E &eos, const C2PArgs &a, int *queue, unsigned *qcount,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_qcount, dim3(1), dim3(1), 0, s, qcount);
+  hipLaunchKernelGGL((k_c2p_ye_main<PRIME, E>), g256(nz), dim3(256), 0, s, nz, cons, g, seed,
+                     prim, flag, iters, eos, a, queue, qcount);
+  const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
+  hipLaunchKernelGGL(k_c2p_ye_rest<E>, dim3(g2), dim3(256), 0, s, nz, cons, g, seed, prim,
+                     flag, iters, eos, a, queue, qcount);
+  return hipGetLastError();
+}
+hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const double *seed,
+                         double *prim, double *flag, int32_t *iters, const IdealGas &eos,
+                         const C2PArgs &a, int *queue, unsigned *qcount, hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  if (a.prime == ASX_C2P_NOBLE)
+    return c2p_ye_t<ASX_C2P_NOBLE>(nz, cons, g, seed, prim, flag, iters, eos, a, queue, qcount, s);
+  return c2p_ye_t<ASX_C2P_PALENZUELA>(nz, cons, g, seed, prim, flag, iters, eos, a, queue,
+                                      qcount, s);
+}
+hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const double *seed,
+                         double *prim, double *flag, int32_t *iters, const TabEOS &eos,
+                         const C2PArgs &a, int *queue, unsigned *qcount, hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  return c2p_ye_t<ASX_C2P_PALENZUELA>(nz, cons, g, seed, prim, flag, iters, eos, a, queue,
+                                      qcount, s);
+}
+
+} // namespace asx

@@ -201,6 +201,27 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
                                 double *guess, int *queue, unsigned *qcount, hipStream_t s);
 
+// Independent zones with composition, any 3-parameter EOS (asx_eos.hip).
+// mode 0: (rho, T, Ye) -> press, eps, c_s; mode 1: (rho, eps, Ye) -> T,
+// press, eps after clamping; out[3][n]
+hipError_t launch_eos_points(int mode, int64_t n, const double *rho, const double *x,
+                             const double *ye, double *out, const IdealGas &eos,
+                             hipStream_t s);
+hipError_t launch_eos_points(int mode, int64_t n, const double *rho, const double *x,
+                             const double *ye, double *out, const TabEOS &eos,
+                             hipStream_t s);
+hipError_t launch_prim2con_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                              double *thermo, const IdealGas &eos, hipStream_t s);
+hipError_t launch_prim2con_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                              double *thermo, const TabEOS &eos, hipStream_t s);
+// queue: nz ints, qcount: one unsigned (device)
+hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const double *seed,
+                         double *prim, double *flag, int32_t *iters, const IdealGas &eos,
+                         const C2PArgs &a, int *queue, unsigned *qcount, hipStream_t s);
+hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const double *seed,
+                         double *prim, double *flag, int32_t *iters, const TabEOS &eos,
+                         const C2PArgs &a, int *queue, unsigned *qcount, hipStream_t s);
+
 hipError_t launch_c2p_batch(int64_t nz, const double *cons, const double *g,
                             const double *seed, double *prim, double *flag,
                             int32_t *iters, const IdealGas &eos,

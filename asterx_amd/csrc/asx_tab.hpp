@@ -15,14 +15,17 @@ namespace asx {
 // Brent's ZERO (Brent 1973, "Algorithms for Minimization without
 // Derivatives", ch. 4): the root of f in [a, b] (f(a) f(b) <= 0) to
 // 2 eps |x| + t, returning the last iterate -- the form of
-// EOSX/src/utils/eos_brent.hxx:8-166.
+// EOSX/src/utils/eos_brent.hxx:8-166.  The loop is capped at kBrentCap
+// iterations: on a valid bracket ZERO needs far fewer, and the cap only ends
+// a NaN residual, which would otherwise step by t across the whole bracket.
+constexpr int kBrentCap = 1000;
 template <typename F>
 ASX_DEV double zero_brent(double a, double b, double t, const F &f) {
   double sa = a, sb = b;
   double fa = f(sa), fb = f(sb);
   double c = sa, fc = fa, e = sb - sa, d = e;
   const double macheps = 2.220446049250313e-16;
-  for (;;) {
+  for (int it = 0; it < kBrentCap; ++it) {
     if (fabs(fc) < fabs(fb)) {
       sa = sb; sb = c; c = sa;
       fa = fb; fb = fc; fc = fa;

@@ -66,3 +66,31 @@ def make_batch(nz, seed=0, gamma=5.0 / 3.0, seed_rel=1e-2, cold=False):
                        vel[2] * u()])
     true = dict(rho=rho, eps=eps, press=press, vel=vel, W=W, B=B, b2=b2)
     return cons, np.ascontiguousarray(g), np.ascontiguousarray(sd), true
+
+
+def make_batch_ye(nz, seed, rho_range, temp_range, ye_range, W_max=10.0):
+    """Zones with composition for a 3-parameter EOS: rho log-uniform in
+    rho_range, T log-uniform in temp_range, Ye uniform in ye_range, W
+    uniform in [1, W_max] along a random direction, b^2/rho log-uniform in
+    [1e-4, 1e2] along a random direction, SPD metric within 10% of the
+    identity.  Returns prim[9][nz] = rho, T, Ye, v^i, B^i (the layout of
+    asx_prim2con_batch_ye) and g[6][nz]."""
+    rng = np.random.default_rng(seed)
+    lu = lambda a, b: np.exp(rng.uniform(np.log(a), np.log(b), nz))
+    rho = lu(*rho_range)
+    temp = lu(*temp_range)
+    ye = rng.uniform(ye_range[0], ye_range[1], nz)
+    W = rng.uniform(1.0, W_max, nz)
+    M = rng.uniform(-1.0, 1.0, (6, nz)) * (0.1 / 3.0)
+    g = np.stack([1 + M[0], M[1], M[2], 1 + M[3], M[4], 1 + M[5]])
+
+    def unit(vec):
+        n = np.sqrt(np.einsum("in,in->n", vec, _lower(g, vec)))
+        return vec / n
+    vel = unit(rng.normal(size=(3, nz))) * np.sqrt(1.0 - 1.0 / (W * W))
+    nb = unit(rng.normal(size=(3, nz)))
+    b2 = rho * lu(1e-4, 1e2)
+    ndotv = np.einsum("in,in->n", nb, _lower(g, vel))
+    B = nb * np.sqrt(b2 / (1.0 / (W * W) + ndotv * ndotv))
+    prim = np.ascontiguousarray(np.concatenate([rho[None], temp[None], ye[None], vel, B]))
+    return prim, np.ascontiguousarray(g)

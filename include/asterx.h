@@ -87,6 +87,26 @@ enum {
  * files (AsterX/param.ccl, ReconX/param.ccl, EOSX/param.ccl,
  * Con2PrimFactory/param.ccl).  asx_params_default() sets the defaults.
  */
+/* A tabulated EOS as EOSX holds it after reading its table
+ * (EOSX/src/eos_3p_tabulated3d/eos_3p_tabulated3d.hxx:60-183): uniform axes
+ * log rho and log T (code units, T in MeV) and Ye, and per node the
+ * variables log p, log(eps + energy_shift) and c_s^2 in code units, the
+ * variable index fastest: data[v + 3 (i + n[0] (j + n[1] k))].  Device
+ * memory; the library reads it in place.  eps_min / eps_max are the
+ * table's eps range (EOSX rgeps): min and max of exp(log(eps + shift)) -
+ * shift over the nodes, the minimum floored at 1e-15. */
+typedef struct asx_eos_table_s {
+  const double *data;
+  const double *axis[3]; /* log rho (n[0] values), log T (n[1]), Ye (n[2]) */
+  int32_t n[3];          /* >= 2 each */
+  int32_t nvars;         /* 3 */
+  double energy_shift;
+  double eps_min, eps_max;
+  /* host copies of axis[d][0], axis[d][1] - axis[d][0], axis[d][n[d]-1]
+   * (the library does not read the device table from the host) */
+  double x_lo[3], x_step[3], x_hi[3];
+} asx_eos_table;
+
 typedef struct {
   /* ReconX */
   int32_t reconstruction_method;        /* "Godunov"       */
@@ -142,6 +162,8 @@ typedef struct {
   double alp_thresh;        /* 0            */
   double rho_BH, eps_BH, vwlim_BH; /* 1e100 each */
   double cons_error_limit;  /* -1           */
+  /* EOSX (evolution_eos = "Tabulated3d") */
+  const struct asx_eos_table_s *eos_table; /* NULL */
 } asx_params;
 
 /* ------------------------------------------------------------------------ */
@@ -447,6 +469,38 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                              const double *seed, const float *blob, int32_t kguess,
                              double *prim, double *flag, int32_t *iters, int32_t *fallback,
                              const asx_params *par, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Independent zones with composition, any 3-parameter EOS (evolution_eos  */
+/* IdealGas or Tabulated3d with eos_table)                                  */
+/* ------------------------------------------------------------------------ */
+/* The EOSX interface on n points (eos_3p.hxx; eos_3p_tabulated3d.hxx:
+ * 186-336 for a table).  From (rho, T, Ye): out[3][n] = press, eps, c_s
+ * (press_/eps_/csnd_from_valid_rho_temp_ye).  From (rho, eps, Ye):
+ * out[3][n] = T, press and eps after the EOS's clamping of eps
+ * (temp_from_valid_rho_eps_ye, then press_from_valid_rho_eps_ye). */
+asx_status asx_eos_from_rho_temp(int64_t n, const double *rho, const double *temp,
+                                 const double *ye, double *out, const asx_params *par,
+                                 void *stream);
+asx_status asx_eos_from_rho_eps(int64_t n, const double *rho, const double *eps,
+                                const double *ye, double *out, const asx_params *par,
+                                void *stream);
+/* Valencia prim -> cons with composition (Con2PrimFactory/src/cons.hxx:
+ * 49-91): prim[9][nz] = rho, T, Ye, v^i, B^i (undensitized); eps and press
+ * from the EOS at (rho, T, Ye); cons[9][nz] = dens, S_i, tau, dB^i, DYe;
+ * thermo[2][nz] (optional) = eps, press. */
+asx_status asx_prim2con_batch_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                                 double *thermo, const asx_params *par, void *stream);
+/* asx_c2p_batch with composition: cons[9][nz] (asx_c2p_batch rows, then
+ * DYe), seed[7][nz] = saved rho, eps, v^i, Ye and the temperature;
+ * prim[9][nz] = the asx_c2p_batch rows, then Ye.  With a table: Palenzuela
+ * (c2p_prime), use_temperature = yes and use_press_atmo = no, as the
+ * reference's table requires (eps_from_valid_rho_press_ye is refused there).
+ * Two passes as asx_con2prim (primary solver, then the queue of the zones
+ * it failed through the full chain); nz < 2^31. */
+asx_status asx_c2p_batch_ye(int64_t nz, const double *cons, const double *g,
+                            const double *seed, double *prim, double *flag, int32_t *iters,
+                            const asx_params *par, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched patches (replaces the driver loop that calls the scheduled        */

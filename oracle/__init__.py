@@ -34,6 +34,15 @@ _SIGS = {
     "orc_c2p_batch": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.c_void_p, C.c_void_p, C.c_void_p, _PP],
     "orc_prim2con_batch": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, _PP],
+    "orc_c2p_batch_ye": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                         C.c_void_p, C.c_void_p, C.c_void_p, _PP],
+    "orc_prim2con_batch_ye": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p, _PP],
+    "orc_eos_from_rho_temp": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p, _PP],
+    "orc_eos_from_rho_eps": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p, _PP],
+    "orc_tab_interp": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _PP],
     "asx_gf_centering": [C.c_int],
 }
 

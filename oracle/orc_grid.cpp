@@ -20,6 +20,7 @@
 // parity unpinned below 1e-8.
 #include "asterx.h"
 #include "orc_c2p.hpp"
+#include "orc_tab.hpp"
 #include "orc_physics.hpp"
 
 #include <algorithm>
@@ -117,6 +118,43 @@ Ctx make_ctx(const asx_patch *pt, const asx_params *p) {
              p->rho_min, p->rho_max, p->ye_min, p->ye_max);
   c.poly.init(p->poly_gamma, p->poly_k);
   return c;
+}
+
+// The tabulated EOS as EOSX sets it up from its table
+// (eos_3p_tabulated3d.hxx:165-183): idx from the first spacing, ranges from
+// the axis ends.
+TabEOS make_tab(const asx_eos_table *t) {
+  TabEOS e;
+  e.y = t->data;
+  for (int d = 0; d < 3; ++d) {
+    e.x[d] = t->axis[d];
+    e.n[d] = t->n[d];
+    e.idx[d] = 1. / t->x_step[d];
+  }
+  e.shift = t->energy_shift;
+  e.rho_min = std::exp(t->x_lo[0]);
+  e.rho_max = std::exp(t->x_hi[0]);
+  e.temp_min = std::exp(t->x_lo[1]);
+  e.temp_max = std::exp(t->x_hi[1]);
+  e.ye_min = t->x_lo[2];
+  e.ye_max = t->x_hi[2];
+  e.eps_min = t->eps_min;
+  e.eps_max = t->eps_max;
+  return e;
+}
+
+int check_table(const asx_params *p) {
+  const asx_eos_table *t = p->eos_table;
+  if (!t || !t->data || !t->axis[0] || !t->axis[1] || !t->axis[2]) return ASX_ERR_NULL;
+  if (t->nvars != 3 || t->n[0] < 2 || t->n[1] < 2 || t->n[2] < 2) return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+
+IdealGas make_ideal(const asx_params *p) {
+  IdealGas eos;
+  eos.init(p->gl_gamma, p->particle_mass, p->eps_min, p->eps_max, p->rho_min, p->rho_max,
+           p->ye_min, p->ye_max);
+  return eos;
 }
 
 int check_par(const asx_params *p) {
@@ -1236,6 +1274,176 @@ int orc_prim2con_batch(int64_t nz, const double *prim, const double *g,
     cons[4 * nz + z] = cv.tau;
     for (int i = 0; i < 3; ++i) cons[(5 + i) * nz + z] = cv.dB[i];
   }
+  return ASX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Independent zones with composition, any 3-parameter EOS
+// ---------------------------------------------------------------------------
+static int check_eos(const asx_params *par) {
+  if (!par) return ASX_ERR_NULL;
+  if (par->evolution_eos == ASX_EOS_TABULATED) return check_table(par);
+  if (par->evolution_eos != ASX_EOS_IDEALGAS) return ASX_ERR_UNSUPPORTED;
+  return ASX_OK;
+}
+
+} // extern "C"
+
+namespace {
+template <class E>
+void c2p_batch_ye(const E &eos, int64_t nz, const double *cons, const double *g,
+                         const double *seed, double *prim, double *flag, int32_t *iters,
+                         const asx_params *par) {
+  Polytrope poly;
+  poly.init(par->poly_gamma, par->poly_k);
+  const C2PDriver dp = make_driver(par);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t z = 0; z < nz; ++z) {
+    C2PCellIn in;
+    in.r = 0.0;
+    in.alp = 1.0;
+    in.glo = Sym3{g[0 * nz + z], g[1 * nz + z], g[2 * nz + z],
+                  g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+    in.mask = 1.0;
+    in.cv.dens = cons[0 * nz + z];
+    for (int i = 0; i < 3; ++i) in.cv.mom[i] = cons[(1 + i) * nz + z];
+    in.cv.tau = cons[4 * nz + z];
+    for (int i = 0; i < 3; ++i) in.cv.dB[i] = cons[(5 + i) * nz + z];
+    in.cv.DYe = cons[8 * nz + z];
+    in.cv.DEnt = 0.0;
+    in.saved_rho = seed[0 * nz + z];
+    in.saved_eps = seed[1 * nz + z];
+    for (int i = 0; i < 3; ++i) in.saved_vel[i] = seed[(2 + i) * nz + z];
+    in.saved_Ye = seed[5 * nz + z];
+    in.temperature = seed[6 * nz + z];
+    C2PCellOut out;
+    con2prim_cell(eos, poly, dp, in, out);
+    const Prim &pv = out.pv;
+    prim[0 * nz + z] = pv.rho;
+    prim[1 * nz + z] = pv.eps;
+    prim[2 * nz + z] = pv.press;
+    for (int i = 0; i < 3; ++i) prim[(3 + i) * nz + z] = pv.vel[i];
+    prim[6 * nz + z] = pv.w_lor;
+    prim[7 * nz + z] = pv.temperature;
+    prim[8 * nz + z] = pv.Ye;
+    flag[z] = out.flag;
+    iters[z] = out.iters;
+  }
+}
+
+template <class E>
+void prim2con_batch_ye(const E &eos, int64_t nz, const double *prim, const double *g,
+                       double *cons, double *thermo) {
+#pragma omp parallel for schedule(static)
+  for (int64_t z = 0; z < nz; ++z) {
+    Sym3 gl{g[0 * nz + z], g[1 * nz + z], g[2 * nz + z],
+            g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+    Prim pv;
+    pv.rho = prim[0 * nz + z];
+    pv.temperature = prim[1 * nz + z];
+    pv.Ye = prim[2 * nz + z];
+    for (int i = 0; i < 3; ++i) pv.vel[i] = prim[(3 + i) * nz + z];
+    for (int i = 0; i < 3; ++i) pv.B[i] = prim[(6 + i) * nz + z];
+    pv.eps = eos.eps_from_rho_temp_ye(pv.rho, pv.temperature, pv.Ye);
+    pv.press = eos.press_from_rho_temp_ye(pv.rho, pv.temperature, pv.Ye);
+    pv.entropy = 0.0;
+    double v_low[3];
+    lower(gl, pv.vel, v_low);
+    pv.w_lor = 1.0 / std::sqrt(1.0 - dot3(v_low, pv.vel));
+    Cons cv;
+    cv.from_prim(pv, gl);
+    cons[0 * nz + z] = cv.dens;
+    for (int i = 0; i < 3; ++i) cons[(1 + i) * nz + z] = cv.mom[i];
+    cons[4 * nz + z] = cv.tau;
+    for (int i = 0; i < 3; ++i) cons[(5 + i) * nz + z] = cv.dB[i];
+    cons[8 * nz + z] = cv.DYe;
+    if (thermo) {
+      thermo[z] = pv.eps;
+      thermo[nz + z] = pv.press;
+    }
+  }
+}
+} // namespace
+
+extern "C" {
+
+// asx_c2p_batch_ye: cons[9][nz] (asx_c2p_batch rows, DYe), seed[7][nz]
+// (rho, eps, vel^i, Ye, temperature), prim[9][nz] (asx_c2p_batch rows, Ye)
+int orc_c2p_batch_ye(int64_t nz, const double *cons, const double *g, const double *seed,
+                     double *prim, double *flag, int32_t *iters, const asx_params *par) {
+  int st = check_eos(par);
+  if (st) return st;
+  if (par->use_entropy_fix || par->max_iter < 1) return ASX_ERR_PARAM;
+  if (par->evolution_eos == ASX_EOS_TABULATED) {
+    if (par->c2p_prime != ASX_C2P_PALENZUELA || !par->use_temperature ||
+        par->use_press_atmo)
+      return ASX_ERR_UNSUPPORTED;
+    c2p_batch_ye(make_tab(par->eos_table), nz, cons, g, seed, prim, flag, iters, par);
+  } else {
+    c2p_batch_ye(make_ideal(par), nz, cons, g, seed, prim, flag, iters, par);
+  }
+  return ASX_OK;
+}
+
+// asx_prim2con_batch_ye: prim[9][nz] = rho, temperature, Ye, vel^i, B^i
+// (undensitized) -> cons[9][nz]; thermo[2][nz] (optional) = eps, press
+int orc_prim2con_batch_ye(int64_t nz, const double *prim, const double *g, double *cons,
+                          double *thermo, const asx_params *par) {
+  int st = check_eos(par);
+  if (st) return st;
+  if (par->evolution_eos == ASX_EOS_TABULATED)
+    prim2con_batch_ye(make_tab(par->eos_table), nz, prim, g, cons, thermo);
+  else
+    prim2con_batch_ye(make_ideal(par), nz, prim, g, cons, thermo);
+  return ASX_OK;
+}
+
+// asx_eos_from_rho_temp: out[3][n] = press, eps, c_s
+int orc_eos_from_rho_temp(int64_t n, const double *rho, const double *temp, const double *ye,
+                          double *out, const asx_params *par) {
+  int st = check_eos(par);
+  if (st) return st;
+  auto run = [&](const auto &eos) {
+    for (int64_t i = 0; i < n; ++i) {
+      out[i] = eos.press_from_rho_temp_ye(rho[i], temp[i], ye[i]);
+      out[n + i] = eos.eps_from_rho_temp_ye(rho[i], temp[i], ye[i]);
+      out[2 * n + i] = eos.csnd_from_rho_temp_ye(rho[i], temp[i], ye[i]);
+    }
+  };
+  if (par->evolution_eos == ASX_EOS_TABULATED) run(make_tab(par->eos_table));
+  else run(make_ideal(par));
+  return ASX_OK;
+}
+
+// asx_eos_from_rho_eps: out[3][n] = temperature, press, eps after the EOS's
+// clamping (temp_from_valid_rho_eps_ye then press_from_valid_rho_eps_ye)
+int orc_eos_from_rho_eps(int64_t n, const double *rho, const double *eps, const double *ye,
+                         double *out, const asx_params *par) {
+  int st = check_eos(par);
+  if (st) return st;
+  auto run = [&](const auto &eos) {
+    for (int64_t i = 0; i < n; ++i) {
+      double e = eps[i];
+      out[i] = eos.temp_from_rho_eps_ye(rho[i], e, ye[i]);
+      out[n + i] = eos.press_from_rho_eps_ye(rho[i], e, ye[i]);
+      out[2 * n + i] = e;
+    }
+  };
+  if (par->evolution_eos == ASX_EOS_TABULATED) run(make_tab(par->eos_table));
+  else run(make_ideal(par));
+  return ASX_OK;
+}
+
+// The table's interpolation alone (pinning): out[3][n] = log p,
+// log(eps + shift), c_s^2 at (log rho, log T, Ye)
+int orc_tab_interp(int64_t n, const double *lr, const double *lt, const double *ye,
+                   double *out, const asx_params *par) {
+  if (!par || par->evolution_eos != ASX_EOS_TABULATED) return ASX_ERR_PARAM;
+  int st = check_table(par);
+  if (st) return st;
+  const TabEOS e = make_tab(par->eos_table);
+  for (int64_t i = 0; i < n; ++i)
+    for (int v = 0; v < 3; ++v) out[v * n + i] = e.interpolate(v, lr[i], lt[i], ye[i]);
   return ASX_OK;
 }
 

@@ -426,6 +426,9 @@ struct IdealGas {
   double eps_from_rho_press_ye(double rho, double press, double) const {
     return eps_from_rho_press(rho, press);
   }
+  double csnd_from_rho_temp_ye(double rho, double temp, double) const {
+    return csnd_from_rho_eps(rho, eps_from_rho_temp(rho, temp));
+  }
 
 };
 

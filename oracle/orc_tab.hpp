@@ -15,13 +15,15 @@
 
 namespace orc {
 
-// zero_brent (EOSX/src/utils/eos_brent.hxx:8-166)
+// zero_brent (EOSX/src/utils/eos_brent.hxx:8-166), with the loop capped
+// at 1000 iterations (the reference's for (;;) never ends on a NaN
+// residual; a valid bracket converges long before the cap)
 template <typename F> double zero_brent(double a, double b, double t, F &f) {
   double sa = a, sb = b;
   double fa = f(sa), fb = f(sb);
   double c = sa, fc = fa, e = sb - sa, d = e;
   const double macheps = std::numeric_limits<double>::epsilon();
-  for (;;) {
+  for (int it = 0; it < 1000; ++it) {
     if (std::fabs(fc) < std::fabs(fb)) {
       sa = sb;
       sb = c;

@@ -1,0 +1,189 @@
+"""Tabulated EOS and zones with composition on the GPU against the oracle.
+
+The synthetic table (asterx_amd/tabeos.py) sits in device memory; the
+oracle reads the same values on the host.  Device and host logarithms and
+exponentials may differ in the last bit, so interpolated values agree to
+rounding (1e-14 relative), the temperature inversion to its tolerance
+(1e-14 in log T, checked at 1e-12), and Palenzuela con2prim, which sits on
+both, to the solver tolerance (c2p_tol 1e-10, checked at 1e-9) with
+identical flags.  The ideal gas runs through the same entry points.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from asterx_amd import abi, tabeos, zones
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return torch
+
+
+def _L():
+    from asterx_amd import lib
+    return lib.lib()
+
+
+_T = {}
+
+
+def _tables():
+    if "t" not in _T:
+        _torch()
+        tab = tabeos.build()
+        _T["t"] = (tabeos.Table(tab, device="cuda"), tabeos.Table(tab, device=None))
+    return _T["t"]
+
+
+def _par(table, **kw):
+    base = dict(c2p_prime="Palenzuela", c2p_second="None", use_temperature=1,
+                use_press_atmo=0, thermal_eos_atmo=1, rho_abs_min=1e-20, t_atmo=0.01,
+                Ye_atmo=0.3, c2p_tol=1e-10, max_iter=100, vw_lim=1e3, B_lim=1e3,
+                ye_min=0.035, ye_max=0.55)
+    base.update(kw)
+    return table.attach(abi.params(**base))
+
+
+def _dev(a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+
+
+def _p(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _points(tab, n, seed):
+    (r0, r1), (t0, t1), (y0, y1) = tab.ranges()
+    rng = np.random.default_rng(seed)
+    rho = np.exp(rng.uniform(np.log(r0) - 0.5, np.log(r1) + 0.5, n))  # some outside
+    temp = np.exp(rng.uniform(np.log(t0) - 0.3, np.log(t1) + 0.3, n))
+    ye = rng.uniform(y0, y1, n)
+    return rho, temp, ye
+
+
+def test_eos_points_match_oracle():
+    torch = _torch()
+    dtab, htab = _tables()
+    n = 4000
+    rho, temp, ye = _points(htab, n, 1)
+    L, O = _L(), oracle.lib()
+    out = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    r_, t_, y_ = _dev(rho), _dev(temp), _dev(ye)
+    abi.check("asx_eos_from_rho_temp", L.asx_eos_from_rho_temp(
+        n, _p(r_), _p(t_), _p(y_), _p(out), C.byref(_par(dtab)), None))
+    torch.cuda.synchronize()
+    ref = np.zeros((3, n))
+    abi.check("orc_eos_from_rho_temp", O.orc_eos_from_rho_temp(
+        n, rho.ctypes.data, temp.ctypes.data, ye.ctypes.data, ref.ctypes.data,
+        C.byref(_par(htab))))
+    got = out.cpu().numpy()
+    assert np.allclose(got, ref, rtol=1e-13, atol=0), np.max(np.abs(got / ref - 1))
+    # inversion from (rho, eps): eps from the table at T, and eps off the
+    # table on both sides
+    eps = ref[1] * np.where(np.arange(n) % 7 == 0, 1e-3, np.where(np.arange(n) % 7 == 1, 1e3, 1))
+    e_ = _dev(eps)
+    abi.check("asx_eos_from_rho_eps", L.asx_eos_from_rho_eps(
+        n, _p(r_), _p(e_), _p(y_), _p(out), C.byref(_par(dtab)), None))
+    torch.cuda.synchronize()
+    abi.check("orc_eos_from_rho_eps", O.orc_eos_from_rho_eps(
+        n, rho.ctypes.data, eps.ctypes.data, ye.ctypes.data, ref.ctypes.data,
+        C.byref(_par(htab))))
+    got = out.cpu().numpy()
+    assert np.max(np.abs(np.log(got[0] / ref[0]))) < 1e-12
+    assert np.allclose(got[1], ref[1], rtol=1e-11)
+    assert np.allclose(got[2], ref[2], rtol=1e-13)
+
+
+def _c2p_case(table_d, table_h, nz, seed, ideal=False):
+    torch = _torch()
+    if ideal:
+        par_d = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8, rho_abs_min=1e-20,
+                           vw_lim=1e3, B_lim=1e3, c2p_tol=1e-10, c2p_prime="Noble",
+                           c2p_second="Palenzuela", ye_min=0.0, ye_max=1.0)
+        par_h = par_d
+        rho_r, t_r = (1e-10, 1e-2), (1e-6, 1e2)
+    else:
+        par_d, par_h = _par(table_d), _par(table_h)
+        rho_r, t_r = (1e-12, 1e-4), (0.05, 80.0)
+    prim, g = zones.make_batch_ye(nz, seed, rho_r, t_r, (0.06, 0.5))
+    L, O = _L(), oracle.lib()
+    cons = np.zeros((9, nz))
+    thermo = np.zeros((2, nz))
+    abi.check("orc_prim2con_batch_ye", O.orc_prim2con_batch_ye(
+        nz, prim.ctypes.data, g.ctypes.data, cons.ctypes.data, thermo.ctypes.data,
+        C.byref(par_h)))
+    # prim2con on the device against the oracle
+    pd, gd = _dev(prim), _dev(g)
+    cd = torch.empty((9, nz), dtype=torch.float64, device="cuda")
+    td = torch.empty((2, nz), dtype=torch.float64, device="cuda")
+    abi.check("asx_prim2con_batch_ye", L.asx_prim2con_batch_ye(
+        nz, _p(pd), _p(gd), _p(cd), _p(td), C.byref(par_d), None))
+    torch.cuda.synchronize()
+    assert np.allclose(cd.cpu().numpy(), cons, rtol=1e-13, atol=0)
+    assert np.allclose(td.cpu().numpy(), thermo, rtol=1e-13, atol=0)
+    rng = np.random.default_rng(seed + 1)
+    u = lambda: 1 + 1e-2 * rng.uniform(-1, 1, nz)
+    seed_a = np.ascontiguousarray(np.stack(
+        [prim[0] * u(), thermo[0] * u(), prim[3] * u(), prim[4] * u(), prim[5] * u(),
+         prim[2], prim[1]]))
+    out = np.zeros((9, nz))
+    flag = np.zeros(nz)
+    iters = np.zeros(nz, dtype=np.int32)
+    abi.check("orc_c2p_batch_ye", O.orc_c2p_batch_ye(
+        nz, cons.ctypes.data, g.ctypes.data, seed_a.ctypes.data, out.ctypes.data,
+        flag.ctypes.data, iters.ctypes.data, C.byref(par_h)))
+    sd = _dev(seed_a)
+    od = torch.empty((9, nz), dtype=torch.float64, device="cuda")
+    fd = torch.empty(nz, dtype=torch.float64, device="cuda")
+    idd = torch.empty(nz, dtype=torch.int32, device="cuda")
+    abi.check("asx_c2p_batch_ye", L.asx_c2p_batch_ye(
+        nz, _p(_dev(cons)), _p(gd), _p(sd), _p(od), _p(fd), _p(idd), C.byref(par_d), None))
+    torch.cuda.synchronize()
+    got, gflag = od.cpu().numpy(), fd.cpu().numpy()
+    assert np.array_equal(gflag, flag), np.unique(gflag - flag, return_counts=True)
+    ok = flag == 1
+    assert ok.mean() > 0.95
+    for row in (0, 1, 2, 6, 7):
+        rel = np.abs(got[row, ok] - out[row, ok]) / np.abs(out[row, ok])
+        assert np.nanmax(rel) < 1e-9, (row, np.nanmax(rel))
+    for row in (3, 4, 5):
+        assert np.nanmax(np.abs(got[row, ok] - out[row, ok])) < 1e-9, row
+    assert np.array_equal(got[8], out[8])  # Ye = DYe / D
+    return got, out, flag
+
+
+def test_c2p_batch_ye_table_matches_oracle():
+    dtab, htab = _tables()
+    _c2p_case(dtab, htab, 4000, 21)
+
+
+def test_c2p_batch_ye_ideal_gas_matches_oracle():
+    dtab, htab = _tables()
+    _c2p_case(dtab, htab, 4000, 22, ideal=True)
+
+
+def test_table_parameters_refused():
+    dtab, _ = _tables()
+    L = _L()
+    n = 4
+    torch = _torch()
+    buf = torch.zeros((9, n), dtype=torch.float64, device="cuda")
+    fl = torch.zeros(n, dtype=torch.float64, device="cuda")
+    it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    for kw in (dict(c2p_prime="Noble"), dict(use_temperature=0), dict(use_press_atmo=1)):
+        par = _par(dtab, **kw)
+        r = L.asx_c2p_batch_ye(n, _p(buf), _p(buf), _p(buf), _p(buf), _p(fl), _p(it),
+                               C.byref(par), None)
+        assert r == 4, (kw, r)  # ASX_ERR_UNSUPPORTED
+    par = _par(dtab)
+    par.eos_table = None
+    assert L.asx_eos_from_rho_temp(n, _p(buf), _p(buf), _p(buf), _p(buf), C.byref(par),
+                                   None) == 1
