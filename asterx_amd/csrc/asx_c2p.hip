@@ -132,8 +132,8 @@ ASX_DEV void store_zone(const KPatch &P, int oc, const C2POut &o) {
 // so the backup solvers cost no registers here).  Zones whose primary solve
 // fails store nothing; a wavefront ballot packs their indices into `queue`
 // with one atomic per wavefront.
-template <bool FROM_A, int PRIME>
-__global__ __launch_bounds__(256) void k_c2p_main(KPatch P, Box B, IdealGas eos, C2PArgs a,
+template <bool FROM_A, int PRIME, class E>
+__global__ __launch_bounds__(256) void k_c2p_main(KPatch P, Box B, E eos, C2PArgs a,
                                                   unsigned long long *hist, int *queue,
                                                   unsigned *qcount) {
   __shared__ HistLDS h;
@@ -169,8 +169,8 @@ __global__ __launch_bounds__(256) void k_c2p_main(KPatch P, Box B, IdealGas eos,
 
 // Pass 2: the queued zones through the reference's full chain (primary
 // again, then the backup), grid-stride over the device-side count.
-template <bool FROM_A>
-__global__ __launch_bounds__(256) void k_c2p_rest(KPatch P, Box B, IdealGas eos, C2PArgs a,
+template <bool FROM_A, class E>
+__global__ __launch_bounds__(256) void k_c2p_rest(KPatch P, Box B, E eos, C2PArgs a,
                                                   unsigned long long *hist, const int *queue,
                                                   const unsigned *qcount) {
   __shared__ HistLDS h;
@@ -288,25 +288,49 @@ static Box c2p_box(const KPatch &P) {
   return b;
 }
 
+template <bool FROM_A, int PRIME, class E>
+static void c2p_main_t(const KPatch &P, const Box &b, const E &eos, const C2PArgs &a,
+                       unsigned long long *hist, int *queue, unsigned *qcount, hipStream_t s) {
+  hipLaunchKernelGGL((k_c2p_main<FROM_A, PRIME, E>), b.grid(), dim3(256), 0, s, P, b, eos, a,
+                     hist, queue, qcount);
+}
+template <class E>
+static void c2p_rest_t(const KPatch &P, const Box &b, const E &eos, const C2PArgs &a,
+                       bool from_A, unsigned long long *hist, const int *queue,
+                       const unsigned *qcount, hipStream_t s) {
+  const int64_t nz = b.htotal();
+  // enough work-items for a few percent of the zones; more stride
+  const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
+  if (from_A)
+    hipLaunchKernelGGL((k_c2p_rest<true, E>), dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
+  else
+    hipLaunchKernelGGL((k_c2p_rest<false, E>), dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
+}
+
 hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
                                 bool from_A, unsigned long long *hist, int *queue,
                                 unsigned *qcount, hipStream_t s) {
   const Box b = c2p_box(P);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, qcount);
   if (b.htotal() <= 0) return hipGetLastError();
-  const dim3 grid = b.grid(), block(256);
   const bool noble = a.prime == ASX_C2P_NOBLE;
   if (from_A) {
-    if (noble)
-      hipLaunchKernelGGL((k_c2p_main<true, ASX_C2P_NOBLE>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
-    else
-      hipLaunchKernelGGL((k_c2p_main<true, ASX_C2P_PALENZUELA>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
+    if (noble) c2p_main_t<true, ASX_C2P_NOBLE>(P, b, eos, a, hist, queue, qcount, s);
+    else c2p_main_t<true, ASX_C2P_PALENZUELA>(P, b, eos, a, hist, queue, qcount, s);
   } else {
-    if (noble)
-      hipLaunchKernelGGL((k_c2p_main<false, ASX_C2P_NOBLE>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
-    else
-      hipLaunchKernelGGL((k_c2p_main<false, ASX_C2P_PALENZUELA>), grid, block, 0, s, P, b, eos, a, hist, queue, qcount);
+    if (noble) c2p_main_t<false, ASX_C2P_NOBLE>(P, b, eos, a, hist, queue, qcount, s);
+    else c2p_main_t<false, ASX_C2P_PALENZUELA>(P, b, eos, a, hist, queue, qcount, s);
   }
+  return hipGetLastError();
+}
+hipError_t launch_con2prim_main(const KPatch &P, const TabEOS &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, int *queue,
+                                unsigned *qcount, hipStream_t s) {
+  const Box b = c2p_box(P);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, qcount);
+  if (b.htotal() <= 0) return hipGetLastError();
+  if (from_A) c2p_main_t<true, ASX_C2P_PALENZUELA>(P, b, eos, a, hist, queue, qcount, s);
+  else c2p_main_t<false, ASX_C2P_PALENZUELA>(P, b, eos, a, hist, queue, qcount, s);
   return hipGetLastError();
 }
 
@@ -314,14 +338,16 @@ hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PA
                                 bool from_A, unsigned long long *hist, const int *queue,
                                 const unsigned *qcount, hipStream_t s) {
   const Box b = c2p_box(P);
-  const int64_t nz = b.htotal();
-  if (nz <= 0) return hipSuccess;
-  // enough work-items for a few percent of the zones; more stride
-  const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
-  if (from_A)
-    hipLaunchKernelGGL(k_c2p_rest<true>, dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
-  else
-    hipLaunchKernelGGL(k_c2p_rest<false>, dim3(g2), dim3(256), 0, s, P, b, eos, a, hist, queue, qcount);
+  if (b.htotal() <= 0) return hipSuccess;
+  c2p_rest_t(P, b, eos, a, from_A, hist, queue, qcount, s);
+  return hipGetLastError();
+}
+hipError_t launch_con2prim_rest(const KPatch &P, const TabEOS &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, const int *queue,
+                                const unsigned *qcount, hipStream_t s) {
+  const Box b = c2p_box(P);
+  if (b.htotal() <= 0) return hipSuccess;
+  c2p_rest_t(P, b, eos, a, from_A, hist, queue, qcount, s);
   return hipGetLastError();
 }
 

@@ -202,9 +202,17 @@ C2PArgs c2p_args(const asx_params *p) {
   return a;
 }
 
+asx_status check_table(const asx_params *p);
+bool tabulated(const asx_params *p) { return p->evolution_eos == ASX_EOS_TABULATED; }
+
+// A table (with its descriptor) or the ideal gas.
 asx_status check_par(const asx_params *p) {
   if (!p) return ASX_ERR_NULL;
-  if (p->evolution_eos != ASX_EOS_IDEALGAS) return ASX_ERR_UNSUPPORTED;
+  if (tabulated(p)) {
+    if (asx_status st = check_table(p)) return st;
+  } else if (p->evolution_eos != ASX_EOS_IDEALGAS) {
+    return ASX_ERR_UNSUPPORTED;
+  }
   if (p->reconstruction_method == ASX_RECON_EPPM) return ASX_ERR_UNSUPPORTED;
   if (p->reconstruction_method < 0 || p->reconstruction_method > 6) return ASX_ERR_PARAM;
   if (p->loworder_method < 0 || p->loworder_method > 2) return ASX_ERR_PARAM;
@@ -222,7 +230,7 @@ asx_status check_par(const asx_params *p) {
 asx_status check_table(const asx_params *p) {
   const asx_eos_table *t = p->eos_table;
   if (!t || !t->data || !t->axis[0] || !t->axis[1] || !t->axis[2]) return ASX_ERR_NULL;
-  if (t->nvars != 3) return ASX_ERR_PARAM;
+  if (t->nvars != 4) return ASX_ERR_PARAM;
   for (int d = 0; d < 3; ++d)
     if (t->n[d] < 2 || !(t->x_step[d] > 0.0)) return ASX_ERR_PARAM;
   if (!(t->eps_max >= t->eps_min)) return ASX_ERR_PARAM;
@@ -261,6 +269,11 @@ asx_status check_c2p_par(const asx_params *p) {
   asx_status st = check_par(p);
   if (st) return st;
   if (p->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
+  // a table: Palenzuela on temperature without a pressure atmosphere (Noble
+  // needs the ideal-fluid Gamma, eps_from_valid_rho_press_ye is refused)
+  if (tabulated(p) && (p->c2p_prime != ASX_C2P_PALENZUELA || p->c2p_second == ASX_C2P_NOBLE ||
+                       !p->use_temperature || p->use_press_atmo))
+    return ASX_ERR_UNSUPPORTED;
   if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA)
     return ASX_ERR_UNSUPPORTED;
   if (p->c2p_second != ASX_C2P_NOBLE && p->c2p_second != ASX_C2P_PALENZUELA &&
@@ -268,6 +281,15 @@ asx_status check_c2p_par(const asx_params *p) {
     return ASX_ERR_UNSUPPORTED;
   if (p->max_iter < 1) return ASX_ERR_PARAM;
   if (!(p->c2p_tol > 0.0)) return ASX_ERR_PARAM;
+  return ASX_OK;
+}
+
+// Faces with a table: reconstruction in T (a table has no eps(rho, p)) and
+// the high-order methods the table kernels are built for.
+asx_status check_flux_eos(const asx_params *p) {
+  if (!tabulated(p)) return ASX_OK;
+  if (!p->reconstruct_with_temperature || !flux_tab_method(p->reconstruction_method))
+    return ASX_ERR_UNSUPPORTED;
   return ASX_OK;
 }
 
@@ -562,6 +584,7 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
   asx_status st = check_geom(pt);
   if (st) return st;
   if ((st = check_par(par))) return st;
+  if ((st = check_flux_eos(par))) return st;
   if (face_halo < 0 || face_halo > 1) return ASX_ERR_PARAM;
   if (pt->ng < recon_ghosts(par->reconstruction_method) ||
       pt->ng < recon_ghosts(par->loworder_method) || (face_halo > 0 && pt->ng < 3))
@@ -597,6 +620,8 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
   FluxArgs A;
   A.rp = rparams(par);
   A.eos = ideal_gas(par);
+  const bool tab = tabulated(par);
+  if (tab) A.tab = tab_eos(par);
   A.lo_method = par->loworder_method;
   A.flux_type = par->flux_type;
   A.rec_var = par->recon_type;
@@ -611,7 +636,7 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
-    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s); })))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s, tab); })))) return st;
   }
   if (aux) {
     const int h = 2 * face_halo;
@@ -757,15 +782,17 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
   }
   C2PQueue *Q = c2p_queue(stream, (int64_t)pt->n[0] * pt->n[1] * pt->n[2]);
   if (!Q) return ASX_ERR_DEVICE;
-  const IdealGas eos = ideal_gas(par);
   const C2PArgs ca = c2p_args(par);
   unsigned long long *hist = cd ? cd + 2 : nullptr;
+  const bool tab = tabulated(par);
   if ((st = dev_status(timed(ASX_KC_C2P, s, [&] {
-         return launch_con2prim_main(P, eos, ca, from_A, hist, Q->q, Q->n, s);
+         return tab ? launch_con2prim_main(P, tab_eos(par), ca, from_A, hist, Q->q, Q->n, s)
+                    : launch_con2prim_main(P, ideal_gas(par), ca, from_A, hist, Q->q, Q->n, s);
        }))))
     return st;
   if ((st = dev_status(timed(ASX_KC_C2P_REST, s, [&] {
-         return launch_con2prim_rest(P, eos, ca, from_A, hist, Q->q, Q->n, s);
+         return tab ? launch_con2prim_rest(P, tab_eos(par), ca, from_A, hist, Q->q, Q->n, s)
+                    : launch_con2prim_rest(P, ideal_gas(par), ca, from_A, hist, Q->q, Q->n, s);
        }))))
     return st;
   if (cnt) {
@@ -880,6 +907,7 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
   asx_status st = check_geom(stage);
   if (st) return st;
   if ((st = check_par(par))) return st;
+  if ((st = check_flux_eos(par))) return st;
   if (!out || (c0 != 0.0 && !u0)) return ASX_ERR_NULL;
   // Upwind CT reconstructs face data three rows into the transverse ghost
   // region, which the fused path does not compute; use the unfused entry
@@ -926,6 +954,8 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
   FluxArgs A;
   A.rp = rparams(par);
   A.eos = ideal_gas(par);
+  const bool tab = tabulated(par);
+  if (tab) A.tab = tab_eos(par);
   A.lo_method = par->loworder_method;
   A.flux_type = par->flux_type;
   A.rec_var = par->recon_type;
@@ -940,7 +970,7 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
-    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s); })))) return st;
+    if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s, tab); })))) return st;
   }
   {
     const int halo[3] = {2, 2, 2};
@@ -959,6 +989,7 @@ asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,
                          const asx_params *par, void *stream) {
   asx_status st = check_c2p_par(par);
   if (st) return st;
+  if (tabulated(par)) return ASX_ERR_UNSUPPORTED; // no Ye in this layout: asx_c2p_batch_ye
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !seed || !prim || !flag || !iters)) return ASX_ERR_NULL;
   return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] { return launch_c2p_batch(nz, cons, g, seed, prim, flag, iters, ideal_gas(par),
@@ -1063,6 +1094,7 @@ asx_status asx_prim2con_batch(int64_t nz, const double *prim, const double *g, d
                               double *cold_seed, const asx_params *par, void *stream) {
   asx_status st = check_par(par);
   if (st) return st;
+  if (tabulated(par)) return ASX_ERR_UNSUPPORTED; // asx_prim2con_batch_ye
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!prim || !g || !cons)) return ASX_ERR_NULL;
   const IdealGas eos = ideal_gas(par);
@@ -1115,7 +1147,7 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                              const asx_params *par, void *stream) {
   asx_status st = check_c2p_par(par);
   if (st) return st;
-  if (par->c2p_prime != ASX_C2P_NOBLE) return ASX_ERR_UNSUPPORTED;
+  if (par->c2p_prime != ASX_C2P_NOBLE || tabulated(par)) return ASX_ERR_UNSUPPORTED;
   if (nz < 0) return ASX_ERR_SIZE;
   if (kguess < 1) return ASX_ERR_PARAM;
   if (nz >= (int64_t(1) << 31)) return ASX_ERR_SIZE;
@@ -1589,7 +1621,7 @@ asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_
   if (npatch < 1) return ASX_ERR_SIZE;
   asx_status st;
   if ((st = check_c2p_par(par))) return st;
-  if (par->use_uct) return ASX_ERR_UNSUPPORTED;
+  if (par->use_uct || tabulated(par)) return ASX_ERR_UNSUPPORTED;
   if (bc_kind < ASX_BC_LINEAR || bc_kind > ASX_BC_COPY) return ASX_ERR_PARAM;
   if (!std::isfinite(dt)) return ASX_ERR_PARAM;
   bool ubst = patches[0].gf[ASX_DBX_STAG].data != nullptr;

@@ -392,6 +392,10 @@ struct IdealGas {
   ASX_DEV double press_rt(double rho, double temp, double) const { return press_rt(rho, temp); }
   ASX_DEV double eps_rp(double rho, double press, double) const { return eps_rp(rho, press); }
   ASX_DEV double csnd_rt(double, double temp, double) const { return csnd(eps_rt(temp)); }
+  // eos_3p_idealgas.hxx: the entropy of (rho, T) is kappa(rho, eps(T))
+  ASX_DEV double entropy_rt(double rho, double temp, double) const {
+    return kappa(rho, eps_rt(temp));
+  }
 };
 
 // v2f metric average at a face normal to D (AsterUtils aster_interp.hxx:237-249)

@@ -6,8 +6,8 @@
 
 namespace asx {
 
-// EOSX on points: from (rho, T, Ye) -> press, eps, c_s
-// (press_from_valid_rho_temp_ye, eps_from_..., csnd_from_valid_rho_temp_ye)
+// EOSX on points: from (rho, T, Ye) -> press, eps, c_s, entropy
+// (press_from_valid_rho_temp_ye, eps_from_..., csnd_from_..., entropy_from_...)
 template <class E>
 __global__ __launch_bounds__(256) void k_eos_rt(int64_t n, const double *rho,
                                                 const double *temp, const double *ye,
@@ -17,6 +17,7 @@ __global__ __launch_bounds__(256) void k_eos_rt(int64_t n, const double *rho,
   out[i] = eos.press_rt(rho[i], temp[i], ye[i]);
   out[n + i] = eos.eps_rt(rho[i], temp[i], ye[i]);
   out[2 * n + i] = eos.csnd_rt(rho[i], temp[i], ye[i]);
+  out[3 * n + i] = eos.entropy_rt(rho[i], temp[i], ye[i]);
 }
 // from (rho, eps, Ye) -> T, press, eps after the EOS's clamping
 template <class E>

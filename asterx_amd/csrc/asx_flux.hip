@@ -127,7 +127,7 @@ ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Si
 
 // Numerical flux of every conserved variable at face (i, j, k) from the
 // left (0) and right (1) states.
-template <int D>
+template <int D, bool TAB = false>
 ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
                        const Side &L, const Side &R, bool lo0, bool lo1) {
   constexpr int CF = 1 << D;      // face centering
@@ -158,10 +158,15 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
   const double rho[2] = {L.rho, R.rho}, ent[2] = {L.ent, R.ent}, ye[2] = {L.ye, R.ye};
   const double th[2] = {L.th, R.th};
   const double irho[2] = {1.0 / rho[0], 1.0 / rho[1]};
-  double eps[2], press[2];
+  double eps[2], press[2], cs2t[2];
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
-    if (A.with_T) {
+    if constexpr (TAB) {
+      // fluxes.cxx:298-304 and the sound speed of :555-559 from one lookup
+      // at the reconstructed (rho, T, Ye); the reference's csnd from
+      // (rho, eps) inverts eps back to that T within 1e-14 in log T
+      A.tab.state_rt(rho[f], th[f], ye[f], eps[f], press[f], cs2t[f]);
+    } else if (A.with_T) {
       eps[f] = eos.eps_rt(th[f]);
       press[f] = eos.press_rt(rho[f], th[f]);
     } else {
@@ -273,8 +278,12 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
   double mom[3][2], Bt[3][2];
 #pragma unroll
   for (int f = 0; f < 2; ++f) {
-    const double cs = eos.csnd(eps[f]);
-    cs2[f] = cs * cs;
+    if constexpr (TAB) {
+      cs2[f] = cs2t[f];
+    } else {
+      const double cs = eos.csnd(eps[f]);
+      cs2[f] = cs * cs;
+    }
     h[f] = 1 + eps[f] + press[f] * irho[f];
     dens[f] = sqrtg * rho[f] * W[f];
     dent[f] = sqrtg * rho[f] * W[f] * ent[f];
@@ -407,7 +416,7 @@ template <int D, int V> using FaceTile = typename FaceTileT<D, V>::T;
 
 // One tile of a face box (lo, cnt), tile index b of a grid of ntx x nty x ntz
 // tiles (ordered x, y, z).  Every work-item of the workgroup calls it.
-template <int D, int M, int V>
+template <int D, int M, int V, bool TAB = false>
 ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], const int cnt[3],
                        int b, int ntx, int nty) {
   using T = FaceTile<D, V>;
@@ -456,17 +465,18 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     L.v[0] = sh[6][tl];
     L.v[1] = sh[7][tl];
     L.v[2] = sh[8][tl];
-    face_flux<D>(P, A, i, j, k, L, lo, shf[tl] != 0, flo);
+    face_flux<D, TAB>(P, A, i, j, k, L, lo, shf[tl] != 0, flo);
   }
 }
 
 #ifndef ASX_FACE_WAVES
 #define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
 #endif
-template <int D, int M, int V>
+template <int D, int M, int V, bool TAB = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES)))
 void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
-  face_tile<D, M, V>(P, A, A.lo, A.cnt, xcd_tile((int)blockIdx.x, ntx * nty * ntz), ntx, nty);
+  face_tile<D, M, V, TAB>(P, A, A.lo, A.cnt, xcd_tile((int)blockIdx.x, ntx * nty * ntz), ntx,
+                          nty);
 }
 
 // CalcAuxForAvecPsi (AsterX/src/fluxes.cxx:803-825) at vertex (i, j, k)
@@ -603,7 +613,7 @@ __global__ __launch_bounds__(256) void k_prim2con(KPatch P, Box B) {
 // ---------------------------------------------------------------------------
 template <int D, int V>
 static hipError_t launch_flux_dv(const KPatch &P, const FluxArgs &A, int method,
-                                 hipStream_t s) {
+                                 hipStream_t s, bool tab) {
   using T = FaceTile<D, V>;
   if ((int64_t)A.cnt[0] * A.cnt[1] * A.cnt[2] <= 0) return hipSuccess;
   const int ex[3] = {T::X, T::Y, T::Z};
@@ -613,6 +623,15 @@ static hipError_t launch_flux_dv(const KPatch &P, const FluxArgs &A, int method,
     nt[q] = (A.cnt[q] + per - 1) / per;
   }
   const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2])), block(256);
+  if (tab) {
+    switch (method) {
+    case ASX_RECON_PPM: hipLaunchKernelGGL((k_face<D, ASX_RECON_PPM, V, true>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+    case ASX_RECON_WENOZ: hipLaunchKernelGGL((k_face<D, ASX_RECON_WENOZ, V, true>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+    case ASX_RECON_MP5: hipLaunchKernelGGL((k_face<D, ASX_RECON_MP5, V, true>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+  }
   switch (method) {
   case ASX_RECON_GODUNOV: hipLaunchKernelGGL((k_face<D, ASX_RECON_GODUNOV, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
   case ASX_RECON_MINMOD: hipLaunchKernelGGL((k_face<D, ASX_RECON_MINMOD, V>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;
@@ -625,21 +644,25 @@ static hipError_t launch_flux_dv(const KPatch &P, const FluxArgs &A, int method,
 }
 template <int D>
 static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method, int variant,
-                                hipStream_t s) {
+                                hipStream_t s, bool tab) {
   switch (variant) {
-  case 1: return launch_flux_dv<D, 1>(P, A, method, s);
-  case 2: return launch_flux_dv<D, 2>(P, A, method, s);
-  case 3: return launch_flux_dv<D, 3>(P, A, method, s);
-  case 4: return launch_flux_dv<D, 4>(P, A, method, s);
-  default: return launch_flux_dv<D, 0>(P, A, method, s);
+  case 1: return launch_flux_dv<D, 1>(P, A, method, s, tab);
+  case 2: return launch_flux_dv<D, 2>(P, A, method, s, tab);
+  case 3: return launch_flux_dv<D, 3>(P, A, method, s, tab);
+  case 4: return launch_flux_dv<D, 4>(P, A, method, s, tab);
+  default: return launch_flux_dv<D, 0>(P, A, method, s, tab);
   }
 }
 
+bool flux_tab_method(int method) {
+  return method == ASX_RECON_PPM || method == ASX_RECON_WENOZ || method == ASX_RECON_MP5;
+}
+
 hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, int variant,
-                       hipStream_t s) {
-  if (d == 0) return launch_flux_d<0>(P, A, method, variant, s);
-  if (d == 1) return launch_flux_d<1>(P, A, method, variant, s);
-  return launch_flux_d<2>(P, A, method, variant, s);
+                       hipStream_t s, bool tab) {
+  if (d == 0) return launch_flux_d<0>(P, A, method, variant, s, tab);
+  if (d == 1) return launch_flux_d<1>(P, A, method, variant, s, tab);
+  return launch_flux_d<2>(P, A, method, variant, s, tab);
 }
 
 hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s) {

@@ -71,6 +71,7 @@ struct BatchBox {
 struct FluxArgs {
   RParams rp;
   IdealGas eos;
+  TabEOS tab; // used by the table instantiations (evolution_eos = Tabulated3d)
   int lo_method, flux_type, rec_var, with_T, write_uct;
   int lo[3], cnt[3];
   unsigned long long *cnt_dev; // [nan_faces, lo_faces] or null
@@ -87,8 +88,10 @@ struct StatePtrs {
   double *g[ASX_NUM_STATE];
 };
 
+// tab: the EOS is A.tab (PPM, WENO-Z and MP5 only; reconstruction in T)
 hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, int variant,
-                       hipStream_t s);
+                       hipStream_t s, bool tab = false);
+bool flux_tab_method(int method);
 hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s);
 hipError_t launch_bstag(int d, const KPatch &P, const Box &B, double *out,
                         hipStream_t s);
@@ -121,6 +124,13 @@ hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PA
                                 bool from_A, unsigned long long *hist, int *queue,
                                 unsigned *qcount, hipStream_t s);
 hipError_t launch_con2prim_rest(const KPatch &P, const IdealGas &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, const int *queue,
+                                const unsigned *qcount, hipStream_t s);
+// with a table: Palenzuela primary
+hipError_t launch_con2prim_main(const KPatch &P, const TabEOS &eos, const C2PArgs &a,
+                                bool from_A, unsigned long long *hist, int *queue,
+                                unsigned *qcount, hipStream_t s);
+hipError_t launch_con2prim_rest(const KPatch &P, const TabEOS &eos, const C2PArgs &a,
                                 bool from_A, unsigned long long *hist, const int *queue,
                                 const unsigned *qcount, hipStream_t s);
 
@@ -202,8 +212,8 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 double *guess, int *queue, unsigned *qcount, hipStream_t s);
 
 // Independent zones with composition, any 3-parameter EOS (asx_eos.hip).
-// mode 0: (rho, T, Ye) -> press, eps, c_s; mode 1: (rho, eps, Ye) -> T,
-// press, eps after clamping; out[3][n]
+// mode 0: (rho, T, Ye) -> out[4][n] = press, eps, c_s, entropy;
+// mode 1: (rho, eps, Ye) -> out[3][n] = T, press, eps after clamping
 hipError_t launch_eos_points(int mode, int64_t n, const double *rho, const double *x,
                              const double *ye, double *out, const IdealGas &eos,
                              hipStream_t s);

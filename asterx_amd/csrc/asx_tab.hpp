@@ -1,6 +1,6 @@
 // Tabulated 3-parameter EOS for gfx950 (EOSX eos_3p_tabulated3d): a table
-// of log p, log(eps + shift) and c_s^2 over axes log rho, log T, Ye (code
-// units; axes uniform), the variable index fastest, and the EOSX interface
+// of log p, log(eps + shift), c_s^2 and entropy over axes log rho, log T,
+// Ye (code units; axes uniform), the variable index fastest, and the EOSX interface
 // on it (EOSX/src/eos_3p_tabulated3d/eos_3p_tabulated3d.hxx:186-357):
 // trilinear interpolation in the order of eos_linear_interp_ND.hxx (rho,
 // then T, then Ye), temperature from (rho, eps, Ye) by Brent's ZERO, and
@@ -75,7 +75,7 @@ ASX_DEV double zero_brent(double a, double b, double t, const F &f) {
 }
 
 struct TabEOS {
-  enum { PRESS = 0, EPS = 1, CS2 = 2, NV = 3 };
+  enum { PRESS = 0, EPS = 1, CS2 = 2, ENT = 3, NV = 4 };
   const double *tab;    // [v + NV * (ir + n0 * (it + n1 * iy))]
   const double *ax[3];  // axis values: log rho, log T, Ye
   int n[3];
@@ -114,6 +114,38 @@ struct TabEOS {
     return lin(b0, b1, y0, y1, ye);
   }
   // std::fmin / std::fmax as the reference (a NaN argument is dropped)
+  // p, eps and c_s^2 at one point (indices found once)
+  ASX_DEV void interp3(double lr, double lt, double ye, double out[3]) const {
+    const int i0 = find(0, lr), i1 = find(1, lt), i2 = find(2, ye);
+    const int s1 = NV * n[0], s2 = NV * n[0] * n[1];
+    const double *p = tab + NV * i0 + s1 * i1 + s2 * i2;
+    const double r0 = ax[0][i0], r1 = ax[0][i0 + 1];
+    const double t0 = ax[1][i1], t1 = ax[1][i1 + 1];
+    const double y0 = ax[2][i2], y1 = ax[2][i2 + 1];
+#pragma unroll
+    for (int v = 0; v < 3; ++v) {
+      const double a00 = lin(p[v], p[v + NV], r0, r1, lr);
+      const double a10 = lin(p[v + s1], p[v + s1 + NV], r0, r1, lr);
+      const double a01 = lin(p[v + s2], p[v + s2 + NV], r0, r1, lr);
+      const double a11 = lin(p[v + s2 + s1], p[v + s2 + s1 + NV], r0, r1, lr);
+      const double b0 = lin(a00, a10, t0, t1, lt);
+      const double b1 = lin(a01, a11, t0, t1, lt);
+      out[v] = lin(b0, b1, y0, y1, ye);
+    }
+  }
+  // eps_rt, press_rt and csnd_rt^2 at one (rho, T, Ye) from one lookup; the
+  // pressure is looked up again at the unbounded (rho, T) when either lies
+  // outside the table (press_rt takes the logs of the inputs)
+  ASX_DEV void state_rt(double rho, double temp, double ye, double &eps, double &press,
+                        double &cs2) const {
+    const double rc = crho(rho), tc = ctemp(temp);
+    double v[3];
+    interp3(log(rc), log(tc), ye, v);
+    eps = exp(v[EPS]) - shift;
+    press = (rc == rho && tc == temp) ? exp(v[PRESS]) : press_rt(rho, temp, ye);
+    const double cs = sqrt(v[CS2]);
+    cs2 = cs * cs;
+  }
   ASX_DEV double crho(double rho) const { return fmin(fmax(rho, rho_min), rho_max); }
   ASX_DEV double ctemp(double t) const { return fmin(fmax(t, temp_min), temp_max); }
   ASX_DEV double xmin1() const { return ax[1][0]; }
@@ -161,24 +193,13 @@ struct TabEOS {
   ASX_DEV double temp_re(double rho, double &eps, double ye) const {
     return exp(logtemp_re(rho, eps, ye));
   }
-  // the change logtemp_re makes to eps, without the root find
-  ASX_DEV void eps_clamp(double rho, double &eps, double ye) const {
-    eps = fmax(eps + shift, eps_min + shift) - shift;
-    const double lrho = log(crho(rho));
-    const double leps = log(eps + shift);
-    const double vmin = interp(EPS, lrho, xmin1(), ye);
-    if (leps <= vmin) {
-      eps = exp(vmin) - shift;
-      return;
-    }
-    const double vmax = interp(EPS, lrho, xmax1(), ye);
-    if (leps >= vmax) eps = exp(vmax) - shift;
+  // entropy_from_valid_rho_temp_ye (:320-329)
+  ASX_DEV double entropy_rt(double rho, double temp, double ye) const {
+    return interp(ENT, log(crho(rho)), log(ctemp(temp)), ye);
   }
-  // the table holds no entropy: kappa is 0, with the change of eps of the
-  // temperature inversion kappa_from_valid_rho_eps_ye runs
+  // kappa_from_valid_rho_eps_ye (:331-336): the entropy at T(rho, eps, Ye)
   ASX_DEV double kappa_re(double rho, double &eps, double ye) const {
-    eps_clamp(rho, eps, ye);
-    return 0.0;
+    return entropy_rt(rho, temp_re(rho, eps, ye), ye);
   }
   // eps_from_valid_rho_press_ye is refused by the reference for tables;
   // the callers that reach it (pressure atmosphere, use_temperature = no)

@@ -12,10 +12,14 @@ rho_t = 5e14 g/cc, 3.0 above, continuous), eps_c its cold energy
 (d eps_c = P_c / rho^2 d rho, eps_c(0) = 0), an ideal thermal part, and
 
   c_s^2 = [dP/drho|_eps + (P/rho^2) dP/deps|_rho] / h,
-  h = 1 + (eps + P/rho)/c^2.
+  h = 1 + (eps + P/rho)/c^2,
+  s = 45 + (3/2) ln(T / MeV) - ln(rho / (g/cc))   (entropy per baryon of
+                                                   the thermal gas, > 0)
 
 Stored like a stellar-collapse table: log10 P, log10(eps + energy_shift)
-(shift 2e19 erg/g), c_s^2.  `code_units` converts it the way EOSX reads a
+(shift 2e19 erg/g), c_s^2, and the entropy, which AsterX's fluxes read
+(their low-order fallback triggers on a reconstructed entropy <= 0) and
+its con2prim writes.  `code_units` converts it the way EOSX reads a
 table (EOSX/src/eos_3p_tabulated3d/eos_3p_tabulated3d.hxx:151-165, the
 constants of EOSX/src/utils/eos_constants.hxx:7-12): natural logs, code
 units, variable index fastest.
@@ -69,6 +73,12 @@ def thermal_eps(temp):
     return 1.5 * np.asarray(temp) * MEV_TO_ERG / MNUC_CGS
 
 
+def entropy(rho, temp):
+    """Entropy per baryon (k_B) of the thermal gas, offset to stay positive
+    on the table."""
+    return 45.0 + 1.5 * np.log(np.asarray(temp)) - np.log(np.asarray(rho))
+
+
 def state_cgs(rho, temp, ye):
     """P, eps, c_s^2 (CGS) at (rho [g/cc], T [MeV], Ye)."""
     Pc, ec, G = cold(rho)
@@ -85,20 +95,21 @@ def state_cgs(rho, temp, ye):
 
 def make_cgs(nrho=NRHO, ntemp=NTEMP, nye=NYE):
     """Axes (log10 rho, log10 T, Ye) and the CGS tables log10 P,
-    log10(eps + shift), c_s^2, each [nye][ntemp][nrho]."""
+    log10(eps + shift), c_s^2 and entropy, each [nye][ntemp][nrho]."""
     lr = np.linspace(LOGRHO[0], LOGRHO[1], nrho)
     lt = np.linspace(LOGTEMP[0], LOGTEMP[1], ntemp)
     ye = np.linspace(YE[0], YE[1], nye)
     Y, T, R = np.meshgrid(ye, 10.0 ** lt, 10.0 ** lr, indexing="ij")
     P, eps, cs2 = state_cgs(R, T, Y)
-    return lr, lt, ye, np.log10(P), np.log10(eps + SHIFT_CGS), cs2
+    return lr, lt, ye, np.log10(P), np.log10(eps + SHIFT_CGS), cs2, entropy(R, T)
 
 
-def code_units(lr10, lt10, ye, logp10, loge10, cs2, shift_cgs=SHIFT_CGS):
+def code_units(lr10, lt10, ye, logp10, loge10, cs2, ent, shift_cgs=SHIFT_CGS):
     """The table as EOSX holds it after reading (eos_3p_tabulated3d.hxx:
     151-165): natural-log axes in code units, log p and log(eps + shift)
-    in code units, c_s^2 in code units, interleaved [.., 3] with the
-    variable fastest; plus the energy shift and the eps range."""
+    in code units, c_s^2 in code units, the entropy as it is, interleaved
+    [.., 4] with the variable fastest; plus the energy shift and the eps
+    range."""
     ln10 = math.log(10.0)
     shift = shift_cgs * EPSGF
     ax0 = lr10 * ln10 + math.log(RHOGF)
@@ -108,7 +119,7 @@ def code_units(lr10, lt10, ye, logp10, loge10, cs2, shift_cgs=SHIFT_CGS):
     lp = logp10 * ln10 + math.log(PRESSGF)
     le = loge10 * ln10 + math.log(EPSGF)
     c2 = cs2 * LENGTHGF * LENGTHGF * inv_time2
-    data = np.stack([lp, le, c2], axis=-1)  # [ye][T][rho][var]
+    data = np.stack([lp, le, c2, np.asarray(ent, dtype=np.float64)], axis=-1)
     eps = np.exp(le) - shift
     # EOSX's rgeps: min / max of eps over the nodes, the minimum floored
     # (eos_3p_tabulated3d.hxx:341-357)
@@ -144,7 +155,7 @@ class Table:
         for k in range(3):
             d.axis[k] = ptr(self.axes[k])
             d.n[k] = len(tab["axes"][k])
-        d.nvars = 3
+        d.nvars = 4
         d.energy_shift = tab["shift"]
         d.eps_min = tab["eps_min"]
         d.eps_max = tab["eps_max"]

@@ -90,8 +90,9 @@ enum {
 /* A tabulated EOS as EOSX holds it after reading its table
  * (EOSX/src/eos_3p_tabulated3d/eos_3p_tabulated3d.hxx:60-183): uniform axes
  * log rho and log T (code units, T in MeV) and Ye, and per node the
- * variables log p, log(eps + energy_shift) and c_s^2 in code units, the
- * variable index fastest: data[v + 3 (i + n[0] (j + n[1] k))].  Device
+ * variables log p, log(eps + energy_shift), c_s^2 (code units) and the
+ * entropy per baryon, the variable index fastest:
+ * data[v + 4 (i + n[0] (j + n[1] k))].  Device
  * memory; the library reads it in place.  eps_min / eps_max are the
  * table's eps range (EOSX rgeps): min and max of exp(log(eps + shift)) -
  * shift over the nodes, the minimum floored at 1e-15. */
@@ -99,7 +100,7 @@ typedef struct asx_eos_table_s {
   const double *data;
   const double *axis[3]; /* log rho (n[0] values), log T (n[1]), Ye (n[2]) */
   int32_t n[3];          /* >= 2 each */
-  int32_t nvars;         /* 3 */
+  int32_t nvars;         /* 4 */
   double energy_shift;
   double eps_min, eps_max;
   /* host copies of axis[d][0], axis[d][1] - axis[d][0], axis[d][n[d]-1]
@@ -475,8 +476,8 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
 /* IdealGas or Tabulated3d with eos_table)                                  */
 /* ------------------------------------------------------------------------ */
 /* The EOSX interface on n points (eos_3p.hxx; eos_3p_tabulated3d.hxx:
- * 186-336 for a table).  From (rho, T, Ye): out[3][n] = press, eps, c_s
- * (press_/eps_/csnd_from_valid_rho_temp_ye).  From (rho, eps, Ye):
+ * 186-336 for a table).  From (rho, T, Ye): out[4][n] = press, eps, c_s,
+ * entropy (press_/eps_/csnd_/entropy_from_valid_rho_temp_ye).  From (rho, eps, Ye):
  * out[3][n] = T, press and eps after the EOS's clamping of eps
  * (temp_from_valid_rho_eps_ye, then press_from_valid_rho_eps_ye). */
 asx_status asx_eos_from_rho_temp(int64_t n, const double *rho, const double *temp,

@@ -95,7 +95,11 @@ struct Ctx {
   ReconParams rp;
   IdealGas eos;
   Polytrope poly;
+  bool tab = false; // evolution_eos = Tabulated3d: teos instead of eos
+  TabEOS teos;
 };
+
+TabEOS make_tab(const asx_eos_table *t);
 
 Ctx make_ctx(const asx_patch *pt, const asx_params *p) {
   Ctx c;
@@ -117,6 +121,10 @@ Ctx make_ctx(const asx_patch *pt, const asx_params *p) {
   c.eos.init(p->gl_gamma, p->particle_mass, p->eps_min, p->eps_max,
              p->rho_min, p->rho_max, p->ye_min, p->ye_max);
   c.poly.init(p->poly_gamma, p->poly_k);
+  if (p->evolution_eos == ASX_EOS_TABULATED && p->eos_table) {
+    c.tab = true;
+    c.teos = make_tab(p->eos_table);
+  }
   return c;
 }
 
@@ -146,7 +154,7 @@ TabEOS make_tab(const asx_eos_table *t) {
 int check_table(const asx_params *p) {
   const asx_eos_table *t = p->eos_table;
   if (!t || !t->data || !t->axis[0] || !t->axis[1] || !t->axis[2]) return ASX_ERR_NULL;
-  if (t->nvars != 3 || t->n[0] < 2 || t->n[1] < 2 || t->n[2] < 2) return ASX_ERR_PARAM;
+  if (t->nvars != 4 || t->n[0] < 2 || t->n[1] < 2 || t->n[2] < 2) return ASX_ERR_PARAM;
   return ASX_OK;
 }
 
@@ -159,7 +167,11 @@ IdealGas make_ideal(const asx_params *p) {
 
 int check_par(const asx_params *p) {
   if (!p) return ASX_ERR_NULL;
-  if (p->evolution_eos != ASX_EOS_IDEALGAS) return ASX_ERR_UNSUPPORTED;
+  if (p->evolution_eos == ASX_EOS_TABULATED) {
+    if (int st = check_table(p)) return st;
+  } else if (p->evolution_eos != ASX_EOS_IDEALGAS) {
+    return ASX_ERR_UNSUPPORTED;
+  }
   if (p->reconstruction_method < 0 || p->reconstruction_method > 6 ||
       p->reconstruction_method == ASX_RECON_EPPM)
     return ASX_ERR_UNSUPPORTED;
@@ -254,17 +266,21 @@ void calc_flux_face(const Ctx &c, int dir, const int I[3]) {
     R.run(LO, thermo, false, t);
     th[f] = t[f];
   }
-  for (int f = 0; f < 2; ++f) {
-    if (with_T) { // fluxes.cxx:298-304
-      temp[f] = th[f];
-      eps[f] = c.eos.eps_from_rho_temp(rho[f], temp[f]);
-      press[f] = c.eos.press_from_rho_temp(rho[f], temp[f]);
-    } else { // fluxes.cxx:347-353
-      press[f] = th[f];
-      eps[f] = c.eos.eps_from_rho_press(rho[f], press[f]);
-      temp[f] = c.eos.temp_from_rho_eps(rho[f], eps[f]);
+  auto thermo_from = [&](const auto &eos) {
+    for (int f = 0; f < 2; ++f) {
+      if (with_T) { // fluxes.cxx:298-304
+        temp[f] = th[f];
+        eps[f] = eos.eps_from_rho_temp_ye(rho[f], temp[f], ye[f]);
+        press[f] = eos.press_from_rho_temp_ye(rho[f], temp[f], ye[f]);
+      } else { // fluxes.cxx:347-353
+        press[f] = th[f];
+        eps[f] = eos.eps_from_rho_press_ye(rho[f], press[f], ye[f]);
+        temp[f] = eos.temp_from_rho_eps_ye(rho[f], eps[f], ye[f]);
+      }
     }
-  }
+  };
+  if (c.tab) thermo_from(c.teos);
+  else thermo_from(c.eos);
   double rhoh[2];
   for (int f = 0; f < 2; ++f) rhoh[f] = rho[f] + rho[f] * eps[f] + press[f];
 
@@ -348,9 +364,17 @@ void calc_flux_face(const Ctx &c, int dir, const int I[3]) {
   for (int f = 0; f < 2; ++f) bsq[f] = (B2[f] + pow2(ab0[f])) / pow2(W[f]);
   const double beta_avg = betas_avg[dir];
   double cs2[2], h[2];
-  for (int f = 0; f < 2; ++f)
-    cs2[f] = c.eos.csnd_from_rho_eps(rho[f], eps[f]) *
-             c.eos.csnd_from_rho_eps(rho[f], eps[f]);
+  // fluxes.cxx:555-559: both calls take eps_rc(f) by reference (a table
+  // clamps it through the temperature inversion); h below sees the result
+  auto cs2_from = [&](const auto &eos) {
+    for (int f = 0; f < 2; ++f) {
+      const double a = eos.csnd_from_rho_eps_ye(rho[f], eps[f], ye[f]);
+      const double b = eos.csnd_from_rho_eps_ye(rho[f], eps[f], ye[f]);
+      cs2[f] = a * b;
+    }
+  };
+  if (c.tab) cs2_from(c.teos);
+  else cs2_from(c.eos);
   for (int f = 0; f < 2; ++f) h[f] = 1 + eps[f] + press[f] / rho[f];
   double dens[2], dent[2], dhW[2], dhWb[2], ppm[2], mom[3][2], tau[2], Bt[3][2];
   for (int f = 0; f < 2; ++f) dens[f] = sqrtg * rho[f] * W[f];
@@ -773,6 +797,9 @@ int orc_fluxes(const asx_patch *pt, const asx_params *par, int face_halo) {
   int st = check_par(par);
   if (st) return st;
   if (!pt) return ASX_ERR_NULL;
+  // a table has no eps(rho, p) (eos_3p_tabulated3d.hxx:249-258)
+  if (par->evolution_eos == ASX_EOS_TABULATED && !par->reconstruct_with_temperature)
+    return ASX_ERR_UNSUPPORTED;
   if (face_halo < 0 || face_halo > 1) return ASX_ERR_PARAM;
   int m0, m1;
   stencil_range(par->reconstruction_method, m0, m1);
@@ -1024,6 +1051,10 @@ static int check_c2p_par(const asx_params *par) {
   int st = check_par(par);
   if (st) return st;
   if (par->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
+  if (par->evolution_eos == ASX_EOS_TABULATED &&
+      (par->c2p_prime != ASX_C2P_PALENZUELA || par->c2p_second == ASX_C2P_NOBLE ||
+       !par->use_temperature || par->use_press_atmo))
+    return ASX_ERR_UNSUPPORTED;
   if (par->c2p_prime != ASX_C2P_NOBLE && par->c2p_prime != ASX_C2P_PALENZUELA)
     return ASX_ERR_UNSUPPORTED;
   if (par->c2p_second != ASX_C2P_NOBLE && par->c2p_second != ASX_C2P_PALENZUELA &&
@@ -1066,7 +1097,8 @@ int orc_con2prim(const asx_patch *pt, const asx_params *par) {
     for (int i = 0; i < 3; ++i) in.saved_vel[i] = v.g(ASX_SAVED_VELX + i).at(I);
     in.temperature = v.g(ASX_TEMPERATURE).at(I);
     C2PCellOut out;
-    con2prim_cell(c.eos, c.poly, dp, in, out);
+    if (c.tab) con2prim_cell(c.teos, c.poly, dp, in, out);
+    else con2prim_cell(c.eos, c.poly, dp, in, out);
     v.g(ASX_CON2PRIM_FLAG).at(I) = out.flag;
     if (v.g(ASX_MASK_CC).ok()) v.g(ASX_MASK_CC).at(I) = out.mask_cc;
     const Prim &pv = out.pv;
@@ -1398,7 +1430,7 @@ int orc_prim2con_batch_ye(int64_t nz, const double *prim, const double *g, doubl
   return ASX_OK;
 }
 
-// asx_eos_from_rho_temp: out[3][n] = press, eps, c_s
+// asx_eos_from_rho_temp: out[4][n] = press, eps, c_s, entropy
 int orc_eos_from_rho_temp(int64_t n, const double *rho, const double *temp, const double *ye,
                           double *out, const asx_params *par) {
   int st = check_eos(par);
@@ -1408,6 +1440,7 @@ int orc_eos_from_rho_temp(int64_t n, const double *rho, const double *temp, cons
       out[i] = eos.press_from_rho_temp_ye(rho[i], temp[i], ye[i]);
       out[n + i] = eos.eps_from_rho_temp_ye(rho[i], temp[i], ye[i]);
       out[2 * n + i] = eos.csnd_from_rho_temp_ye(rho[i], temp[i], ye[i]);
+      out[3 * n + i] = eos.entropy_from_rho_temp_ye(rho[i], temp[i], ye[i]);
     }
   };
   if (par->evolution_eos == ASX_EOS_TABULATED) run(make_tab(par->eos_table));
@@ -1434,8 +1467,8 @@ int orc_eos_from_rho_eps(int64_t n, const double *rho, const double *eps, const 
   return ASX_OK;
 }
 
-// The table's interpolation alone (pinning): out[3][n] = log p,
-// log(eps + shift), c_s^2 at (log rho, log T, Ye)
+// The table's interpolation alone (pinning): out[4][n] = log p,
+// log(eps + shift), c_s^2, entropy at (log rho, log T, Ye)
 int orc_tab_interp(int64_t n, const double *lr, const double *lt, const double *ye,
                    double *out, const asx_params *par) {
   if (!par || par->evolution_eos != ASX_EOS_TABULATED) return ASX_ERR_PARAM;
@@ -1443,7 +1476,7 @@ int orc_tab_interp(int64_t n, const double *lr, const double *lt, const double *
   if (st) return st;
   const TabEOS e = make_tab(par->eos_table);
   for (int64_t i = 0; i < n; ++i)
-    for (int v = 0; v < 3; ++v) out[v * n + i] = e.interpolate(v, lr[i], lt[i], ye[i]);
+    for (int v = 0; v < 4; ++v) out[v * n + i] = e.interpolate(v, lr[i], lt[i], ye[i]);
   return ASX_OK;
 }
 

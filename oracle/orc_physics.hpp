@@ -429,6 +429,9 @@ struct IdealGas {
   double csnd_from_rho_temp_ye(double rho, double temp, double) const {
     return csnd_from_rho_eps(rho, eps_from_rho_temp(rho, temp));
   }
+  double entropy_from_rho_temp_ye(double rho, double temp, double) const {
+    return kappa_from_rho_eps(rho, eps_from_rho_temp(rho, temp));
+  }
 
 };
 

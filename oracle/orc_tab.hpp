@@ -5,9 +5,9 @@
 //   interpolation: lower node by truncation of (x - x0)/dx, clamped to
 //   [0, n-2]; interpolation in rho, then T, then Ye),
 //   utils/eos_brent.hxx:8-166 (zero_brent, Brent's ZERO).
-// The table holds 3 variables, the variable index fastest: log p,
-// log(eps + shift) and c_s^2 (the reference's table holds 19 and reads 3 of
-// them on this path; the entropy is not tabulated here, so kappa is 0).
+// The table holds 4 variables, the variable index fastest: log p,
+// log(eps + shift), c_s^2 and the entropy (the reference's table holds 19
+// and reads these 4 on this path).
 #pragma once
 
 #include <cmath>
@@ -77,7 +77,7 @@ template <typename F> double zero_brent(double a, double b, double t, F &f) {
 }
 
 struct TabEOS {
-  enum { PRESS = 0, EPS = 1, CS2 = 2, NV = 3 };
+  enum { PRESS = 0, EPS = 1, CS2 = 2, ENT = 3, NV = 4 };
   const double *y;       // table, variable fastest
   const double *x[3];    // axes: log rho, log T, Ye
   int n[3];
@@ -167,11 +167,13 @@ struct TabEOS {
   double temp_from_rho_eps_ye(double rho, double &eps, double ye) const {
     return std::exp(logtemp_from_rho_eps_ye(rho, eps, ye));
   }
-  // :331-336 -> entropy_from_valid_rho_temp_ye(rho, temp_from(...), ye); no
-  // entropy table here: 0 after the inversion (which may change eps)
+  // :320-329
+  double entropy_from_rho_temp_ye(double rho, double temp, double ye) const {
+    return interpolate(ENT, std::log(crho(rho)), std::log(ctemp(temp)), ye);
+  }
+  // :331-336
   double kappa_from_rho_eps_ye(double rho, double &eps, double ye) const {
-    (void)temp_from_rho_eps_ye(rho, eps, ye);
-    return 0.0;
+    return entropy_from_rho_temp_ye(rho, temp_from_rho_eps_ye(rho, eps, ye), ye);
   }
   // :249-258: refused by the reference for tables
   double eps_from_rho_press_ye(double, double, double) const {

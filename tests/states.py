@@ -46,8 +46,23 @@ def prim2con_np(g, rho, eps, press, vel, B, Ye, ent):
     return dens, mom, tau, [sq * B[i] for i in range(3)], dens * Ye, dens * ent
 
 
+def _table_thermo(table, par, rho, temp, ye):
+    """press, eps, entropy from the oracle's EOS at (rho, T, Ye)."""
+    import ctypes as C
+    import oracle
+    shp = rho.shape
+    r, t, y = (np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (rho, temp, ye))
+    out = np.zeros((4, r.size))
+    abi.check("orc_eos_from_rho_temp", oracle.lib().orc_eos_from_rho_temp(
+        r.size, r.ctypes.data, t.ctypes.data, y.ctypes.data, out.ctypes.data, C.byref(par)))
+    return out[0].reshape(shp), out[1].reshape(shp), out[3].reshape(shp)
+
+
 def make_state(n=(18, 14, 12), ng=3, curved=True, seed=7, gamma=5.0 / 3.0,
-               uct=True, zero_entropy_block=False, jump=False):
+               uct=True, zero_entropy_block=False, jump=False, table_par=None):
+    """With table_par (asx_params of a host table): rho ~ 1e-6 (code),
+    T ~ 5 MeV, Ye ~ 0.3, and eps, press, entropy from the table; magnetic
+    field and vector potential scaled by 1e-3."""
     rng = np.random.default_rng(seed)
     dx = [0.1, 0.12, 0.09]
     p = Patch(n, ng, (-0.9, -0.8, -0.6), dx, slots=all_slots(use_uct=uct))
@@ -95,12 +110,21 @@ def make_state(n=(18, 14, 12), ng=3, curved=True, seed=7, gamma=5.0 / 3.0,
          np.broadcast_to(-0.2 + 0.05 * x, cs) + noise("Bvecz", 0.01)]
     eps = press / ((gamma - 1.0) * rho)
     ent = (gamma - 1.0) * eps * np.power(rho, 1.0 - gamma)
+    Ye = 0.5 + noise("Ye", 0.01)
+    temp = (gamma - 1.0) * eps
+    bscale = 1.0
+    if table_par is not None:
+        rho = 1e-6 * rho
+        temp = 5.0 * press / 0.6
+        Ye = 0.3 + noise("Ye", 0.02)
+        press, eps, ent = _table_thermo(None, table_par, rho, temp, Ye)
+        bscale = 1e-3
+        B = [bscale * b for b in B]
     if zero_entropy_block:
         ent = ent.copy()
         ent[ng + 2:ng + 4, ng + 3:ng + 5, ng + 4:ng + 6] = 0.0
-    Ye = 0.5 + noise("Ye", 0.01)
     p["rho"], p["press"], p["eps"], p["entropy"], p["Ye"] = rho, press, eps, ent, Ye
-    p["temperature"] = (gamma - 1.0) * eps
+    p["temperature"] = temp
     for i, c in enumerate("xyz"):
         p["vel" + c] = vel[i]
         p["Bvec" + c] = B[i]
@@ -131,14 +155,14 @@ def make_state(n=(18, 14, 12), ng=3, curved=True, seed=7, gamma=5.0 / 3.0,
                     ("Avec_y", lambda z, y, x: 0.2 * x + 0.05 * np.cos(z)),
                     ("Avec_z", lambda z, y, x: 0.4 * y + 0.03 * x * x)):
         zz, yy, xx = p.coords(abi.centering(name))
-        p[name] = np.broadcast_to(f(zz, yy, xx), p.shape(name)) + noise(name, 1e-3)
+        p[name] = bscale * (np.broadcast_to(f(zz, yy, xx), p.shape(name)) + noise(name, 1e-3))
     zz, yy, xx = p.coords(7)
     p["Psi"] = np.broadcast_to(0.01 * np.sin(xx + yy + zz), p.shape("Psi")) + \
         noise("Psi", 1e-3)
     for d, name in enumerate(abi.DBSTAG):
         zz, yy, xx = p.coords(1 << d)
-        p[name] = np.broadcast_to(B[d].mean() + 0.01 * np.sin(xx - zz),
-                                  p.shape(name)) + noise(name, 1e-3)
+        p[name] = np.broadcast_to(B[d].mean() + bscale * 0.01 * np.sin(xx - zz),
+                                  p.shape(name)) + bscale * noise(name, 1e-3)
     return p
 
 

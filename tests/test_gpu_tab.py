@@ -75,12 +75,12 @@ def test_eos_points_match_oracle():
     n = 4000
     rho, temp, ye = _points(htab, n, 1)
     L, O = _L(), oracle.lib()
-    out = torch.empty((3, n), dtype=torch.float64, device="cuda")
+    out = torch.empty((4, n), dtype=torch.float64, device="cuda")
     r_, t_, y_ = _dev(rho), _dev(temp), _dev(ye)
     abi.check("asx_eos_from_rho_temp", L.asx_eos_from_rho_temp(
         n, _p(r_), _p(t_), _p(y_), _p(out), C.byref(_par(dtab)), None))
     torch.cuda.synchronize()
-    ref = np.zeros((3, n))
+    ref = np.zeros((4, n))
     abi.check("orc_eos_from_rho_temp", O.orc_eos_from_rho_temp(
         n, rho.ctypes.data, temp.ctypes.data, ye.ctypes.data, ref.ctypes.data,
         C.byref(_par(htab))))
@@ -88,11 +88,13 @@ def test_eos_points_match_oracle():
     assert np.allclose(got, ref, rtol=1e-13, atol=0), np.max(np.abs(got / ref - 1))
     # inversion from (rho, eps): eps from the table at T, and eps off the
     # table on both sides
+    out = out[:3]
     eps = ref[1] * np.where(np.arange(n) % 7 == 0, 1e-3, np.where(np.arange(n) % 7 == 1, 1e3, 1))
     e_ = _dev(eps)
     abi.check("asx_eos_from_rho_eps", L.asx_eos_from_rho_eps(
         n, _p(r_), _p(e_), _p(y_), _p(out), C.byref(_par(dtab)), None))
     torch.cuda.synchronize()
+    ref = np.zeros((3, n))
     abi.check("orc_eos_from_rho_eps", O.orc_eos_from_rho_eps(
         n, rho.ctypes.data, eps.ctypes.data, ye.ctypes.data, ref.ctypes.data,
         C.byref(_par(htab))))
@@ -187,3 +189,122 @@ def test_table_parameters_refused():
     par.eos_table = None
     assert L.asx_eos_from_rho_temp(n, _p(buf), _p(buf), _p(buf), _p(buf), C.byref(par),
                                    None) == 1
+
+
+# ---------------------------------------------------------------------------
+# A patch on the table: faces, con2prim, the fused substep
+# ---------------------------------------------------------------------------
+def _patch_pair(par_h, **kw):
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(__file__))
+    import states
+    h = states.make_state(table_par=par_h, **kw)
+    return h, h.to("cuda"), states
+
+
+@pytest.mark.parametrize("recon", ["wenoz", "ppm", "mp5"])
+@pytest.mark.parametrize("flux", ["HLLE", "LxF"])
+def test_fluxes_on_table(recon, flux):
+    """The device reads eps, p and c_s^2 at the reconstructed (rho, T, Ye)
+    from one lookup; the oracle follows the reference (c_s from (rho, eps)
+    through the temperature inversion): equal to 1e-12."""
+    _torch()
+    dtab, htab = _tables()
+    kw = dict(reconstruction_method=recon, flux_type=flux, reconstruct_with_temperature=1,
+              loworder_method="minmod")
+    h, d, states = _patch_pair(_par(htab), zero_entropy_block=True)
+    from asterx_amd import lib
+    cnt = abi.asx_counters()
+    lib.backend()._call("fluxes", C.byref(d.struct()), C.byref(_par(dtab, **kw)), 1,
+                        C.byref(cnt), None)
+    _torch().cuda.synchronize()
+    abi.check("orc_fluxes", oracle.lib().orc_fluxes(C.byref(h.struct()),
+                                                   C.byref(_par(htab, **kw)), 1))
+    assert cnt.lo_faces > 0
+    bad = {}
+    for name in abi.FLUX:
+        c = abi.centering(name)
+        sl = []
+        for ax in (2, 1, 0):
+            v = (c >> ax) & 1
+            hh = 0 if v else 1
+            sl.append(slice(h.ng - hh, h.ng + h.n[ax] + v + hh))
+        e = states.rel_err(d.numpy(name)[tuple(sl)], h.numpy(name)[tuple(sl)])
+        if not e <= 1e-12:
+            bad[name] = e
+    assert not bad, bad
+
+
+def test_fluxes_on_table_refuse_pressure_reconstruction():
+    _torch()
+    dtab, _ = _tables()
+    h, d, _ = _patch_pair(_par(_tables()[1]))
+    from asterx_amd import lib
+    r = lib.lib().asx_fluxes(C.byref(d.struct()),
+                             C.byref(_par(dtab, reconstruction_method="wenoz",
+                                          reconstruct_with_temperature=0)), 0, None, None)
+    assert r == 4  # ASX_ERR_UNSUPPORTED
+
+
+def test_con2prim_on_table():
+    _torch()
+    dtab, htab = _tables()
+    h, d, states = _patch_pair(_par(htab))
+    for p in (h, d):
+        dens = p.numpy("dens").copy()
+        dens[5, 6, 7] = 1e-25  # atmosphere
+        p["dens"] = dens
+    par_kw = dict(rho_abs_min=1e-12)
+    from asterx_amd import lib
+    cnt = abi.asx_counters()
+    lib.backend()._call("con2prim", C.byref(d.struct()), C.byref(_par(dtab, **par_kw)),
+                        C.byref(cnt), None)
+    _torch().cuda.synchronize()
+    abi.check("orc_con2prim", oracle.lib().orc_con2prim(C.byref(h.struct()),
+                                                       C.byref(_par(htab, **par_kw))))
+    sl = h.interior("rho")
+    fl_h, fl_d = h.numpy("con2prim_flag")[sl], d.numpy("con2prim_flag")[sl]
+    assert np.array_equal(fl_h, fl_d)
+    assert (fl_h == 4).sum() >= 1 and (fl_h == 1).sum() > 0.9 * fl_h.size
+    bad = {}
+    for name in ["rho", "velx", "vely", "velz", "eps", "press", "temperature", "entropy",
+                 "Ye"] + abi.CONS:
+        e = states.rel_err(d.numpy(name)[sl], h.numpy(name)[sl])
+        if not e <= 1e-9:
+            bad[name] = e
+    assert not bad, bad
+
+
+def test_substep_on_table_matches_oracle():
+    torch = _torch()
+    dtab, htab = _tables()
+    from asterx_amd.patch import StateBuf
+    from asterx_amd import lib
+    h, d, states = _patch_pair(_par(htab))
+    kw = dict(reconstruction_method="wenoz", flux_type="HLLE", reconstruct_with_temperature=1)
+    c0, c1, c2 = 0.75, 0.25, 0.25 * 0.01
+    res = []
+    for p, dev in ((d, True), (h, False)):
+        u0 = StateBuf(p, fill=0.0)
+        out = StateBuf(p, fill=0.0)
+        for name in abi.STATE:
+            u0.arr[name] = p[name].clone() * 1.01 if dev else p[name] * 1.01
+        ps, u0s, os_ = p.struct(), u0.struct(), out.struct()
+        if dev:
+            abi.check("asx_substep", lib.lib().asx_substep(
+                C.byref(ps), C.byref(u0s), C.byref(os_), c0, c1, c2,
+                C.byref(_par(dtab, **kw)), None))
+            torch.cuda.synchronize()
+        else:
+            abi.check("orc_substep", oracle.lib().orc_substep(
+                C.byref(ps), C.byref(u0s), C.byref(os_), c0, c1, c2, C.byref(_par(htab, **kw))))
+        res.append(out)
+    dg, ho = res
+    bad = {}
+    for name in abi.STATE:
+        sl = h.interior(name)
+        e = states.rel_err(dg.arr[name].cpu().numpy()[sl], ho.arr[name][sl])
+        if not e <= 1e-12:
+            bad[name] = e
+    assert not bad, bad

@@ -60,13 +60,14 @@ def test_table_sound_speed_and_monotonicity():
     assert np.allclose(cs2, (dPdr + P / rho ** 2 * dPde) / h, rtol=1e-6)
     assert np.allclose(P_re(rho, eps), P, rtol=1e-12)
     assert (cs2 > 0).all() and (cs2 < tabeos.C2_CGS).all()
-    lr, lt, ye_ax, lp, le, c2 = tabeos.make_cgs(20, 30, 5)
+    lr, lt, ye_ax, lp, le, c2, ent = tabeos.make_cgs(20, 30, 5)
+    assert (ent > 0).all()
     assert (np.diff(le, axis=1) > 0).all()  # eps increases with T: invertible
 
 
 def test_code_units_follow_the_reader():
-    lr, lt, ye, lp, le, c2 = tabeos.make_cgs(6, 5, 4)
-    t = tabeos.code_units(lr, lt, ye, lp, le, c2)
+    lr, lt, ye, lp, le, c2, ent = tabeos.make_cgs(6, 5, 4)
+    t = tabeos.code_units(lr, lt, ye, lp, le, c2, ent)
     assert np.allclose(np.exp(t["axes"][0]), 10.0 ** lr * tabeos.RHOGF, rtol=1e-13)
     assert np.allclose(np.exp(t["axes"][1]), 10.0 ** lt, rtol=1e-13)
     k, j, i = 2, 3, 4
@@ -75,13 +76,14 @@ def test_code_units_follow_the_reader():
     assert math.isclose(math.exp(row[1]), (10.0 ** le[k, j, i]) * tabeos.EPSGF, rel_tol=1e-12)
     # c_s^2 in units of c^2
     assert math.isclose(row[2], c2[k, j, i] / tabeos.C2_CGS, rel_tol=1e-9)
+    assert row[3] == ent[k, j, i]  # entropy per baryon, no units
     assert t["eps_min"] >= 1e-15 and t["eps_max"] > t["eps_min"]
 
 
 def _interp(table, lr, lt, ye):
     lr, lt, ye = _d(lr), _d(lt), _d(ye)  # keep the arrays alive over the call
     n = len(lr)
-    out = np.zeros((3, n))
+    out = np.zeros((4, n))
     abi.check("orc_tab_interp", oracle.lib().orc_tab_interp(
         n, lr.ctypes.data, lt.ctypes.data, ye.ctypes.data, out.ctypes.data,
         C.byref(_par(table))))
@@ -102,7 +104,7 @@ def test_interpolation_nodes_and_scipy(table):
     n = 2000
     pts = [rng.uniform(a[0], a[-1], n) for a in ax]
     out = _interp(table, *pts)
-    for v in range(3):
+    for v in range(4):
         f = RegularGridInterpolator(ax, np.transpose(data[..., v], (2, 1, 0)))
         ref = f(np.stack(pts, axis=1))
         assert np.allclose(out[v], ref, rtol=1e-12, atol=1e-14 * np.abs(ref).max()), v
@@ -112,7 +114,7 @@ def test_interpolation_exact_for_multilinear(table):
     ax = table.host["axes"]
     Y, T, R = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
     f = lambda r, t, y: 0.3 + 1.7 * r - 0.4 * t + 2.5 * y + 0.01 * r * t * y - 0.2 * t * y
-    data = np.stack([f(R, T, Y), 2 * f(R, T, Y), -f(R, T, Y)], axis=-1)
+    data = np.stack([f(R, T, Y), 2 * f(R, T, Y), -f(R, T, Y), 3 * f(R, T, Y)], axis=-1)
     t2 = dict(table.host, data=np.ascontiguousarray(data))
     tab2 = tabeos.Table(t2, device=None)
     rng = np.random.default_rng(1)
@@ -121,12 +123,13 @@ def test_interpolation_exact_for_multilinear(table):
     ref = f(*pts)
     assert np.allclose(out[0], ref, rtol=1e-12, atol=1e-12)
     assert np.allclose(out[1], 2 * ref, rtol=1e-12, atol=1e-12)
+    assert np.allclose(out[3], 3 * ref, rtol=1e-12, atol=1e-12)
 
 
 def _eos_rt(table, rho, temp, ye):
     rho, temp, ye = _d(rho), _d(temp), _d(ye)
     n = len(rho)
-    out = np.zeros((3, n))
+    out = np.zeros((4, n))
     abi.check("orc_eos_from_rho_temp", oracle.lib().orc_eos_from_rho_temp(
         n, rho.ctypes.data, temp.ctypes.data, ye.ctypes.data, out.ctypes.data,
         C.byref(_par(table))))
@@ -150,7 +153,8 @@ def test_temperature_inversion_round_trip(table):
     rho = np.exp(rng.uniform(np.log(r0) + 0.1, np.log(r1) - 0.1, n))
     temp = np.exp(rng.uniform(np.log(t0) + 0.05, np.log(t1) - 0.05, n))
     ye = rng.uniform(y0, y1, n)
-    p, eps, cs = _eos_rt(table, rho, temp, ye)
+    p, eps, cs, ent = _eos_rt(table, rho, temp, ye)
+    assert np.allclose(ent, tabeos.entropy(rho / tabeos.RHOGF, temp), rtol=1e-6)
     T2, p2, eps2 = _eos_re(table, rho, eps, ye)
     assert np.max(np.abs(np.log(T2 / temp))) < 1e-12
     assert np.allclose(p2, p, rtol=1e-11)
