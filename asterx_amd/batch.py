@@ -18,8 +18,8 @@ EDGES = (16, 24, 32, 48, 64)
 SSPRK3 = [(0.0, 1.0, 1.0), (0.75, 0.25, 0.25), (1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0)]
 STAGE_IN = (0, 1, 2)
 STAGE_OUT = (1, 2, 0)
-SYNC_PRIMS = ["rho", "velx", "vely", "velz", "eps", "press", "entropy", "Ye",
-              "Bvecx", "Bvecy", "Bvecz"]
+SYNC_PRIMS = ["rho", "velx", "vely", "velz", "eps", "press", "temperature", "entropy",
+              "Ye", "Bvecx", "Bvecy", "Bvecz"]
 # grid functions a batch patch carries (the plan owns fluxes, aux, dB_stag)
 PATCH_SLOTS = abi.ADM + abi.PRIMS + abi.CONS + abi.DB + abi.SAVED + abi.ZS + \
     ["con2prim_flag", "aster_mask_cc"] + abi.AVEC

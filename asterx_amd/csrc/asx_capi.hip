@@ -1267,9 +1267,12 @@ asx_status asx_c2p_batch_ye(int64_t nz, const double *cons, const double *g,
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kSyncPrims[11] = {ASX_RHO,     ASX_VELX, ASX_VELY,  ASX_VELZ,
-                                ASX_EPS,     ASX_PRESS, ASX_ENTROPY, ASX_YE,
-                                ASX_BVECX,   ASX_BVECY, ASX_BVECZ};
+// the primitives SYNCed after con2prim (AsterX/schedule.ccl:78, with Bvec)
+constexpr int kNumSyncPrims = 12;
+constexpr int kSyncPrims[kNumSyncPrims] = {ASX_RHO,   ASX_VELX,        ASX_VELY,    ASX_VELZ,
+                                           ASX_EPS,   ASX_PRESS,       ASX_TEMPERATURE,
+                                           ASX_ENTROPY, ASX_YE,        ASX_BVECX,
+                                           ASX_BVECY, ASX_BVECZ};
 // SSPRK3 stages: out = c0 u0 + c1 u + c2 dt rhs(u)
 constexpr double kRK3[3][3] = {{0.0, 1.0, 1.0}, {0.75, 0.25, 0.25},
                                {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
@@ -1529,7 +1532,7 @@ asx_status plan_stage(asx_batch *b, int s, hipStream_t str) {
                                            b->qcount, str);
        }))))
     return st;
-  return plan_bc(b, U, kSyncPrims, 11, str);
+  return plan_bc(b, U, kSyncPrims, kNumSyncPrims, str);
 }
 
 // Build every launch list the stages use with the current variants (no

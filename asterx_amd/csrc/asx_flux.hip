@@ -472,8 +472,13 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
 #ifndef ASX_FACE_WAVES
 #define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
 #endif
+// the table's face kernels keep the lookup's 8 corners x 3 variables and
+// the temperature bracket live beside the reconstruction: a 256-register
+// budget (2 waves per SIMD) instead of spilling ~300 B per lane at 3
+#define ASX_FACE_WAVES_TAB 2
 template <int D, int M, int V, bool TAB = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASX_FACE_WAVES)))
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(TAB ? ASX_FACE_WAVES_TAB : ASX_FACE_WAVES)))
 void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
   face_tile<D, M, V, TAB>(P, A, A.lo, A.cnt, xcd_tile((int)blockIdx.x, ntx * nty * ntz), ntx,
                           nty);

@@ -170,6 +170,7 @@ class Table:
         """Point asx_params at this table (evolution_eos = Tabulated3d)."""
         par.evolution_eos = abi.EOS["Tabulated3d"]
         par.eos_table = C.addressof(self.desc)
+        par._table = self  # the table's arrays live as long as the parameters
         return par
 
     def ranges(self):

@@ -34,9 +34,10 @@ if ROOT not in sys.path:
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6           # MI355X FP64, TFLOP/s
 
-# Grid functions the next substep reads in the ghost zones.
-SYNC_PRIMS = ["rho", "velx", "vely", "velz", "eps", "press", "entropy", "Ye",
-              "Bvecx", "Bvecy", "Bvecz"]
+# The primitives the reference SYNCs after con2prim (AsterX/schedule.ccl:78,
+# with Bvec): what the next substep may read in the ghost zones.
+SYNC_PRIMS = ["rho", "velx", "vely", "velz", "eps", "press", "temperature", "entropy",
+              "Ye", "Bvecx", "Bvecy", "Bvecz"]
 
 
 def parse():
@@ -64,6 +65,12 @@ def parse():
                     help="also time con2prim with the learned guess against "
                          "the plain seed on 2^24 independent zones")
     ap.add_argument("--c2p-zones", type=int, default=1 << 24)
+    ap.add_argument("--tab", choices=["on", "off"], default="on",
+                    help="also run the tabulated-EOS workload (torus on the table and "
+                         "con2prim with composition)")
+    ap.add_argument("--tab-n", type=int, default=128)
+    ap.add_argument("--tab-steps", type=int, default=12)
+    ap.add_argument("--tab-c2p-zones", type=int, default=1 << 22)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -308,6 +315,142 @@ def bench_c2p_guess(args, L):
     return out
 
 
+def table_params(table, **kw):
+    """asx_params on the table: Palenzuela on the temperature, atmosphere
+    T = 0.01 MeV at rho 1e-14 (code), WENO-Z + HLLE reconstructing T."""
+    from asterx_amd import abi
+    base = dict(c2p_prime="Palenzuela", c2p_second="None", use_temperature=1,
+                use_press_atmo=0, thermal_eos_atmo=1, rho_abs_min=1e-14, t_atmo=0.01,
+                Ye_atmo=0.5, c2p_tol=1e-10, max_iter=100, vw_lim=1e3, B_lim=1e3,
+                ye_min=0.035, ye_max=0.55, reconstruction_method="wenoz", flux_type="HLLE",
+                reconstruct_with_temperature=1)
+    base.update(kw)
+    return table.attach(abi.params(**base))
+
+
+def bench_tabulated(args, L, sh, stream):
+    """The tabulated-EOS workload: the 234x180x50 synthetic table; the
+    torus geometry at `--tab-n`^3 on the table (SSPRK3 substeps, as the
+    headline), and Palenzuela con2prim with composition on 2^22 independent
+    zones (rho 1e-12..1e-4 code, T 0.05..80 MeV, Ye 0.06..0.5, W up to 10,
+    b^2/rho 1e-4..1e2, previous-step seed = truth x (1 + 1e-2 u))."""
+    import numpy as np
+    import torch
+    from asterx_amd import abi, tabeos, torus, zones
+    from asterx_amd.patch import Patch
+    host_tab = tabeos.build()
+    table = tabeos.Table(host_tab, device="cuda")
+    par = table_params(table)
+    from asterx_amd import lib
+    be = lib.backend(sh)
+    n, ng, x0, dx = torus.torus_patch_geometry(n=args.tab_n)
+    slots = abi.ADM + abi.PRIMS + abi.CONS + abi.DB + abi.SAVED + abi.ZS + \
+        ["con2prim_flag", "aster_mask_cc"] + abi.DBSTAG + abi.AVEC
+    with torch.cuda.stream(stream):
+        p = Patch(n, ng, x0, dx, slots, device="cuda:0")
+        torus.build_torus_tab(p, be, par, table.ranges())
+    torch.cuda.synchronize()
+    for s in abi.DBSTAG:
+        del p.arr[s]
+    zn = n[0] * n[1] * n[2]
+    dt = 0.2 * dx[0]
+    st = Stepper(p, par, be, dt, sh)
+    for _ in range(args.warmup):
+        st.substep()
+    torch.cuda.synchronize()
+    K = args.tab_steps
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    t0 = time.perf_counter()
+    for _ in range(K):
+        st.substep()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = {}
+    for name, kc in abi.KCLASS.items():
+        nl, ms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(kc, C.byref(nl), C.byref(ms)))
+        if nl.value:
+            kt[name] = round(ms.value / K, 4)
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    cnt = abi.asx_counters()
+    ps = p.struct()
+    abi.check("asx_con2prim_group", L.asx_con2prim_group(C.byref(ps), C.byref(par),
+                                                         C.byref(cnt), sh))
+    torch.cuda.synchronize()
+    for name in abi.CONS:
+        if not bool(torch.isfinite(p.arr[name][p.interior(name)]).all()):
+            raise RuntimeError("non-finite %s on the table" % name)
+    patch_out = {"workload": "torus %d^3 +3 ghosts on the table, WENO-Z + HLLE on (rho, T, "
+                             "Ye), Palenzuela on T" % args.tab_n,
+                 "zones": zn, "steps": K, "ms_per_step": round(1e3 * wall / K, 4),
+                 "value": round(zn * K / wall, 1), "unit": "zone-updates/s",
+                 "kernels_ms_per_step": kt, "c2p_flags": list(cnt.c2p_flags)}
+    del st, p
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+    # con2prim with composition on independent zones
+    nz = args.tab_c2p_zones
+    prim, g = zones.make_batch_ye(nz, 5, (1e-12, 1e-4), (0.05, 80.0), (0.06, 0.5))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+    pd, gd = dev(prim), dev(g)
+    del prim
+    cd = torch.empty((9, nz), dtype=torch.float64, device="cuda")
+    td = torch.empty((2, nz), dtype=torch.float64, device="cuda")
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    abi.check("asx_prim2con_batch_ye", L.asx_prim2con_batch_ye(
+        nz, ptr(pd), ptr(gd), ptr(cd), ptr(td), C.byref(par), sh))
+    torch.cuda.synchronize()
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    u = lambda: 1 + 1e-2 * (2 * torch.rand(nz, device="cuda", dtype=torch.float64,
+                                           generator=gen) - 1)
+    seed = torch.stack([pd[0] * u(), td[0] * u(), pd[3] * u(), pd[4] * u(), pd[5] * u(),
+                        pd[2], pd[1]]).contiguous()
+    od = torch.empty((9, nz), dtype=torch.float64, device="cuda")
+    fd = torch.empty(nz, dtype=torch.float64, device="cuda")
+    idd = torch.empty(nz, dtype=torch.int32, device="cuda")
+    args_c = (nz, ptr(cd), ptr(gd), ptr(seed), ptr(od), ptr(fd), ptr(idd), C.byref(par), sh)
+    abi.check("asx_c2p_batch_ye", L.asx_c2p_batch_ye(*args_c))   # warm-up
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        abi.check("asx_c2p_batch_ye", L.asx_c2p_batch_ye(*args_c))
+    torch.cuda.synchronize()
+    cwall = (time.perf_counter() - t0) / reps
+    flags = torch.bincount(fd.to(torch.int64).clamp(0, 6), minlength=7).tolist()
+    ok = fd == 1
+    rel_rho = float(((od[0] - pd[0]).abs() / pd[0])[ok].max())
+    rel_t = float(((od[7] - pd[1]).abs() / pd[1])[ok].max())
+    c2p_out = {"zones": nz, "zones_per_s": round(nz / cwall, 1),
+               "ms_per_batch": round(1e3 * cwall, 4),
+               "iters_mean": round(float(idd[ok].double().mean()), 3), "flags": flags,
+               "max_rel_err_rho": rel_rho, "max_rel_err_T": rel_t}
+    log(args, "tabulated patch:", patch_out, "\ntabulated c2p batch:", c2p_out)
+    # the CPU oracle on the first 2^15 zones of the same batch
+    m = min(nz, 1 << 15)
+    hc = np.ascontiguousarray(cd[:, :m].cpu().numpy())
+    hg = np.ascontiguousarray(gd[:, :m].cpu().numpy())
+    hs = np.ascontiguousarray(seed[:, :m].cpu().numpy())
+    htable = tabeos.Table(host_tab, device=None)  # holds the arrays hpar points into
+    hpar = table_params(htable)
+    import oracle
+    ho, hf, hi = np.zeros((9, m)), np.zeros(m), np.zeros(m, dtype=np.int32)
+    t0 = time.perf_counter()
+    abi.check("orc_c2p_batch_ye", oracle.lib().orc_c2p_batch_ye(
+        m, hc.ctypes.data, hg.ctypes.data, hs.ctypes.data, ho.ctypes.data, hf.ctypes.data,
+        hi.ctypes.data, C.byref(hpar)))
+    c2p_out["cpu_oracle_zones_per_s"] = round(m / (time.perf_counter() - t0), 1)
+    c2p_out["cpu_sample_zones"] = m
+    c2p_out["flags_equal_to_oracle_on_sample"] = bool(
+        np.array_equal(hf, fd[:m].cpu().numpy()))
+    del htable, pd, gd, cd, td, seed, od, fd, idd
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return {"table": "234x180x50, 4 variables (log p, log(eps+shift), c_s^2, entropy)",
+            "patch": patch_out, "c2p_batch": c2p_out}
+
+
 def cpu_baseline(host_patch, par, dt, steps):
     """The CPU oracle (OpenMP over k-planes) on the same patch and inputs:
     stage-1 substep + SYNC + B from A + con2prim + SYNC, per substep."""
@@ -488,6 +631,12 @@ def main():
         guess = bench_c2p_guess(args, L)
         log(args, "c2p guess: %.1f s" % (time.perf_counter() - t0), guess)
 
+    tab = None
+    if args.tab == "on":
+        t0 = time.perf_counter()
+        tab = bench_tabulated(args, L, sh, stream)
+        log(args, "tabulated: %.1f s" % (time.perf_counter() - t0), tab)
+
     cpu = None
     if host is not None:
         ts = cpu_baseline(host, par, dt, args.cpu_steps)
@@ -511,7 +660,7 @@ def main():
                                   (args.n, args.recon, args.flux),
                       "zones": zones, "substep": "SSPRK3 stages cycled"},
            "roofline": roof, "cpu_baseline": cpu,
-           "batched": batched, "c2p_guess": guess}
+           "batched": batched, "c2p_guess": guess, "tabulated": tab}
     print(json.dumps(out), flush=True)
 
 

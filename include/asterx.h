@@ -513,8 +513,8 @@ asx_status asx_c2p_batch_ye(int64_t nz, const double *cons, const double *g,
  * (c0, c1, c2) = (0, 1, dt), (3/4, 1/4, dt/4), (1/3, 2/3, 2dt/3)):
  *   asx_substep from state buffer b_in = s into b_out = (1, 2, 0)[s] with
  *   u0 = buffer 0, boundary fill of the 11 state fields of b_out,
- *   asx_con2prim_group on b_out, boundary fill of the 11 synced primitives
- *   (rho, vel^i, eps, press, entropy, Ye, Bvec^i),
+ *   asx_con2prim_group on b_out, boundary fill of the 12 synced primitives
+ *   (rho, vel^i, eps, press, temperature, entropy, Ye, Bvec^i),
  * so a step leaves the new state in buffer 0.  Buffer 0 is the state slots
  * of patches[p]; bufs[2p] and bufs[2p+1] are buffers 1 and 2 of patch p.
  * Requirements per patch are those of asx_substep and asx_con2prim_group;

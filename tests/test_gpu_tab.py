@@ -85,7 +85,15 @@ def test_eos_points_match_oracle():
         n, rho.ctypes.data, temp.ctypes.data, ye.ctypes.data, ref.ctypes.data,
         C.byref(_par(htab))))
     got = out.cpu().numpy()
-    assert np.allclose(got, ref, rtol=1e-13, atol=0), np.max(np.abs(got / ref - 1))
+    for row in (0, 2, 3):  # press, c_s, entropy
+        assert np.allclose(got[row], ref[row], rtol=1e-13, atol=0), \
+            (row, np.max(np.abs(got[row] / ref[row] - 1)))
+    # eps = exp(log(eps + shift)) - shift: the device's and the host's exp
+    # and log may differ by an ulp of eps + shift, which the subtraction
+    # turns into up to shift/eps ulps of eps
+    shift = htab.host["shift"]
+    assert np.all(np.abs(got[1] - ref[1]) <= 1e-13 * (np.abs(ref[1]) + shift)), \
+        np.max(np.abs(got[1] - ref[1]) / (np.abs(ref[1]) + shift))
     # inversion from (rho, eps): eps from the table at T, and eps off the
     # table on both sides
     out = out[:3]
@@ -153,11 +161,17 @@ def _c2p_case(table_d, table_h, nz, seed, ideal=False):
     assert np.array_equal(gflag, flag), np.unique(gflag - flag, return_counts=True)
     ok = flag == 1
     assert ok.mean() > 0.95
+    # ideal gas (Noble, Newton to 1e-10): 1e-9.  Table (Palenzuela's Brent
+    # on x = hW to 1e-10, eps -> T by Brent per residual): the stop leaves
+    # rho and T set to ~1e-8 only -- the oracle itself moves by 7e-9 when
+    # its inputs change by one ulp -- and the device's exp/log differ from
+    # the host's by an ulp, so 3e-8
+    tol = 1e-9 if ideal else 3e-8
     for row in (0, 1, 2, 6, 7):
         rel = np.abs(got[row, ok] - out[row, ok]) / np.abs(out[row, ok])
-        assert np.nanmax(rel) < 1e-9, (row, np.nanmax(rel))
+        assert np.nanmax(rel) < tol, (row, np.nanmax(rel))
     for row in (3, 4, 5):
-        assert np.nanmax(np.abs(got[row, ok] - out[row, ok])) < 1e-9, row
+        assert np.nanmax(np.abs(got[row, ok] - out[row, ok])) < tol, row
     assert np.array_equal(got[8], out[8])  # Ye = DYe / D
     return got, out, flag
 
