@@ -133,7 +133,8 @@ ASX_DEV void store_zone(const KPatch &P, int oc, const C2POut &o) {
 // fails store nothing; a wavefront ballot packs their indices into `queue`
 // with one atomic per wavefront.
 template <bool FROM_A, int PRIME, class E>
-__global__ __launch_bounds__(256) void k_c2p_main(KPatch P, Box B, E eos, C2PArgs a,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(c2p_waves<E>())))
+void k_c2p_main(KPatch P, Box B, E eos, C2PArgs a,
                                                   unsigned long long *hist, int *queue,
                                                   unsigned *qcount) {
   __shared__ HistLDS h;

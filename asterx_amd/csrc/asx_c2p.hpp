@@ -15,6 +15,12 @@
 
 namespace asx {
 
+// Waves per SIMD the primary-solver kernels are budgeted for: on a table
+// the nested temperature solve would otherwise take every register (one
+// wave per SIMD, with the latency of the table reads exposed)
+template <class E> constexpr int c2p_waves() { return std::is_same<E, TabEOS>::value ? 2 : 1; }
+
+
 enum C2PStatus {
   ST_SUCCESS = 0, ST_INVALID_DETG, ST_NEG_BSQR, ST_NANS_IN_CONS, ST_RANGE_RHO,
   ST_RANGE_EPS, ST_SPEED_LIMIT, ST_RANGE_YE, ST_B_LIMIT, ST_ROOT_FAIL_CONV,
@@ -493,12 +499,14 @@ template <class E> struct PalF {
 // Brent's ZERO (Brent 1973; the form of EOSX/src/utils/eos_brent.hxx)
 // returning the final bracket (b, c): stops when |b - c| <= rel*min(|b|,|c|),
 // f(b) == 0, or after maxiters evaluations; steps are at least half that
-// width so the bracket collapses once b has converged.
+// width so the bracket collapses once b has converged.  fa0, fb0 are f(a),
+// f(b), which the caller has evaluated to choose the bracket (the residual
+// is deterministic, so this equals evaluating them again).
 template <typename F>
-ASX_DEV void brent(const F &fn, double a, double b, double rel, int maxiters,
-                   double &ra, double &rb, int &iters) {
+ASX_DEV void brent(const F &fn, double a, double b, double fa0, double fb0, double rel,
+                   int maxiters, double &ra, double &rb, int &iters) {
   double sa = a, sb = b;
-  double fa = fn(sa), fb = fn(sb);
+  double fa = fa0, fb = fb0;
   iters = 0;
   if (fa == 0.0) { ra = rb = sa; return; }
   if (fb == 0.0) { ra = rb = sb; return; }
@@ -559,7 +567,8 @@ ASX_DEV void brent(const F &fn, double a, double b, double rel, int maxiters,
 // Palenzuela 1D, c2p_1DPalenzuela.hxx:306-556
 template <class E>
 ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
-                              Prim &pv, Cons &cv, const Sym3 &glo, Report &rep) {
+                              Prim &pv, Cons &cv, const Sym3 &glo, Report &rep,
+                              double xguess = -1.0, double xrel = 0.0) {
   rep.iters = 0;
   rep.adjust_cons = false;
   rep.set_atmo = false;
@@ -599,9 +608,34 @@ ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
   const PalF<E> fn{eos, atmo, a, q, r, s, t, cv.dens, cv.DYe / cv.dens};
   const double xa = smax(1e-10, 1.0 + q - s);
   double xb = 2.0 + 2.0 * q - s;
-  if (fn(xa) * fn(xb) > 0) xb = 3.0 + 3.0 * q - 1.5 * s;
+  double lo = xa, hi = xb, flo = 0.0, fhi = 0.0;
+  bool hinted = false;
+  if (xguess > 0.0) {
+    // a bracket of relative half-width xrel around the guess of x = h W,
+    // inside the widest bracket the reference may take, used when the
+    // residual changes sign across it (the root it holds is the root)
+    const double xw = 3.0 + 3.0 * q - 1.5 * s;
+    const double gl = smax(xa, xguess * (1.0 - xrel)), gh = smin(xw, xguess * (1.0 + xrel));
+    if (gl < gh) {
+      flo = fn(gl);
+      fhi = fn(gh);
+      if (flo * fhi <= 0.0) {
+        lo = gl;
+        hi = gh;
+        hinted = true;
+      }
+    }
+  }
+  if (!hinted) {
+    flo = fn(xa);
+    fhi = fn(xb);
+    if (flo * fhi > 0) {
+      hi = 3.0 + 3.0 * q - 1.5 * s;
+      fhi = fn(hi);
+    }
+  }
   double ra, rb;
-  brent(fn, xa, xb, a.tol0, a.max_iter, ra, rb, rep.iters);
+  brent(fn, lo, hi, flo, fhi, a.tol0, a.max_iter, ra, rb, rep.iters);
   const double x = 0.5 * (ra + rb);
   {
     double Wm2 = 1.0 - (x * x * r + (2.0 * x + s) * t * t) / (x * x * (x + s) * (x + s));
@@ -616,9 +650,7 @@ ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
     }
     vel_from_Z(x * pv.rho * W, W, Bsq, BiSi, cv, gup, glo, a.use_z, pv);
     pv.Ye = cv.DYe / cv.dens;
-    pv.press = eos.press_re(pv.rho, pv.eps, pv.Ye);
-    pv.temperature = eos.temp_re(pv.rho, pv.eps, pv.Ye);
-    pv.entropy = eos.kappa_re(pv.rho, pv.eps, pv.Ye);
+    eos.state_re(pv.rho, pv.eps, pv.Ye, pv.press, pv.temperature, pv.entropy);
     pv.B[0] = cv.dB[0]; pv.B[1] = cv.dB[1]; pv.B[2] = cv.dB[2];
   }
   if (pv.rho <= 0.0) {
@@ -679,6 +711,10 @@ struct C2POut {
 // compiled for that method (a.prime must equal it); returns false, with
 // `out` not set, when it fails: the zone then needs the full chain, which
 // repeats the same primary solve before the backup.
+// zguess, vguess (optional, > 0) guide the primary solver: Noble starts
+// from Z = zguess, v^2 = vguess; Palenzuela brackets x = h W within the
+// relative half-width vguess of zguess when the residual changes sign
+// there, else takes the reference's bracket.
 template <int PRIME, class E>
 ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
                            C2POut &out, double zguess = -1.0, double vguess = -1.0) {
@@ -710,11 +746,10 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
   seed.eps = in.saved_eps;
   seed.Ye = in.saved_Ye;
   {
-    // both calls take the saved eps by reference (a table may clamp it for
-    // the second); the seed keeps the value as loaded
-    double e = in.saved_eps;
-    seed.press = eos.press_re(in.saved_rho, e, in.saved_Ye);
-    seed.entropy = eos.kappa_re(in.saved_rho, e, in.saved_Ye);
+    // p and the entropy of the saved state; the EOS takes eps by reference
+    // (a table may clamp it), the seed keeps the value as loaded
+    double e = in.saved_eps, temp;
+    eos.state_re(in.saved_rho, e, in.saved_Ye, seed.press, temp, seed.entropy);
   }
   seed.temperature = in.temperature;
   seed.vel[0] = v_up[0]; seed.vel[1] = v_up[1]; seed.vel[2] = v_up[2];
@@ -745,10 +780,10 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
     const int prime = PRIME < 0 ? a.prime : PRIME;
     if constexpr (std::is_same<E, IdealGas>::value) {
       if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
-      else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1);
+      else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess);
     } else {
       (void)prime; // Noble needs the ideal-fluid Gamma; refused for tables at the C ABI
-      palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1);
+      palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess);
     }
     out.iters = rep1.iters;
     if (PRIME >= 0 && rep1.failed()) return false;

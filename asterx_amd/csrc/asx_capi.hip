@@ -254,7 +254,6 @@ TabEOS tab_eos(const asx_params *p) {
   e.ye_max = t->x_hi[2];
   e.eps_min = t->eps_min;
   e.eps_max = t->eps_max;
-  e.tol = 1.e-14;
   return e;
 }
 // An ideal gas or a table (entry points with composition)
@@ -503,6 +502,9 @@ Scratch *scratch_for(const asx_patch *pt, void *stream) {
 }
 
 } // namespace
+
+static_assert(ASX_MLP_NPARAM == kMlpNParam && ASX_MLP_NBLOB == kMlpNBlob, "MLP blob");
+static_assert(ASX_MLP_YE_NPARAM == kMlpYeNParam && ASX_MLP_YE_NBLOB == kMlpYeNBlob, "MLP blob");
 
 extern "C" {
 
@@ -1109,7 +1111,7 @@ asx_status asx_mlp_features(int64_t nz, const double *cons, const double *g, flo
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !x)) return ASX_ERR_NULL;
   return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] {
-    return launch_mlp_features(nz, cons, g, x, (hipStream_t)stream);
+    return launch_mlp_features<kMlpIdeal>(nz, cons, g, x, (hipStream_t)stream);
   }));
 }
 
@@ -1118,7 +1120,7 @@ asx_status asx_mlp_forward(int64_t nz, const double *cons, const double *g, cons
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !blob || !out)) return ASX_ERR_NULL;
   return dev_status(timed(ASX_KC_MLP, (hipStream_t)stream, [&] {
-    return launch_mlp_forward(nz, cons, g, blob, out, (hipStream_t)stream);
+    return launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, out, (hipStream_t)stream);
   }));
 }
 
@@ -1132,11 +1134,11 @@ asx_status asx_mlp_train_step(int64_t nz, const double *cons, const double *g,
   hipStream_t s = (hipStream_t)stream;
   const int nw = (int)((nz + 63) / 64);
   asx_status st = dev_status(timed(ASX_KC_MLP, s, [&] {
-    return launch_mlp_grad(nz, cons, g, target, blob, partial, s);
+    return launch_mlp_grad<kMlpIdeal>(nz, cons, g, target, blob, partial, s);
   }));
   if (st) return st;
   return dev_status(timed(ASX_KC_MLP, s, [&] {
-    return launch_mlp_adam(nw, partial, blob, m, v, lr, 0.9f, 0.999f, 1e-8f,
+    return launch_mlp_adam<kMlpIdeal>(nw, partial, blob, m, v, lr, 0.9f, 0.999f, 1e-8f,
                            step < 1 ? 1 : step, loss, grad, s);
   }));
 }
@@ -1257,6 +1259,86 @@ asx_status asx_c2p_batch_ye(int64_t nz, const double *cons, const double *g,
                                Q->n, s)
                : launch_c2p_ye(nz, cons, g, seed, prim, flag, iters, ideal_gas(par), ca, Q->q,
                                Q->n, s);
+  }));
+}
+
+asx_status asx_zone_sample_ye(int64_t nz, uint64_t seed, const double *box, double seed_rel,
+                              double *prim, double *g, double *seed_out, double *target,
+                              const asx_params *par, void *stream) {
+  asx_status st = check_eos(par);
+  if (st) return st;
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (!box || (nz > 0 && (!prim || !g))) return ASX_ERR_NULL;
+  Box7 b;
+  for (int i = 0; i < 7; ++i) b.v[i] = box[i];
+  if (!(b.v[0] > 0.0 && b.v[1] >= b.v[0] && b.v[2] > 0.0 && b.v[3] >= b.v[2] &&
+        b.v[5] >= b.v[4] && b.v[6] >= 1.0 && std::isfinite(b.v[1]) && std::isfinite(b.v[3]) &&
+        std::isfinite(b.v[6]) && seed_rel >= 0.0))
+    return ASX_ERR_PARAM;
+  hipStream_t s = (hipStream_t)stream;
+  return dev_status(timed(ASX_KC_OTHER, s, [&] {
+    return tabulated(par) ? launch_zone_sample_ye(nz, seed, b, seed_rel, prim, g, seed_out,
+                                                  target, tab_eos(par), s)
+                          : launch_zone_sample_ye(nz, seed, b, seed_rel, prim, g, seed_out,
+                                                  target, ideal_gas(par), s);
+  }));
+}
+
+asx_status asx_mlp_ye_features(int64_t nz, const double *cons, const double *g, float *x,
+                               void *stream) {
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !x)) return ASX_ERR_NULL;
+  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] {
+    return launch_mlp_features<kMlpTab>(nz, cons, g, x, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_mlp_ye_forward(int64_t nz, const double *cons, const double *g,
+                              const float *blob, double *out, void *stream) {
+  if (nz < 0) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !blob || !out)) return ASX_ERR_NULL;
+  return dev_status(timed(ASX_KC_MLP, (hipStream_t)stream, [&] {
+    return launch_mlp_forward<kMlpTab>(nz, cons, g, blob, out, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_mlp_ye_train_step(int64_t nz, const double *cons, const double *g,
+                                 const double *target, float *blob, float *m, float *v,
+                                 float lr, int64_t step, float *partial, float *loss,
+                                 float *grad, void *stream) {
+  if (nz < 1 || nz > (int64_t)1 << 30) return ASX_ERR_SIZE;
+  if (!cons || !g || !target || !blob || !partial) return ASX_ERR_NULL;
+  if (lr != 0.0f && (!m || !v || step < 1)) return ASX_ERR_PARAM;
+  hipStream_t s = (hipStream_t)stream;
+  const int nw = (int)((nz + 63) / 64);
+  asx_status st = dev_status(timed(ASX_KC_MLP, s, [&] {
+    return launch_mlp_grad<kMlpTab>(nz, cons, g, target, blob, partial, s);
+  }));
+  if (st) return st;
+  return dev_status(timed(ASX_KC_MLP, s, [&] {
+    return launch_mlp_adam<kMlpTab>(nw, partial, blob, m, v, lr, 0.9f, 0.999f, 1e-8f,
+                                    step < 1 ? 1 : step, loss, grad, s);
+  }));
+}
+
+asx_status asx_c2p_batch_ye_mlp(int64_t nz, const double *cons, const double *g,
+                                const double *seed, const float *blob, double bracket,
+                                double *prim, double *flag, int32_t *iters,
+                                const asx_params *par, void *stream) {
+  asx_status st = check_c2p_eos(par);
+  if (st) return st;
+  if (!tabulated(par)) return ASX_ERR_UNSUPPORTED; // the ideal gas: asx_c2p_batch_mlp
+  if (!(bracket > 0.0 && bracket < 1.0)) return ASX_ERR_PARAM;
+  if (nz < 0 || nz >= (int64_t(1) << 31)) return ASX_ERR_SIZE;
+  if (nz > 0 && (!cons || !g || !seed || !blob || !prim || !flag || !iters)) return ASX_ERR_NULL;
+  if (nz == 0) return ASX_OK;
+  double *guess = (double *)dev_scratch(SCR_MLP_GUESS, stream, (size_t)nz * 2 * sizeof(double));
+  C2PQueue *Q = c2p_queue(stream, nz);
+  if (!guess || !Q) return ASX_ERR_DEVICE;
+  hipStream_t s = (hipStream_t)stream;
+  return dev_status(timed(ASX_KC_C2P, s, [&] {
+    return launch_c2p_ye_mlp(nz, cons, g, seed, blob, bracket, prim, flag, iters, tab_eos(par),
+                             c2p_args(par), guess, Q->q, Q->n, s);
   }));
 }
 

@@ -388,6 +388,13 @@ struct IdealGas {
   ASX_DEV double temp_re(double, double &eps, double) const { return temp_re(eps); }
   ASX_DEV double kappa_re(double rho, double &eps, double) const { return kappa(rho, eps); }
   ASX_DEV double csnd_re(double, double &eps, double) const { return csnd(eps); }
+  // press_re, temp_re and kappa_re at one (rho, eps, Ye)
+  ASX_DEV void state_re(double rho, double &eps, double ye, double &press, double &temp,
+                        double &kap) const {
+    press = press_re(rho, eps, ye);
+    temp = temp_re(rho, eps, ye);
+    kap = kappa_re(rho, eps, ye);
+  }
   ASX_DEV double eps_rt(double, double temp, double) const { return eps_rt(temp); }
   ASX_DEV double press_rt(double rho, double temp, double) const { return press_rt(rho, temp); }
   ASX_DEV double eps_rp(double rho, double press, double) const { return eps_rp(rho, press); }

@@ -3,6 +3,7 @@
 // passes (primary solver with a wavefront-ballot queue of the zones it
 // failed, then the reference's full chain on the queue).
 #include "asx_kernels.hpp"
+#include "asx_rand.hpp"
 
 namespace asx {
 
@@ -99,19 +100,24 @@ ASX_DEV void zone_out_ye(int64_t z, int64_t nz, const C2POut &o, double *prim, d
   iters[z] = spent + o.iters;
 }
 
+// guess (optional, Palenzuela primary): guess[2][nz] = log W, log x from the
+// network; the primary brackets x = h W within the relative half-width xrel
+// of exp(guess[1]) where the residual changes sign there
 template <int PRIME, class E>
-__global__ __launch_bounds__(256) void k_c2p_ye_main(int64_t nz, const double *cons,
-                                                     const double *g, const double *seed,
-                                                     double *prim, double *flag,
-                                                     int32_t *iters, E eos, C2PArgs a,
-                                                     int *queue, unsigned *qcount) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(c2p_waves<E>())))
+void k_c2p_ye_main(int64_t nz, const double *cons, const double *g, const double *seed,
+                   double *prim, double *flag, int32_t *iters, E eos, C2PArgs a, int *queue,
+                   unsigned *qcount, const double *guess, double xrel) {
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool fail = false;
   if (z < nz) {
     C2PIn in;
     zone_in_ye(z, nz, cons, g, seed, in);
+    double xg = -1.0;
+    if constexpr (PRIME == ASX_C2P_PALENZUELA)
+      if (guess) xg = exp(guess[nz + z]);
     C2POut o;
-    if (con2prim_zone<PRIME>(eos, a, in, o)) zone_out_ye(z, nz, o, prim, flag, iters, 0);
+    if (con2prim_zone<PRIME>(eos, a, in, o, xg, xrel)) zone_out_ye(z, nz, o, prim, flag, iters, 0);
     else fail = true;
   }
   queue_push(fail, (int)z, queue, qcount);
@@ -180,10 +186,10 @@ template <int PRIME, class E>
 static hipError_t c2p_ye_t(int64_t nz, const double *cons, const double *g,
                            const double *seed, double *prim, double *flag, int32_t *iters,
                            const E &eos, const C2PArgs &a, int *queue, unsigned *qcount,
-                           hipStream_t s) {
+                           hipStream_t s, const double *guess = nullptr, double xrel = 0.0) {
   hipLaunchKernelGGL(k_zero_qcount, dim3(1), dim3(1), 0, s, qcount);
   hipLaunchKernelGGL((k_c2p_ye_main<PRIME, E>), g256(nz), dim3(256), 0, s, nz, cons, g, seed,
-                     prim, flag, iters, eos, a, queue, qcount);
+                     prim, flag, iters, eos, a, queue, qcount, guess, xrel);
   const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
   hipLaunchKernelGGL(k_c2p_ye_rest<E>, dim3(g2), dim3(256), 0, s, nz, cons, g, seed, prim,
                      flag, iters, eos, a, queue, qcount);
@@ -204,6 +210,100 @@ hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const 
   if (nz <= 0) return hipSuccess;
   return c2p_ye_t<ASX_C2P_PALENZUELA>(nz, cons, g, seed, prim, flag, iters, eos, a, queue,
                                       qcount, s);
+}
+
+// con2prim on a table from the table network's guess of x = h W: the
+// forward pass writes guess[2][nz], then the two passes above with the
+// guided Palenzuela primary
+hipError_t launch_c2p_ye_mlp(int64_t nz, const double *cons, const double *g,
+                             const double *seed, const float *blob, double xrel, double *prim,
+                             double *flag, int32_t *iters, const TabEOS &eos, const C2PArgs &a,
+                             double *guess, int *queue, unsigned *qcount, hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipError_t e = launch_mlp_forward<kMlpTab>(nz, cons, g, blob, guess, s);
+  if (e != hipSuccess) return e;
+  return c2p_ye_t<ASX_C2P_PALENZUELA>(nz, cons, g, seed, prim, flag, iters, eos, a, queue,
+                                      qcount, s, guess, xrel);
+}
+
+// Zones with composition for a 3-parameter EOS, drawn on the device
+// (draws as asx_zone_sample): rho log-uniform in [box[0], box[1]], T
+// log-uniform in [box[2], box[3]], Ye uniform in [box[4], box[5]], W
+// uniform in [1, box[6]] along a random direction, b^2/rho log-uniform in
+// [1e-4, 1e2] along a random direction, a symmetric 3-metric within 10% of
+// the identity.  prim[9] = rho, T, Ye, v^i, B^i (asx_prim2con_batch_ye);
+// sd[7] (optional) = rho, eps, v^i, Ye, T with rho, eps, v^i and T times
+// (1 + s u); target[2] (optional) = log W, log(h W) with h from the EOS.
+template <class E>
+__global__ __launch_bounds__(256) void k_zone_sample_ye(int64_t nz, uint64_t seed, Box7 box,
+                                                        double seed_rel, double *prim,
+                                                        double *g, double *sd, double *target,
+                                                        E eos) {
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (z >= nz) return;
+  const double rho = loguni(seed, z, 0, box.v[0], box.v[1]);
+  const double temp = loguni(seed, z, 1, box.v[2], box.v[3]);
+  const double ye = box.v[4] + (box.v[5] - box.v[4]) * urand(seed, z, 2);
+  const double W = 1.0 + (box.v[6] - 1.0) * urand(seed, z, 3);
+  double M[6];
+  for (int q = 0; q < 6; ++q) M[q] = (2.0 * urand(seed, z, 4 + q) - 1.0) * (0.1 / 3.0);
+  const Sym3 gl{1.0 + M[0], M[1], M[2], 1.0 + M[3], M[4], 1.0 + M[5]};
+  double nv[3], nb[3];
+  normal3(seed, z, 10, nv);
+  unit_in(gl, nv);
+  normal3(seed, z, 14, nb);
+  unit_in(gl, nb);
+  const double vabs = sqrt(1.0 - 1.0 / (W * W));
+  const double vel[3] = {nv[0] * vabs, nv[1] * vabs, nv[2] * vabs};
+  const double b2 = rho * loguni(seed, z, 18, 1e-4, 1e2);
+  double vl[3];
+  lower(gl, vel, vl);
+  const double ndv = dot3(nb, vl);
+  const double beta = sqrt(b2 / (1.0 / (W * W) + ndv * ndv));
+  prim[z] = rho;
+  prim[nz + z] = temp;
+  prim[2 * nz + z] = ye;
+  for (int i = 0; i < 3; ++i) prim[(3 + i) * nz + z] = vel[i];
+  for (int i = 0; i < 3; ++i) prim[(6 + i) * nz + z] = nb[i] * beta;
+  g[z] = gl.xx;
+  g[nz + z] = gl.xy;
+  g[2 * nz + z] = gl.xz;
+  g[3 * nz + z] = gl.yy;
+  g[4 * nz + z] = gl.yz;
+  g[5 * nz + z] = gl.zz;
+  const double eps = eos.eps_rt(rho, temp, ye);
+  if (sd) {
+    sd[z] = rho * (1.0 + seed_rel * (2.0 * urand(seed, z, 19) - 1.0));
+    sd[nz + z] = eps * (1.0 + seed_rel * (2.0 * urand(seed, z, 20) - 1.0));
+    for (int i = 0; i < 3; ++i)
+      sd[(2 + i) * nz + z] = vel[i] * (1.0 + seed_rel * (2.0 * urand(seed, z, 21 + i) - 1.0));
+    sd[5 * nz + z] = ye;
+    sd[6 * nz + z] = temp * (1.0 + seed_rel * (2.0 * urand(seed, z, 24) - 1.0));
+  }
+  if (target) {
+    const double press = eos.press_rt(rho, temp, ye);
+    target[z] = log(W);
+    target[nz + z] = log((1.0 + eps + press / rho) * W);
+  }
+}
+template <class E>
+static hipError_t zone_sample_ye_t(int64_t nz, uint64_t seed, const Box7 &box, double seed_rel,
+                                   double *prim, double *g, double *sd, double *target,
+                                   const E &eos, hipStream_t s) {
+  if (nz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_zone_sample_ye<E>, g256(nz), dim3(256), 0, s, nz, seed, box, seed_rel,
+                     prim, g, sd, target, eos);
+  return hipGetLastError();
+}
+hipError_t launch_zone_sample_ye(int64_t nz, uint64_t seed, const Box7 &box, double seed_rel,
+                                 double *prim, double *g, double *sd, double *target,
+                                 const IdealGas &eos, hipStream_t s) {
+  return zone_sample_ye_t(nz, seed, box, seed_rel, prim, g, sd, target, eos, s);
+}
+hipError_t launch_zone_sample_ye(int64_t nz, uint64_t seed, const Box7 &box, double seed_rel,
+                                 double *prim, double *g, double *sd, double *target,
+                                 const TabEOS &eos, hipStream_t s) {
+  return zone_sample_ye_t(nz, seed, box, seed_rel, prim, g, sd, target, eos, s);
 }
 
 } // namespace asx

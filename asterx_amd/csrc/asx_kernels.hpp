@@ -179,14 +179,22 @@ hipError_t launch_con2prim_rest_batch(const KPatch *tab, const int *zbase, int n
 // Network: 5 inputs -> 64 tanh -> 64 tanh -> 2 outputs.  One FP32 blob:
 //   W1[64][5] b1[64] W2[64][64] b2[64] W3[2][64] b3[2]   (4674 parameters)
 //   in_mean[5] in_istd[5] out_mean[2] out_std[2]         (normalisation)
-constexpr int kMlpIn = 5, kMlpH = 64, kMlpOut = 2;
-constexpr int kMlpW1 = 0, kMlpB1 = kMlpW1 + kMlpH * kMlpIn, kMlpW2 = kMlpB1 + kMlpH,
-              kMlpB2 = kMlpW2 + kMlpH * kMlpH, kMlpW3 = kMlpB2 + kMlpH,
-              kMlpB3 = kMlpW3 + kMlpOut * kMlpH, kMlpNParam = kMlpB3 + kMlpOut;
-constexpr int kMlpInMean = kMlpNParam, kMlpInIstd = kMlpInMean + kMlpIn,
-              kMlpOutMean = kMlpInIstd + kMlpIn, kMlpOutStd = kMlpOutMean + kMlpOut,
-              kMlpNBlob = kMlpOutStd + kMlpOut;
-constexpr int kMlpPartial = kMlpNParam + 1; // per-wavefront gradient + loss
+// Network layout of NIN inputs -> 64 tanh -> 64 tanh -> 2: offsets into
+// the float32 blob (weights, biases, then the normalisation statistics).
+// NIN = 5 for the ideal gas, 7 for a table (the 5 scale-free inputs,
+// log D and Ye).
+template <int NIN> struct MlpLayout {
+  static constexpr int In = NIN, H = 64, Out = 2;
+  static constexpr int W1 = 0, B1 = W1 + H * In, W2 = B1 + H, B2 = W2 + H * H, W3 = B2 + H,
+                       B3 = W3 + Out * H, NParam = B3 + Out;
+  static constexpr int InMean = NParam, InIstd = InMean + In, OutMean = InIstd + In,
+                       OutStd = OutMean + Out, NBlob = OutStd + Out;
+  static constexpr int Partial = NParam + 1; // per-wavefront gradient + loss
+  static constexpr int NCons = NIN == 5 ? 8 : 9; // rows of the cons batch read
+};
+constexpr int kMlpIdeal = 5, kMlpTab = 7;
+constexpr int kMlpNParam = MlpLayout<kMlpIdeal>::NParam, kMlpNBlob = MlpLayout<kMlpIdeal>::NBlob;
+constexpr int kMlpYeNParam = MlpLayout<kMlpTab>::NParam, kMlpYeNBlob = MlpLayout<kMlpTab>::NBlob;
 
 hipError_t launch_zone_sample(int64_t nz, uint64_t seed, double gamma, double seed_rel,
                               double *prim, double *g, double *sd, double *target,
@@ -194,16 +202,38 @@ hipError_t launch_zone_sample(int64_t nz, uint64_t seed, double gamma, double se
 hipError_t launch_prim2con_batch(int64_t nz, const double *prim, const double *g,
                                  double *cons, double *cold, const IdealGas &eos, double Ye,
                                  hipStream_t s);
+// the network of NIN inputs (kMlpIdeal or kMlpTab) on cons of
+// MlpLayout<NIN>::NCons rows
+template <int NIN>
 hipError_t launch_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
                                hipStream_t s);
+template <int NIN>
 hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
                               const float *blob, double *out, hipStream_t s);
+template <int NIN>
 hipError_t launch_mlp_grad(int64_t nz, const double *cons, const double *g,
                            const double *target, const float *blob, float *partial,
                            hipStream_t s);
+template <int NIN>
 hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float *m,
                            float *v, float lr, float b1, float b2, float eps, int64_t step,
                            float *loss, float *grad, hipStream_t s);
+// the table network's guided con2prim on a table; guess: scratch of 2 nz
+// doubles
+hipError_t launch_c2p_ye_mlp(int64_t nz, const double *cons, const double *g,
+                             const double *seed, const float *blob, double xrel, double *prim,
+                             double *flag, int32_t *iters, const TabEOS &eos, const C2PArgs &a,
+                             double *guess, int *queue, unsigned *qcount, hipStream_t s);
+// zones with composition: box = rho lo, hi, T lo, hi, Ye lo, hi, W max
+struct Box7 {
+  double v[7];
+};
+hipError_t launch_zone_sample_ye(int64_t nz, uint64_t seed, const Box7 &box, double seed_rel,
+                                 double *prim, double *g, double *sd, double *target,
+                                 const IdealGas &eos, hipStream_t s);
+hipError_t launch_zone_sample_ye(int64_t nz, uint64_t seed, const Box7 &box, double seed_rel,
+                                 double *prim, double *g, double *sd, double *target,
+                                 const TabEOS &eos, hipStream_t s);
 // guess: scratch of 2 nz doubles; queue: nz ints; qcount: one unsigned
 hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,

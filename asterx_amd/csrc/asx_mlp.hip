@@ -10,45 +10,9 @@
 // operands and fp32 accumulation.  Weights are kept in fp32 (the blob) and
 // rounded to bf16 where they enter a product.
 #include "asx_kernels.hpp"
+#include "asx_rand.hpp"
 
 namespace asx {
-
-// ---------------------------------------------------------------------------
-// Counter-based random numbers: splitmix64 (Steele, Lea & Flood, OOPSLA 2014)
-// of (seed, zone, draw), so a zone's draws do not depend on the launch shape.
-// ---------------------------------------------------------------------------
-ASX_DEV uint64_t mix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
-}
-// uniform in (0, 1)
-ASX_DEV double urand(uint64_t seed, int64_t z, int draw) {
-  const uint64_t h = mix64(seed ^ mix64((uint64_t)z * 64u + (uint64_t)draw));
-  return ((double)(h >> 11) + 0.5) * 0x1.0p-53;
-}
-ASX_DEV double loguni(uint64_t seed, int64_t z, int d, double a, double b) {
-  return exp(log(a) + (log(b) - log(a)) * urand(seed, z, d));
-}
-// three standard normals (Box-Muller), draws d .. d+3
-ASX_DEV void normal3(uint64_t seed, int64_t z, int d, double n[3]) {
-  const double u1 = urand(seed, z, d), u2 = urand(seed, z, d + 1);
-  const double u3 = urand(seed, z, d + 2), u4 = urand(seed, z, d + 3);
-  const double r1 = sqrt(-2.0 * log(u1)), r2 = sqrt(-2.0 * log(u3));
-  const double tp = 6.283185307179586;
-  n[0] = r1 * cos(tp * u2);
-  n[1] = r1 * sin(tp * u2);
-  n[2] = r2 * cos(tp * u4);
-}
-ASX_DEV void unit_in(const Sym3 &g, double v[3]) {
-  double l[3];
-  lower(g, v, l);
-  const double n = sqrt(dot3(v, l));
-  v[0] /= n;
-  v[1] /= n;
-  v[2] /= n;
-}
 
 // One zone of the learned-guess workload: rho log-uniform in [1e-10, 1e-2],
 // p/rho log-uniform in [1e-6, 1e2], W uniform in [1, 10] along a random
@@ -165,6 +129,13 @@ ASX_DEV void mlp_features(const double c[8], const Sym3 &gl, float x[5]) {
   const double den = sqrt(S2 * B2);
   x[4] = den > 0.0 ? (float)(SB / den) : 0.0f;
 }
+// The two further inputs on a table, which sets a density scale of its
+// own: log D (undensitized, floored at 1e-37) and Ye = DYe / D (c[8]).
+ASX_DEV void mlp_features_tab(const double c[9], const Sym3 &gl, float x[2]) {
+  const double D = c[0] / sqrt(det3(gl));
+  x[0] = __logf((float)(D > 1e-37 ? D : 1e-37));
+  x[1] = (float)(c[8] / c[0]);
+}
 
 // tanh from the hardware exp2 and reciprocal, 1 - 2 / (2^(2 x / ln 2) + 1):
 // within about 1e-7 of tanh, saturating to +-1 (the network's activation in
@@ -199,25 +170,35 @@ ASX_DEV bf16x4 frag(const uint16_t *t, int row, int col) {
   return *reinterpret_cast<const bf16x4 *>(t + row * kTS + col);
 }
 
+// Raw inputs of the network of NIN inputs from cons (MlpLayout<NIN>::NCons
+// values) of one zone.
+template <int NIN> ASX_DEV void net_features(const double *c, const Sym3 &gl, float *x) {
+  mlp_features(c, gl, x);
+  if constexpr (NIN == kMlpTab) mlp_features_tab(c, gl, x + 5);
+}
+
 // Normalised inputs of zone z (zeros for lanes past the batch).
+template <int NIN>
 ASX_DEV void zone_inputs(bool live, int64_t z, int64_t nz, const double *cons,
-                         const double *g, const float *blob, float xn[5]) {
-  float x[5] = {0, 0, 0, 0, 0};
+                         const double *g, const float *blob, float xn[NIN]) {
+  using L = MlpLayout<NIN>;
+  float x[NIN] = {};
   if (live) {
-    double c[8];
-    for (int q = 0; q < 8; ++q) c[q] = cons[q * nz + z];
+    double c[L::NCons];
+    for (int q = 0; q < L::NCons; ++q) c[q] = cons[q * nz + z];
     const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
-    mlp_features(c, gl, x);
+    net_features<NIN>(c, gl, x);
   }
-  for (int i = 0; i < 5; ++i)
-    xn[i] = live ? (x[i] - blob[kMlpInMean + i]) * blob[kMlpInIstd + i] : 0.0f;
+  for (int i = 0; i < NIN; ++i)
+    xn[i] = live ? (x[i] - blob[L::InMean + i]) * blob[L::InIstd + i] : 0.0f;
 }
 
 // Stage W2 (bf16, [j][k]) of the blob into LDS; every work-item of the block.
-ASX_DEV void stage_w2(const float *blob, uint16_t *w2s, uint16_t *w2t) {
-  for (int q = threadIdx.x; q < kMlpH * kMlpH; q += blockDim.x) {
-    const int j = q / kMlpH, k = q % kMlpH;
-    const uint16_t b = f2bf(blob[kMlpW2 + q]);
+template <int NIN> ASX_DEV void stage_w2(const float *blob, uint16_t *w2s, uint16_t *w2t) {
+  using L = MlpLayout<NIN>;
+  for (int q = threadIdx.x; q < L::H * L::H; q += blockDim.x) {
+    const int j = q / L::H, k = q % L::H;
+    const uint16_t b = f2bf(blob[L::W2 + q]);
     w2s[j * kTS + k] = b;
     if (w2t) w2t[k * kTS + j] = b;
   }
@@ -225,10 +206,12 @@ ASX_DEV void stage_w2(const float *blob, uint16_t *w2s, uint16_t *w2t) {
 
 // Layer 1 for this lane's zone: h1 = tanh(W1 x + b1), written as bf16 to
 // the zone's row of h1 (and column of h1t when given).
-ASX_DEV void layer1(const float *blob, const float xb[5], uint16_t *h1, uint16_t *h1t, int l) {
-  for (int j = 0; j < kMlpH; ++j) {
-    float zj = blob[kMlpB1 + j];
-    for (int i = 0; i < kMlpIn; ++i) zj += bfr(blob[kMlpW1 + j * kMlpIn + i]) * xb[i];
+template <int NIN>
+ASX_DEV void layer1(const float *blob, const float xb[NIN], uint16_t *h1, uint16_t *h1t, int l) {
+  using L = MlpLayout<NIN>;
+  for (int j = 0; j < L::H; ++j) {
+    float zj = blob[L::B1 + j];
+    for (int i = 0; i < L::In; ++i) zj += bfr(blob[L::W1 + j * L::In + i]) * xb[i];
     const uint16_t hb = f2bf(fast_tanh(zj));
     h1[l * kTS + j] = hb;
     if (h1t) h1t[j * kTS + l] = hb;
@@ -238,6 +221,7 @@ ASX_DEV void layer1(const float *blob, const float xb[5], uint16_t *h1, uint16_t
 // Layer 2 products Z2[z][j] = sum_k H1[z][k] W2[j][k] for the wavefront's
 // 64 zones: 4 zone tiles x 4 output tiles x 4 k-steps of 16.
 ASX_DEV void layer2(const uint16_t *h1, const uint16_t *w2s, f32x4 acc[4][4], int l) {
+  constexpr int kH = 64;
   const int r16 = l & 15, k4 = 4 * (l >> 4);
   for (int rt = 0; rt < 4; ++rt)
     for (int ob = 0; ob < 4; ++ob) acc[rt][ob] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -258,12 +242,14 @@ ASX_DEV void layer2(const uint16_t *h1, const uint16_t *w2s, f32x4 acc[4][4], in
 // Forward pass of one wavefront (64 zones, lane = zone).  act is the
 // wavefront's 64x64 scratch tile.  Every work-item of the block calls it
 // (it synchronises the block).
-ASX_DEV void mlp_wave(const float *blob, const uint16_t *w2s, uint16_t *act, const float xn[5],
+template <int NIN>
+ASX_DEV void mlp_wave(const float *blob, const uint16_t *w2s, uint16_t *act, const float xn[NIN],
                       float y[2]) {
+  using L = MlpLayout<NIN>;
   const int l = threadIdx.x & 63;
-  float xb[5];
-  for (int i = 0; i < kMlpIn; ++i) xb[i] = bfr(xn[i]);
-  layer1(blob, xb, act, nullptr, l);
+  float xb[NIN];
+  for (int i = 0; i < L::In; ++i) xb[i] = bfr(xn[i]);
+  layer1<NIN>(blob, xb, act, nullptr, l);
   __syncthreads();
   f32x4 acc[4][4];
   layer2(act, w2s, acc, l);
@@ -272,45 +258,49 @@ ASX_DEV void mlp_wave(const float *blob, const uint16_t *w2s, uint16_t *act, con
     for (int ob = 0; ob < 4; ++ob)
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + 4 * (l >> 4) + r, o = ob * 16 + (l & 15);
-        act[zr * kTS + o] = f2bf(fast_tanh(acc[rt][ob][r] + blob[kMlpB2 + o]));
+        act[zr * kTS + o] = f2bf(fast_tanh(acc[rt][ob][r] + blob[L::B2 + o]));
       }
   __syncthreads();
-  for (int o = 0; o < kMlpOut; ++o) {
-    float s = blob[kMlpB3 + o];
-    for (int k = 0; k < kMlpH; ++k) s += bfr(blob[kMlpW3 + o * kMlpH + k]) * bf2f(act[l * kTS + k]);
+  for (int o = 0; o < L::Out; ++o) {
+    float s = blob[L::B3 + o];
+    for (int k = 0; k < L::H; ++k) s += bfr(blob[L::W3 + o * L::H + k]) * bf2f(act[l * kTS + k]);
     y[o] = s;
   }
 }
 
 // Network outputs in target units (log W, log(Z/D)) for every zone.
+template <int NIN>
 __global__ __launch_bounds__(256) void k_mlp_forward(int64_t nz, const double *cons,
                                                      const double *g, const float *blob,
                                                      double *out) {
-  __shared__ uint16_t w2s[kMlpH * kTS];
-  __shared__ uint16_t act[4][kMlpH * kTS];
-  stage_w2(blob, w2s, nullptr);
+  using L = MlpLayout<NIN>;
+  __shared__ uint16_t w2s[L::H * kTS];
+  __shared__ uint16_t act[4][L::H * kTS];
+  stage_w2<NIN>(blob, w2s, nullptr);
   __syncthreads();
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = z < nz;
-  float xn[5], y[2];
-  zone_inputs(live, z, nz, cons, g, blob, xn);
-  mlp_wave(blob, w2s, act[threadIdx.x >> 6], xn, y);
+  float xn[NIN], y[2];
+  zone_inputs<NIN>(live, z, nz, cons, g, blob, xn);
+  mlp_wave<NIN>(blob, w2s, act[threadIdx.x >> 6], xn, y);
   if (live) {
-    out[z] = (double)(y[0] * blob[kMlpOutStd] + blob[kMlpOutMean]);
-    out[nz + z] = (double)(y[1] * blob[kMlpOutStd + 1] + blob[kMlpOutMean + 1]);
+    out[z] = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
+    out[nz + z] = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
   }
 }
 
+template <int NIN>
 __global__ __launch_bounds__(256) void k_mlp_features(int64_t nz, const double *cons,
                                                       const double *g, float *xo) {
+  using L = MlpLayout<NIN>;
   const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= nz) return;
-  double c[8];
-  float x[5];
-  for (int q = 0; q < 8; ++q) c[q] = cons[q * nz + z];
+  double c[L::NCons];
+  float x[NIN];
+  for (int q = 0; q < L::NCons; ++q) c[q] = cons[q * nz + z];
   const Sym3 gl{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
-  mlp_features(c, gl, x);
-  for (int i = 0; i < 5; ++i) xo[i * nz + z] = x[i];
+  net_features<NIN>(c, gl, x);
+  for (int i = 0; i < NIN; ++i) xo[i * nz + z] = x[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -321,34 +311,36 @@ __global__ __launch_bounds__(256) void k_mlp_features(int64_t nz, const double *
 // run on the matrix cores use the bf16-rounded operands, as in the forward
 // pass.
 // ---------------------------------------------------------------------------
+template <int NIN>
 __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons, const double *g,
                                                  const double *target, const float *blob,
                                                  float *partial) {
-  __shared__ uint16_t w2s[kMlpH * kTS], w2t[kMlpH * kTS];
-  __shared__ uint16_t h1[kMlpH * kTS], h1t[kMlpH * kTS];
-  __shared__ uint16_t a2[kMlpH * kTS], dz2t[kMlpH * kTS];
-  __shared__ float xs[64][kMlpIn];
-  __shared__ float dys[64][kMlpOut];
-  __shared__ float red[4][kMlpH][kMlpIn + 1];
+  using L = MlpLayout<NIN>;
+  __shared__ uint16_t w2s[L::H * kTS], w2t[L::H * kTS];
+  __shared__ uint16_t h1[L::H * kTS], h1t[L::H * kTS];
+  __shared__ uint16_t a2[L::H * kTS], dz2t[L::H * kTS];
+  __shared__ float xs[64][L::In];
+  __shared__ float dys[64][L::Out];
+  __shared__ float red[4][L::H][L::In + 1];
   __shared__ float lsum[64];
   const int l = threadIdx.x;
   const int r16 = l & 15, grp = l >> 4, k4 = 4 * grp;
-  stage_w2(blob, w2s, w2t);
+  stage_w2<NIN>(blob, w2s, w2t);
   const int64_t z = (int64_t)blockIdx.x * 64 + l;
   const bool live = z < nz;
-  float xn[5];
-  zone_inputs(live, z, nz, cons, g, blob, xn);
-  float xb[5];
-  for (int i = 0; i < kMlpIn; ++i) {
+  float xn[NIN];
+  zone_inputs<NIN>(live, z, nz, cons, g, blob, xn);
+  float xb[NIN];
+  for (int i = 0; i < L::In; ++i) {
     xb[i] = bfr(xn[i]);
     xs[l][i] = xb[i];
   }
   float tn[2] = {0.f, 0.f};
   if (live)
-    for (int o = 0; o < kMlpOut; ++o)
-      tn[o] = ((float)target[o * nz + z] - blob[kMlpOutMean + o]) / blob[kMlpOutStd + o];
+    for (int o = 0; o < L::Out; ++o)
+      tn[o] = ((float)target[o * nz + z] - blob[L::OutMean + o]) / blob[L::OutStd + o];
   __syncthreads();
-  layer1(blob, xb, h1, h1t, l);
+  layer1<NIN>(blob, xb, h1, h1t, l);
   __syncthreads();
   f32x4 acc[4][4];
   layer2(h1, w2s, acc, l);
@@ -357,7 +349,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
     for (int ob = 0; ob < 4; ++ob)
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + k4 + r, o = ob * 16 + r16;
-        const float h = fast_tanh(acc[rt][ob][r] + blob[kMlpB2 + o]);
+        const float h = fast_tanh(acc[rt][ob][r] + blob[L::B2 + o]);
         acc[rt][ob][r] = h;
         a2[zr * kTS + o] = f2bf(h);
       }
@@ -365,31 +357,31 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
   // layer 3 and the loss
   const float inv_n = 1.0f / (float)nz;
   float loss = 0.f;
-  for (int o = 0; o < kMlpOut; ++o) {
-    float s = blob[kMlpB3 + o];
-    for (int k = 0; k < kMlpH; ++k) s += bfr(blob[kMlpW3 + o * kMlpH + k]) * bf2f(a2[l * kTS + k]);
+  for (int o = 0; o < L::Out; ++o) {
+    float s = blob[L::B3 + o];
+    for (int k = 0; k < L::H; ++k) s += bfr(blob[L::W3 + o * L::H + k]) * bf2f(a2[l * kTS + k]);
     const float e = live ? s - tn[o] : 0.f;
     loss += e * e * inv_n;
     dys[l][o] = 2.f * e * inv_n;
   }
   lsum[l] = loss;
   __syncthreads();
-  float *out = partial + (int64_t)blockIdx.x * kMlpPartial;
+  float *out = partial + (int64_t)blockIdx.x * L::Partial;
   // dW3[o][k] (lane = k), db3, loss
-  for (int o = 0; o < kMlpOut; ++o) {
+  for (int o = 0; o < L::Out; ++o) {
     float s = 0.f;
     for (int q = 0; q < 64; ++q) s += dys[q][o] * bf2f(a2[q * kTS + l]);
-    out[kMlpW3 + o * kMlpH + l] = s;
+    out[L::W3 + o * L::H + l] = s;
   }
-  if (l < kMlpOut) {
+  if (l < L::Out) {
     float s = 0.f;
     for (int q = 0; q < 64; ++q) s += dys[q][l];
-    out[kMlpB3 + l] = s;
+    out[L::B3 + l] = s;
   }
   if (l == 0) {
     float s = 0.f;
     for (int q = 0; q < 64; ++q) s += lsum[q];
-    out[kMlpNParam] = s;
+    out[L::NParam] = s;
   }
   // dz2 = (W3^T dy) (1 - h2^2), in place of acc; db2 partials per lane group
   float db2p[4] = {0.f, 0.f, 0.f, 0.f};
@@ -398,7 +390,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + k4 + r, j = ob * 16 + r16;
         float dh = 0.f;
-        for (int o = 0; o < kMlpOut; ++o) dh += dys[zr][o] * bfr(blob[kMlpW3 + o * kMlpH + j]);
+        for (int o = 0; o < L::Out; ++o) dh += dys[zr][o] * bfr(blob[L::W3 + o * L::H + j]);
         const float h = acc[rt][ob][r];
         const float dz = dh * (1.f - h * h);
         acc[rt][ob][r] = dz;
@@ -415,7 +407,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
         dz2t[j * kTS + zr] = b;
       }
   __syncthreads();
-  out[kMlpB2 + l] = red[0][l][0] + red[1][l][0] + red[2][l][0] + red[3][l][0];
+  out[L::B2 + l] = red[0][l][0] + red[1][l][0] + red[2][l][0] + red[3][l][0];
   // dW2[j][k] = sum_z dz2[z][j] H1[z][k]: A = dz2^T (rows j), B = H1 (cols k)
   {
     f32x4 gw[4][4];
@@ -436,7 +428,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
     for (int jt = 0; jt < 4; ++jt)
       for (int kt = 0; kt < 4; ++kt)
         for (int r = 0; r < 4; ++r)
-          out[kMlpW2 + (jt * 16 + k4 + r) * kMlpH + kt * 16 + r16] = gw[jt][kt][r];
+          out[L::W2 + (jt * 16 + k4 + r) * L::H + kt * 16 + r16] = gw[jt][kt][r];
   }
   // dH1[z][k] = sum_j dz2[z][j] W2[j][k]: A = dz2 (rows z), B = W2 (cols k)
   f32x4 dh1[4][4];
@@ -459,39 +451,41 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
   __syncthreads();
   for (int kt = 0; kt < 4; ++kt) {
     const int k = kt * 16 + r16;
-    float sd = 0.f, sw[kMlpIn] = {0.f, 0.f, 0.f, 0.f, 0.f};
-    float wk[kMlpIn];
-    for (int i = 0; i < kMlpIn; ++i) wk[i] = bfr(blob[kMlpW1 + k * kMlpIn + i]);
+    float sd = 0.f, sw[L::In] = {};
+    float wk[L::In];
+    for (int i = 0; i < L::In; ++i) wk[i] = bfr(blob[L::W1 + k * L::In + i]);
     for (int rt = 0; rt < 4; ++rt)
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + k4 + r;
-        float zk = blob[kMlpB1 + k];
-        for (int i = 0; i < kMlpIn; ++i) zk += wk[i] * xs[zr][i];
+        float zk = blob[L::B1 + k];
+        for (int i = 0; i < L::In; ++i) zk += wk[i] * xs[zr][i];
         const float h = fast_tanh(zk);
         const float dz = dh1[rt][kt][r] * (1.f - h * h);
         sd += dz;
-        for (int i = 0; i < kMlpIn; ++i) sw[i] += dz * xs[zr][i];
+        for (int i = 0; i < L::In; ++i) sw[i] += dz * xs[zr][i];
       }
-    for (int i = 0; i < kMlpIn; ++i) red[grp][k][i] = sw[i];
-    red[grp][k][kMlpIn] = sd;
+    for (int i = 0; i < L::In; ++i) red[grp][k][i] = sw[i];
+    red[grp][k][L::In] = sd;
   }
   __syncthreads();
-  for (int i = 0; i < kMlpIn; ++i)
-    out[kMlpW1 + l * kMlpIn + i] = red[0][l][i] + red[1][l][i] + red[2][l][i] + red[3][l][i];
-  out[kMlpB1 + l] = red[0][l][kMlpIn] + red[1][l][kMlpIn] + red[2][l][kMlpIn] + red[3][l][kMlpIn];
+  for (int i = 0; i < L::In; ++i)
+    out[L::W1 + l * L::In + i] = red[0][l][i] + red[1][l][i] + red[2][l][i] + red[3][l][i];
+  out[L::B1 + l] = red[0][l][L::In] + red[1][l][L::In] + red[2][l][L::In] + red[3][l][L::In];
 }
 
 // Sum the per-workgroup partials in a fixed order (deterministic) and take
 // one Adam step (Kingma & Ba 2015) on the fp32 parameters.
+template <int NIN>
 __global__ __launch_bounds__(256) void k_mlp_adam(int nw, const float *partial, float *blob,
                                                   float *m, float *v, float lr, float b1,
                                                   float b2, float eps, float c1, float c2,
                                                   float *loss, float *grad) {
+  using L = MlpLayout<NIN>;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p > kMlpNParam) return;
+  if (p > L::NParam) return;
   float s = 0.f;
-  for (int w = 0; w < nw; ++w) s += partial[(int64_t)w * kMlpPartial + p];
-  if (p == kMlpNParam) {
+  for (int w = 0; w < nw; ++w) s += partial[(int64_t)w * L::Partial + p];
+  if (p == L::NParam) {
     if (loss) *loss = s;
     return;
   }
@@ -596,41 +590,58 @@ hipError_t launch_prim2con_batch(int64_t nz, const double *prim, const double *g
                      eos, Ye);
   return hipGetLastError();
 }
+template <int NIN>
 hipError_t launch_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
                                hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_features, grid256(nz), dim3(256), 0, s, nz, cons, g, x);
+  hipLaunchKernelGGL(k_mlp_features<NIN>, grid256(nz), dim3(256), 0, s, nz, cons, g, x);
   return hipGetLastError();
 }
+template <int NIN>
 hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
                               const float *blob, double *out, hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_forward, grid256(nz), dim3(256), 0, s, nz, cons, g, blob, out);
+  hipLaunchKernelGGL(k_mlp_forward<NIN>, grid256(nz), dim3(256), 0, s, nz, cons, g, blob, out);
   return hipGetLastError();
 }
+template <int NIN>
 hipError_t launch_mlp_grad(int64_t nz, const double *cons, const double *g,
                            const double *target, const float *blob, float *partial,
                            hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)((nz + 63) / 64)), dim3(64), 0, s, nz, cons, g,
-                     target, blob, partial);
+  hipLaunchKernelGGL(k_mlp_grad<NIN>, dim3((unsigned)((nz + 63) / 64)), dim3(64), 0, s, nz,
+                     cons, g, target, blob, partial);
   return hipGetLastError();
 }
+template <int NIN>
 hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float *m,
                            float *v, float lr, float b1, float b2, float eps, int64_t step,
                            float *loss, float *grad, hipStream_t s) {
   const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_mlp_adam, grid256(kMlpNParam + 1), dim3(256), 0, s, nwaves, partial, blob,
-                     m, v, lr, b1, b2, eps, c1, c2, loss, grad);
+  hipLaunchKernelGGL(k_mlp_adam<NIN>, grid256(MlpLayout<NIN>::NParam + 1), dim3(256), 0, s,
+                     nwaves, partial, blob, m, v, lr, b1, b2, eps, c1, c2, loss, grad);
   return hipGetLastError();
 }
+#define ASX_MLP_INSTANTIATE(N)                                                                \
+  template hipError_t launch_mlp_features<N>(int64_t, const double *, const double *, float *, \
+                                             hipStream_t);                                     \
+  template hipError_t launch_mlp_forward<N>(int64_t, const double *, const double *,          \
+                                            const float *, double *, hipStream_t);             \
+  template hipError_t launch_mlp_grad<N>(int64_t, const double *, const double *,             \
+                                         const double *, const float *, float *, hipStream_t); \
+  template hipError_t launch_mlp_adam<N>(int, const float *, float *, float *, float *, float, \
+                                         float, float, float, int64_t, float *, float *,       \
+                                         hipStream_t);
+ASX_MLP_INSTANTIATE(kMlpIdeal)
+ASX_MLP_INSTANTIATE(kMlpTab)
+#undef ASX_MLP_INSTANTIATE
 hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
                                 double *guess, int *queue, unsigned *qcount, hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipError_t e = launch_mlp_forward(nz, cons, g, blob, guess, s);
+  hipError_t e = launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, guess, s);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
   hipLaunchKernelGGL(k_c2p_guess_main, grid256(nz), dim3(256), 0, s, nz, cons, g, seed, guess,

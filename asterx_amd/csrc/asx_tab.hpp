@@ -3,76 +3,16 @@
 // Ye (code units; axes uniform), the variable index fastest, and the EOSX interface
 // on it (EOSX/src/eos_3p_tabulated3d/eos_3p_tabulated3d.hxx:186-357):
 // trilinear interpolation in the order of eos_linear_interp_ND.hxx (rho,
-// then T, then Ye), temperature from (rho, eps, Ye) by Brent's ZERO, and
-// the same clamping of the inputs, including the change of eps the
-// reference makes through its reference argument.
+// then T, then Ye) and the same clamping of the inputs, including the
+// change of eps the reference makes through its reference argument.  The
+// temperature from (rho, eps, Ye) is the root the reference's Brent
+// iteration (EOSX/src/utils/eos_brent.hxx, tolerance 1e-14 in log T)
+// approaches, solved exactly on the piecewise-linear interpolant.
 #pragma once
 
 #include "asx_device.hpp"
 
 namespace asx {
-
-// Brent's ZERO (Brent 1973, "Algorithms for Minimization without
-// Derivatives", ch. 4): the root of f in [a, b] (f(a) f(b) <= 0) to
-// 2 eps |x| + t, returning the last iterate -- the form of
-// EOSX/src/utils/eos_brent.hxx:8-166.  The loop is capped at kBrentCap
-// iterations: on a valid bracket ZERO needs far fewer, and the cap only ends
-// a NaN residual, which would otherwise step by t across the whole bracket.
-constexpr int kBrentCap = 1000;
-template <typename F>
-ASX_DEV double zero_brent(double a, double b, double t, const F &f) {
-  double sa = a, sb = b;
-  double fa = f(sa), fb = f(sb);
-  double c = sa, fc = fa, e = sb - sa, d = e;
-  const double macheps = 2.220446049250313e-16;
-  for (int it = 0; it < kBrentCap; ++it) {
-    if (fabs(fc) < fabs(fb)) {
-      sa = sb; sb = c; c = sa;
-      fa = fb; fb = fc; fc = fa;
-    }
-    const double tol = 2.0 * macheps * fabs(sb) + t;
-    const double m = 0.5 * (c - sb);
-    if (fabs(m) <= tol || fb == 0.0) break;
-    if (fabs(e) < tol || fabs(fa) <= fabs(fb)) {
-      e = m;
-      d = e;
-    } else {
-      double s = fb / fa, p, q;
-      if (sa == c) {
-        p = 2.0 * m * s;
-        q = 1.0 - s;
-      } else {
-        q = fa / fc;
-        const double r = fb / fc;
-        p = s * (2.0 * m * q * (q - r) - (sb - sa) * (r - 1.0));
-        q = (q - 1.0) * (r - 1.0) * (s - 1.0);
-      }
-      if (0.0 < p) q = -q;
-      else p = -p;
-      s = e;
-      e = d;
-      if (2.0 * p < 3.0 * m * q - fabs(tol * q) && p < fabs(0.5 * s * q)) {
-        d = p / q;
-      } else {
-        e = m;
-        d = e;
-      }
-    }
-    sa = sb;
-    fa = fb;
-    if (tol < fabs(d)) sb = sb + d;
-    else if (0.0 < m) sb = sb + tol;
-    else sb = sb - tol;
-    fb = f(sb);
-    if ((0.0 < fb && 0.0 < fc) || (fb <= 0.0 && fc <= 0.0)) {
-      c = sa;
-      fc = fa;
-      e = sb - sa;
-      d = e;
-    }
-  }
-  return sb;
-}
 
 struct TabEOS {
   enum { PRESS = 0, EPS = 1, CS2 = 2, ENT = 3, NV = 4 };
@@ -83,7 +23,6 @@ struct TabEOS {
   double shift;         // energy shift
   double rho_min, rho_max, temp_min, temp_max, ye_min, ye_max; // rgrho, rgtemp, rgye
   double eps_min, eps_max;                                     // rgeps
-  double tol;           // temperature inversion tolerance in log T
 
   // nearest lower node, uniform axis (eos_linear_interp_ND.hxx:57-73)
   ASX_DEV int find(int d, double x) const {
@@ -167,8 +106,43 @@ struct TabEOS {
       eps = exp(vmax) - shift;
       return xmax1();
     }
-    auto f = [&](double lt) { return leps - interp(EPS, lrho, lt, ye); };
-    return zero_brent(xmin1(), xmax1(), tol, f);
+    return logtemp_nodes(lrho, leps, ye, vmin, vmax);
+  }
+  // The root in log T of interp(EPS, lrho, lt, ye) = leps for
+  // vmin < leps < vmax (the interpolant's values at the ends of the T
+  // axis).  At fixed (lrho, ye) the interpolant is linear in lt inside each
+  // cell [t_j, t_j+1], between the node values E_j (the bilinear rho, Ye
+  // interpolation at T node j, in the reference's order), so the root is
+  // found exactly: a bisection over the node index keeps
+  // E_lo <= leps < E_hi -- a sign change, hence a root, in between -- down
+  // to adjacent nodes, then that cell's linear equation is solved.  Brent's
+  // iteration on the same function stops within 2 eps |lt| + 1e-14 of this
+  // root, after about twice as many dependent steps of twice the reads.
+  ASX_DEV double logtemp_nodes(double lrho, double leps, double ye, double vmin,
+                               double vmax) const {
+    const int i0 = find(0, lrho), i2 = find(2, ye);
+    const int s1 = NV * n[0], s2 = NV * n[0] * n[1];
+    const double *p = tab + EPS + NV * i0 + s2 * i2;
+    const double r0 = ax[0][i0], r1 = ax[0][i0 + 1];
+    const double y0 = ax[2][i2], y1 = ax[2][i2 + 1];
+    int lo = 0, hi = n[1] - 1;
+    double elo = vmin, ehi = vmax;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      const double *q = p + s1 * mid;
+      const double a0 = lin(q[0], q[NV], r0, r1, lrho);
+      const double a1 = lin(q[s2], q[s2 + NV], r0, r1, lrho);
+      const double em = lin(a0, a1, y0, y1, ye);
+      if (em <= leps) {
+        lo = mid;
+        elo = em;
+      } else {
+        hi = mid;
+        ehi = em;
+      }
+    }
+    const double t0 = ax[1][lo], t1 = ax[1][hi];
+    return t0 + (leps - elo) * (t1 - t0) / (ehi - elo);
   }
   // press_from_valid_rho_temp_ye: the bounded rho, T are not used there
   ASX_DEV double press_rt(double rho, double temp, double ye) const {
@@ -192,6 +166,18 @@ struct TabEOS {
   }
   ASX_DEV double temp_re(double rho, double &eps, double ye) const {
     return exp(logtemp_re(rho, eps, ye));
+  }
+  // press_re, temp_re and kappa_re at one (rho, eps, Ye) from one
+  // temperature solve.  Called in turn, the reference's functions solve
+  // three times, the second and third time at the eps the first clamped;
+  // where that clamp applied, the later solves may land an ulp away from
+  // the table edge, which is the only difference
+  ASX_DEV void state_re(double rho, double &eps, double ye, double &press, double &temp,
+                        double &kap) const {
+    const double lt = logtemp_re(rho, eps, ye);
+    press = exp(interp(PRESS, log(crho(rho)), lt, ye));
+    temp = exp(lt);
+    kap = entropy_rt(rho, temp, ye);
   }
   // entropy_from_valid_rho_temp_ye (:320-329)
   ASX_DEV double entropy_rt(double rho, double temp, double ye) const {

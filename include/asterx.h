@@ -503,6 +503,53 @@ asx_status asx_c2p_batch_ye(int64_t nz, const double *cons, const double *g,
                             const double *seed, double *prim, double *flag, int32_t *iters,
                             const asx_params *par, void *stream);
 
+/* The learned con2prim guess on a table (no counterpart in the reference:
+ * the network around the Palenzuela solver of
+ * Con2PrimFactory/src/c2p_1DPalenzuela.hxx:306-556).  A table sets a density
+ * scale, so the network takes the 5 inputs of asx_mlp_features, log D
+ * (undensitized) and Ye = DYe/D -> 64 tanh -> 64 tanh -> 2 outputs (log W,
+ * log x, x = h W: Palenzuela's unknown), on a float32 blob of
+ * ASX_MLP_YE_NBLOB values laid out as for ASX_MLP_NBLOB with W1[64][7] and
+ * 7 input statistics.  cons are the asx_c2p_batch_ye rows (9). */
+#define ASX_MLP_YE_NPARAM 4802
+#define ASX_MLP_YE_NBLOB 4820
+/* Zones with composition drawn on the device for any 3-parameter EOS:
+ * box[7] = rho lo, hi; T lo, hi (log-uniform); Ye lo, hi (uniform); W max
+ * (W uniform in [1, W max] along a random direction); b^2/rho log-uniform in
+ * [1e-4, 1e2] along a random direction; a symmetric 3-metric within 10% of
+ * the identity.  prim[9][nz] (asx_prim2con_batch_ye layout), g[6][nz];
+ * seed_out[7][nz] (optional, the asx_c2p_batch_ye seed) = rho, eps, v^i,
+ * Ye, T with rho, eps, v^i and T times (1 + seed_rel u), u ~ U[-1, 1];
+ * target[2][nz] (optional) = log W, log(h W) from the EOS.  Zone z depends
+ * only on (seed, z). */
+asx_status asx_zone_sample_ye(int64_t nz, uint64_t seed, const double *box, double seed_rel,
+                              double *prim, double *g, double *seed_out, double *target,
+                              const asx_params *par, void *stream);
+/* x[7][nz] (float32): the table network's raw inputs. */
+asx_status asx_mlp_ye_features(int64_t nz, const double *cons, const double *g, float *x,
+                               void *stream);
+/* out[2][nz]: the table network's outputs (log W, log x). */
+asx_status asx_mlp_ye_forward(int64_t nz, const double *cons, const double *g,
+                              const float *blob, double *out, void *stream);
+/* asx_mlp_train_step for the table network (ASX_MLP_YE_NPARAM parameters;
+ * partial: ceil(nz/64) * (ASX_MLP_YE_NPARAM + 1) floats). */
+asx_status asx_mlp_ye_train_step(int64_t nz, const double *cons, const double *g,
+                                 const double *target, float *blob, float *m, float *v,
+                                 float lr, int64_t step, float *partial, float *loss,
+                                 float *grad, void *stream);
+/* asx_c2p_batch_ye on a table with the table network's guess: the
+ * Palenzuela primary brackets x in [x_g (1 - bracket), x_g (1 + bracket)]
+ * (inside the widest bracket the reference may take) when the residual
+ * changes sign there, and takes the reference's bracket otherwise; the
+ * root, hence the result, is the same to the solver tolerance.
+ * 0 < bracket < 1.  The guesses and the queue live in library scratch per
+ * device and stream.  An ideal gas returns ASX_ERR_UNSUPPORTED (use
+ * asx_c2p_batch_mlp). */
+asx_status asx_c2p_batch_ye_mlp(int64_t nz, const double *cons, const double *g,
+                                const double *seed, const float *blob, double bracket,
+                                double *prim, double *flag, int32_t *iters,
+                                const asx_params *par, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched patches (replaces the driver loop that calls the scheduled        */
 /* functions once per local box and RK stage: AsterX/schedule.ccl:119-232)  */
