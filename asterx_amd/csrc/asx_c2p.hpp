@@ -1,4 +1,4 @@
-// Device con2prim for the ideal-gas EOS: the per-cell chain of
+// Device con2prim for a 3-parameter EOS (ideal gas or table): the per-cell chain of
 // AsterX/src/con2prim.cxx:81-453 with Con2PrimFactory's Noble 2D
 // (c2p_2DNoble.hxx:275-594) and Palenzuela 1D (c2p_1DPalenzuela.hxx:306-556)
 // solvers, atmosphere (atmo.hxx), floors (c2p.hxx:86-369).
@@ -25,7 +25,8 @@ enum C2PStatus {
   ST_SUCCESS = 0, ST_INVALID_DETG, ST_NEG_BSQR, ST_NANS_IN_CONS, ST_RANGE_RHO,
   ST_RANGE_EPS, ST_SPEED_LIMIT, ST_RANGE_YE, ST_B_LIMIT, ST_ROOT_FAIL_CONV,
   ST_ROOT_FAIL_BRACKET, ST_PREP_ROOT_FAIL_CONV, ST_PREP_ROOT_FAIL_BRACKET,
-  ST_NOT_SET
+  ST_NOT_SET,
+  ST_GUESS_MISS // a guided primary whose guessed bracket holds no root (internal)
 };
 struct Report {
   int status;
@@ -568,7 +569,8 @@ ASX_DEV void brent(const F &fn, double a, double b, double fa0, double fb0, doub
 template <class E>
 ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
                               Prim &pv, Cons &cv, const Sym3 &glo, Report &rep,
-                              double xguess = -1.0, double xrel = 0.0) {
+                              double xguess = -1.0, double xrel = 0.0,
+                              bool miss_fails = false) {
   rep.iters = 0;
   rep.adjust_cons = false;
   rep.set_atmo = false;
@@ -625,6 +627,12 @@ ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
         hinted = true;
       }
     }
+  }
+  if (xguess > 0.0 && !hinted && miss_fails) {
+    // the zone goes to the full chain (a wavefront then keeps to the narrow
+    // brackets instead of waiting on a lane that searches the wide one)
+    rep.status = ST_GUESS_MISS;
+    return;
   }
   if (!hinted) {
     flo = fn(xa);
@@ -714,7 +722,8 @@ struct C2POut {
 // zguess, vguess (optional, > 0) guide the primary solver: Noble starts
 // from Z = zguess, v^2 = vguess; Palenzuela brackets x = h W within the
 // relative half-width vguess of zguess when the residual changes sign
-// there, else takes the reference's bracket.
+// there; otherwise the primary-only form fails (the zone is queued for the
+// full chain) and the full chain takes the reference's bracket.
 template <int PRIME, class E>
 ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
                            C2POut &out, double zguess = -1.0, double vguess = -1.0) {
@@ -780,10 +789,10 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
     const int prime = PRIME < 0 ? a.prime : PRIME;
     if constexpr (std::is_same<E, IdealGas>::value) {
       if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
-      else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess);
+      else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess, PRIME >= 0);
     } else {
       (void)prime; // Noble needs the ideal-fluid Gamma; refused for tables at the C ABI
-      palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess);
+      palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess, PRIME >= 0);
     }
     out.iters = rep1.iters;
     if (PRIME >= 0 && rep1.failed()) return false;

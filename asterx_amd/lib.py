@@ -56,6 +56,14 @@ SIGNATURES = {
     "asx_eos_from_rho_eps": ([C.c_int64, _V, _V, _V, _V, _PP, _V], C.c_int),
     "asx_prim2con_batch_ye": ([C.c_int64, _V, _V, _V, _V, _PP, _V], C.c_int),
     "asx_c2p_batch_ye": ([C.c_int64, _V, _V, _V, _V, _V, _V, _PP, _V], C.c_int),
+    "asx_zone_sample_ye": ([C.c_int64, C.c_uint64, _V, C.c_double, _V, _V, _V, _V, _PP, _V],
+                           C.c_int),
+    "asx_mlp_ye_features": ([C.c_int64, _V, _V, _V, _V], C.c_int),
+    "asx_mlp_ye_forward": ([C.c_int64, _V, _V, _V, _V, _V], C.c_int),
+    "asx_mlp_ye_train_step": ([C.c_int64, _V, _V, _V, _V, _V, _V, C.c_float, C.c_int64, _V,
+                               _V, _V, _V], C.c_int),
+    "asx_c2p_batch_ye_mlp": ([C.c_int64, _V, _V, _V, _V, C.c_double, _V, _V, _V, _PP, _V],
+                             C.c_int),
     "asx_batch_create": ([C.c_int32, _P, _S, _PP, C.c_double, C.c_int32,
                           C.POINTER(C.c_void_p)], C.c_int),
     "asx_batch_stage": ([_V, C.c_int32, _V], C.c_int),

@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--tab-n", type=int, default=128)
     ap.add_argument("--tab-steps", type=int, default=12)
     ap.add_argument("--tab-c2p-zones", type=int, default=1 << 22)
+    ap.add_argument("--tab-bracket", type=float, default=0.03,
+                    help="relative half-width of the guided Palenzuela bracket")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -328,6 +330,41 @@ def table_params(table, **kw):
     return table.attach(abi.params(**base))
 
 
+def bench_guided_ye(args, L, sh, par, nz, cd, gd, seed, od, fd, plain_s):
+    """The same zones through asx_c2p_batch_ye_mlp: the table network's
+    guess of x = h W brackets the Palenzuela root (relative half-width
+    --tab-bracket); zones whose root lies outside go to the full chain.
+    Against the plain solve: throughput, Brent iterations, identical flags,
+    largest relative difference of rho and T."""
+    import numpy as np
+    import torch
+    from asterx_amd import abi, mlp_ye
+    if not os.path.exists(mlp_ye.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained table weights at %s" % mlp_ye.DEFAULT_WEIGHTS}
+    bt = torch.from_numpy(mlp_ye.load()).cuda()
+    og = torch.empty_like(od)
+    fg = torch.empty_like(fd)
+    ig = torch.empty(nz, dtype=torch.int32, device="cuda")
+    ptr = lambda t: C.c_void_p(t.data_ptr())
+    a = (nz, ptr(cd), ptr(gd), ptr(seed), ptr(bt), args.tab_bracket, ptr(og), ptr(fg), ptr(ig),
+         C.byref(par), sh)
+    abi.check("asx_c2p_batch_ye_mlp", L.asx_c2p_batch_ye_mlp(*a))
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        abi.check("asx_c2p_batch_ye_mlp", L.asx_c2p_batch_ye_mlp(*a))
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / reps
+    ok = (fd == 1) & (fg == 1)
+    d = lambda r: float(((og[r] - od[r]).abs() / od[r].abs())[ok].max())
+    return {"bracket": args.tab_bracket, "zones_per_s": round(nz / wall, 1),
+            "ms_per_batch": round(1e3 * wall, 4), "speedup": round(plain_s / wall, 3),
+            "iters_mean": round(float(ig[fg == 1].double().mean()), 3),
+            "flags_equal": bool(torch.equal(fg, fd)),
+            "max_rel_diff_rho": d(0), "max_rel_diff_T": d(7)}
+
+
 def bench_tabulated(args, L, sh, stream):
     """The tabulated-EOS workload: the 234x180x50 synthetic table; the
     torus geometry at `--tab-n`^3 on the table (SSPRK3 substeps, as the
@@ -426,6 +463,7 @@ def bench_tabulated(args, L, sh, stream):
                "ms_per_batch": round(1e3 * cwall, 4),
                "iters_mean": round(float(idd[ok].double().mean()), 3), "flags": flags,
                "max_rel_err_rho": rel_rho, "max_rel_err_T": rel_t}
+    c2p_out["guided"] = bench_guided_ye(args, L, sh, par, nz, cd, gd, seed, od, fd, cwall)
     log(args, "tabulated patch:", patch_out, "\ntabulated c2p batch:", c2p_out)
     # the CPU oracle on the first 2^15 zones of the same batch
     m = min(nz, 1 << 15)

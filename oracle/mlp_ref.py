@@ -12,19 +12,32 @@ finite differences.
 """
 import numpy as np
 
-NIN, NH, NOUT = 5, 64, 2
-W1 = 0
-B1 = W1 + NH * NIN
-W2 = B1 + NH
-B2 = W2 + NH * NH
-W3 = B2 + NH
-B3 = W3 + NOUT * NH
-NPARAM = B3 + NOUT
-IN_MEAN = NPARAM
-IN_ISTD = IN_MEAN + NIN
-OUT_MEAN = IN_ISTD + NIN
-OUT_STD = OUT_MEAN + NOUT
-NBLOB = OUT_STD + NOUT
+NH, NOUT = 64, 2
+
+
+def layout(nin):
+    """Offsets of the blob of the network with nin inputs (5: the ideal
+    gas; 7: a table, with log D and Ye added)."""
+    L = dict(NIN=nin, W1=0)
+    L["B1"] = L["W1"] + NH * nin
+    L["W2"] = L["B1"] + NH
+    L["B2"] = L["W2"] + NH * NH
+    L["W3"] = L["B2"] + NH
+    L["B3"] = L["W3"] + NOUT * NH
+    L["NPARAM"] = L["B3"] + NOUT
+    L["IN_MEAN"] = L["NPARAM"]
+    L["IN_ISTD"] = L["IN_MEAN"] + nin
+    L["OUT_MEAN"] = L["IN_ISTD"] + nin
+    L["OUT_STD"] = L["OUT_MEAN"] + NOUT
+    L["NBLOB"] = L["OUT_STD"] + NOUT
+    return L
+
+
+_L5 = layout(5)
+NIN = 5
+W1, B1, W2, B2, W3, B3 = (_L5[k] for k in ("W1", "B1", "W2", "B2", "W3", "B3"))
+NPARAM, IN_MEAN, IN_ISTD = _L5["NPARAM"], _L5["IN_MEAN"], _L5["IN_ISTD"]
+OUT_MEAN, OUT_STD, NBLOB = _L5["OUT_MEAN"], _L5["OUT_STD"], _L5["NBLOB"]
 
 
 def bf16(x):
@@ -68,25 +81,52 @@ def features(cons, g):
                      np.log(taup / E), np.log(np.maximum(B2 / E, 1e-30)), cosv], axis=1)
 
 
+def features_tab(cons, g):
+    """x[nz, 7]: the 5 inputs of features(), log D (undensitized, floored at
+    1e-37) and Ye = DYe / D (cons row 8)."""
+    gxx, gxy, gxz, gyy, gyz, gzz = g
+    det = 2 * gxy * gxz * gyz + gxx * gyy * gzz - gyy * gxz ** 2 - \
+        gxx * gyz ** 2 - gzz * gxy ** 2
+    D = cons[0] / np.sqrt(det)
+    extra = np.stack([np.log(np.maximum(D, 1e-37)), cons[8] / cons[0]], axis=1)
+    return np.concatenate([features(cons, g), extra], axis=1)
+
+
+def _nin(blob):
+    for nin in (5, 7):
+        if len(blob) == layout(nin)["NBLOB"]:
+            return nin
+    raise ValueError("not a network blob: %d values" % len(blob))
+
+
 def _params(blob, bf):
+    L = layout(_nin(blob))
     f = bf16 if bf else (lambda a: np.asarray(a, dtype=np.float64))
     dt = np.float32 if bf else np.float64
-    p = dict(W1=f(blob[W1:B1].reshape(NH, NIN)), b1=blob[B1:W2].astype(dt),
-             W2=f(blob[W2:B2].reshape(NH, NH)), b2=blob[B2:W3].astype(dt),
-             W3=f(blob[W3:B3].reshape(NOUT, NH)), b3=blob[B3:NPARAM].astype(dt))
+    p = dict(W1=f(blob[L["W1"]:L["B1"]].reshape(NH, L["NIN"])),
+             b1=blob[L["B1"]:L["W2"]].astype(dt),
+             W2=f(blob[L["W2"]:L["B2"]].reshape(NH, NH)), b2=blob[L["B2"]:L["W3"]].astype(dt),
+             W3=f(blob[L["W3"]:L["B3"]].reshape(NOUT, NH)),
+             b3=blob[L["B3"]:L["NPARAM"]].astype(dt))
     return p
 
 
 def normalized_inputs(blob, cons, g, bf=True):
-    x = features(cons, g)
+    """The network's inputs (5 or 7, from the blob's size), normalised.
+    In float32 (bf=True) the logarithms are of float32-rounded ratios, as on
+    the device, to within float32 rounding."""
+    L = layout(_nin(blob))
+    x = features(cons, g) if L["NIN"] == 5 else features_tab(cons, g)
     dt = np.float32 if bf else np.float64
     x = x.astype(np.float32) if bf else x
-    xn = (x - blob[IN_MEAN:IN_ISTD].astype(dt)) * blob[IN_ISTD:OUT_MEAN].astype(dt)
+    xn = (x - blob[L["IN_MEAN"]:L["IN_ISTD"]].astype(dt)) * \
+        blob[L["IN_ISTD"]:L["OUT_MEAN"]].astype(dt)
     return xn.astype(dt)
 
 
 def forward(blob, cons, g, bf=True):
-    """Normalised outputs y[nz, 2] and the cache for backward."""
+    """Normalised outputs y[nz, 2] and the cache for backward (5 or 7
+    inputs, from the blob's size)."""
     r = bf16 if bf else (lambda a: a)
     p = _params(blob, bf)
     xb = r(normalized_inputs(blob, cons, g, bf))
@@ -101,9 +141,11 @@ def forward(blob, cons, g, bf=True):
 
 
 def normalized_targets(blob, target, bf=True):
+    L = layout(_nin(blob))
     dt = np.float32 if bf else np.float64
     t = np.asarray(target).T.astype(dt)
-    return (t - blob[OUT_MEAN:OUT_STD].astype(dt)) / blob[OUT_STD:NBLOB].astype(dt)
+    return (t - blob[L["OUT_MEAN"]:L["OUT_STD"]].astype(dt)) / \
+        blob[L["OUT_STD"]:L["NBLOB"]].astype(dt)
 
 
 def loss_and_grad(blob, cons, g, target, bf=True):

@@ -86,3 +86,50 @@ def test_oracle_prim2con_matches_generator():
     abi.check("orc_prim2con_batch", oracle.lib().orc_prim2con_batch(
         nz, prim.ctypes.data, g.ctypes.data, out.ctypes.data, C.byref(par)))
     assert np.max(np.abs(out - cons) / (np.abs(cons).max(1, keepdims=True))) < 1e-13
+
+
+def test_table_network_backprop_and_inputs():
+    """The 7-input table network: backpropagation against finite differences
+    (float64), the two extra inputs equal log D and DYe / D of the zones, and
+    the first five equal the ideal network's."""
+    rng = np.random.default_rng(3)
+    L = M.layout(7)
+    nz = 40
+    prim, g = zones.make_batch_ye(nz, 4, (1e-12, 1e-4), (0.05, 80.0), (0.06, 0.5))
+    # cons of an ideal gas stand in (the network does not use the EOS)
+    rho, temp, ye = prim[0], prim[1], prim[2]
+    gam = 5.0 / 3.0
+    eps = temp / (gam - 1.0)
+    par = abi.params(gl_gamma=gam, eps_max=1e8, rho_max=1e8, rho_abs_min=1e-20)
+    cons = np.zeros((9, nz))
+    thermo = np.zeros((2, nz))
+    abi.check("orc_prim2con_batch_ye", oracle.lib().orc_prim2con_batch_ye(
+        nz, prim.ctypes.data, g.ctypes.data, cons.ctypes.data, thermo.ctypes.data,
+        C.byref(par)))
+    x = M.features_tab(cons, g)
+    det = np.linalg.det(np.array([[g[0], g[1], g[2]], [g[1], g[3], g[4]],
+                                  [g[2], g[4], g[5]]]).transpose(2, 0, 1))
+    assert np.allclose(x[:, :5], M.features(cons, g), rtol=0, atol=0)
+    assert np.allclose(x[:, 5], np.log(cons[0] / np.sqrt(det)), rtol=1e-13)
+    assert np.allclose(x[:, 6], ye, rtol=1e-13)
+    b = np.zeros(L["NBLOB"])
+    b[:L["NPARAM"]] = rng.normal(0, 0.3, L["NPARAM"])
+    b[L["IN_MEAN"]:L["IN_ISTD"]] = [-12.0, -20.0, -12.0, -15.0, 0.0, -15.0, 0.3]
+    b[L["IN_ISTD"]:L["OUT_MEAN"]] = [0.2, 0.1, 0.2, 0.1, 1.0, 0.2, 6.0]
+    b[L["OUT_MEAN"]:L["OUT_STD"]] = [1.5, 2.0]
+    b[L["OUT_STD"]:L["NBLOB"]] = [0.5, 1.0]
+    W = 1.0 / np.sqrt(1.0 - 0.5 * rng.uniform(0, 1, nz))
+    target = np.stack([np.log(W), np.log(W * (1 + eps))])
+    loss, grad = M.loss_and_grad(b, cons, g, target, bf=False)
+    assert grad.shape == (L["NPARAM"],)
+    idx = np.concatenate([rng.choice(np.arange(L["W1"], L["B1"]), 10),
+                          rng.choice(np.arange(L["W2"], L["B2"]), 10),
+                          rng.choice(np.arange(L["W3"], L["NPARAM"]), 6)])
+    h = 1e-6
+    for p in idx:
+        bp, bm = b.copy(), b.copy()
+        bp[p] += h
+        bm[p] -= h
+        fd = (M.loss_and_grad(bp, cons, g, target, bf=False)[0] -
+              M.loss_and_grad(bm, cons, g, target, bf=False)[0]) / (2 * h)
+        assert abs(fd - grad[p]) <= 1e-6 * (abs(fd) + 1e-3), (p, fd, grad[p])
