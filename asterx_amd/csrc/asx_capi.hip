@@ -473,10 +473,10 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
-// Scratch slots the fused substep computes (fluxes, UCT data, aux, dB_stag).
+// Scratch slots the fused substep computes (fluxes, UCT data, aux; the
+// faces take their normal field from A, so dB_stag is not among them).
 bool scratch_slot(int s) {
-  return (s >= ASX_FXDENS && s <= ASX_AMIN_ZFACE) || (s >= ASX_FX && s <= ASX_G) ||
-         (s >= ASX_DBX_STAG && s <= ASX_DBZ_STAG);
+  return (s >= ASX_FXDENS && s <= ASX_AMIN_ZFACE) || (s >= ASX_FX && s <= ASX_G);
 }
 
 Scratch *scratch_for(const asx_patch *pt, void *stream) {
@@ -629,6 +629,7 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
   A.rec_var = par->recon_type;
   A.with_T = par->reconstruct_with_temperature;
   A.write_uct = uct;
+  A.bstag_from_A = 0; // AsterX_Fluxes reads dB_stag (schedule.ccl:173-194)
   A.cnt_dev = cd;
   for (int d = 0; d < 3; ++d) {
     int halo[3] = {face_halo, face_halo, face_halo};
@@ -944,15 +945,6 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
   KPatch P = kpatch(stage);
   for (int q = 0; q < ASX_NUM_GF; ++q)
     if (sc->slot[q]) P.g[q] = sc->slot[q];
-  // dB_stag on every face whose curl stencil lies inside the arrays
-  for (int d = 0; d < 3; ++d) {
-    Box b;
-    for (int q = 0; q < 3; ++q) {
-      b.lo[q] = 0;
-      b.cnt[q] = (q == d) ? stage->n[q] + 2 * stage->ng + 1 : stage->n[q] + 2 * stage->ng;
-    }
-    if ((st = dev_status(timed(ASX_KC_BSTAG, s, [&] { return launch_bstag(d, P, b, P.g[ASX_DBX_STAG + d], s); })))) return st;
-  }
   FluxArgs A;
   A.rp = rparams(par);
   A.eos = ideal_gas(par);
@@ -963,6 +955,9 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
   A.rec_var = par->recon_type;
   A.with_T = par->reconstruct_with_temperature;
   A.write_uct = par->use_uct;
+  // the faces take their normal field as the curl of the stage's A (what
+  // ComputedBstagFromA would store; nothing else here reads dB_stag)
+  A.bstag_from_A = 1;
   A.cnt_dev = nullptr;
   for (int d = 0; d < 3; ++d) {
     int halo[3] = {1, 1, 1};
@@ -1362,14 +1357,13 @@ constexpr int kStageIn[3] = {0, 1, 2}, kStageOut[3] = {1, 2, 0};
 
 // Launch kinds of a plan, each with its per-patch boxes and workgroup list.
 enum BatchKind {
-  BK_BSTAG = 0,  // + d: dB_stag scratch over the whole array
-  BK_FACE = 3,   // + d: faces, tiles of the face variant
-  BK_AUX = 6,    // vertices, two layers into the ghosts
-  BK_UPDATE = 7, // interior vertices, tiles of the update variant
-  BK_BC = 8,     // + d: ghost layers normal to d
-  BK_UBSTAG = 11, // + d: interior faces of the caller's dB_stag
-  BK_C2P = 14,   // interior cells
-  BK_COUNT = 15
+  BK_FACE = 0,   // + d: faces, tiles of the face variant
+  BK_AUX = 3,    // vertices, two layers into the ghosts
+  BK_UPDATE = 4, // interior vertices, tiles of the update variant
+  BK_BC = 5,     // + d: ghost layers normal to d
+  BK_UBSTAG = 8, // + d: interior faces of the caller's dB_stag
+  BK_C2P = 11,   // interior cells
+  BK_COUNT = 12
 };
 
 struct BatchLaunch {
@@ -1443,13 +1437,7 @@ asx_status plan_launch(asx_batch *b, int k, int v, BatchLaunch **out) {
     const asx_patch &pt = b->pt[p];
     BatchBox &bb = boxes[p];
     int tile[3] = {0, 0, 0}; // 0: 256-point chunks of the flattened box
-    if (k >= BK_BSTAG && k < BK_BSTAG + 3) {
-      const int d = k - BK_BSTAG;
-      for (int q = 0; q < 3; ++q) {
-        bb.lo[q] = 0;
-        bb.cnt[q] = pt.n[q] + 2 * pt.ng + (q == d);
-      }
-    } else if (k >= BK_FACE && k < BK_FACE + 3) {
+    if (k >= BK_FACE && k < BK_FACE + 3) {
       const int d = k - BK_FACE;
       int halo[3] = {1, 1, 1};
       halo[d] = 0;
@@ -1526,24 +1514,19 @@ FluxArgs plan_flux_args(const asx_params *par) {
   A.rec_var = par->recon_type;
   A.with_T = par->reconstruct_with_temperature;
   A.write_uct = 0;
+  A.bstag_from_A = 1; // the faces take the curl of the stage's A
   A.cnt_dev = nullptr;
   for (int q = 0; q < 3; ++q) A.lo[q] = A.cnt[q] = 0;
   return A;
 }
 
-// Stage-s stencil work: dB_stag, faces, aux, update into buffer (1,2,0)[s].
+// Stage-s stencil work: faces (normal field from A), aux, update into
+// buffer (1,2,0)[s].
 asx_status plan_stencil(asx_batch *b, int s, hipStream_t str) {
   const int bi = kStageIn[s], bo = kStageOut[s];
   const KPatch *T = b->stab[bi];
   BatchLaunch *L;
   asx_status st;
-  for (int d = 0; d < 3; ++d) {
-    if ((st = plan_launch(b, BK_BSTAG + d, 0, &L))) return st;
-    if ((st = dev_status(timed(ASX_KC_BSTAG, str, [&] {
-           return launch_bstag_batch(d, T, L->box, L->map, L->nmap, str);
-         }))))
-      return st;
-  }
   const FluxArgs A = plan_flux_args(&b->par);
   for (int d = 0; d < 3; ++d) {
     const int v = b->var[d];
@@ -1751,11 +1734,11 @@ asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_
   if (hipGetDevice(&b->dev) != hipSuccess ||
       hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess)
     return fail(ASX_ERR_DEVICE);
-  // one scratch pool: face fluxes, aux and dB_stag of every patch
+  // one scratch pool: face fluxes and aux of every patch (the faces take
+  // their normal field from A, so no dB_stag scratch)
   std::vector<int> sslots;
   for (int q = ASX_FXDENS; q <= ASX_FZBZ; ++q) sslots.push_back(q);
   for (int q = ASX_FX; q <= ASX_G; ++q) sslots.push_back(q);
-  for (int q = ASX_DBX_STAG; q <= ASX_DBZ_STAG; ++q) sslots.push_back(q);
   size_t pool = 0;
   for (int p = 0; p < npatch; ++p)
     for (int q : sslots)

@@ -127,6 +127,25 @@ ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Si
 
 // Numerical flux of every conserved variable at face (i, j, k) from the
 // left (0) and right (1) states.
+// dB_stag on the D-face at (i, j, k): the curl of the edge vector
+// potential with first differences (ComputeStaggeredB<D>,
+// AsterX/src/computeBfromA.cxx:32-55).  One expression for the curl kernel
+// and for the face kernels that take it inline, so both give the same value.
+template <int D> ASX_DEV double bstag_value(const KPatch &P, int i, int j, int k) {
+  const double *idx = P.idx;
+  const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
+  if (D == 0) {
+    const int oz = off(P, 3, i, j, k), oy = off(P, 5, i, j, k);
+    return idx[1] * (Az[oz + P.sj[3]] - Az[oz]) - idx[2] * (Ay[oy + P.sk[5]] - Ay[oy]);
+  } else if (D == 1) {
+    const int ox = off(P, 6, i, j, k), oz = off(P, 3, i, j, k);
+    return idx[2] * (Ax[ox + P.sk[6]] - Ax[ox]) - idx[0] * (Az[oz + 1] - Az[oz]);
+  } else {
+    const int oy = off(P, 5, i, j, k), ox = off(P, 6, i, j, k);
+    return idx[0] * (Ay[oy + 1] - Ay[oy]) - idx[1] * (Ax[ox + P.sj[6]] - Ax[ox]);
+  }
+}
+
 template <int D, bool TAB = false>
 ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
                        const Side &L, const Side &R, bool lo0, bool lo1) {
@@ -181,7 +200,8 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
   // magnetic field: normal from the staggered field, transverse recon
   double Bs[3][2];
   {
-    const double val = P.g[ASX_DBX_STAG + D][ofc] * isqrtg;
+    const double bn = A.bstag_from_A ? bstag_value<D>(P, i, j, k) : P.g[ASX_DBX_STAG + D][ofc];
+    const double val = bn * isqrtg;
     Bs[D][0] = val;
     Bs[D][1] = val;
     Bs[D1][0] = L.b1;
@@ -528,20 +548,7 @@ __global__ __launch_bounds__(256) void k_aux(KPatch P, Box B) {
 // ComputeStaggeredB<D> (AsterX/src/computeBfromA.cxx:32-55)
 template <int D>
 ASX_DEV void bstag_point(const KPatch &P, int i, int j, int k, double *out) {
-  const double *idx = P.idx;
-  const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
-  double r;
-  if (D == 0) {
-    const int oz = off(P, 3, i, j, k), oy = off(P, 5, i, j, k);
-    r = idx[1] * (Az[oz + P.sj[3]] - Az[oz]) - idx[2] * (Ay[oy + P.sk[5]] - Ay[oy]);
-  } else if (D == 1) {
-    const int ox = off(P, 6, i, j, k), oz = off(P, 3, i, j, k);
-    r = idx[2] * (Ax[ox + P.sk[6]] - Ax[ox]) - idx[0] * (Az[oz + 1] - Az[oz]);
-  } else {
-    const int oy = off(P, 5, i, j, k), ox = off(P, 6, i, j, k);
-    r = idx[0] * (Ay[oy + 1] - Ay[oy]) - idx[1] * (Ax[ox + P.sj[6]] - Ax[ox]);
-  }
-  out[off(P, 1 << D, i, j, k)] = r;
+  out[off(P, 1 << D, i, j, k)] = bstag_value<D>(P, i, j, k);
 }
 template <int D>
 __global__ __launch_bounds__(256) void k_bstag(KPatch P, Box B, double *out) {

@@ -73,6 +73,7 @@ struct FluxArgs {
   IdealGas eos;
   TabEOS tab; // used by the table instantiations (evolution_eos = Tabulated3d)
   int lo_method, flux_type, rec_var, with_T, write_uct;
+  int bstag_from_A; // normal face field as the curl of A at the face (no dB_stag read)
   int lo[3], cnt[3];
   unsigned long long *cnt_dev; // [nan_faces, lo_faces] or null
 };

@@ -379,8 +379,9 @@ asx_status asx_bc_fill_many(const asx_patch *patch, int32_t nslot,
 /* ODESolvers update + AsterX_Con2PrimGroup on one patch)                   */
 /* ------------------------------------------------------------------------ */
 /* Stencil half of a substep.  Requires valid ghost data for prims, Avec and
- * Psi on `stage` (as after a SYNC); dB*_stag is recomputed from Avec exactly
- * as AsterX_ComputedBstagFromA does, so its ghosts are not needed.  Computes
+ * Psi on `stage` (as after a SYNC); the faces take dB*_stag as the curl of
+ * Avec at the face, exactly as AsterX_ComputedBstagFromA computes it, so
+ * dB*_stag is neither read nor written.  Computes
  * out = c0*u0 + c1*u + c2*rhs(u) on interior points, where u is the state
  * held in `stage` and rhs is SourceTerms + RHS of the reference.  u0 is not
  * read when c0 == 0.  Face scratch is allocated on first use per patch shape
@@ -566,7 +567,7 @@ asx_status asx_c2p_batch_ye_mlp(int64_t nz, const double *cons, const double *g,
  * of patches[p]; bufs[2p] and bufs[2p+1] are buffers 1 and 2 of patch p.
  * Requirements per patch are those of asx_substep and asx_con2prim_group;
  * ghost zones are filled with `bc_kind` on every face (there is no
- * inter-patch exchange).  The plan owns the face/aux/dB_stag scratch of all
+ * inter-patch exchange).  The plan owns the face/aux scratch of all
  * patches and the con2prim queue; it holds the descriptors, not the data. */
 typedef struct asx_batch_s asx_batch;
 asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_state *bufs,
@@ -581,7 +582,7 @@ asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, vo
 /* Launch shapes: a plan starts with the device's current variants
  * (asx_set_variant) and keeps its own from then on.  asx_batch_autotune
  * runs, for every face and update variant, the stage-0 stencil work
- * (dB_stag, faces, aux, update into buffer 1) once untimed and `reps` times
+ * (faces, aux, update into buffer 1) once untimed and `reps` times
  * timed, and keeps the fastest variant per class in the plan.  It writes
  * only buffer 1 and the plan's scratch. */
 asx_status asx_batch_autotune(asx_batch *plan, int32_t reps, void *stream);
