@@ -314,12 +314,14 @@ hipError_t launch_con2prim_main(const KPatch &P, const IdealGas &eos, const C2PA
   const Box b = c2p_box(P);
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, qcount);
   if (b.htotal() <= 0) return hipGetLastError();
-  const bool noble = a.prime == ASX_C2P_NOBLE;
+  const int pr = a.prime;
   if (from_A) {
-    if (noble) c2p_main_t<true, ASX_C2P_NOBLE>(P, b, eos, a, hist, queue, qcount, s);
+    if (pr == ASX_C2P_NOBLE) c2p_main_t<true, ASX_C2P_NOBLE>(P, b, eos, a, hist, queue, qcount, s);
+    else if (pr == ASX_C2P_ENTROPY) c2p_main_t<true, ASX_C2P_ENTROPY>(P, b, eos, a, hist, queue, qcount, s);
     else c2p_main_t<true, ASX_C2P_PALENZUELA>(P, b, eos, a, hist, queue, qcount, s);
   } else {
-    if (noble) c2p_main_t<false, ASX_C2P_NOBLE>(P, b, eos, a, hist, queue, qcount, s);
+    if (pr == ASX_C2P_NOBLE) c2p_main_t<false, ASX_C2P_NOBLE>(P, b, eos, a, hist, queue, qcount, s);
+    else if (pr == ASX_C2P_ENTROPY) c2p_main_t<false, ASX_C2P_ENTROPY>(P, b, eos, a, hist, queue, qcount, s);
     else c2p_main_t<false, ASX_C2P_PALENZUELA>(P, b, eos, a, hist, queue, qcount, s);
   }
   return hipGetLastError();
@@ -362,6 +364,8 @@ hipError_t launch_con2prim_main_batch(const KPatch *tab, const int *zbase, int n
   const dim3 grid((unsigned)nmap), block(256);
   if (a.prime == ASX_C2P_NOBLE)
     hipLaunchKernelGGL(k_c2p_main_batch<ASX_C2P_NOBLE>, grid, block, 0, s, tab, zbase, eos, a, map, nmap, queue, qcount);
+  else if (a.prime == ASX_C2P_ENTROPY)
+    hipLaunchKernelGGL(k_c2p_main_batch<ASX_C2P_ENTROPY>, grid, block, 0, s, tab, zbase, eos, a, map, nmap, queue, qcount);
   else
     hipLaunchKernelGGL(k_c2p_main_batch<ASX_C2P_PALENZUELA>, grid, block, 0, s, tab, zbase, eos, a, map, nmap, queue, qcount);
   return hipGetLastError();

@@ -120,6 +120,7 @@ struct Atmo {
 // Solver and atmosphere parameters, resolved once on the host.
 struct C2PArgs {
   int prime, second, max_iter, excise, use_mask;
+  int entropy_fix; // use_entropy_fix: the Entropy solver after both failed
   int use_z, use_temp, use_press_atmo, thermal_eos_atmo;
   double tol, alp_thresh, cons_error, vw_lim, w_lim, v_lim, Bsq_lim, rho_BH,
       eps_BH, vwlim_BH;
@@ -698,6 +699,116 @@ ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
   else cv.scale(sqrt_detg);
 }
 
+// Con2PrimFactory/src/c2p_1DEntropy.hxx:229-255 (Appendix A.4 of
+// arXiv:1112.0568): x - rho(x), h from the conserved entropy
+template <class E> struct EntF {
+  const E &eos;
+  double Ssq, Bsq, BiSi, dens, ent, ye;
+  ASX_DEV double operator()(double x) const {
+    const double press = eos.press_rk(x, ent, ye);
+    const double eps = eos.eps_rk(x, ent, ye);
+    const double lor = dens / x;
+    const double h = (1.0 + eps + press / x);
+    const double W = x * h * lor * lor;
+    const double Sfsq = (W * W * Ssq + BiSi * BiSi * (Bsq + 2.0 * W)) / ((W + Bsq) * (W + Bsq));
+    const double rho = dens / sqrt(1.0 + Sfsq / (dens * dens * h * h));
+    return x - rho;
+  }
+};
+
+// Entropy 1D, c2p_1DEntropy.hxx:257-466 (the ideal gas; refused for tables)
+template <class E>
+ASX_DEV void entropy_solve(const E &eos, const Atmo &atmo, const C2PArgs &a, Prim &pv,
+                           Cons &cv, const Sym3 &glo, Report &rep) {
+  rep.iters = 0;
+  rep.adjust_cons = false;
+  rep.set_atmo = false;
+  rep.status = ST_SUCCESS;
+  const double detg = det3(glo);
+  const double sqrt_detg = sqrt(detg);
+  if (!((glo.xx > 0.0) && (glo.xx * glo.yy - glo.xy * glo.xy > 0.0) && (detg > 0.0))) {
+    rep.status = ST_INVALID_DETG; rep.adjust_cons = true;
+    prim_nan(pv); cv.set_nan();
+    return;
+  }
+  const Sym3 gup = inv3(glo, detg);
+  cv.divide(sqrt_detg);
+  const double *m = cv.mom;
+  double Ssq = m[0] * (gup.xx * m[0] + gup.xy * m[1] + gup.xz * m[2]);
+  Ssq += m[1] * (gup.xy * m[0] + gup.yy * m[1] + gup.yz * m[2]);
+  Ssq += m[2] * (gup.xz * m[0] + gup.yz * m[1] + gup.zz * m[2]);
+  double B_low[3];
+  lower(glo, cv.dB, B_low);
+  const double Bsq = dot3(B_low, cv.dB);
+  const double BiSi = cv.dB[0] * m[0] + cv.dB[1] * m[1] + cv.dB[2] * m[2];
+  if (!isfinite(cv.dens) || !isfinite(Ssq) || !isfinite(Bsq) || !isfinite(BiSi) ||
+      !isfinite(cv.DYe) || !isfinite(cv.DEnt)) {
+    rep.status = ST_NANS_IN_CONS; rep.adjust_cons = true;
+    prim_nan(pv); cv.set_nan();
+    return;
+  }
+  if (Bsq > a.Bsq_lim) {
+    rep.status = ST_B_LIMIT; rep.adjust_cons = true;
+    prim_nan(pv); cv.set_nan();
+    return;
+  }
+  // (A59): the root lies in [D / sqrt(1 + S^2/D^2), D]
+  const EntF<E> fn{eos, Ssq, Bsq, BiSi, cv.dens, cv.DEnt / cv.dens, cv.DYe / cv.dens};
+  const double xa = cv.dens / sqrt(1.0 + Ssq / (cv.dens * cv.dens));
+  const double xb = cv.dens;
+  double ra, rb;
+  brent(fn, xa, xb, fn(xa), fn(xb), a.tol0, a.max_iter, ra, rb, rep.iters);
+  const double x = 0.5 * (ra + rb);
+  // xEntropyToPrim, c2p_1DEntropy.hxx:142-226
+  pv.rho = x;
+  pv.entropy = cv.DEnt / cv.dens;
+  pv.Ye = cv.DYe / cv.dens;
+  pv.w_lor = cv.dens / x;
+  pv.press = eos.press_rk(x, pv.entropy, pv.Ye);
+  pv.eps = eos.eps_rk(x, pv.entropy, pv.Ye);
+  pv.temperature = eos.temp_re(x, pv.eps, pv.Ye);
+  const double Z_Sol = (x + pv.eps * x + pv.press) * pv.w_lor * pv.w_lor;
+  vel_from_Z(Z_Sol, pv.w_lor, Bsq, BiSi, cv, gup, glo, a.use_z, pv);
+  pv.B[0] = cv.dB[0]; pv.B[1] = cv.dB[1]; pv.B[2] = cv.dB[2];
+  if (pv.rho <= 0.0) {
+    rep.status = ST_RANGE_RHO; rep.adjust_cons = true;
+    cv.scale(sqrt_detg);
+    return;
+  }
+  if (fabs(ra - rb) > a.tol0 * smin(fabs(ra), fabs(rb))) {
+    // tau is not checked: the inversion uses the conserved entropy
+    Cons ck;
+    cons_from_prim(glo, pv, ck);
+    ck.divide(sqrt_detg);
+    const double small = 1e-50;
+    double e = pow2((ck.dens - cv.dens) / (cv.dens + small));
+    e = smax(e, pow2((ck.mom[0] - cv.mom[0]) / (cv.mom[0] + small)));
+    e = smax(e, pow2((ck.mom[1] - cv.mom[1]) / (cv.mom[1] + small)));
+    e = smax(e, pow2((ck.mom[2] - cv.mom[2]) / (cv.mom[2] + small)));
+    if (sqrt(e) >= a.cons_error) {
+      rep.status = ST_ROOT_FAIL_CONV; rep.adjust_cons = true;
+      cv.scale(sqrt_detg);
+      return;
+    }
+  }
+  // tau from the primitives (:427-436)
+  double v_low[3];
+  lower(glo, pv.vel, v_low);
+  const double bst = pv.w_lor * dot3(pv.B, v_low);
+  const double bs2 = (Bsq + bst * bst) / (pv.w_lor * pv.w_lor);
+  cv.tau = (pv.w_lor * pv.w_lor * (pv.rho * (1.0 + pv.eps) + pv.press + bs2) -
+            (pv.press + 0.5 * bs2) - bst * bst) -
+           cv.dens;
+  if (pv.rho < atmo.rho_cut) {
+    rep.status = ST_SUCCESS; rep.set_atmo = true; rep.adjust_cons = true;
+    atmo.set(pv, cv, glo);
+    return;
+  }
+  prims_floors(eos, atmo, a, pv, cv, glo, rep);
+  if (rep.adjust_cons) cons_from_prim(glo, pv, cv);
+  else cv.scale(sqrt_detg);
+}
+
 // Inputs of one cell of AsterX_Con2Prim_typeEoS.
 struct C2PIn {
   double r, alp, mask;
@@ -789,6 +900,7 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
     const int prime = PRIME < 0 ? a.prime : PRIME;
     if constexpr (std::is_same<E, IdealGas>::value) {
       if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
+      else if (prime == ASX_C2P_ENTROPY) entropy_solve(eos, atmo, a, pv, cv, glo, rep1);
       else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess, PRIME >= 0);
     } else {
       (void)prime; // Noble needs the ideal-fluid Gamma; refused for tables at the C ABI
@@ -801,11 +913,21 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
       if constexpr (std::is_same<E, IdealGas>::value) {
         if (a.second == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep2);
         else if (a.second == ASX_C2P_PALENZUELA) palenzuela_solve(eos, atmo, a, pv, cv, glo, rep2);
+        else if (a.second == ASX_C2P_ENTROPY) entropy_solve(eos, atmo, a, pv, cv, glo, rep2);
       } else {
         if (a.second == ASX_C2P_PALENZUELA) palenzuela_solve(eos, atmo, a, pv, cv, glo, rep2);
       }
     }
-    if (rep1.failed() && rep2.failed()) {
+    bool failed = rep1.failed() && rep2.failed();
+    if constexpr (std::is_same<E, IdealGas>::value) {
+      if (failed && a.entropy_fix) { // con2prim.cxx:310-313
+        code = ASX_C2P_FLAG_ENTROPY;
+        Report rep3{ST_NOT_SET, false, false, 0};
+        entropy_solve(eos, atmo, a, pv, cv, glo, rep3);
+        failed = rep3.failed();
+      }
+    }
+    if (failed) {
       flag_local = false;
       code = ASX_C2P_FLAG_FAIL;
       if (mask_local != 1.0) {

@@ -168,6 +168,7 @@ C2PArgs c2p_args(const asx_params *p) {
   a.max_iter = p->max_iter;
   a.excise = p->excise;
   a.use_mask = p->use_mask;
+  a.entropy_fix = p->use_entropy_fix;
   a.use_z = p->use_z;
   a.use_temp = p->use_temperature;
   a.use_press_atmo = p->use_press_atmo;
@@ -267,16 +268,18 @@ asx_status check_eos(const asx_params *p) {
 asx_status check_c2p_par(const asx_params *p) {
   asx_status st = check_par(p);
   if (st) return st;
-  if (p->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
   // a table: Palenzuela on temperature without a pressure atmosphere (Noble
-  // needs the ideal-fluid Gamma, eps_from_valid_rho_press_ye is refused)
+  // needs the ideal-fluid Gamma; eps_from_valid_rho_press_ye and p, eps
+  // from (rho, kappa), which the Entropy solver needs, are refused)
   if (tabulated(p) && (p->c2p_prime != ASX_C2P_PALENZUELA || p->c2p_second == ASX_C2P_NOBLE ||
+                       p->c2p_second == ASX_C2P_ENTROPY || p->use_entropy_fix ||
                        !p->use_temperature || p->use_press_atmo))
     return ASX_ERR_UNSUPPORTED;
-  if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA)
+  if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA &&
+      p->c2p_prime != ASX_C2P_ENTROPY)
     return ASX_ERR_UNSUPPORTED;
   if (p->c2p_second != ASX_C2P_NOBLE && p->c2p_second != ASX_C2P_PALENZUELA &&
-      p->c2p_second != ASX_C2P_NONE)
+      p->c2p_second != ASX_C2P_ENTROPY && p->c2p_second != ASX_C2P_NONE)
     return ASX_ERR_UNSUPPORTED;
   if (p->max_iter < 1) return ASX_ERR_PARAM;
   if (!(p->c2p_tol > 0.0)) return ASX_ERR_PARAM;
@@ -1172,19 +1175,19 @@ namespace {
 asx_status check_c2p_eos(const asx_params *p) {
   asx_status st = check_eos(p);
   if (st) return st;
-  if (p->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
   if (p->max_iter < 1 || !(p->c2p_tol > 0.0)) return ASX_ERR_PARAM;
   if (p->evolution_eos == ASX_EOS_TABULATED) {
-    if (p->c2p_prime != ASX_C2P_PALENZUELA) return ASX_ERR_UNSUPPORTED;
+    if (p->c2p_prime != ASX_C2P_PALENZUELA || p->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
     if (p->c2p_second != ASX_C2P_PALENZUELA && p->c2p_second != ASX_C2P_NONE)
       return ASX_ERR_UNSUPPORTED;
     if (!p->use_temperature || p->use_press_atmo) return ASX_ERR_UNSUPPORTED;
     return ASX_OK;
   }
-  if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA)
+  if (p->c2p_prime != ASX_C2P_NOBLE && p->c2p_prime != ASX_C2P_PALENZUELA &&
+      p->c2p_prime != ASX_C2P_ENTROPY)
     return ASX_ERR_UNSUPPORTED;
   if (p->c2p_second != ASX_C2P_NOBLE && p->c2p_second != ASX_C2P_PALENZUELA &&
-      p->c2p_second != ASX_C2P_NONE)
+      p->c2p_second != ASX_C2P_ENTROPY && p->c2p_second != ASX_C2P_NONE)
     return ASX_ERR_UNSUPPORTED;
   return ASX_OK;
 }

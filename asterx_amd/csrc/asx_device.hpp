@@ -388,6 +388,12 @@ struct IdealGas {
   ASX_DEV double temp_re(double, double &eps, double) const { return temp_re(eps); }
   ASX_DEV double kappa_re(double rho, double &eps, double) const { return kappa(rho, eps); }
   ASX_DEV double csnd_re(double, double &eps, double) const { return csnd(eps); }
+  // p and eps from (rho, kappa) (eos_3p_idealgas.hxx:201-215):
+  // p = kappa rho^gamma, eps = kappa rho^(gamma-1) / (gamma-1)
+  ASX_DEV double press_rk(double rho, double kap, double) const { return kap * pow(rho, gamma); }
+  ASX_DEV double eps_rk(double rho, double kap, double) const {
+    return kap * pow(rho, gamma - 1.0) / (gamma - 1.0);
+  }
   // press_re, temp_re and kappa_re at one (rho, eps, Ye)
   ASX_DEV void state_re(double rho, double &eps, double ye, double &press, double &temp,
                         double &kap) const {

@@ -201,6 +201,9 @@ hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const 
   if (nz <= 0) return hipSuccess;
   if (a.prime == ASX_C2P_NOBLE)
     return c2p_ye_t<ASX_C2P_NOBLE>(nz, cons, g, seed, prim, flag, iters, eos, a, queue, qcount, s);
+  if (a.prime == ASX_C2P_ENTROPY)
+    return c2p_ye_t<ASX_C2P_ENTROPY>(nz, cons, g, seed, prim, flag, iters, eos, a, queue, qcount,
+                                     s);
   return c2p_ye_t<ASX_C2P_PALENZUELA>(nz, cons, g, seed, prim, flag, iters, eos, a, queue,
                                       qcount, s);
 }

@@ -191,6 +191,10 @@ struct TabEOS {
   // the callers that reach it (pressure atmosphere, use_temperature = no)
   // are refused at the C ABI
   ASX_DEV double eps_rp(double, double, double) const { return __builtin_nan(""); }
+  // p and eps from (rho, kappa): refused by the reference for tables
+  // (eos_3p_tabulated3d.hxx:315-333), and at the C ABI here
+  ASX_DEV double press_rk(double, double, double) const { return __builtin_nan(""); }
+  ASX_DEV double eps_rk(double, double, double) const { return __builtin_nan(""); }
 };
 
 } // namespace asx

@@ -161,3 +161,23 @@ def magnetic_rotor(patch, gamma):
     _set_hydro(patch, rho, (vx, vy, 0.0), 1.0, gamma)
     zero = lambda x, y, z: 0.0 * (x + y + z)
     _set_avec(patch, zero, zero, lambda x, y, z: y + 0.0 * (x + z))
+
+
+def cylindrical_blast(patch, gamma):
+    """AsterSeeds/src/2D_tests.cxx:135-190 ("cylindrical blast"): rho 1e-2,
+    p 1 inside r_cyl < 0.8, rho 1e-4, p 3e-5 outside r_cyl > 1, both
+    log-linear in r_cyl between; at rest; A = (0, 0, 0.1 y)."""
+    z, y, x = patch.coords(0)
+    shp = patch.shape("rho")
+    x, y = np.broadcast_to(x, shp), np.broadcast_to(y, shp)
+    r = np.sqrt(x * x + y * y)
+    r_in, r_out = 0.8, 1.0
+    rho_in, rho_out, p_in, p_out = 1e-2, 1e-4, 1.0, 3.0e-5
+
+    def prof(a_in, a_out):
+        mid = np.exp(((r_out - r) * math.log(a_in) + (r - r_in) * math.log(a_out)) /
+                     (r_out - r_in))
+        return np.where(r < r_in, a_in, np.where(r > r_out, a_out, mid))
+    _set_hydro(patch, prof(rho_in, rho_out), (0.0, 0.0, 0.0), prof(p_in, p_out), gamma)
+    zero = lambda x, y, z: 0.0 * (x + y + z)
+    _set_avec(patch, zero, zero, lambda x, y, z: 0.1 * y + 0.0 * (x + z))

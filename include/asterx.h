@@ -63,7 +63,7 @@ enum { ASX_RECVAR_V = 0, ASX_RECVAR_Z = 1, ASX_RECVAR_S = 2 };
 enum { ASX_GAUGE_ALGEBRAIC = 0, ASX_GAUGE_GENERALIZED_LORENZ = 1 };
 /* EOSX/param.ccl:14-20 (Hybrid is not built here) */
 enum { ASX_EOS_IDEALGAS = 0, ASX_EOS_HYBRID = 1, ASX_EOS_TABULATED = 2 };
-/* Con2PrimFactory/param.ccl:8-22 (Entropy is not built here) */
+/* Con2PrimFactory/param.ccl:8-22 (Entropy: ideal gas only, as in the reference) */
 enum {
   ASX_C2P_NONE = 0,
   ASX_C2P_NOBLE = 1,

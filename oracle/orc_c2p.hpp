@@ -682,6 +682,124 @@ inline void palenzuela_solve(const E &eos, const Atmo &atmo,
   }
 }
 
+// Con2PrimFactory/src/c2p_1DEntropy.hxx:229-255 (Appendix A.4 of
+// arXiv:1112.0568): x - rho(x), with h from the conserved entropy
+inline double entropy_f(const IdealGas &eos, double Ssq, double Bsq, double BiSi, double x,
+                        const Cons &cv) {
+  const double ent = cv.DEnt / cv.dens;
+  const double ye = cv.DYe / cv.dens;
+  const double press = eos.press_from_rho_kappa_ye(x, ent, ye);
+  const double eps = eos.eps_from_rho_kappa_ye(x, ent, ye);
+  const double lor = cv.dens / x;
+  const double h = (1.0 + eps + press / x);
+  const double W = x * h * lor * lor;
+  const double Sfsq = (W * W * Ssq + BiSi * BiSi * (Bsq + 2.0 * W)) / ((W + Bsq) * (W + Bsq));
+  const double rho = cv.dens / std::sqrt(1.0 + Sfsq / (cv.dens * cv.dens * h * h));
+  return x - rho;
+}
+
+// Con2PrimFactory/src/c2p_1DEntropy.hxx:257-466 (ideal gas: the tabulated
+// EOS of the reference refuses p and eps from (rho, kappa))
+inline void entropy_solve(const IdealGas &eos, const Atmo &atmo, const C2PParams &cp,
+                          Prim &pv, Cons &cv, const Sym3 &glo, Report &rep) {
+  rep.iters = 0;
+  rep.adjust_cons = false;
+  rep.set_atmo = false;
+  rep.status = ST_SUCCESS;
+  const double detg = calc_det(glo);
+  const double sqrt_detg = std::sqrt(detg);
+  const bool minor1 = glo.xx > 0.0;
+  const bool minor2 = glo.xx * glo.yy - glo.xy * glo.xy > 0.0;
+  const bool minor3 = detg > 0.0;
+  if (!(minor1 && minor2 && minor3)) {
+    rep.status = ST_INVALID_DETG; rep.adjust_cons = true;
+    prim_set_nan(pv); cv.set_nan();
+    return;
+  }
+  const Sym3 gup = calc_inv(glo, detg);
+  cv.divide(sqrt_detg);
+  const double *m = cv.mom;
+  double Ssq = m[0] * (gup.xx * m[0] + gup.xy * m[1] + gup.xz * m[2]);
+  Ssq += m[1] * (gup.xy * m[0] + gup.yy * m[1] + gup.yz * m[2]);
+  Ssq += m[2] * (gup.xz * m[0] + gup.yz * m[1] + gup.zz * m[2]);
+  double B_low[3];
+  lower(glo, cv.dB, B_low);
+  const double Bsq = dot3(B_low, cv.dB);
+  const double BiSi = cv.dB[0] * m[0] + cv.dB[1] * m[1] + cv.dB[2] * m[2];
+  if ((!std::isfinite(cv.dens)) || (!std::isfinite(Ssq)) || (!std::isfinite(Bsq)) ||
+      (!std::isfinite(BiSi)) || (!std::isfinite(cv.DYe)) || (!std::isfinite(cv.DEnt))) {
+    rep.status = ST_NANS_IN_CONS; rep.adjust_cons = true;
+    prim_set_nan(pv); cv.set_nan();
+    return;
+  }
+  if (Bsq > cp.Bsq_lim) {
+    rep.status = ST_B_LIMIT; rep.adjust_cons = true;
+    prim_set_nan(pv); cv.set_nan();
+    return;
+  }
+  // (A59): the root lies in [D / sqrt(1 + S^2/D^2), D]
+  const double a = cv.dens / std::sqrt(1.0 + Ssq / (cv.dens * cv.dens));
+  const double b = cv.dens;
+  auto fn = [&](double x) { return entropy_f(eos, Ssq, Bsq, BiSi, x, cv); };
+  const double log2 = std::log(2.0);
+  const int minbits = int(std::fabs(std::log(cp.tol)) / log2);
+  const double tol0 = std::ldexp(double(1.0), -minbits);
+  const auto res = brent(fn, a, b, minbits, cp.max_iter, rep.iters);
+  const double x_sol = 0.5 * (res.first + res.second);
+  // xEntropyToPrim, c2p_1DEntropy.hxx:142-226
+  pv.rho = x_sol;
+  pv.entropy = cv.DEnt / cv.dens;
+  pv.Ye = cv.DYe / cv.dens;
+  pv.w_lor = cv.dens / x_sol;
+  pv.press = eos.press_from_rho_kappa_ye(x_sol, pv.entropy, pv.Ye);
+  pv.eps = eos.eps_from_rho_kappa_ye(x_sol, pv.entropy, pv.Ye);
+  pv.temperature = eos.temp_from_rho_eps_ye(x_sol, pv.eps, pv.Ye);
+  const double Z_Sol = (x_sol + pv.eps * x_sol + pv.press) * pv.w_lor * pv.w_lor;
+  vel_from_Z(Z_Sol, pv.w_lor, Bsq, BiSi, cv, gup, glo, cp.use_z, pv);
+  for (int i = 0; i < 3; ++i) pv.B[i] = cv.dB[i];
+  if (pv.rho <= 0.0) {
+    rep.status = ST_RANGE_RHO; rep.adjust_cons = true;
+    cv.scale(sqrt_detg);
+    return;
+  }
+  if (std::fabs(res.first - res.second) >
+      tol0 * smin(std::fabs(res.first), std::fabs(res.second))) {
+    // tau is not checked: the inversion uses the conserved entropy
+    Cons ck;
+    ck.from_prim(pv, glo);
+    ck.divide(sqrt_detg);
+    const double small = 1e-50;
+    double e = pow2((ck.dens - cv.dens) / (cv.dens + small));
+    e = smax(e, pow2((ck.mom[0] - cv.mom[0]) / (cv.mom[0] + small)));
+    e = smax(e, pow2((ck.mom[1] - cv.mom[1]) / (cv.mom[1] + small)));
+    e = smax(e, pow2((ck.mom[2] - cv.mom[2]) / (cv.mom[2] + small)));
+    if (std::sqrt(e) >= cp.cons_error) {
+      rep.status = ST_ROOT_FAIL_CONV; rep.adjust_cons = true;
+      cv.scale(sqrt_detg);
+      return;
+    }
+  }
+  // tau from the primitives (:427-436)
+  double v_low[3];
+  lower(glo, pv.vel, v_low);
+  const double bst = pv.w_lor * dot3(pv.B, v_low);
+  const double bs2 = (Bsq + bst * bst) / (pv.w_lor * pv.w_lor);
+  cv.tau = (pv.w_lor * pv.w_lor * (pv.rho * (1.0 + pv.eps) + pv.press + bs2) -
+            (pv.press + 0.5 * bs2) - bst * bst) -
+           cv.dens;
+  if (pv.rho < atmo.rho_cut) {
+    rep.status = ST_SUCCESS; rep.set_atmo = true; rep.adjust_cons = true;
+    atmo.set(pv, cv, glo);
+    return;
+  }
+  prims_floors(eos, atmo, cp, pv, cv, glo, rep);
+  if (rep.adjust_cons) {
+    cv.from_prim(pv, glo);
+  } else {
+    cv.scale(sqrt_detg);
+  }
+}
+
 // Atmosphere of AsterX/src/con2prim.cxx:82-133 at radius r.
 struct AtmoParams {
   double rho_abs_min, atmo_tol, p_atmo, t_atmo, Ye_atmo, r_atmo, n_rho_atmo,
@@ -742,6 +860,7 @@ struct C2PCellOut {
 struct C2PDriver {
   int prime, second; // ASX_C2P_*
   bool excise;
+  bool entropy_fix; // use_entropy_fix: Entropy after both solvers failed
   AtmoParams ap;
   C2PParams cp;
 };
@@ -812,7 +931,7 @@ inline void con2prim_cell(const E &eos, const Polytrope &poly,
       call_c2p = false;
     }
   }
-  Report rep1, rep2;
+  Report rep1, rep2, rep3;
   cons_floors(cv, glo);
   out.iters = 0;
   if (call_c2p) {
@@ -822,6 +941,9 @@ inline void con2prim_cell(const E &eos, const Polytrope &poly,
         noble_solve(eos, atmo, dp.cp, pv, seed, cv, glo, rep1);
     } else if (dp.prime == 2) {
       palenzuela_solve(eos, atmo, dp.cp, pv, cv, glo, rep1);
+    } else if (dp.prime == 3) {
+      if constexpr (std::is_same<E, IdealGas>::value)
+        entropy_solve(eos, atmo, dp.cp, pv, cv, glo, rep1);
     }
     out.iters = rep1.iters;
     if (rep1.failed()) {
@@ -831,9 +953,19 @@ inline void con2prim_cell(const E &eos, const Polytrope &poly,
           noble_solve(eos, atmo, dp.cp, pv, seed, cv, glo, rep2);
       } else if (dp.second == 2) {
         palenzuela_solve(eos, atmo, dp.cp, pv, cv, glo, rep2);
+      } else if (dp.second == 3) {
+        if constexpr (std::is_same<E, IdealGas>::value)
+          entropy_solve(eos, atmo, dp.cp, pv, cv, glo, rep2);
       }
     }
-    if (rep1.failed() && rep2.failed()) {
+    bool failed = rep1.failed() && rep2.failed();
+    if (failed && dp.entropy_fix) { // con2prim.cxx:310-313
+      code = 3; // C2P_ENTROPY
+      if constexpr (std::is_same<E, IdealGas>::value)
+        entropy_solve(eos, atmo, dp.cp, pv, cv, glo, rep3);
+      failed = rep3.failed();
+    }
+    if (failed) {
       flag_local = false;
       code = 6; // C2P_FAIL
       if (mask_local != 1.0) {

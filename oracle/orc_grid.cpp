@@ -732,17 +732,6 @@ void bstag_face(const View &v, int dir, const int I[3]) {
     v.g(ASX_DBZ_STAG).at(I) = idx[0] * (Ay.at(ip) - Ay.at(I)) - idx[1] * (Ax.at(jp) - Ax.at(I));
 }
 
-int check_patch(const asx_patch *pt, std::initializer_list<int> slots) {
-  if (!pt) return ASX_ERR_NULL;
-  for (int s : slots) {
-    if (!pt->gf[s].data) return ASX_ERR_NULL;
-    int64_t sj, sk;
-    asx_gf_strides(pt->n, pt->ng, s, &sj, &sk);
-    if (pt->gf[s].sj != sj || pt->gf[s].sk != sk) return ASX_ERR_LAYOUT;
-  }
-  return ASX_OK;
-}
-
 const std::initializer_list<int> kAdm = {ASX_ALP, ASX_BETAX, ASX_BETAY, ASX_BETAZ,
                                          ASX_GXX, ASX_GXY,   ASX_GXZ,   ASX_GYY,
                                          ASX_GYZ, ASX_GZZ};
@@ -793,10 +782,20 @@ extern "C" void asx_gf_strides(const int32_t n[3], int32_t ng, int s,
 
 extern "C" {
 
+// The upwind-CT face data (vtilde, amax, amin) are written by the fluxes
+// and read by the RHS when use_uct is set.
+static int need_uct(const asx_patch *pt, const asx_params *par) {
+  if (!par->use_uct) return ASX_OK;
+  for (int s = ASX_VTILDE_Y_XFACE; s <= ASX_AMIN_ZFACE; ++s)
+    if (!pt->gf[s].data) return ASX_ERR_NULL;
+  return ASX_OK;
+}
+
 int orc_fluxes(const asx_patch *pt, const asx_params *par, int face_halo) {
   int st = check_par(par);
   if (st) return st;
   if (!pt) return ASX_ERR_NULL;
+  if ((st = need_uct(pt, par))) return st;
   // a table has no eps(rho, p) (eos_3p_tabulated3d.hxx:249-258)
   if (par->evolution_eos == ASX_EOS_TABULATED && !par->reconstruct_with_temperature)
     return ASX_ERR_UNSUPPORTED;
@@ -835,6 +834,8 @@ int orc_source_terms(const asx_patch *pt, const asx_params *par) {
 int orc_rhs(const asx_patch *pt, const asx_params *par) {
   int st = check_par(par);
   if (st) return st;
+  if (!pt) return ASX_ERR_NULL;
+  if ((st = need_uct(pt, par))) return st;
   const Ctx c = make_ctx(pt, par);
   int halo[3] = {0, 0, 0}, lo[3], hi[3];
   interior(pt, 0, halo, lo, hi);
@@ -1026,6 +1027,7 @@ static C2PDriver make_driver(const asx_params *par) {
   d.prime = par->c2p_prime;
   d.second = par->c2p_second;
   d.excise = par->excise != 0;
+  d.entropy_fix = par->use_entropy_fix != 0;
   d.ap = AtmoParams{par->rho_abs_min, par->atmo_tol, par->p_atmo, par->t_atmo,
                     par->Ye_atmo, par->r_atmo, par->n_rho_atmo, par->n_press_atmo,
                     par->n_temp_atmo, par->thermal_eos_atmo != 0,
@@ -1050,15 +1052,18 @@ static C2PDriver make_driver(const asx_params *par) {
 static int check_c2p_par(const asx_params *par) {
   int st = check_par(par);
   if (st) return st;
-  if (par->use_entropy_fix) return ASX_ERR_UNSUPPORTED;
+  // a table: Palenzuela on the temperature only (the reference's table
+  // refuses eps from (rho, p) and p, eps from (rho, kappa))
   if (par->evolution_eos == ASX_EOS_TABULATED &&
       (par->c2p_prime != ASX_C2P_PALENZUELA || par->c2p_second == ASX_C2P_NOBLE ||
+       par->c2p_second == ASX_C2P_ENTROPY || par->use_entropy_fix ||
        !par->use_temperature || par->use_press_atmo))
     return ASX_ERR_UNSUPPORTED;
-  if (par->c2p_prime != ASX_C2P_NOBLE && par->c2p_prime != ASX_C2P_PALENZUELA)
+  if (par->c2p_prime != ASX_C2P_NOBLE && par->c2p_prime != ASX_C2P_PALENZUELA &&
+      par->c2p_prime != ASX_C2P_ENTROPY)
     return ASX_ERR_UNSUPPORTED;
   if (par->c2p_second != ASX_C2P_NOBLE && par->c2p_second != ASX_C2P_PALENZUELA &&
-      par->c2p_second != ASX_C2P_NONE)
+      par->c2p_second != ASX_C2P_ENTROPY && par->c2p_second != ASX_C2P_NONE)
     return ASX_ERR_UNSUPPORTED;
   if (par->max_iter < 1) return ASX_ERR_PARAM;
   return ASX_OK;

@@ -414,6 +414,14 @@ struct IdealGas {
   double kappa_from_rho_eps_ye(double rho, double &eps, double) const {
     return kappa_from_rho_eps(rho, eps);
   }
+  // eos_3p_idealgas.hxx:201-215: p = kappa rho^gamma,
+  // eps = kappa rho^(gamma-1) / (gamma-1)
+  double press_from_rho_kappa_ye(double rho, double kappa, double) const {
+    return kappa * std::pow(rho, gamma);
+  }
+  double eps_from_rho_kappa_ye(double rho, double kappa, double) const {
+    return kappa * std::pow(rho, gamma - 1.0) / (gamma - 1.0);
+  }
   double csnd_from_rho_eps_ye(double rho, double &eps, double) const {
     return csnd_from_rho_eps(rho, eps);
   }

@@ -35,6 +35,7 @@ CASES = [
     "Alfven_wave",
     "Sound_wave",
     "Magnetic_rotor",
+    "Cylindrical_blast",
 ]
 VARS = ["rho", "vel", "eps", "press", "bvec"]
 

@@ -189,6 +189,35 @@ def test_con2prim(prime, second):
     _cmp(h, d, C2P_OUT, tol=tol)
 
 
+@pytest.mark.parametrize("prime,second,fix", [("Entropy", "None", 0),
+                                              ("Noble", "None", 1),
+                                              ("Palenzuela", "Entropy", 0)])
+def test_con2prim_entropy(prime, second, fix):
+    """The Entropy solver as primary, as backup, and as the entropy fix after
+    both solvers failed (zones whose energy was cut to 1/100): flags
+    bit-exact, primitives to the solver tolerance."""
+    _torch()
+    h, d = _pair(curved=True)
+    for p in (h, d):
+        tau = p.numpy("tau").copy()
+        for k in ((5, 6, 7), (7, 5, 6), (6, 8, 9)):
+            tau[k] *= 1e-2
+        p["tau"] = tau
+    par = abi.params(c2p_prime=prime, c2p_second=second, use_entropy_fix=fix,
+                     gl_gamma=5.0 / 3.0, rho_abs_min=1e-12, rho_max=1e8, eps_max=1e8)
+    cnt = abi.asx_counters()
+    _gpu()._call("con2prim", C.byref(d.struct()), C.byref(par), C.byref(cnt), None)
+    _sync()
+    abi.check("orc_con2prim", oracle.lib().orc_con2prim(C.byref(h.struct()), C.byref(par)))
+    fl_h = h.numpy("con2prim_flag")[h.interior("rho")]
+    fl_d = d.numpy("con2prim_flag")[h.interior("rho")]
+    assert np.array_equal(fl_h, fl_d)
+    if fix:
+        assert (fl_h == 3).sum() == 3
+    assert list(np.bincount(fl_h.astype(int).ravel(), minlength=7)) == list(cnt.c2p_flags)
+    _cmp(h, d, C2P_OUT, tol=1e-9)
+
+
 def test_substep_fused_matches_unfused():
     _torch()
     from asterx_amd.patch import StateBuf

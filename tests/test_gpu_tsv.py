@@ -20,7 +20,7 @@ MULTISTEP_TOL = 1e-10
 GPU_CASES = ["Balsara1_shocktube_xdir_ppm", "Balsara1_shocktube_xdir_wenoz",
              "Balsara1_shocktube_xdir_ppm_PalenzuelaC2P",
              "Balsara3_shocktube_xdir_minmod_LxF", "Balsara1_shocktube_zdir_ppm",
-             "Alfven_wave", "Sound_wave", "Magnetic_rotor"]
+             "Alfven_wave", "Sound_wave", "Magnetic_rotor", "Cylindrical_blast"]
 
 
 @pytest.mark.parametrize("name", GPU_CASES)

@@ -67,6 +67,13 @@ CASES = {
         _B1, kind="rotor", n=(40, 40, 2), lo=(-0.5, -0.5, -0.004),
         hi=(0.5, 0.5, 0.004), ng=3, bc="copy", dtfac=0.35,
         gamma=1.66666666666667, eps_min=1e-6, dirs="xyz"),
+    # AsterX/test/Cylindrical_blast.par: 2-D minmod, upwind CT, z_vec,
+    # entropy fix, neumann, RK4
+    "Cylindrical_blast": dict(
+        _B1, kind="blast", n=(50, 50, 2), lo=(-6.0, -6.0, -0.06),
+        hi=(6.0, 6.0, 0.06), ng=2, bc="copy", recon="minmod", dtfac=0.35,
+        max_iter=1000, rho_abs_min=1e-6, gamma=1.3333333333, eps_min=1e-7,
+        uct=1, recon_type="z_vec", entropy_fix=1, dirs="xyz"),
 }
 
 
@@ -78,6 +85,10 @@ def case_params(c):
         kw["atmo_tol"] = c["atmo_tol"]
     if "eps_min" in c:
         kw["eps_min"] = c["eps_min"]
+    for k, name in (("uct", "use_uct"), ("recon_type", "recon_type"),
+                    ("entropy_fix", "use_entropy_fix")):
+        if k in c:
+            kw[name] = c[k]
     return abi.params(
         reconstruction_method=c["recon"], flux_type=c["flux"],
         c2p_prime=c["prime"], c2p_second=c["second"], c2p_tol=c["tol"],
@@ -100,7 +111,7 @@ def make_patch(c, device=None):
     else:
         n, lo, hi = list(c["n"]), c["lo"], c["hi"]
     dx = [(hi[q] - lo[q]) / n[q] for q in range(3)]
-    p = Patch(n, c["ng"], lo, dx, slots=all_slots(), device=None)
+    p = Patch(n, c["ng"], lo, dx, slots=all_slots(use_uct=bool(c.get("uct"))), device=None)
     seeds.minkowski(p)
     if c["kind"] == "balsara":
         seeds.balsara(p, c["test"], c["shock_dir"], c["gamma"])
@@ -108,6 +119,8 @@ def make_patch(c, device=None):
         seeds.alfven_wave(p, c["gamma"])
     elif c["kind"] == "sound":
         seeds.sound_wave(p, c["gamma"])
+    elif c["kind"] == "blast":
+        seeds.cylindrical_blast(p, c["gamma"])
     else:
         seeds.magnetic_rotor(p, c["gamma"])
     seeds.post_initial(p, c["gamma"])
