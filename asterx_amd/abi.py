@@ -27,9 +27,11 @@ UCT = ["vtilde_y_xface", "vtilde_z_xface", "vtilde_x_yface", "vtilde_z_yface",
 AUX = ["Fx", "Fy", "Fz", "Fbetax", "Fbetay", "Fbetaz", "G"]
 RHS = ["densrhs", "momxrhs", "momyrhs", "momzrhs", "taurhs", "DEntrhs",
        "DYe_rhs", "Avec_x_rhs", "Avec_y_rhs", "Avec_z_rhs", "Psi_rhs"]
+TMUNU = ["eTtt", "eTtx", "eTty", "eTtz", "eTxx", "eTxy", "eTxz", "eTyy",
+         "eTyz", "eTzz"]
 
 SLOTS = ADM + PRIMS + CONS + DB + SAVED + ZS + FLAGS + DBSTAG + AVEC + FLUX + \
-    UCT + AUX + RHS
+    UCT + AUX + RHS + TMUNU
 NUM_GF = len(SLOTS)
 SLOT = {name: i for i, name in enumerate(SLOTS)}
 
@@ -42,7 +44,8 @@ STATE_RHS = ["densrhs", "momxrhs", "momyrhs", "momzrhs", "taurhs", "DEntrhs",
 
 def centering(name):
     """Bit d set = vertex-centred along axis d (AsterX/interface.ccl)."""
-    if name in ADM or name in ("aster_mask_vc", "Psi", "Psi_rhs") or name in AUX:
+    if name in ADM or name in ("aster_mask_vc", "Psi", "Psi_rhs") or name in AUX \
+            or name in TMUNU:
         return 7
     if name in ("dBx_stag",) or name.startswith("fx") or name.endswith("_xface"):
         return 1
@@ -76,7 +79,7 @@ BC = {"linear": 0, "periodic": 1, "copy": 2}
 # kernel classes of asx_timing_read
 KCLASS = {"bstag": 0, "face_x": 1, "face_y": 2, "face_z": 3, "aux": 4,
           "update": 5, "c2p": 6, "bc": 7, "rhs": 8, "other": 9,
-          "c2p_rest": 10, "mlp": 11, "graph": 12}
+          "c2p_rest": 10, "mlp": 11, "graph": 12, "tmunu": 13}
 C2P_FLAG = {"INIT": 0, "PRIME": 1, "SECOND": 2, "ENTROPY": 3, "ATMO": 4,
             "AVG": 5, "FAIL": 6}
 
@@ -166,6 +169,7 @@ _PARAM_FIELDS = [
     ("rho_BH", C.c_double, 1e100), ("eps_BH", C.c_double, 1e100),
     ("vwlim_BH", C.c_double, 1e100), ("cons_error_limit", C.c_double, -1.0),
     ("eos_table", C.c_void_p, None),
+    ("tmunu_interp_order", C.c_int32, 4), ("update_tmunu", C.c_int32, 1),
 ]
 
 _KEYWORDS = {"reconstruction_method": RECON, "loworder_method": RECON,

@@ -45,6 +45,7 @@ int centering(int s) {
   if (s == ASX_AVEC_Y_RHS) return 5;
   if (s == ASX_AVEC_Z_RHS) return 3;
   if (s == ASX_PSI_RHS) return 7;
+  if (s >= ASX_ETTT && s <= ASX_ETZZ) return 7;
   return -1;
 }
 
@@ -582,6 +583,8 @@ void asx_params_default(asx_params *p) {
   p->eps_BH = 1.0e100;
   p->vwlim_BH = 1.0e100;
   p->cons_error_limit = -1;
+  p->tmunu_interp_order = 4;
+  p->update_tmunu = 1;
 }
 
 asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_halo,
@@ -688,6 +691,26 @@ asx_status asx_source_terms(const asx_patch *pt, const asx_params *par, void *st
   const int halo[3] = {0, 0, 0};
   return dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_sources(kpatch(pt), box(pt, 0, halo), rhs_args(par),
                                    (hipStream_t)stream); }));
+}
+
+asx_status asx_tmunu(const asx_patch *pt, const asx_params *par, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (!par) return ASX_ERR_NULL;
+  if (par->tmunu_interp_order != 2 && par->tmunu_interp_order != 4) return ASX_ERR_PARAM;
+  if (!par->update_tmunu) return ASX_OK;
+  if (pt->ng < par->tmunu_interp_order / 2) return ASX_ERR_GHOSTS;
+  const int use_z = par->recon_type != ASX_RECVAR_V;
+  if ((st = need_range(pt, ASX_ALP, ASX_GZZ))) return st;
+  if ((st = need(pt, {ASX_RHO, ASX_EPS, ASX_PRESS, ASX_BVECX, ASX_BVECY, ASX_BVECZ})))
+    return st;
+  if ((st = use_z ? need_range(pt, ASX_ZVEC_X, ASX_ZVEC_Z)
+                  : need(pt, {ASX_VELX, ASX_VELY, ASX_VELZ})))
+    return st;
+  if ((st = need_range(pt, ASX_ETTT, ASX_ETZZ))) return st;
+  return dev_status(timed(ASX_KC_TMUNU, (hipStream_t)stream, [&] {
+    return launch_tmunu(kpatch(pt), par->tmunu_interp_order, use_z, (hipStream_t)stream);
+  }));
 }
 
 asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {

@@ -100,6 +100,9 @@ hipError_t launch_dB(const KPatch &P, const Box &B, hipStream_t s);
 hipError_t launch_B_from_dB(const KPatch &P, const Box &B, hipStream_t s);
 hipError_t launch_prim2con(const KPatch &P, const Box &B, hipStream_t s);
 
+// eT_{mu nu} += T_{mu nu} at the interior vertices (order 2 or 4; use_z:
+// velocities from zvec instead of vel)
+hipError_t launch_tmunu(const KPatch &P, int order, int use_z, hipStream_t s);
 hipError_t launch_sources(const KPatch &P, const Box &B, const RhsArgs &A,
                           hipStream_t s);
 hipError_t launch_rhs(const KPatch &P, const RhsArgs &A, hipStream_t s);

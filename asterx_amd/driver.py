@@ -157,6 +157,10 @@ class Backend:
     def rhs(self, ps, par):
         self._call("rhs", ps_ref(ps), ref(par), *self._s())
 
+    def tmunu(self, ps, par):
+        """eT_{mu nu} += T_{mu nu} at the interior vertices (AsterX_Tmunu)."""
+        self._call("tmunu", ps_ref(ps), ref(par), *self._s())
+
     def bstag_from_A(self, ps):
         self._call("bstag_from_A", ps_ref(ps), *self._s())
 

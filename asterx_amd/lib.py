@@ -22,6 +22,7 @@ SIGNATURES = {
     "asx_fluxes": ([_P, _PP, C.c_int32, _CNT, _V], C.c_int),
     "asx_source_terms": ([_P, _PP, _V], C.c_int),
     "asx_rhs": ([_P, _PP, _V], C.c_int),
+    "asx_tmunu": ([_P, _PP, _V], C.c_int),
     "asx_bstag_from_A": ([_P, _V], C.c_int),
     "asx_dB_from_bstag": ([_P, _V], C.c_int),
     "asx_B_from_dB": ([_P, _V], C.c_int),

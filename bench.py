@@ -489,6 +489,74 @@ def bench_tabulated(args, L, sh, stream):
             "patch": patch_out, "c2p_batch": c2p_out}
 
 
+def tmunu_bytes_per_vertex():
+    """Algorithmic bytes of AsterX_Tmunu per interior vertex: 9 cell
+    primitives (rho, eps, p, B^i, v^i) and 10 vertex metric fields (alpha,
+    beta^i, gamma_ij) read once, the 10 eT components read and written."""
+    return 8 * (9 + 10 + 10 + 10)
+
+
+def bench_tmunu(args, p, par, L, sh, workload):
+    """AsterX_Tmunu on the evolved torus patch (order-4 interpolation):
+    average launch time from HIP events, HBM roofline, and the CPU oracle
+    on a 48^3-cell window of the same data."""
+    import torch
+    from asterx_amd import abi
+    from asterx_amd.patch import Patch
+    for nm in abi.TMUNU:
+        p.alloc(nm)
+    ps = p.struct()
+    for _ in range(2):
+        abi.check("asx_tmunu", L.asx_tmunu(C.byref(ps), C.byref(par), sh))
+    torch.cuda.synchronize()
+    reps = 20
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    for _ in range(reps):
+        abi.check("asx_tmunu", L.asx_tmunu(C.byref(ps), C.byref(par), sh))
+    torch.cuda.synchronize()
+    nl, ms = C.c_int64(), C.c_double()
+    abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["tmunu"], C.byref(nl),
+                                                  C.byref(ms)))
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    avg = ms.value / nl.value
+    nv = (p.n[0] + 1) * (p.n[1] + 1) * (p.n[2] + 1)
+    per = tmunu_bytes_per_vertex() * nv
+    ach = per / (avg * 1e-3) / 1e9
+    ett = p.arr["eTtt"][p.interior("eTtt")]
+    if not bool(torch.isfinite(ett).all()):
+        raise RuntimeError("non-finite eTtt")
+    out = {"vertices": nv, "order": int(par.tmunu_interp_order),
+           "avg_ms": round(avg, 4), "vertices_per_s": round(nv / (avg * 1e-3), 1),
+           "roofline": {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                        "traffic": read_traffic("tmunu", workload),
+                        "bytes_per_launch": int(per)}}
+    for nm in abi.TMUNU:
+        del p.arr[nm]
+    if args.cpu == "on":
+        import oracle
+        m, ng = 48, p.ng
+        hp = Patch((m,) * 3, ng, (0.0,) * 3, p.dx)
+        c0 = p.ng + (p.n[0] - m) // 2
+        for nm in abi.ADM[:10] + ["rho", "eps", "press", "velx", "vely", "velz",
+                                  "Bvecx", "Bvecy", "Bvecz", "zvec_x", "zvec_y",
+                                  "zvec_z"]:
+            e = m + 2 * ng + (abi.centering(nm) & 1)
+            a = p.arr[nm][c0 - ng:c0 - ng + e, c0 - ng:c0 - ng + e, c0 - ng:c0 - ng + e]
+            hp[nm] = a.cpu().numpy()
+        for nm in abi.TMUNU:
+            hp.alloc(nm)
+        hs = hp.struct()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            abi.check("orc_tmunu", oracle.lib().orc_tmunu(C.byref(hs), C.byref(par)))
+            ts.append(time.perf_counter() - t0)
+        out["cpu_oracle_vertices_per_s"] = round((m + 1) ** 3 / min(ts), 1)
+        out["cpu_sample"] = "oracle (OpenMP) on a %d^3-cell window of the torus, best of 3" % m
+    return out
+
+
 def cpu_baseline(host_patch, par, dt, steps):
     """The CPU oracle (OpenMP over k-planes) on the same patch and inputs:
     stage-1 substep + SYNC + B from A + con2prim + SYNC, per substep."""
@@ -675,6 +743,10 @@ def main():
         tab = bench_tabulated(args, L, sh, stream)
         log(args, "tabulated: %.1f s" % (time.perf_counter() - t0), tab)
 
+    t0 = time.perf_counter()
+    tmunu = bench_tmunu(args, p, par, L, sh, workload)
+    log(args, "tmunu: %.1f s" % (time.perf_counter() - t0), tmunu)
+
     cpu = None
     if host is not None:
         ts = cpu_baseline(host, par, dt, args.cpu_steps)
@@ -698,7 +770,8 @@ def main():
                                   (args.n, args.recon, args.flux),
                       "zones": zones, "substep": "SSPRK3 stages cycled"},
            "roofline": roof, "cpu_baseline": cpu,
-           "batched": batched, "c2p_guess": guess, "tabulated": tab}
+           "batched": batched, "c2p_guess": guess, "tabulated": tab,
+           "tmunu": tmunu}
     print(json.dumps(out), flush=True)
 
 

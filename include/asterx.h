@@ -165,6 +165,9 @@ typedef struct {
   double cons_error_limit;  /* -1           */
   /* EOSX (evolution_eos = "Tabulated3d") */
   const struct asx_eos_table_s *eos_table; /* NULL */
+  /* AsterX Tmunu */
+  int32_t tmunu_interp_order; /* 4 (2 or 4) */
+  int32_t update_tmunu;       /* yes        */
 } asx_params;
 
 /* ------------------------------------------------------------------------ */
@@ -228,6 +231,9 @@ enum {
   ASX_DENTRHS, ASX_DYE_RHS,                       /* cell            */
   ASX_AVEC_X_RHS, ASX_AVEC_Y_RHS, ASX_AVEC_Z_RHS, /* {cvv},{vcv},{vvc} */
   ASX_PSI_RHS,                                    /* vertex          */
+  /* TmunuBaseX stress-energy tensor, vertex (accumulated by asx_tmunu) */
+  ASX_ETTT, ASX_ETTX, ASX_ETTY, ASX_ETTZ,
+  ASX_ETXX, ASX_ETXY, ASX_ETXZ, ASX_ETYY, ASX_ETYZ, ASX_ETZZ,
   ASX_NUM_GF
 };
 
@@ -292,7 +298,8 @@ enum {
   ASX_KC_C2P_REST = 10, /* con2prim: zones the primary solver failed */
   ASX_KC_MLP = 11,      /* learned con2prim guess: inference and training */
   ASX_KC_GRAPH = 12,    /* replays of a batch plan's captured step */
-  ASX_KC_COUNT = 13
+  ASX_KC_TMUNU = 13,    /* stress-energy tensor at the vertices */
+  ASX_KC_COUNT = 14
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
@@ -329,6 +336,18 @@ asx_status asx_fluxes(const asx_patch *patch, const asx_params *par,
  * source.cxx:20-168). Writes densrhs..DYe_rhs on interior cells. */
 asx_status asx_source_terms(const asx_patch *patch, const asx_params *par,
                             void *stream);
+
+/* AsterX_Tmunu (AsterX/src/tmunu.cxx:19-171, scheduled in
+ * TmunuBaseX_AddToTmunu when update_tmunu, AsterX/schedule.ccl:253-266):
+ * interpolates rho, eps, press, Bvec and vel (recon_type v_vec) or zvec
+ * (otherwise) from cells to the interior vertices with calc_avg_c2v of order
+ * tmunu_interp_order (AsterUtils/src/aster_interp.hxx:97-148) and ADDS the
+ * lower-index T_{mu nu} of the fluid and the field into eTtt..eTzz (the
+ * caller zeroes them first, as TmunuBaseX does).  Reads alp, beta^i, g_ij at
+ * the vertices and the cell primitives on ghosts (ng >= order/2, else
+ * ASX_ERR_GHOSTS).  tmunu_interp_order other than 2 or 4: ASX_ERR_PARAM (the
+ * reference's CCTK_VERROR).  update_tmunu = no: nothing is done. */
+asx_status asx_tmunu(const asx_patch *patch, const asx_params *par, void *stream);
 
 /* AsterX_RHS (AsterX/src/rhs.cxx:18-253): adds the flux divergence to the
  * hydro RHS, writes Avec_*_rhs (flux-CT or UCT EMF + gauge) and Psi_rhs. */

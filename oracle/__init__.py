@@ -21,6 +21,7 @@ _SIGS = {
     "orc_fluxes": [_P, _PP, C.c_int],
     "orc_source_terms": [_P, _PP],
     "orc_rhs": [_P, _PP],
+    "orc_tmunu": [_P, _PP],
     "orc_bstag_from_A": [_P],
     "orc_bstag_from_A_halo": [_P, C.c_int],
     "orc_dB_from_bstag": [_P],

@@ -6,6 +6,7 @@
 //   orc_fluxes        AsterX/src/fluxes.cxx:38-826 (CalcFlux + CalcAux)
 //   orc_source_terms  AsterX/src/source.cxx:20-185
 //   orc_rhs           AsterX/src/rhs.cxx:18-253
+//   orc_tmunu         AsterX/src/tmunu.cxx:19-176
 //   orc_bstag_from_A  AsterX/src/computeBfromA.cxx:20-65
 //   orc_dB_from_bstag AsterX/src/computeBfromA.cxx:67-85
 //   orc_B_from_dB     AsterX/src/computeBfromA.cxx:87-113
@@ -475,6 +476,96 @@ void calc_aux_vertex(const Ctx &c, const int I[3]) {
   v.g(ASX_G).at(I) = alp * psi / sqrtg - dot3(beta, A);
 }
 
+// calc_avg_c2v<2>, AsterUtils/src/aster_interp.hxx:97-111: the mean of the 8
+// cells around vertex I (cells I - {0,1} on each axis).
+double avg_c2v2(const GF &g, const int I[3]) {
+  double s = 0.0;
+  for (int dk = 0; dk < 2; ++dk)
+    for (int dj = 0; dj < 2; ++dj)
+      for (int di = 0; di < 2; ++di) s += g(I[0] - di, I[1] - dj, I[2] - dk);
+  return s / 8.0;
+}
+// calc_avg_c2v<4>, AsterUtils/src/aster_interp.hxx:113-148: the tensor
+// product of the 4-point weights (-1, 9, 9, -1)/16 on cells I-2..I+1,
+// evaluated in each of the three cyclic axis orders (x y z, y z x, z x y)
+// and averaged.
+double avg_c2v4(const GF &g, const int I[3]) {
+  static const double wt[4] = {-1.0 / 16.0, 9.0 / 16.0, 9.0 / 16.0, -1.0 / 16.0};
+  double avg = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    const int b = (a + 1) % 3, c = (a + 2) % 3;
+    double gi[4][4] = {};
+    for (int dk = 0; dk < 4; ++dk)
+      for (int dj = 0; dj < 4; ++dj)
+        for (int di = 0; di < 4; ++di) {
+          int O[3] = {I[0], I[1], I[2]};
+          O[a] += di - 2;
+          O[b] += dj - 2;
+          O[c] += dk - 2;
+          gi[dk][dj] += wt[di] * g.at(O);
+        }
+    double gj[4] = {};
+    for (int dk = 0; dk < 4; ++dk)
+      for (int dj = 0; dj < 4; ++dj) gj[dk] += wt[dj] * gi[dk][dj];
+    for (int dk = 0; dk < 4; ++dk) avg += wt[dk] * gj[dk];
+  }
+  return avg / 3.0;
+}
+
+// Tmunu<interp_order>, AsterX/src/tmunu.cxx:44-170: primitives interpolated
+// to the vertex, lower-index T_{mu nu} of fluid + field added into eT*.
+void tmunu_vertex(const Ctx &c, const int I[3]) {
+  const View &v = c.v;
+  const int order = c.par.tmunu_interp_order;
+  auto avg = [&](int s) { return order == 2 ? avg_c2v2(v.g(s), I) : avg_c2v4(v.g(s), I); };
+  const double rho = avg(ASX_RHO), eps = avg(ASX_EPS), press = avg(ASX_PRESS);
+  const double B[3] = {avg(ASX_BVECX), avg(ASX_BVECY), avg(ASX_BVECZ)};
+  const Sym3 g{v.g(ASX_GXX).at(I), v.g(ASX_GXY).at(I), v.g(ASX_GXZ).at(I),
+               v.g(ASX_GYY).at(I), v.g(ASX_GYZ).at(I), v.g(ASX_GZZ).at(I)};
+  const double betau[3] = {v.g(ASX_BETAX).at(I), v.g(ASX_BETAY).at(I),
+                           v.g(ASX_BETAZ).at(I)};
+  const double alp = v.g(ASX_ALP).at(I);
+  double betal[3];
+  lower(g, betau, betal);
+  const double beta2 = dot3(betal, betau);
+  double vu[3], vl[3], w;
+  if (c.par.recon_type == ASX_RECVAR_V) { // tmunu.cxx:71-82
+    for (int d = 0; d < 3; ++d) vu[d] = avg(ASX_VELX + d);
+    lower(g, vu, vl);
+    w = 1.0 / std::sqrt(1.0 - dot3(vu, vl));
+  } else { // tmunu.cxx:84-95
+    double zu[3], zl[3];
+    for (int d = 0; d < 3; ++d) zu[d] = avg(ASX_ZVEC_X + d);
+    lower(g, zu, zl);
+    w = std::sqrt(1.0 + dot3(zu, zl));
+    for (int d = 0; d < 3; ++d) {
+      vu[d] = zu[d] / w;
+      vl[d] = zl[d] / w;
+    }
+  }
+  const double rhw2 = (rho * (1.0 + eps) + press) * w * w;
+  const double utl = -alp + dot3(betal, vu);
+  double uiu[3], bsu[3], gb[3], bsl[3];
+  for (int d = 0; d < 3; ++d) uiu[d] = vu[d] - betau[d] / alp;
+  const double btu = w * dot3(B, vl) / alp;
+  for (int d = 0; d < 3; ++d) bsu[d] = (B[d] + alp * btu * w * uiu[d]) / w;
+  const double btl = btu * (-alp * alp + beta2) + dot3(bsu, betal);
+  lower(g, bsu, gb);
+  for (int d = 0; d < 3; ++d) bsl[d] = btu * betal[d] + gb[d];
+  const double b2 = btu * btl + dot3(bsu, bsl);
+  const double g00 = -alp * alp + beta2;
+  v.g(ASX_ETTT).at(I) += rhw2 * utl * utl + press * g00 +
+                         (w * w * utl * utl + 0.5 * g00) * b2 - btl * btl;
+  for (int d = 0; d < 3; ++d)
+    v.g(ASX_ETTX + d).at(I) += rhw2 * utl * vl[d] + press * betal[d] +
+                               (w * w * utl * vl[d] + 0.5 * betal[d]) * b2 - btl * bsl[d];
+  int s = ASX_ETXX;
+  for (int a = 0; a < 3; ++a)
+    for (int b = a; b < 3; ++b, ++s)
+      v.g(s).at(I) += rhw2 * vl[a] * vl[b] + press * g(a, b) +
+                      (w * w * vl[a] * vl[b] + 0.5 * g(a, b)) * b2 - bsl[a] * bsl[b];
+}
+
 // Loop over a box of points [lo, hi] (inclusive) in k, j, i order.
 template <typename F>
 void loop_box(const int lo[3], const int hi[3], const F &f) {
@@ -768,6 +859,7 @@ extern "C" int asx_gf_centering(int s) {
   if (s == ASX_AVEC_Y_RHS) return 5;
   if (s == ASX_AVEC_Z_RHS) return 3;
   if (s == ASX_PSI_RHS) return 7;
+  if (s >= ASX_ETTT && s <= ASX_ETZZ) return 7;
   return -1;
 }
 
@@ -828,6 +920,18 @@ int orc_source_terms(const asx_patch *pt, const asx_params *par) {
   int halo[3] = {0, 0, 0}, lo[3], hi[3];
   interior(pt, 0, halo, lo, hi);
   loop_box(lo, hi, [&](const int I[3]) { source_cell(c, I); });
+  return ASX_OK;
+}
+
+int orc_tmunu(const asx_patch *pt, const asx_params *par) {
+  if (!pt || !par) return ASX_ERR_NULL;
+  if (par->tmunu_interp_order != 2 && par->tmunu_interp_order != 4) return ASX_ERR_PARAM;
+  if (!par->update_tmunu) return ASX_OK;
+  if (pt->ng < par->tmunu_interp_order / 2) return ASX_ERR_GHOSTS;
+  const Ctx c = make_ctx(pt, par);
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 7, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) { tmunu_vertex(c, I); });
   return ASX_OK;
 }
 
