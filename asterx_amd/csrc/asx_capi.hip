@@ -387,6 +387,7 @@ std::map<int, std::array<int, ASX_KC_COUNT>> g_var;
 int variant_count(int kc) {
   if (kc == ASX_KC_FACE_X || kc == ASX_KC_FACE_Y || kc == ASX_KC_FACE_Z) return kFaceVariants;
   if (kc == ASX_KC_UPDATE) return kUpdateVariants;
+  if (kc == ASX_KC_TMUNU) return kTmunuVariants;
   return 1;
 }
 int variant_of(int kc) {
@@ -708,8 +709,11 @@ asx_status asx_tmunu(const asx_patch *pt, const asx_params *par, void *stream) {
                   : need(pt, {ASX_VELX, ASX_VELY, ASX_VELZ})))
     return st;
   if ((st = need_range(pt, ASX_ETTT, ASX_ETZZ))) return st;
+  // the kernel addresses every grid function with 32-bit byte offsets
+  if (npoints(pt->n, pt->ng, 7) * 8 >= (int64_t(1) << 32)) return ASX_ERR_SIZE;
+  const int v = variant_of(ASX_KC_TMUNU);
   return dev_status(timed(ASX_KC_TMUNU, (hipStream_t)stream, [&] {
-    return launch_tmunu(kpatch(pt), par->tmunu_interp_order, use_z, (hipStream_t)stream);
+    return launch_tmunu(kpatch(pt), par->tmunu_interp_order, use_z, v, (hipStream_t)stream);
   }));
 }
 

@@ -57,6 +57,7 @@ template <int TX, int TY, int TZ = 1> struct TileGrid {
 // asx_set_variant / asx_autotune_substep in the C ABI).
 constexpr int kUpdateVariants = 5;
 constexpr int kFaceVariants = 5;
+constexpr int kTmunuVariants = 5;
 
 // Batched launches over many patches: the workgroups of one launch are
 // listed as (patch, chunk) pairs, chunk = tile index or 256-point block of
@@ -101,8 +102,9 @@ hipError_t launch_B_from_dB(const KPatch &P, const Box &B, hipStream_t s);
 hipError_t launch_prim2con(const KPatch &P, const Box &B, hipStream_t s);
 
 // eT_{mu nu} += T_{mu nu} at the interior vertices (order 2 or 4; use_z:
-// velocities from zvec instead of vel)
-hipError_t launch_tmunu(const KPatch &P, int order, int use_z, hipStream_t s);
+// velocities from zvec instead of vel; variant: tile shape and wave budget,
+// 0 .. kTmunuVariants-1).  The cell grid functions must span < 4 GiB each.
+hipError_t launch_tmunu(const KPatch &P, int order, int use_z, int variant, hipStream_t s);
 hipError_t launch_sources(const KPatch &P, const Box &B, const RhsArgs &A,
                           hipStream_t s);
 hipError_t launch_rhs(const KPatch &P, const RhsArgs &A, hipStream_t s);

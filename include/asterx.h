@@ -346,7 +346,8 @@ asx_status asx_source_terms(const asx_patch *patch, const asx_params *par,
  * caller zeroes them first, as TmunuBaseX does).  Reads alp, beta^i, g_ij at
  * the vertices and the cell primitives on ghosts (ng >= order/2, else
  * ASX_ERR_GHOSTS).  tmunu_interp_order other than 2 or 4: ASX_ERR_PARAM (the
- * reference's CCTK_VERROR).  update_tmunu = no: nothing is done. */
+ * reference's CCTK_VERROR).  update_tmunu = no: nothing is done.  Grid functions
+ * of 4 GiB or more (vertex extent): ASX_ERR_SIZE. */
 asx_status asx_tmunu(const asx_patch *patch, const asx_params *par, void *stream);
 
 /* AsterX_RHS (AsterX/src/rhs.cxx:18-253): adds the flux divergence to the

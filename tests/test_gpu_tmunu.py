@@ -38,15 +38,28 @@ def _with_eT(h, seed=11):
     return h
 
 
+def _variants():
+    from asterx_amd import lib
+    return range(lib.lib().asx_variant_count(abi.KCLASS["tmunu"]))
+
+
 @pytest.mark.parametrize("order", [2, 4])
 @pytest.mark.parametrize("rec", ["v_vec", "z_vec"])
-def test_tmunu_parity(order, rec):
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_tmunu_parity(order, rec, variant):
     torch = _torch()
+    if variant not in _variants():
+        pytest.skip("variant not built")
     h = _with_eT(make_state(curved=True))
     d = h.to("cuda")
     par = abi.params(tmunu_interp_order=order, recon_type=rec)
-    abi.check("asx_tmunu", _lib().asx_tmunu(C.byref(d.struct()), C.byref(par), None))
-    torch.cuda.synchronize()
+    L = _lib()
+    abi.check("asx_set_variant", L.asx_set_variant(abi.KCLASS["tmunu"], variant))
+    try:
+        abi.check("asx_tmunu", L.asx_tmunu(C.byref(d.struct()), C.byref(par), None))
+        torch.cuda.synchronize()
+    finally:
+        abi.check("asx_set_variant", L.asx_set_variant(abi.KCLASS["tmunu"], 0))
     before = {n: h.numpy(n).copy() for n in abi.TMUNU}
     abi.check("orc_tmunu", oracle.lib().orc_tmunu(C.byref(h.struct()), C.byref(par)))
     bad = {}
