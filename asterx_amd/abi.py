@@ -31,7 +31,7 @@ TMUNU = ["eTtt", "eTtx", "eTty", "eTtz", "eTxx", "eTxy", "eTxz", "eTyy",
          "eTyz", "eTzz"]
 
 SLOTS = ADM + PRIMS + CONS + DB + SAVED + ZS + FLAGS + DBSTAG + AVEC + FLUX + \
-    UCT + AUX + RHS + TMUNU
+    UCT + AUX + RHS + TMUNU + ["regrid_error"]
 NUM_GF = len(SLOTS)
 SLOT = {name: i for i, name in enumerate(SLOTS)}
 
@@ -62,6 +62,33 @@ def centering(name):
     return 0
 
 
+# Cactus group -> slots, for the regrid_groups parameter (AsterX/param.ccl:52)
+GROUPS = {
+    "HydroBaseX::rho": ["rho"], "HydroBaseX::vel": ["velx", "vely", "velz"],
+    "HydroBaseX::eps": ["eps"], "HydroBaseX::press": ["press"],
+    "HydroBaseX::temperature": ["temperature"], "HydroBaseX::entropy": ["entropy"],
+    "HydroBaseX::Ye": ["Ye"], "HydroBaseX::Bvec": ["Bvecx", "Bvecy", "Bvecz"],
+    "AsterX::dens": ["dens"], "AsterX::mom": ["momx", "momy", "momz"],
+    "AsterX::tau": ["tau"], "AsterX::DEnt": ["DEnt"], "AsterX::DYe": ["DYe"],
+    "AsterX::dB": ["dBx", "dBy", "dBz"], "AsterX::Avec_x": ["Avec_x"],
+    "AsterX::Avec_y": ["Avec_y"], "AsterX::Avec_z": ["Avec_z"], "AsterX::Psi": ["Psi"],
+    "ADMBaseX::lapse": ["alp"], "ADMBaseX::shift": ["betax", "betay", "betaz"],
+    "ADMBaseX::metric": ["gxx", "gxy", "gxz", "gyy", "gyz", "gzz"],
+    "ADMBaseX::curv": ["kxx", "kxy", "kxz", "kyy", "kyz", "kzz"],
+}
+
+
+def regrid_slots(groups="HydroBaseX::rho"):
+    """Slot indices of the variables of a whitespace-separated list of full
+    group names, in order (AsterX_EstimateError_Setup,
+    AsterX/src/estimate_error.cxx:26-70, which asserts on an unknown group:
+    KeyError here)."""
+    out = []
+    for g in groups.split():
+        out += [SLOT[n] for n in GROUPS[g]]
+    return out
+
+
 def extents(n, ng, cent):
     """(ex, ey, ez) of a centering on a patch."""
     return tuple(n[d] + 2 * ng + ((cent >> d) & 1) for d in range(3))
@@ -75,11 +102,13 @@ RECVAR = {"v_vec": 0, "z_vec": 1, "s_vec": 2}
 GAUGE = {"algebraic": 0, "generalized Lorentz": 1}
 EOS = {"IdealGas": 0, "Hybrid": 1, "Tabulated3d": 2}
 C2P = {"None": 0, "Noble": 1, "Palenzuela": 2, "Entropy": 3}
+REGRID = {"first derivative": 0, "second derivative": 1,
+          "second derivative norm": 2, "first gradient": 3}
 BC = {"linear": 0, "periodic": 1, "copy": 2}
 # kernel classes of asx_timing_read
 KCLASS = {"bstag": 0, "face_x": 1, "face_y": 2, "face_z": 3, "aux": 4,
           "update": 5, "c2p": 6, "bc": 7, "rhs": 8, "other": 9,
-          "c2p_rest": 10, "mlp": 11, "graph": 12, "tmunu": 13}
+          "c2p_rest": 10, "mlp": 11, "graph": 12, "tmunu": 13, "regrid": 14}
 C2P_FLAG = {"INIT": 0, "PRIME": 1, "SECOND": 2, "ENTROPY": 3, "ATMO": 4,
             "AVG": 5, "FAIL": 6}
 
@@ -170,12 +199,14 @@ _PARAM_FIELDS = [
     ("vwlim_BH", C.c_double, 1e100), ("cons_error_limit", C.c_double, -1.0),
     ("eos_table", C.c_void_p, None),
     ("tmunu_interp_order", C.c_int32, 4), ("update_tmunu", C.c_int32, 1),
+    ("local_estimate_error", C.c_int32, 1), ("regrid_method", C.c_int32, "first gradient"),
+    ("epsilon_err", C.c_double, 0.01),
 ]
 
 _KEYWORDS = {"reconstruction_method": RECON, "loworder_method": RECON,
              "flux_type": FLUXTYPE, "recon_type": RECVAR,
              "vector_potential_gauge": GAUGE, "evolution_eos": EOS,
-             "c2p_prime": C2P, "c2p_second": C2P}
+             "c2p_prime": C2P, "c2p_second": C2P, "regrid_method": REGRID}
 
 
 class asx_params(C.Structure):

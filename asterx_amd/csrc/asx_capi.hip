@@ -46,6 +46,7 @@ int centering(int s) {
   if (s == ASX_AVEC_Z_RHS) return 3;
   if (s == ASX_PSI_RHS) return 7;
   if (s >= ASX_ETTT && s <= ASX_ETZZ) return 7;
+  if (s == ASX_REGRID_ERROR) return 0;
   return -1;
 }
 
@@ -586,6 +587,9 @@ void asx_params_default(asx_params *p) {
   p->cons_error_limit = -1;
   p->tmunu_interp_order = 4;
   p->update_tmunu = 1;
+  p->local_estimate_error = 1;
+  p->regrid_method = ASX_REGRID_FIRST_GRAD;
+  p->epsilon_err = 0.01;
 }
 
 asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_halo,
@@ -714,6 +718,33 @@ asx_status asx_tmunu(const asx_patch *pt, const asx_params *par, void *stream) {
   const int v = variant_of(ASX_KC_TMUNU);
   return dev_status(timed(ASX_KC_TMUNU, (hipStream_t)stream, [&] {
     return launch_tmunu(kpatch(pt), par->tmunu_interp_order, use_z, v, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_estimate_error(const asx_patch *pt, const asx_params *par, int32_t nvars,
+                              const int32_t *slots, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (!par) return ASX_ERR_NULL;
+  if (par->regrid_method < ASX_REGRID_FIRST_DERIV || par->regrid_method > ASX_REGRID_FIRST_GRAD)
+    return ASX_ERR_PARAM;
+  if (!par->local_estimate_error) return ASX_OK;
+  if (nvars < 0 || nvars > kMaxRegridVars) return ASX_ERR_SIZE;
+  if (nvars > 0 && !slots) return ASX_ERR_NULL;
+  RegridArgs A{};
+  A.nv = nvars;
+  A.method = par->regrid_method;
+  A.eps = par->epsilon_err;
+  for (int q = 0; q < nvars; ++q) {
+    if (slots[q] < 0 || slots[q] >= ASX_NUM_GF || centering(slots[q]) < 0) return ASX_ERR_PARAM;
+    if ((st = need(pt, {slots[q]}))) return st;
+    A.slot[q] = slots[q];
+    A.cent[q] = centering(slots[q]);
+  }
+  if ((st = need(pt, {ASX_REGRID_ERROR}))) return st;
+  const int halo[3] = {0, 0, 0};
+  return dev_status(timed(ASX_KC_REGRID, (hipStream_t)stream, [&] {
+    return launch_regrid(kpatch(pt), box(pt, 0, halo), A, (hipStream_t)stream);
   }));
 }
 

@@ -56,6 +56,7 @@ ASX_DEV constexpr int cent(int s) {
          : (s == ASX_AVEC_X_RHS)    ? 6
          : (s == ASX_AVEC_Y_RHS)    ? 5
          : (s == ASX_AVEC_Z_RHS)    ? 3
+         : (s == ASX_REGRID_ERROR)  ? 0
                                     : 7;
 }
 

@@ -105,6 +105,14 @@ hipError_t launch_prim2con(const KPatch &P, const Box &B, hipStream_t s);
 // velocities from zvec instead of vel; variant: tile shape and wave budget,
 // 0 .. kTmunuVariants-1).  The cell grid functions must span < 4 GiB each.
 hipError_t launch_tmunu(const KPatch &P, int order, int use_z, int variant, hipStream_t s);
+// regrid_error on the cells of B: max over nv grid functions (slot, centering)
+constexpr int kMaxRegridVars = 9;
+struct RegridArgs {
+  int nv, method;
+  double eps;
+  int slot[kMaxRegridVars], cent[kMaxRegridVars];
+};
+hipError_t launch_regrid(const KPatch &P, const Box &B, const RegridArgs &A, hipStream_t s);
 hipError_t launch_sources(const KPatch &P, const Box &B, const RhsArgs &A,
                           hipStream_t s);
 hipError_t launch_rhs(const KPatch &P, const RhsArgs &A, hipStream_t s);

@@ -23,6 +23,7 @@ SIGNATURES = {
     "asx_source_terms": ([_P, _PP, _V], C.c_int),
     "asx_rhs": ([_P, _PP, _V], C.c_int),
     "asx_tmunu": ([_P, _PP, _V], C.c_int),
+    "asx_estimate_error": ([_P, _PP, C.c_int32, C.c_void_p, _V], C.c_int),
     "asx_bstag_from_A": ([_P, _V], C.c_int),
     "asx_dB_from_bstag": ([_P, _V], C.c_int),
     "asx_B_from_dB": ([_P, _V], C.c_int),

@@ -557,6 +557,36 @@ def bench_tmunu(args, p, par, L, sh, workload):
     return out
 
 
+def bench_regrid(p, par, L, sh):
+    """AsterX_EstimateError (first gradient of rho, the reference's default)
+    on the evolved torus: average launch time from HIP events and the HBM
+    roofline at 16 B per cell (rho read once, regrid_error written)."""
+    import torch
+    from asterx_amd import abi
+    p.alloc("regrid_error")
+    ps = p.struct()
+    slots = (C.c_int32 * 1)(*abi.regrid_slots("HydroBaseX::rho"))
+    for _ in range(2):
+        abi.check("asx_estimate_error", L.asx_estimate_error(C.byref(ps), C.byref(par), 1,
+                                                             slots, sh))
+    torch.cuda.synchronize()
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    for _ in range(20):
+        abi.check("asx_estimate_error", L.asx_estimate_error(C.byref(ps), C.byref(par), 1,
+                                                             slots, sh))
+    torch.cuda.synchronize()
+    nl, ms = C.c_int64(), C.c_double()
+    abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["regrid"], C.byref(nl),
+                                                  C.byref(ms)))
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    del p.arr["regrid_error"]
+    avg = ms.value / nl.value
+    cells = p.n[0] * p.n[1] * p.n[2]
+    ach = 16 * cells / (avg * 1e-3) / 1e9
+    return {"method": "first gradient", "vars": 1, "avg_ms": round(avg, 4),
+            "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+
+
 def cpu_baseline(host_patch, par, dt, steps):
     """The CPU oracle (OpenMP over k-planes) on the same patch and inputs:
     stage-1 substep + SYNC + B from A + con2prim + SYNC, per substep."""
@@ -746,6 +776,8 @@ def main():
     t0 = time.perf_counter()
     tmunu = bench_tmunu(args, p, par, L, sh, workload)
     log(args, "tmunu: %.1f s" % (time.perf_counter() - t0), tmunu)
+    regrid = bench_regrid(p, par, L, sh)
+    log(args, "regrid:", regrid)
 
     cpu = None
     if host is not None:
@@ -771,7 +803,7 @@ def main():
                       "zones": zones, "substep": "SSPRK3 stages cycled"},
            "roofline": roof, "cpu_baseline": cpu,
            "batched": batched, "c2p_guess": guess, "tabulated": tab,
-           "tmunu": tmunu}
+           "tmunu": tmunu, "regrid_error": regrid}
     print(json.dumps(out), flush=True)
 
 

@@ -71,6 +71,14 @@ enum {
   ASX_C2P_ENTROPY = 3
 };
 
+/* AsterX/param.ccl:39-45 regrid_method */
+enum {
+  ASX_REGRID_FIRST_DERIV = 0,       /* "first derivative"       */
+  ASX_REGRID_SECOND_DERIV = 1,      /* "second derivative"      */
+  ASX_REGRID_SECOND_DERIV_NORM = 2, /* "second derivative norm" */
+  ASX_REGRID_FIRST_GRAD = 3         /* "first gradient"         */
+};
+
 /* AsterX/src/con2prim.cxx:27-35 -- written to con2prim_flag as FP64 */
 enum {
   ASX_C2P_FLAG_INIT = 0,
@@ -168,6 +176,10 @@ typedef struct {
   /* AsterX Tmunu */
   int32_t tmunu_interp_order; /* 4 (2 or 4) */
   int32_t update_tmunu;       /* yes        */
+  /* AsterX regrid error estimate */
+  int32_t local_estimate_error; /* yes              */
+  int32_t regrid_method;        /* "first gradient" */
+  double epsilon_err;           /* 0.01             */
 } asx_params;
 
 /* ------------------------------------------------------------------------ */
@@ -234,6 +246,8 @@ enum {
   /* TmunuBaseX stress-energy tensor, vertex (accumulated by asx_tmunu) */
   ASX_ETTT, ASX_ETTX, ASX_ETTY, ASX_ETTZ,
   ASX_ETXX, ASX_ETXY, ASX_ETXZ, ASX_ETYY, ASX_ETYZ, ASX_ETZZ,
+  /* CarpetXRegrid::regrid_error, cell (written by asx_estimate_error) */
+  ASX_REGRID_ERROR,
   ASX_NUM_GF
 };
 
@@ -299,7 +313,8 @@ enum {
   ASX_KC_MLP = 11,      /* learned con2prim guess: inference and training */
   ASX_KC_GRAPH = 12,    /* replays of a batch plan's captured step */
   ASX_KC_TMUNU = 13,    /* stress-energy tensor at the vertices */
-  ASX_KC_COUNT = 14
+  ASX_KC_REGRID = 14,   /* regrid error estimate */
+  ASX_KC_COUNT = 15
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
@@ -349,6 +364,20 @@ asx_status asx_source_terms(const asx_patch *patch, const asx_params *par,
  * reference's CCTK_VERROR).  update_tmunu = no: nothing is done.  Grid functions
  * of 4 GiB or more (vertex extent): ASX_ERR_SIZE. */
 asx_status asx_tmunu(const asx_patch *patch, const asx_params *par, void *stream);
+
+/* AsterX_EstimateError (AsterX/src/estimate_error.cxx:72-142, estimators
+ * estimate_error.hxx:14-100), scheduled IN ODESolvers_EstimateError when
+ * local_estimate_error: writes regrid_error on the interior cells as the
+ * maximum over the nvars grid functions slots[] of the regrid_method
+ * estimator (differences to the +-1 neighbours on each axis; epsilon_err for
+ * "second derivative norm").  slots[] replaces the regrid_groups string: the
+ * variables of those groups, each read at the cell's index (i,j,k) in its own
+ * centering's layout, as the reference does.  nvars = 0 writes 0; nvars >= 10
+ * (the reference's assert) or a slot out of range: ASX_ERR_SIZE / ASX_ERR_PARAM;
+ * an unknown regrid_method: ASX_ERR_PARAM; local_estimate_error = no: nothing
+ * is done. */
+asx_status asx_estimate_error(const asx_patch *patch, const asx_params *par, int32_t nvars,
+                              const int32_t *slots, void *stream);
 
 /* AsterX_RHS (AsterX/src/rhs.cxx:18-253): adds the flux divergence to the
  * hydro RHS, writes Avec_*_rhs (flux-CT or UCT EMF + gauge) and Psi_rhs. */

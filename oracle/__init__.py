@@ -22,6 +22,7 @@ _SIGS = {
     "orc_source_terms": [_P, _PP],
     "orc_rhs": [_P, _PP],
     "orc_tmunu": [_P, _PP],
+    "orc_estimate_error": [_P, _PP, C.c_int, C.c_void_p],
     "orc_bstag_from_A": [_P],
     "orc_bstag_from_A_halo": [_P, C.c_int],
     "orc_dB_from_bstag": [_P],

@@ -7,6 +7,7 @@
 //   orc_source_terms  AsterX/src/source.cxx:20-185
 //   orc_rhs           AsterX/src/rhs.cxx:18-253
 //   orc_tmunu         AsterX/src/tmunu.cxx:19-176
+//   orc_estimate_error AsterX/src/estimate_error.cxx:72-142
 //   orc_bstag_from_A  AsterX/src/computeBfromA.cxx:20-65
 //   orc_dB_from_bstag AsterX/src/computeBfromA.cxx:67-85
 //   orc_B_from_dB     AsterX/src/computeBfromA.cxx:87-113
@@ -566,6 +567,43 @@ void tmunu_vertex(const Ctx &c, const int I[3]) {
                       (w * w * vl[a] * vl[b] + 0.5 * g(a, b)) * b2 - bsl[a] * bsl[b];
 }
 
+// The four regrid estimators of AsterX/src/estimate_error.hxx:14-100 for one
+// grid function at point I of its own layout (std::max of the reference:
+// the first argument is kept unless it is less than the next).
+double regrid_estimate(const GF &g, const int I[3], int method, double eps) {
+  auto mx = [](double a, double b) { return std::max({a, b}); };
+  const double var0 = g.at(I);
+  double err = 0.0, errp = 0.0, errm = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    int Im[3], Ip[3];
+    sh(I, d, -1, Im);
+    sh(I, d, 1, Ip);
+    const double varm = g.at(Im), varp = g.at(Ip);
+    switch (method) {
+    case ASX_REGRID_FIRST_DERIV: // calc_deriv_1st, :31-42
+      err = std::max({err, std::fabs(var0 - varm), std::fabs(varp - var0)});
+      break;
+    case ASX_REGRID_SECOND_DERIV: // calc_deriv_2nd, :45-56
+      err = mx(err, std::fabs(varp + varm - 2 * var0));
+      break;
+    case ASX_REGRID_SECOND_DERIV_NORM: { // calc_deriv_2nd_norm, :60-83
+      const double diffp0 = varp - var0, diff0m = var0 - varm;
+      const double varsigma = std::fabs(varp) + 2 * std::fabs(var0) + std::fabs(varm);
+      const double diffpm2 = std::fabs(diffp0 - diff0m) * std::fabs(diffp0 - diff0m);
+      const double denom = (std::fabs(diffp0) + std::fabs(diff0m) + eps * varsigma) *
+                           (std::fabs(diffp0) + std::fabs(diff0m) + eps * varsigma);
+      err = mx(err, std::sqrt(diffpm2 / denom));
+      break;
+    }
+    default: // calc_grad_1st, :14-28
+      errp += (varp - var0) * (varp - var0);
+      errm += (var0 - varm) * (var0 - varm);
+    }
+  }
+  if (method == ASX_REGRID_FIRST_GRAD) return std::sqrt(mx(errp, errm));
+  return err;
+}
+
 // Loop over a box of points [lo, hi] (inclusive) in k, j, i order.
 template <typename F>
 void loop_box(const int lo[3], const int hi[3], const F &f) {
@@ -860,6 +898,7 @@ extern "C" int asx_gf_centering(int s) {
   if (s == ASX_AVEC_Z_RHS) return 3;
   if (s == ASX_PSI_RHS) return 7;
   if (s >= ASX_ETTT && s <= ASX_ETZZ) return 7;
+  if (s == ASX_REGRID_ERROR) return 0;
   return -1;
 }
 
@@ -932,6 +971,32 @@ int orc_tmunu(const asx_patch *pt, const asx_params *par) {
   int halo[3] = {0, 0, 0}, lo[3], hi[3];
   interior(pt, 7, halo, lo, hi);
   loop_box(lo, hi, [&](const int I[3]) { tmunu_vertex(c, I); });
+  return ASX_OK;
+}
+
+// AsterX_EstimateError, AsterX/src/estimate_error.cxx:72-142: regrid_error
+// on the interior cells, the max over the listed grid functions.
+int orc_estimate_error(const asx_patch *pt, const asx_params *par, int nvars,
+                       const int *slots) {
+  if (!pt || !par) return ASX_ERR_NULL;
+  if (par->regrid_method < ASX_REGRID_FIRST_DERIV || par->regrid_method > ASX_REGRID_FIRST_GRAD)
+    return ASX_ERR_PARAM;
+  if (!par->local_estimate_error) return ASX_OK;
+  if (nvars < 0 || nvars >= 10) return ASX_ERR_SIZE;
+  for (int q = 0; q < nvars; ++q) {
+    if (slots[q] < 0 || slots[q] >= ASX_NUM_GF) return ASX_ERR_PARAM;
+    if (!pt->gf[slots[q]].data) return ASX_ERR_NULL;
+  }
+  if (!pt->gf[ASX_REGRID_ERROR].data) return ASX_ERR_NULL;
+  const View v{pt};
+  int halo[3] = {0, 0, 0}, lo[3], hi[3];
+  interior(pt, 0, halo, lo, hi);
+  loop_box(lo, hi, [&](const int I[3]) {
+    double e = 0.0;
+    for (int q = 0; q < nvars; ++q)
+      e = std::max({e, regrid_estimate(v.g(slots[q]), I, par->regrid_method, par->epsilon_err)});
+    v.g(ASX_REGRID_ERROR).at(I) = e;
+  });
   return ASX_OK;
 }
 
