@@ -70,8 +70,23 @@ def convert(case, outdir):
     return path
 
 
+# One cut file kept verbatim, for the text format of the TSV writer.
+FORMAT_FILE = ("Balsara1_shocktube_xdir_ppm", "hydrobasex-rho.it000000.x.tsv")
+
+
+def copy_format_file(outdir):
+    case, name = FORMAT_FILE
+    with open(os.path.join(REF, case, name)) as f:
+        txt = f.read()
+    path = os.path.join(outdir, "%s.%s" % (case, name))
+    with open(path, "w") as f:
+        f.write(txt)
+    return path
+
+
 def main():
     outdir = os.path.dirname(os.path.abspath(__file__))
+    print(copy_format_file(outdir))
     cases = sys.argv[1:] or CASES
     for c in cases:
         print(convert(c, outdir))
