@@ -53,9 +53,42 @@ template <int TX, int TY, int TZ = 1> struct TileGrid {
   }
 };
 
+// The same tiles visited in bands of YS tile rows: x fastest, then the YS
+// rows of the band, then z, then the next band.  Tiles sharing a z plane of
+// vertices or z faces are then YS * ntx launches apart instead of a whole
+// x-y plane of tiles, close enough to meet in L2.
+template <int TX, int TY, int TZ, int YS> struct TileGridBand {
+  static constexpr int kThreads = TX * TY * TZ;
+  int ntx, nty, ntz;
+  __host__ static TileGridBand make(const Box &B) {
+    return TileGridBand{(B.cnt[0] + TX - 1) / TX, (B.cnt[1] + TY - 1) / TY,
+                        (B.cnt[2] + TZ - 1) / TZ};
+  }
+  __host__ dim3 grid() const { return dim3((unsigned)(ntx * nty * ntz)); }
+  ASX_DEV bool point(const Box &B, int &i, int &j, int &k) const {
+    const int b = xcd_tile((int)blockIdx.x, ntx * nty * ntz);
+    const int full = nty / YS, blk = YS * ntz * ntx;
+    int bx, by, bz;
+    if (b < full * blk) {
+      const int yb = b / blk, r = b % blk;
+      bz = r / (YS * ntx);
+      const int r2 = r % (YS * ntx);
+      by = yb * YS + r2 / ntx;
+      bx = r2 % ntx;
+    } else {
+      const int rem = nty - full * YS, b2 = b - full * blk;
+      bz = b2 / (rem * ntx);
+      const int r2 = b2 % (rem * ntx);
+      by = full * YS + r2 / ntx;
+      bx = r2 % ntx;
+    }
+    return tile_point<TX, TY, TZ>(B.lo, B.cnt, bx + ntx * (by + nty * bz), ntx, nty, i, j, k);
+  }
+};
+
 // Launch-shape variants of the tunable kernels (selected per call; see
 // asx_set_variant / asx_autotune_substep in the C ABI).
-constexpr int kUpdateVariants = 5;
+constexpr int kUpdateVariants = 7;
 constexpr int kFaceVariants = 5;
 constexpr int kTmunuVariants = 5;
 
