@@ -5,7 +5,7 @@
 // vertex metric.
 //
 // Work decomposition: a workgroup owns a tile of TX x TY vertex columns
-// and a run of kTZ vertices along z, and marches through the cell planes
+// and a run of TZ vertices along z, and marches through the cell planes
 // the run needs.  Each plane of the 9 interpolated variables is staged in
 // LDS once per tile (coalesced rows, each cell fetched once), every
 // work-item reduces its (ORD x ORD) window of the plane in x and y from LDS,
@@ -30,9 +30,8 @@ template <int ORD> ASX_DEV constexpr double c2v_w(int a) {
 constexpr int kBufferFlags = 0x00020000;
 
 // Tiles of TX x TY vertex columns (TX * TY = 256 work-items) and runs of
-// kTZ vertices along z; the staged plane has (TX + ORD - 1) x (TY + ORD - 1)
+// TZ vertices along z; the staged plane has (TX + ORD - 1) x (TY + ORD - 1)
 // cells per variable.
-constexpr int kTZ = 16;
 template <int ORD, int TX, int TY> struct TmTile {
   static_assert(TX * TY == 256, "one work-item per vertex column");
   static constexpr int H = ORD / 2;
@@ -106,7 +105,7 @@ ASX_DEV void tmunu_point(const KPatch &P, bool use_z, unsigned ob,
                     (w2 * vl[a] * vl[b] + 0.5 * g(a, b)) * b2 - bsl[a] * bsl[b];
 }
 
-template <int ORD, int TX, int TY, int WAVES>
+template <int ORD, int TX, int TY, int TZ, int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_tmunu(
     KPatch P, int ntx, int nty, int nb, int use_z) {
   using T = TmTile<ORD, TX, TY>;
@@ -117,8 +116,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
   const int tid = (int)threadIdx.x, tx = tid % TX, ty = tid / TX;
   const int i0 = P.ng + bx * TX, j0 = P.ng + by * TY;
   const int i = i0 + tx, j = j0 + ty;
-  const int kb = P.ng + bz * kTZ;
-  const int ke = min(kb + kTZ, P.ng + P.n[2] + 1);
+  const int kb = P.ng + bz * TZ;
+  const int ke = min(kb + TZ, P.ng + P.n[2] + 1);
   const bool mine = i <= P.ng + P.n[0] && j <= P.ng + P.n[1];
   // last cell any interior vertex's stencil reads, per axis
   const int cxmax = P.ng + P.n[0] + H - 1, cymax = P.ng + P.n[1] + H - 1;
@@ -204,18 +203,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 
 }  // namespace
 
-// Grid: tiles of TX x TY vertex columns x runs of kTZ vertices over the
+// Grid: tiles of TX x TY vertex columns x runs of TZ vertices over the
 // interior vertices.  The cell stencil of vertex (i, j, k) spans
 // i-ORD/2 .. i+ORD/2-1 per axis, inside the patch for vertices ng..ng+n when
 // ng >= ORD/2 (checked by the caller); cells past the last one any interior
 // vertex needs are not read.
-template <int ORD, int TX, int TY, int WAVES>
+template <int ORD, int TX, int TY, int TZ, int WAVES>
 static hipError_t tmunu_t(const KPatch &P, int use_z, hipStream_t s) {
   const int ntx = (P.n[0] + 1 + TX - 1) / TX, nty = (P.n[1] + 1 + TY - 1) / TY;
-  const int ntz = (P.n[2] + 1 + kTZ - 1) / kTZ;
+  const int ntz = (P.n[2] + 1 + TZ - 1) / TZ;
   const int nb = ntx * nty * ntz;
-  hipLaunchKernelGGL((k_tmunu<ORD, TX, TY, WAVES>), dim3(nb), dim3(256), 0, s, P, ntx, nty, nb,
-                     use_z);
+  hipLaunchKernelGGL((k_tmunu<ORD, TX, TY, TZ, WAVES>), dim3(nb), dim3(256), 0, s, P, ntx, nty,
+                     nb, use_z);
   return hipGetLastError();
 }
 
@@ -224,11 +223,13 @@ static hipError_t tmunu_v(const KPatch &P, int use_z, int variant, hipStream_t s
   // default: 32 x 8 columns at a 3-wave register budget (0.84 ms for order 4
   // on 193^3 vertices against 0.94-0.96 ms for the others)
   switch (variant) {
-  case 1: return tmunu_t<ORD, 32, 8, 1>(P, use_z, s);
-  case 2: return tmunu_t<ORD, 32, 8, 2>(P, use_z, s);
-  case 3: return tmunu_t<ORD, 64, 4, 2>(P, use_z, s);
-  case 4: return tmunu_t<ORD, 16, 16, 2>(P, use_z, s);
-  default: return tmunu_t<ORD, 32, 8, 3>(P, use_z, s);
+  case 1: return tmunu_t<ORD, 32, 8, 16, 1>(P, use_z, s);
+  case 2: return tmunu_t<ORD, 32, 8, 16, 2>(P, use_z, s);
+  case 3: return tmunu_t<ORD, 64, 4, 16, 2>(P, use_z, s);
+  case 4: return tmunu_t<ORD, 16, 16, 16, 2>(P, use_z, s);
+  case 5: return tmunu_t<ORD, 32, 8, 32, 3>(P, use_z, s);
+  case 6: return tmunu_t<ORD, 64, 4, 16, 3>(P, use_z, s);
+  default: return tmunu_t<ORD, 32, 8, 16, 3>(P, use_z, s);
   }
 }
 
