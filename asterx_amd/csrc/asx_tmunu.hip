@@ -1,5 +1,5 @@
 // Vertex-centred stress-energy tensor of the fluid and the magnetic field,
-// added into eTtt..eTzz (AsterX/src/tmunu.cxx:19-171): the primitives are
+// added into eTtt..eTzz (AsterX/src/tmunu.cxx:19-176): the primitives are
 // interpolated from cells to the vertex (calc_avg_c2v,
 // AsterUtils/src/aster_interp.hxx:97-148), then T_{mu nu} is formed with the
 // vertex metric.
@@ -44,7 +44,7 @@ template <int ORD, int TX, int TY> struct TmTile {
 ASX_DEV double &gf_at(double *g, unsigned ob) { return *(double *)((char *)g + ob); }
 
 // T_{mu nu} at the vertex at byte offset ob from the interpolated primitives
-// (AsterX/src/tmunu.cxx:56-170), added into the ten eT slots.
+// (AsterX/src/tmunu.cxx:56-158), added into the ten eT slots.
 ASX_DEV void tmunu_point(const KPatch &P, bool use_z, unsigned ob,
                          const double q[kTmunuVars]) {
   const double rho = q[0], eps = q[1], press = q[2];

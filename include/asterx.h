@@ -352,7 +352,7 @@ asx_status asx_fluxes(const asx_patch *patch, const asx_params *par,
 asx_status asx_source_terms(const asx_patch *patch, const asx_params *par,
                             void *stream);
 
-/* AsterX_Tmunu (AsterX/src/tmunu.cxx:19-171, scheduled in
+/* AsterX_Tmunu (AsterX/src/tmunu.cxx:19-176, scheduled in
  * TmunuBaseX_AddToTmunu when update_tmunu, AsterX/schedule.ccl:253-266):
  * interpolates rho, eps, press, Bvec and vel (recon_type v_vec) or zvec
  * (otherwise) from cells to the interior vertices with calc_avg_c2v of order

@@ -513,7 +513,7 @@ double avg_c2v4(const GF &g, const int I[3]) {
   return avg / 3.0;
 }
 
-// Tmunu<interp_order>, AsterX/src/tmunu.cxx:44-170: primitives interpolated
+// Tmunu<interp_order>, AsterX/src/tmunu.cxx:44-159: primitives interpolated
 // to the vertex, lower-index T_{mu nu} of fluid + field added into eT*.
 void tmunu_vertex(const Ctx &c, const int I[3]) {
   const View &v = c.v;
