@@ -40,12 +40,17 @@ def all_slots(use_uct=False):
 
 
 class Evolution:
-    def __init__(self, backend, patch, par, bc="linear"):
+    def __init__(self, backend, patch, par, bc="linear", tmunu=False):
+        """tmunu: also set the stress-energy tensor after every con2prim
+        (TmunuBaseX's SetTmunuVars, which runs after AsterX_Con2PrimGroup in
+        ODESolvers_PostStep, AsterX/schedule.ccl:119), as a spacetime
+        coupled to the fluid needs."""
         self.b = backend
         self.p = patch
         self.par = par
         self.bc = abi.BC[bc] if isinstance(bc, str) else bc
         self.uct = bool(par.use_uct)
+        self.with_tmunu = tmunu
 
     def sync(self, names):
         ps = self.p.struct()
@@ -85,6 +90,20 @@ class Evolution:
         self.sync(abi.DB)
         b.con2prim(p.struct(), self.par)
         self.sync(SYNC_C2P)
+        if self.with_tmunu:
+            self.tmunu()
+
+    # TmunuBaseX zeroes eT*, the TmunuBaseX_AddToTmunu group adds into it
+    # (AsterX_Tmunu when update_tmunu, AsterX/schedule.ccl:253-266), then
+    # eT* is SYNCed
+    def tmunu(self):
+        p = self.p
+        for n in abi.TMUNU:
+            if n not in p:
+                p.alloc(n)
+            p[n] = 0.0
+        self.b.tmunu(p.struct(), self.par)
+        self.sync(abi.TMUNU)
 
     # ODESolvers "RK4"
     def rk4_step(self, dt):

@@ -12,7 +12,9 @@ the mathematics fixes rather than by retyping its formulas:
   cubics and that of order 2 on multilinear data (wrong weights or a stencil
   shifted by one cell fail this);
 * v_vec and z_vec velocities give the same tensor; the result is added to
-  what eT* held; bad parameters are refused.
+  what eT* held; bad parameters are refused;
+* the driver's post-step (eT* zeroed, tensor added, ghosts filled after
+  con2prim) leaves the interior equal to a fresh evaluation.
 """
 import ctypes as C
 
@@ -197,3 +199,28 @@ def test_refusals_and_switch():
     assert lib.orc_tmunu(C.byref(q.struct()), C.byref(abi.params())) == 5  # ghosts
     _run(p, update_tmunu="no")
     assert all(not p.numpy(n).any() for n in ET)
+
+
+def test_evolution_sets_tmunu_after_con2prim():
+    """The driver's post-step zeroes eT*, adds the tensor of the new
+    primitives and fills the ghosts: the interior equals a fresh evaluation
+    on the same primitives."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    import tsv_cases
+    from asterx_amd.driver import Evolution
+    c = tsv_cases.CASES["Alfven_wave"]
+    p = tsv_cases.make_patch(c)
+    par = tsv_cases.case_params(c)
+    ev = Evolution(oracle.backend(), p, par, bc="periodic", tmunu=True)
+    ev.rk2_step(0.2 * min(p.dx))
+    got = {n: p.numpy(n).copy() for n in ET}
+    for n in ET:
+        p[n] = 0.0
+    _run(p, tmunu_interp_order=par.tmunu_interp_order, recon_type=par.recon_type)
+    for n in ET:
+        sl = p.interior(n)
+        assert np.array_equal(got[n][sl], p.numpy(n)[sl]), n
+    # the ghosts were filled (periodic): not left at zero
+    assert np.all(got["eTtt"] != 0.0)
