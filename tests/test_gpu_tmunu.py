@@ -3,8 +3,9 @@ state: both interpolation orders, v_vec and z_vec velocities, eT* holding
 seeded values beforehand (the entry adds into them).  Bar: 1e-12 relative in
 max norm per component (the device evaluates one of the three equal axis
 orders of the order-4 stencil, the reference averages all three).  Also the
-C-ABI refusals and the update_tmunu switch, and a 128^3 patch compared with
-the oracle on sampled vertices."""
+C-ABI refusals and the update_tmunu switch, a 128^3 patch compared with
+the oracle on sampled vertices, and the driver's post-step tensor after two
+RK2 steps of the Alfven wave on both backends."""
 import ctypes as C
 
 import numpy as np
@@ -135,3 +136,29 @@ def test_tmunu_large_patch_sampled():
             got = big.numpy(nm)[c0:c0 + e, c0:c0 + e, c0:c0 + e][win.interior(nm)]
             want = win.numpy(nm)[win.interior(nm)]
             assert rel_err(got, want) <= TOL, (c0, nm)
+
+
+def test_evolution_with_tmunu_matches_oracle():
+    """Two RK2 steps of the periodic Alfven wave with the stress-energy
+    tensor set after every con2prim, on the device and on the oracle: the
+    tensors agree to the evolution's own agreement (1e-10 relative)."""
+    torch = _torch()
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(__file__))
+    import tsv_cases
+    from asterx_amd import lib
+    from asterx_amd.driver import Evolution
+    c = tsv_cases.CASES["Alfven_wave"]
+    par = tsv_cases.case_params(c)
+    out = []
+    for dev, be in ((None, oracle.backend()), ("cuda", lib.backend())):
+        p = tsv_cases.make_patch(c, device=dev)
+        ev = Evolution(be, p, par, bc="periodic", tmunu=True)
+        for _ in range(2):
+            ev.rk2_step(0.2 * min(p.dx))
+        if dev:
+            torch.cuda.synchronize()
+        out.append({n: p.numpy(n).copy() for n in abi.TMUNU})
+    for n in abi.TMUNU:
+        assert rel_err(out[1][n], out[0][n]) <= 1e-10, (n, rel_err(out[1][n], out[0][n]))
