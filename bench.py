@@ -535,14 +535,17 @@ def bench_tmunu(args, p, par, L, sh, workload):
         del p.arr[nm]
     if args.cpu == "on":
         import oracle
-        m, ng = 48, p.ng
+        ng = p.ng
+        m = min(48, *p.n)
         hp = Patch((m,) * 3, ng, (0.0,) * 3, p.dx)
-        c0 = p.ng + (p.n[0] - m) // 2
+        # window origin (first ghost point) per axis, centred in the patch;
+        # arrays are indexed (z, y, x)
+        o = [(p.n[d] - m) // 2 for d in range(3)]
         for nm in abi.ADM[:10] + ["rho", "eps", "press", "velx", "vely", "velz",
                                   "Bvecx", "Bvecy", "Bvecz", "zvec_x", "zvec_y",
                                   "zvec_z"]:
             e = m + 2 * ng + (abi.centering(nm) & 1)
-            a = p.arr[nm][c0 - ng:c0 - ng + e, c0 - ng:c0 - ng + e, c0 - ng:c0 - ng + e]
+            a = p.arr[nm][o[2]:o[2] + e, o[1]:o[1] + e, o[0]:o[0] + e]
             hp[nm] = a.cpu().numpy()
         for nm in abi.TMUNU:
             hp.alloc(nm)
