@@ -90,7 +90,7 @@ template <int TX, int TY, int TZ, int YS> struct TileGridBand {
 // asx_set_variant / asx_autotune_substep in the C ABI).
 constexpr int kUpdateVariants = 7;
 constexpr int kFaceVariants = 5;
-constexpr int kTmunuVariants = 7;
+constexpr int kTmunuVariants = 9;
 
 // Batched launches over many patches: the workgroups of one launch are
 // listed as (patch, chunk) pairs, chunk = tile index or 256-point block of

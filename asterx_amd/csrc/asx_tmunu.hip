@@ -105,14 +105,39 @@ ASX_DEV void tmunu_point(const KPatch &P, bool use_z, unsigned ob,
                     (w2 * vl[a] * vl[b] + 0.5 * g(a, b)) * b2 - bsl[a] * bsl[b];
 }
 
-template <int ORD, int TX, int TY, int TZ, int WAVES>
+// Tile b of an ntx x nty x ntz grid.  YS = 0: x fastest, then y, then z.
+// YS > 0: bands of YS tile rows (x, the band's rows, then z, then the next
+// band), so the runs above and below a tile, which stage ORD - 1 of the same
+// planes, are YS * ntx launches away rather than a whole x-y layer of tiles.
+template <int YS> ASX_DEV void tile_of(int b, int ntx, int nty, int ntz, int &bx, int &by,
+                                       int &bz) {
+  if (YS == 0) {
+    bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+    return;
+  }
+  const int full = nty / YS, blk = YS * ntz * ntx;
+  if (b < full * blk) {
+    const int r = b % blk;
+    bz = r / (YS * ntx);
+    by = (b / blk) * YS + (r % (YS * ntx)) / ntx;
+    bx = r % ntx;
+  } else {
+    const int rem = nty - full * YS, b2 = b - full * blk;
+    bz = b2 / (rem * ntx);
+    by = full * YS + (b2 % (rem * ntx)) / ntx;
+    bx = b2 % ntx;
+  }
+}
+
+template <int ORD, int TX, int TY, int TZ, int WAVES, int YS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_tmunu(
     KPatch P, int ntx, int nty, int nb, int use_z) {
   using T = TmTile<ORD, TX, TY>;
   constexpr int H = T::H, CX = T::CX, NC = T::NC, PER = T::PER;
   __shared__ double lds[kTmunuVars][NC];
   const int b = xcd_tile((int)blockIdx.x, nb);
-  const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+  int bx, by, bz;
+  tile_of<YS>(b, ntx, nty, nb / (ntx * nty), bx, by, bz);
   const int tid = (int)threadIdx.x, tx = tid % TX, ty = tid / TX;
   const int i0 = P.ng + bx * TX, j0 = P.ng + by * TY;
   const int i = i0 + tx, j = j0 + ty;
@@ -208,20 +233,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES))) vo
 // i-ORD/2 .. i+ORD/2-1 per axis, inside the patch for vertices ng..ng+n when
 // ng >= ORD/2 (checked by the caller); cells past the last one any interior
 // vertex needs are not read.
-template <int ORD, int TX, int TY, int TZ, int WAVES>
+template <int ORD, int TX, int TY, int TZ, int WAVES, int YS = 0>
 static hipError_t tmunu_t(const KPatch &P, int use_z, hipStream_t s) {
   const int ntx = (P.n[0] + 1 + TX - 1) / TX, nty = (P.n[1] + 1 + TY - 1) / TY;
   const int ntz = (P.n[2] + 1 + TZ - 1) / TZ;
   const int nb = ntx * nty * ntz;
-  hipLaunchKernelGGL((k_tmunu<ORD, TX, TY, TZ, WAVES>), dim3(nb), dim3(256), 0, s, P, ntx, nty,
-                     nb, use_z);
+  hipLaunchKernelGGL((k_tmunu<ORD, TX, TY, TZ, WAVES, YS>), dim3(nb), dim3(256), 0, s, P, ntx,
+                     nty, nb, use_z);
   return hipGetLastError();
 }
 
 template <int ORD>
 static hipError_t tmunu_v(const KPatch &P, int use_z, int variant, hipStream_t s) {
-  // default: 32 x 8 columns at a 3-wave register budget (0.84 ms for order 4
-  // on 193^3 vertices against 0.94-0.96 ms for the others)
+  // default: 32 x 8 columns at a 3-wave register budget (0.83-0.89 ms for
+  // order 4 on 193^3 vertices; the other shapes 0.93-0.98 ms; the banded
+  // orders 7 and 8 measure within the same run-to-run spread)
   switch (variant) {
   case 1: return tmunu_t<ORD, 32, 8, 16, 1>(P, use_z, s);
   case 2: return tmunu_t<ORD, 32, 8, 16, 2>(P, use_z, s);
@@ -229,6 +255,8 @@ static hipError_t tmunu_v(const KPatch &P, int use_z, int variant, hipStream_t s
   case 4: return tmunu_t<ORD, 16, 16, 16, 2>(P, use_z, s);
   case 5: return tmunu_t<ORD, 32, 8, 32, 3>(P, use_z, s);
   case 6: return tmunu_t<ORD, 64, 4, 16, 3>(P, use_z, s);
+  case 7: return tmunu_t<ORD, 32, 8, 16, 3, 2>(P, use_z, s);
+  case 8: return tmunu_t<ORD, 32, 8, 16, 3, 4>(P, use_z, s);
   default: return tmunu_t<ORD, 32, 8, 16, 3>(P, use_z, s);
   }
 }

@@ -46,7 +46,7 @@ def _variants():
 
 @pytest.mark.parametrize("order", [2, 4])
 @pytest.mark.parametrize("rec", ["v_vec", "z_vec"])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", list(range(9)))
 def test_tmunu_parity(order, rec, variant):
     torch = _torch()
     if variant not in _variants():
