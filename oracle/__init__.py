@@ -45,6 +45,9 @@ _SIGS = {
     "orc_eos_from_rho_eps": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_void_p, _PP],
     "orc_tab_interp": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _PP],
+    "orc_reconstruct": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _PP,
+                        C.c_void_p],
+    "orc_util": [C.c_int, C.c_void_p, C.c_void_p],
     "asx_gf_centering": [C.c_int],
 }
 
@@ -57,17 +60,35 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE, "-B", "liboracle.so"])
 
 
+def build_planted(k, outdir):
+    """Compile the oracle with planted mistake number k (ORC_PLANT=k, see
+    orc_grid.cpp) into outdir/liboracle_plant<k>.so and return that path.
+    Tests use it to show that a pin fails when the restatement is wrong."""
+    import subprocess
+    out = os.path.join(outdir, "liboracle_plant%d.so" % k)
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-ffp-contract=off",
+                           "-fno-fast-math", "-fopenmp", "-DORC_PLANT=%d" % k,
+                           "-I" + os.path.join(_HERE, "..", "include"), "-shared", "-o", out,
+                           os.path.join(_HERE, "orc_grid.cpp")])
+    return out
+
+
+def load(path):
+    """A ctypes handle of an oracle build at `path` with the signatures set."""
+    L = C.CDLL(path)
+    for name, args in _SIGS.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIBPATH):
             build()
-        L = C.CDLL(LIBPATH)
-        for name, args in _SIGS.items():
-            f = getattr(L, name)
-            f.argtypes = args
-            f.restype = C.c_int
-        _lib = L
+        _lib = load(LIBPATH)
     return _lib
 
 

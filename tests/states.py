@@ -183,3 +183,42 @@ def rel_err(a, b):
     d = np.abs(a[m] - b[m]).max()
     s = np.abs(b[m]).max()
     return d / s if s > 0 else d
+
+
+def kat_state(gamma=2.0):
+    """The round-trip state of Con2PrimFactory/src/test.cxx:21-247 on a
+    2^3-cell flat patch: rho 0.125, eps 0.8, Ye 0.5, v = 0, B = (0.5, -0.5,
+    0), entropy = kappa = (Gamma - 1) eps rho^(1 - Gamma); cons from the
+    Valencia map (densitized), saved primitives = the state."""
+    p = Patch((2, 2, 2), 1, (0.0, 0.0, 0.0), (0.1, 0.1, 0.1), slots=all_slots())
+    p["alp"] = 1.0
+    for c in ("gxx", "gyy", "gzz"):
+        p[c] = 1.0
+    rho, eps, Ye = 0.125, 0.8, 0.5
+    press = (gamma - 1.0) * rho * eps
+    ent = (gamma - 1.0) * eps * rho ** (1.0 - gamma)
+    p["rho"], p["eps"], p["press"], p["Ye"], p["entropy"] = rho, eps, press, Ye, ent
+    p["temperature"] = (gamma - 1.0) * eps
+    p["Bvecx"], p["Bvecy"] = 0.5, -0.5
+    g = [np.ones(p.shape("rho")) if c in (0, 3, 5) else np.zeros(p.shape("rho"))
+         for c in range(6)]
+    vel = [np.zeros(p.shape("rho"))] * 3
+    B = [p.numpy("Bvec" + c) for c in "xyz"]
+    dens, mom, tau, dB, DYe, DEnt = prim2con_np(g, p.numpy("rho"), p.numpy("eps"),
+                                                p.numpy("press"), vel, B, Ye, ent)
+    p["dens"], p["tau"], p["DYe"], p["DEnt"] = dens, tau, DYe, DEnt
+    for i, c in enumerate("xyz"):
+        p["mom" + c] = mom[i]
+        p["dB" + c] = dB[i]
+    p["saved_rho"], p["saved_eps"], p["saved_Ye"] = rho, eps, Ye
+    return p
+
+
+def kat_params(prime):
+    """Con2PrimFactory/src/test.cxx:37-65: atmosphere rho 1e-10, eps 1e-8;
+    max_iter 100, tol 1e-8, alp_thresh -1, cons_error_limit -1, vw_lim 1,
+    B_lim 1, BH limits 1e20, Ye_lenient; one solver."""
+    return abi.params(c2p_prime=prime, c2p_second="None", gl_gamma=2.0, max_iter=100,
+                      c2p_tol=1e-8, alp_thresh=-1.0, cons_error_limit=-1.0, vw_lim=1.0,
+                      B_lim=1.0, rho_BH=1e20, eps_BH=1e20, vwlim_BH=1e20, Ye_lenient=1,
+                      rho_abs_min=1e-10, eps_atmo=1e-8)

@@ -150,7 +150,7 @@ def test_bfield_and_prim2con():
 
 C2P_OUT = ["rho", "velx", "vely", "velz", "eps", "press", "temperature",
            "entropy", "Ye", "Bvecx", "Bvecy", "Bvecz"] + abi.CONS + abi.DB + \
-    abi.SAVED + abi.ZS
+    abi.SAVED + abi.ZS + ["aster_mask_cc"]
 
 
 @pytest.mark.parametrize("prime,second", [("Noble", "Palenzuela"),
@@ -187,6 +187,71 @@ def test_con2prim(prime, second):
     assert list(hist) == list(cnt.c2p_flags)
     tol = 1e-12 if prime == "Noble" else 1e-9
     _cmp(h, d, C2P_OUT, tol=tol)
+
+
+@pytest.mark.parametrize("excise,use_mask", [(0, 0), (1, 0), (1, 1), (0, 1)])
+def test_con2prim_bh_interior(excise, use_mask):
+    """BH-interior limiting (Con2PrimFactory/src/c2p.hxx:218-328) and
+    excision (AsterX/src/con2prim.cxx:225-241): cells whose averaged lapse
+    is below alp_thresh (the centre of the seeded lapse dip, alpha ~ 0.8)
+    or whose vertex mask averages below 1 get aster_mask_cc = 0 and, with
+    excise, the limited seed instead of a solve.  Flags and aster_mask_cc
+    bit-exact, primitives to the solver tolerance."""
+    _torch()
+    h, d = _pair(curved=True)
+    if use_mask:
+        m = np.ones(h.shape("aster_mask_vc"))
+        m[4:7, 5:9, 6:10] = 0.0
+        for p in (h, d):
+            p["aster_mask_vc"] = m
+    par = abi.params(c2p_prime="Noble", c2p_second="Palenzuela", gl_gamma=5.0 / 3.0,
+                     rho_abs_min=1e-12, alp_thresh=0.83, rho_BH=0.9, eps_BH=0.5,
+                     vwlim_BH=1.2, excise=excise, use_mask=use_mask)
+    cnt = abi.asx_counters()
+    _gpu()._call("con2prim", C.byref(d.struct()), C.byref(par), C.byref(cnt), None)
+    _sync()
+    abi.check("orc_con2prim", oracle.lib().orc_con2prim(C.byref(h.struct()), C.byref(par)))
+    sl = h.interior("rho")
+    mk = h.numpy("aster_mask_cc")[sl]
+    assert (mk == 0.0).sum() >= 4 and (mk == 1.0).sum() >= 100, np.unique(mk)
+    assert np.array_equal(mk, d.numpy("aster_mask_cc")[sl])
+    fl_h = h.numpy("con2prim_flag")[sl]
+    assert np.array_equal(fl_h, d.numpy("con2prim_flag")[sl])
+    assert list(np.bincount(fl_h.astype(int).ravel(), minlength=7)) == list(cnt.c2p_flags)
+    _cmp(h, d, C2P_OUT, tol=1e-9)
+
+
+def test_con2prim_pressure_atmosphere():
+    """The torus workload's con2prim settings on the seeded curved state:
+    Gamma 4/3, the pressure atmosphere (use_press_atmo) with rho 1e-8 and
+    p 1e-10, graded atmosphere (r_atmo, n_rho_atmo, n_press_atmo), a few
+    cells forced below the density cut and a few with the pressure cut
+    to 1/1000 so that the pressure floor (c2p.hxx:86-216) acts."""
+    _torch()
+    h, d = _pair(curved=True, gamma=4.0 / 3.0)
+    for p in (h, d):
+        dens = p.numpy("dens").copy()
+        dens[5, 6, 7] = 1e-12
+        dens[6, 7, 8] = 3e-9
+        tau = p.numpy("tau").copy()
+        tau[7, 5, 6] *= 1e-3
+        tau[8, 8, 5] *= 1e-3
+        p["dens"] = dens
+        p["tau"] = tau
+    par = abi.params(c2p_prime="Noble", c2p_second="Palenzuela", gl_gamma=4.0 / 3.0,
+                     poly_gamma=4.0 / 3.0, rho_abs_min=1e-8, p_atmo=1e-10,
+                     use_press_atmo=1, rho_max=1e8, eps_max=1e8, r_atmo=0.5,
+                     n_rho_atmo=1.5, n_press_atmo=2.5)
+    cnt = abi.asx_counters()
+    _gpu()._call("con2prim", C.byref(d.struct()), C.byref(par), C.byref(cnt), None)
+    _sync()
+    abi.check("orc_con2prim", oracle.lib().orc_con2prim(C.byref(h.struct()), C.byref(par)))
+    sl = h.interior("rho")
+    fl_h = h.numpy("con2prim_flag")[sl]
+    assert np.array_equal(fl_h, d.numpy("con2prim_flag")[sl])
+    assert (fl_h == 4).sum() >= 2
+    assert list(np.bincount(fl_h.astype(int).ravel(), minlength=7)) == list(cnt.c2p_flags)
+    _cmp(h, d, C2P_OUT, tol=1e-9)
 
 
 @pytest.mark.parametrize("prime,second,fix", [("Entropy", "None", 0),
@@ -261,7 +326,7 @@ def test_substep_fused_matches_unfused():
 def test_c2p_batch():
     _torch()
     import torch
-    from c2p_batch import make_batch
+    from asterx_amd.zones import make_batch
     nz = 20000
     cons, g, seed, _ = make_batch(nz, seed=3)
     par = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8,
@@ -381,3 +446,25 @@ def test_variants_bit_identical():
         assert 0 <= v.value < L.asx_variant_count(kc)
         abi.check("asx_set_variant", L.asx_set_variant(kc, 0))
     assert L.asx_set_variant(kcs[0], 99) == 3  # ASX_ERR_PARAM
+
+
+@pytest.mark.parametrize("prime", ["Noble", "Palenzuela", "Entropy"])
+def test_con2prim_known_answer(prime):
+    """Con2PrimFactory/src/test.cxx:21-247 (rho 0.125, eps 0.8, B = (0.5,
+    -0.5, 0) at rest, tolerance 1e-8) through asx_con2prim: the same flags as
+    the oracle, primitives within 1e-12 of it and within 1e-7 of the state
+    the conserved variables were built from."""
+    _torch()
+    from states import kat_params, kat_state
+    h = kat_state()
+    d = h.to("cuda")
+    par = kat_params(prime)
+    _gpu()._call("con2prim", C.byref(d.struct()), C.byref(par), None, None)
+    _sync()
+    abi.check("orc_con2prim", oracle.lib().orc_con2prim(C.byref(h.struct()), C.byref(par)))
+    sl = h.interior("rho")
+    assert np.all(d.numpy("con2prim_flag")[sl] == 1.0)
+    assert np.array_equal(d.numpy("con2prim_flag")[sl], h.numpy("con2prim_flag")[sl])
+    _cmp(h, d, C2P_OUT, tol=1e-12)
+    for name, want in (("rho", 0.125), ("eps", 0.8), ("press", 0.1)):
+        assert np.abs(d.numpy(name)[sl] - want).max() <= 1e-7 * want, name
