@@ -115,6 +115,32 @@ def lib():
     return _lib
 
 
+EOSIO_PATH = os.path.join(_HERE, "libasterx_eosio.so")
+_eosio = None
+
+
+class asx_eos_host_table(C.Structure):
+    _fields_ = [("n", C.c_int32 * 3), ("data", C.POINTER(C.c_double)),
+                ("axis", C.POINTER(C.c_double) * 3), ("energy_shift", C.c_double),
+                ("eps_min", C.c_double), ("eps_max", C.c_double)]
+
+
+def eosio():
+    """The host table reader (libasterx_eosio.so, include/asterx_eosio.h)."""
+    global _eosio
+    if _eosio is None:
+        if not os.path.exists(EOSIO_PATH):
+            raise RuntimeError("libasterx_eosio.so is not built (%s); run "
+                               "__graft_entry__.build()" % EOSIO_PATH)
+        L = C.CDLL(EOSIO_PATH)
+        L.asx_eos_read_stellarcollapse.argtypes = [C.c_char_p, C.POINTER(asx_eos_host_table)]
+        L.asx_eos_read_stellarcollapse.restype = C.c_int32
+        L.asx_eos_host_table_free.argtypes = [C.POINTER(asx_eos_host_table)]
+        L.asx_eos_host_table_free.restype = None
+        _eosio = L
+    return _eosio
+
+
 def backend(stream=None):
     from .driver import Backend
     return Backend(lib(), "asx_", stream)

@@ -26,6 +26,7 @@ units, variable index fastest.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -132,6 +133,26 @@ def code_units(lr10, lt10, ye, logp10, loge10, cs2, ent, shift_cgs=SHIFT_CGS):
 def build(nrho=NRHO, ntemp=NTEMP, nye=NYE):
     """The synthetic table in code units (host arrays)."""
     return code_units(*make_cgs(nrho, ntemp, nye))
+
+
+def read_stellarcollapse(path):
+    """A StellarCollapse HDF5 table read by the library's reader
+    (asx_eos_read_stellarcollapse: EOSX's reader and unit conversion), in
+    the form build() returns."""
+    from . import lib
+    L = lib.eosio()
+    t = lib.asx_eos_host_table()
+    st = L.asx_eos_read_stellarcollapse(os.fsencode(path), C.byref(t))
+    if st:
+        raise RuntimeError("asx_eos_read_stellarcollapse(%s): %d" % (path, st))
+    try:
+        n = [t.n[k] for k in range(3)]
+        data = np.ctypeslib.as_array(t.data, shape=(n[2], n[1], n[0], 4)).copy()
+        axes = tuple(np.ctypeslib.as_array(t.axis[k], shape=(n[k],)).copy() for k in range(3))
+        return dict(data=data, axes=axes, shift=t.energy_shift, eps_min=t.eps_min,
+                    eps_max=t.eps_max)
+    finally:
+        L.asx_eos_host_table_free(C.byref(t))
 
 
 class Table:

@@ -97,3 +97,15 @@ def test_one_hip_runtime_when_library_loads_first():
                          text=True, timeout=600)
     assert out.returncode == 0, out.stderr[-2000:]
     assert out.stdout.strip().splitlines()[-1] == "1", out.stdout
+
+
+def test_eosio_library_exports_its_header():
+    """libasterx_eosio.so exports every function include/asterx_eosio.h
+    declares."""
+    from asterx_amd import lib
+    txt = re.sub(r"/\*.*?\*/", "",
+                 open(os.path.join(ROOT, "include", "asterx_eosio.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(asx_[a-z0-9_A-Z]+)\s*\(", txt)))
+    assert names == ["asx_eos_host_table_free", "asx_eos_read_stellarcollapse"]
+    L = lib.eosio()
+    assert all(hasattr(L, f) for f in names)
