@@ -448,6 +448,14 @@ C2PQueue *c2p_queue(void *stream, int64_t nz) {
   return &Q;
 }
 
+// Device buffers of one con2prim call: the queue of zones the primary
+// solver failed (one int per interior zone), its count, and the counters.
+struct C2PScratch {
+  int *queue;
+  unsigned *qcount;
+  unsigned long long *cnt; // 80 counters
+};
+
 // Device scratch per (device, stream, tag), grown on demand.
 enum { SCR_MLP_GUESS = 1 };
 struct DevScratch {
@@ -479,10 +487,12 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
-// Scratch slots the fused substep computes (fluxes, UCT data, aux; the
-// faces take their normal field from A, so dB_stag is not among them).
+// Scratch slots the fused substep computes: fluxes, upwind-CT face data,
+// aux, and dB_stag (written only by asx_substep_bc with use_uct; the faces
+// take their normal field from A).
 bool scratch_slot(int s) {
-  return (s >= ASX_FXDENS && s <= ASX_AMIN_ZFACE) || (s >= ASX_FX && s <= ASX_G);
+  return (s >= ASX_FXDENS && s <= ASX_AMIN_ZFACE) || (s >= ASX_FX && s <= ASX_G) ||
+         (s >= ASX_DBX_STAG && s <= ASX_DBZ_STAG);
 }
 
 Scratch *scratch_for(const asx_patch *pt, void *stream) {
@@ -815,7 +825,8 @@ asx_status asx_prim2con(const asx_patch *pt, void *stream) {
 }
 
 static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool from_A,
-                             asx_counters *cnt, void *stream) {
+                             asx_counters *cnt, void *stream,
+                             const C2PScratch *fixed = nullptr) {
   asx_status st = check_geom(pt);
   if (st) return st;
   if ((st = check_c2p_par(par))) return st;
@@ -838,25 +849,38 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
   }
   unsigned long long *cd = nullptr;
   if (cnt) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    cd = counter_buf(dev);
-    if (!cd) return ASX_ERR_DEVICE;
+    if (fixed) {
+      cd = fixed->cnt;
+    } else {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      cd = counter_buf(dev);
+      if (!cd) return ASX_ERR_DEVICE;
+    }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, cd, 80);
   }
-  C2PQueue *Q = c2p_queue(stream, (int64_t)pt->n[0] * pt->n[1] * pt->n[2]);
-  if (!Q) return ASX_ERR_DEVICE;
+  int *queue;
+  unsigned *qcount;
+  if (fixed) {
+    queue = fixed->queue;
+    qcount = fixed->qcount;
+  } else {
+    C2PQueue *Q = c2p_queue(stream, (int64_t)pt->n[0] * pt->n[1] * pt->n[2]);
+    if (!Q) return ASX_ERR_DEVICE;
+    queue = Q->q;
+    qcount = Q->n;
+  }
   const C2PArgs ca = c2p_args(par);
   unsigned long long *hist = cd ? cd + 2 : nullptr;
   const bool tab = tabulated(par);
   if ((st = dev_status(timed(ASX_KC_C2P, s, [&] {
-         return tab ? launch_con2prim_main(P, tab_eos(par), ca, from_A, hist, Q->q, Q->n, s)
-                    : launch_con2prim_main(P, ideal_gas(par), ca, from_A, hist, Q->q, Q->n, s);
+         return tab ? launch_con2prim_main(P, tab_eos(par), ca, from_A, hist, queue, qcount, s)
+                    : launch_con2prim_main(P, ideal_gas(par), ca, from_A, hist, queue, qcount, s);
        }))))
     return st;
   if ((st = dev_status(timed(ASX_KC_C2P_REST, s, [&] {
-         return tab ? launch_con2prim_rest(P, tab_eos(par), ca, from_A, hist, Q->q, Q->n, s)
-                    : launch_con2prim_rest(P, ideal_gas(par), ca, from_A, hist, Q->q, Q->n, s);
+         return tab ? launch_con2prim_rest(P, tab_eos(par), ca, from_A, hist, queue, qcount, s)
+                    : launch_con2prim_rest(P, ideal_gas(par), ca, from_A, hist, queue, qcount, s);
        }))))
     return st;
   if (cnt) {
@@ -965,20 +989,37 @@ asx_status asx_bc_fill_many(const asx_patch *pt, int32_t nslot, const int32_t *s
   return dev_status(timed(ASX_KC_BC, (hipStream_t)stream, [&] { return launch_bc_many(kpatch(pt), L, kind, (hipStream_t)stream); }));
 }
 
-asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_state *out,
-                       double c0, double c1, double c2, const asx_params *par,
-                       void *stream) {
+// The fused substep.  bc_kind < 0: the face data of the one-cell transverse
+// halo are computed from the ghost cells (what a neighbour patch's SYNC
+// delivers there).  bc_kind >= 0: face data, aux and (with use_uct) the
+// upwind-CT face data and dB_stag on the interior only, their ghost points
+// filled by that boundary kind -- AsterX_Fluxes followed by the SYNC of
+// AsterX/schedule.ccl:191-193 on a single patch.
+static asx_status substep_impl(const asx_patch *stage, const asx_state *u0,
+                               const asx_state *out, double c0, double c1, double c2,
+                               const asx_params *par, int bc_kind, void *stream) {
   asx_status st = check_geom(stage);
   if (st) return st;
   if ((st = check_par(par))) return st;
   if ((st = check_flux_eos(par))) return st;
   if (!out || (c0 != 0.0 && !u0)) return ASX_ERR_NULL;
+  const bool bc = bc_kind >= 0;
+  if (bc && bc_kind > ASX_BC_COPY) return ASX_ERR_PARAM;
   // Upwind CT reconstructs face data three rows into the transverse ghost
-  // region, which the fused path does not compute; use the unfused entry
-  // points (with a SYNC of the face data) for use_uct.
-  if (par->use_uct) return ASX_ERR_UNSUPPORTED;
-  if (stage->ng < 3 || stage->ng < recon_ghosts(par->reconstruction_method))
+  // region, which the halo computation cannot reach with 3 ghost cells; it
+  // needs those faces from a SYNC (bc_kind >= 0).
+  if (par->use_uct && !bc) return ASX_ERR_UNSUPPORTED;
+  // the halo computation reconstructs one cell into the ghosts (3 more
+  // cells of stencil); with SYNCed faces the stencils and the aux of the
+  // Psi equation (+-2 vertices) need 2
+  if (stage->ng < (bc ? 2 : 3) || stage->ng < recon_ghosts(par->reconstruction_method) ||
+      stage->ng < recon_ghosts(par->loworder_method))
     return ASX_ERR_GHOSTS;
+  if (bc)
+    for (int d = 0; d < 3; ++d) {
+      if (bc_kind == ASX_BC_LINEAR && stage->n[d] < 2) return ASX_ERR_SIZE;
+      if (bc_kind == ASX_BC_PERIODIC && stage->n[d] < stage->ng) return ASX_ERR_SIZE;
+    }
   if ((st = need_range(stage, ASX_ALP, ASX_KZZ))) return st;
   if ((st = need(stage, {ASX_RHO, ASX_VELX, ASX_VELY, ASX_VELZ, ASX_EPS, ASX_PRESS,
                          ASX_ENTROPY, ASX_YE, ASX_BVECX, ASX_BVECY, ASX_BVECZ})))
@@ -1017,11 +1058,12 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
   A.with_T = par->reconstruct_with_temperature;
   A.write_uct = par->use_uct;
   // the faces take their normal field as the curl of the stage's A (what
-  // ComputedBstagFromA would store; nothing else here reads dB_stag)
+  // ComputedBstagFromA would store on the interior faces)
   A.bstag_from_A = 1;
   A.cnt_dev = nullptr;
+  const int fh = bc ? 0 : 1;
   for (int d = 0; d < 3; ++d) {
-    int halo[3] = {1, 1, 1};
+    int halo[3] = {fh, fh, fh};
     halo[d] = 0;
     const Box b = box(stage, 1 << d, halo);
     for (int q = 0; q < 3; ++q) {
@@ -1031,8 +1073,34 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
     if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s, tab); })))) return st;
   }
   {
-    const int halo[3] = {2, 2, 2};
+    const int h = 2 * fh;
+    const int halo[3] = {h, h, h};
     if ((st = dev_status(timed(ASX_KC_AUX, s, [&] { return launch_aux(P, box(stage, 7, halo), s); })))) return st;
+  }
+  if (bc) {
+    BCList L;
+    L.nf = 0;
+    auto add = [&](int a, int b) {
+      for (int q = a; q <= b; ++q) {
+        L.g[L.nf] = P.g[q];
+        L.cent[L.nf] = centering(q);
+        ++L.nf;
+      }
+    };
+    add(ASX_FXDENS, ASX_FZBZ);
+    add(ASX_FX, ASX_G);
+    if (par->use_uct) {
+      // dB_stag on the interior faces (AsterX_ComputedBstagFromA), SYNCed
+      // with the face data the upwind CT reconstructs
+      const int halo[3] = {0, 0, 0};
+      for (int d = 0; d < 3; ++d)
+        if ((st = dev_status(timed(ASX_KC_BSTAG, s, [&] { return launch_bstag(d, P, box(stage, 1 << d, halo), P.g[ASX_DBX_STAG + d], s); }))))
+          return st;
+      add(ASX_VTILDE_Y_XFACE, ASX_AMIN_ZFACE);
+      add(ASX_DBX_STAG, ASX_DBZ_STAG);
+    }
+    if ((st = dev_status(timed(ASX_KC_BC, s, [&] { return launch_bc_many(P, L, bc_kind, s); }))))
+      return st;
   }
   StatePtrs pu0, pout;
   for (int q = 0; q < ASX_NUM_STATE; ++q) {
@@ -1040,6 +1108,19 @@ asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_st
     pout.g[q] = out->gf[q].data;
   }
   return dev_status(timed(ASX_KC_UPDATE, s, [&] { return launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, variant_of(ASX_KC_UPDATE), s); }));
+}
+
+asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_state *out,
+                       double c0, double c1, double c2, const asx_params *par,
+                       void *stream) {
+  return substep_impl(stage, u0, out, c0, c1, c2, par, -1, stream);
+}
+
+asx_status asx_substep_bc(const asx_patch *stage, const asx_state *u0, const asx_state *out,
+                          double c0, double c1, double c2, const asx_params *par,
+                          int32_t bc_kind, void *stream) {
+  if (bc_kind < 0) return ASX_ERR_PARAM;
+  return substep_impl(stage, u0, out, c0, c1, c2, par, bc_kind, stream);
 }
 
 asx_status asx_c2p_batch(int64_t nz, const double *cons, const double *g,

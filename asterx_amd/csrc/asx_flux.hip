@@ -149,6 +149,7 @@ template <int D> ASX_DEV double bstag_value(const KPatch &P, int i, int j, int k
 template <int D, bool TAB = false>
 ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
                        const Side &L, const Side &R, bool lo0, bool lo1) {
+  ASX_CONTRACT
   constexpr int CF = 1 << D;      // face centering
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
   const int ofc = off(P, CF, i, j, k);
@@ -506,6 +507,7 @@ void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
 
 // CalcAuxForAvecPsi (AsterX/src/fluxes.cxx:803-825) at vertex (i, j, k)
 ASX_DEV void aux_point(const KPatch &P, int i, int j, int k) {
+  ASX_CONTRACT
   const int ov = off(P, 7, i, j, k);
   double A[3];
   {

@@ -102,6 +102,7 @@ ASX_DEV double cell_avg(const double *g, int o, int sj, int sk) {
 template <int ORD>
 ASX_DEV void source_cell(const KPatch &P, int i, int j, int k, const RhsArgs &A,
                          double r[7]) {
+  ASX_CONTRACT
   const int oc = off(P, 0, i, j, k), ov = off(P, 7, i, j, k);
   const int sj = P.sj[7], sk = P.sk[7];
   double c[8];
@@ -262,6 +263,7 @@ __global__ __launch_bounds__(256) void k_sources(KPatch P, Box B, RhsArgs A) {
 
 // -(idx . dF) of one flux variable at cell o (AsterX/src/rhs.cxx:107-114)
 ASX_DEV double flux_div(const KPatch &P, int i, int j, int k, int var) {
+  ASX_CONTRACT
   const double *idx = P.idx;
   const int o1 = off(P, 1, i, j, k), o2 = off(P, 2, i, j, k), o4 = off(P, 4, i, j, k);
   const double *Fx = P.g[ASX_FXDENS + var], *Fy = P.g[ASX_FYDENS + var],
@@ -352,6 +354,7 @@ ASX_DEV void recon_rt(const KPatch &P, int m, int slot, int dir, int i, int j, i
 // A_rhs on edges along I (AsterX/src/rhs.cxx:116-186)
 template <int I>
 ASX_DEV double avec_rhs(const KPatch &P, int i, int j, int k, const RhsArgs &A) {
+  ASX_CONTRACT
   constexpr int J = (I + 1) % 3, K = (I + 2) % 3;
   constexpr int CE = 7 ^ (1 << I); // edge centering
   double E;
@@ -395,6 +398,7 @@ __global__ __launch_bounds__(256) void k_rhs_avec(KPatch P, Box B, RhsArgs A) {
 
 // Psi_rhs at vertices (AsterX/src/rhs.cxx:227-252, aster_fd.hxx:20-36)
 ASX_DEV double psi_rhs(const KPatch &P, int i, int j, int k, const RhsArgs &A) {
+  ASX_CONTRACT
   const int ov = off(P, 7, i, j, k);
   if (A.gauge == ASX_GAUGE_ALGEBRAIC) return 0.0;
   double dF = 0.0;
@@ -539,6 +543,7 @@ __global__ __launch_bounds__(256) void k_bc_batch(const KPatch *__restrict__ tab
 struct Comb {
   double c0, c1, c2;
   ASX_DEV double operator()(double u0, double u, double r) const {
+  ASX_CONTRACT
     const double acc = (c0 != 0.0) ? c0 * u0 + c1 * u : c1 * u;
     return acc + c2 * r;
   }
@@ -565,6 +570,7 @@ template <int ORD> constexpr int kUpdateWaves = ORD == 2 ? 4 : 3;
 template <int ORD>
 ASX_DEV void update_point(const KPatch &P, int i, int j, int k, const RhsArgs &A,
                           const StatePtrs &u0, const StatePtrs &out, const Comb &cb) {
+  ASX_CONTRACT
   const bool cx = i < P.ng + P.n[0], cy = j < P.ng + P.n[1], cz = k < P.ng + P.n[2];
   if (cx && cy && cz) {
     double r[7];

@@ -40,17 +40,20 @@ def all_slots(use_uct=False):
 
 
 class Evolution:
-    def __init__(self, backend, patch, par, bc="linear", tmunu=False):
+    def __init__(self, backend, patch, par, bc="linear", tmunu=False, fused=False):
         """tmunu: also set the stress-energy tensor after every con2prim
         (TmunuBaseX's SetTmunuVars, which runs after AsterX_Con2PrimGroup in
         ODESolvers_PostStep, AsterX/schedule.ccl:119), as a spacetime
-        coupled to the fluid needs."""
+        coupled to the fluid needs.  fused: the RHS group is one call of
+        substep_bc (the fused substep with the face data SYNCed by the
+        boundary fill) instead of Fluxes, SourceTerms and RHS."""
         self.b = backend
         self.p = patch
         self.par = par
         self.bc = abi.BC[bc] if isinstance(bc, str) else bc
         self.uct = bool(par.use_uct)
         self.with_tmunu = tmunu
+        self.fused = fused
 
     def sync(self, names):
         ps = self.p.struct()
@@ -74,6 +77,11 @@ class Evolution:
 
     def rhs(self):
         p, b, par = self.p, self.b, self.par
+        if self.fused:
+            # out = 0 u0 + 0 u + 1 rhs(u) into the RHS grid functions
+            b.substep_bc(p.struct(), None, p.state(abi.STATE_RHS), 0.0, 0.0, 1.0, par,
+                         self.bc)
+            return
         b.fluxes(p.struct(), par, 0)
         self.sync(SYNC_FLUXES + (SYNC_UCT if self.uct else []))
         b.source_terms(p.struct(), par)
@@ -175,6 +183,13 @@ class Backend:
 
     def rhs(self, ps, par):
         self._call("rhs", ps_ref(ps), ref(par), *self._s())
+
+    def substep_bc(self, ps, u0, out, c0, c1, c2, par, kind):
+        """out = c0 u0 + c1 u + c2 rhs(u), face data SYNCed by the boundary
+        fill `kind` (asx_substep_bc / orc_substep_bc)."""
+        u0r = ref(u0) if u0 is not None else None
+        self._call("substep_bc", ps_ref(ps), u0r, ref(out), c0, c1, c2, ref(par), kind,
+                   *self._s())
 
     def tmunu(self, ps, par):
         """eT_{mu nu} += T_{mu nu} at the interior vertices (AsterX_Tmunu)."""

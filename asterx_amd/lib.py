@@ -44,6 +44,8 @@ SIGNATURES = {
                               C.c_int32, _V], C.c_int),
     "asx_substep": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, _V],
                     C.c_int),
+    "asx_substep_bc": ([_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, C.c_int32,
+                        _V], C.c_int),
     "asx_c2p_batch": ([C.c_int64, _V, _V, _V, _V, _V, _V, _PP, _V], C.c_int),
     "asx_zone_sample": ([C.c_int64, C.c_uint64, C.c_double, C.c_double, _V, _V, _V, _V,
                          _V], C.c_int),

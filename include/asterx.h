@@ -435,10 +435,25 @@ asx_status asx_bc_fill_many(const asx_patch *patch, int32_t nslot,
  * held in `stage` and rhs is SourceTerms + RHS of the reference.  u0 is not
  * read when c0 == 0.  Face scratch is allocated on first use per patch shape
  * and kept (one buffer set per device and stream).  use_uct is not supported
- * here (ASX_ERR_UNSUPPORTED): run the unfused entry points for it. */
+ * here (ASX_ERR_UNSUPPORTED): the upwind CT reads face data three rows into
+ * the ghost region, which needs a SYNC of the faces (asx_substep_bc). */
 asx_status asx_substep(const asx_patch *stage, const asx_state *u0,
                        const asx_state *out, double c0, double c1, double c2,
                        const asx_params *par, void *stream);
+
+/* asx_substep for a patch whose ghost zones are filled by a physical
+ * boundary condition rather than by neighbour patches: the face fluxes, the
+ * vertex aux and, with use_uct, the upwind-CT face data and dB*_stag
+ * (AsterX_ComputedBstagFromA) are computed on the interior only and their
+ * ghost points filled with bc_kind (ASX_BC_*), which is AsterX_Fluxes
+ * followed by the SYNC of AsterX/schedule.ccl:191-193 on a single patch;
+ * then the same RHS and combination as asx_substep.  Supports use_uct (the
+ * upwind CT reconstructs the SYNCed face data, AsterX/src/rhs.cxx:124-168).
+ * bc_kind out of range: ASX_ERR_PARAM; too few interior points for the
+ * fill: ASX_ERR_SIZE. */
+asx_status asx_substep_bc(const asx_patch *stage, const asx_state *u0,
+                          const asx_state *out, double c0, double c1, double c2,
+                          const asx_params *par, int32_t bc_kind, void *stream);
 
 /* PostStep half of a substep, fused: AsterX_ComputedBstagFromA +
  * AsterX_ComputedBFromdBstag + AsterX_Con2Prim on interior cells

@@ -33,6 +33,7 @@ _SIGS = {
     "orc_state_lincomb": [_P, _S, C.c_double, C.c_int, _D, C.POINTER(_S)],
     "orc_bc_fill": [_P, C.c_int, C.c_int],
     "orc_substep": [_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP],
+    "orc_substep_bc": [_P, _S, _S, C.c_double, C.c_double, C.c_double, _PP, C.c_int],
     "orc_c2p_batch": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                       C.c_void_p, C.c_void_p, C.c_void_p, _PP],
     "orc_prim2con_batch": [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, _PP],
@@ -48,6 +49,7 @@ _SIGS = {
     "orc_reconstruct": [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _PP,
                         C.c_void_p],
     "orc_util": [C.c_int, C.c_void_p, C.c_void_p],
+    "orc_set_threads": [C.c_int],
     "asx_gf_centering": [C.c_int],
 }
 

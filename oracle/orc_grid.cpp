@@ -48,6 +48,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <omp.h>
 #include <vector>
 
 using namespace orc;
@@ -1419,6 +1420,9 @@ int orc_bc_fill(const asx_patch *pt, int slot, int kind) {
 // with face_halo = 1, sources, RHS, then out = c0*u0 + c1*u + c2*rhs on
 // interior points (c0 == 0: u0 not read).  Same arithmetic as the unfused
 // chain; used as the CPU baseline of the stencil half of a substep.
+static int combine_state(const asx_patch *pt, const asx_state *u0, const asx_state *out,
+                         double c0, double c1, double c2);
+
 int orc_substep(const asx_patch *pt, const asx_state *u0, const asx_state *out,
                 double c0, double c1, double c2, const asx_params *par) {
   int st = orc_bstag_from_A_halo(pt, 1);
@@ -1429,6 +1433,37 @@ int orc_substep(const asx_patch *pt, const asx_state *u0, const asx_state *out,
   if (st) return st;
   st = orc_rhs(pt, par);
   if (st) return st;
+  return combine_state(pt, u0, out, c0, c1, c2);
+}
+
+// asx_substep_bc on the CPU, the unfused chain of the reference's RHS group
+// on a single patch (AsterX/schedule.ccl:169-228): dB_stag from A on the
+// interior faces, fluxes and aux on the interior, the SYNC of fluxes, aux
+// and (use_uct) the upwind-CT face data and dB_stag as the physical
+// boundary fill bc_kind, sources, RHS, then out = c0 u0 + c1 u + c2 rhs.
+int orc_substep_bc(const asx_patch *pt, const asx_state *u0, const asx_state *out,
+                   double c0, double c1, double c2, const asx_params *par, int bc_kind) {
+  if (bc_kind < ASX_BC_LINEAR || bc_kind > ASX_BC_COPY) return ASX_ERR_PARAM;
+  int st = orc_bstag_from_A(pt);
+  if (st) return st;
+  if (par->use_uct)
+    for (int s = ASX_DBX_STAG; s <= ASX_DBZ_STAG; ++s)
+      if ((st = orc_bc_fill(pt, s, bc_kind))) return st;
+  if ((st = orc_fluxes(pt, par, 0))) return st;
+  for (int s = ASX_FXDENS; s <= ASX_FZBZ; ++s)
+    if ((st = orc_bc_fill(pt, s, bc_kind))) return st;
+  for (int s = ASX_FX; s <= ASX_G; ++s)
+    if ((st = orc_bc_fill(pt, s, bc_kind))) return st;
+  if (par->use_uct)
+    for (int s = ASX_VTILDE_Y_XFACE; s <= ASX_AMIN_ZFACE; ++s)
+      if ((st = orc_bc_fill(pt, s, bc_kind))) return st;
+  if ((st = orc_source_terms(pt, par))) return st;
+  if ((st = orc_rhs(pt, par))) return st;
+  return combine_state(pt, u0, out, c0, c1, c2);
+}
+
+static int combine_state(const asx_patch *pt, const asx_state *u0, const asx_state *out,
+                         double c0, double c1, double c2) {
   static const int sslot[ASX_NUM_STATE] = {ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ,
                                            ASX_TAU,  ASX_DENT, ASX_DYE,  ASX_AVEC_X,
                                            ASX_AVEC_Y, ASX_AVEC_Z, ASX_PSI};
@@ -1698,6 +1733,14 @@ int orc_tab_interp(int64_t n, const double *lr, const double *lt, const double *
   for (int64_t i = 0; i < n; ++i)
     for (int v = 0; v < 4; ++v) out[v * n + i] = e.interpolate(v, lr[i], lt[i], ye[i]);
   return ASX_OK;
+}
+
+// Threads of the oracle's OpenMP loops (the CPU baseline times one core and
+// all cores in one process); returns the previous setting.
+int orc_set_threads(int n) {
+  const int old = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return old;
 }
 
 // Per-point pieces the reference's unit tests (AsterX/src/unit_tests/*.cxx)

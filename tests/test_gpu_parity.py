@@ -468,3 +468,55 @@ def test_con2prim_known_answer(prime):
     _cmp(h, d, C2P_OUT, tol=1e-12)
     for name, want in (("rho", 0.125), ("eps", 0.8), ("press", 0.1)):
         assert np.abs(d.numpy(name)[sl] - want).max() <= 1e-7 * want, name
+
+
+@pytest.mark.parametrize("uct", [0, 1])
+@pytest.mark.parametrize("bc", ["linear", "periodic", "copy"])
+def test_substep_bc_matches_oracle(uct, bc):
+    """asx_substep_bc (face data on the interior, SYNCed by the boundary
+    fill, upwind CT when use_uct) against the oracle's unfused chain
+    orc_substep_bc on a curved seeded state, SSPRK3 stage-2 coefficients."""
+    _torch()
+    from asterx_amd.patch import StateBuf
+    h, d = _pair(curved=True)
+    par = abi.params(reconstruction_method="wenoz", flux_type="HLLE", gl_gamma=5.0 / 3.0,
+                     use_uct=uct, lorenz_damp_fac=0.2)
+    c0, c1, c2 = 0.75, 0.25, 0.25 * 0.01
+    outs = []
+    for p, is_gpu in ((d, True), (h, False)):
+        u0 = StateBuf(p)
+        out = StateBuf(p)
+        for name in abi.STATE:
+            u0.arr[name] = p[name].clone() * 1.01 if is_gpu else p[name] * 1.01
+        ps, u0s, os_ = p.struct(), u0.struct(), out.struct()
+        if is_gpu:
+            abi.check("asx_substep_bc", _gpu().lib.asx_substep_bc(
+                C.byref(ps), C.byref(u0s), C.byref(os_), c0, c1, c2, C.byref(par),
+                abi.BC[bc], None))
+            _sync()
+        else:
+            abi.check("orc_substep_bc", oracle.lib().orc_substep_bc(
+                C.byref(ps), C.byref(u0s), C.byref(os_), c0, c1, c2, C.byref(par),
+                abi.BC[bc]))
+        outs.append(out)
+    bad = {}
+    for name in abi.STATE:
+        sl = h.interior(name)
+        e = rel_err(outs[0].arr[name].cpu().numpy()[sl], outs[1].arr[name][sl])
+        if not e <= TOL:
+            bad[name] = e
+    assert not bad, bad
+
+
+def test_substep_refuses_uct_without_sync():
+    """asx_substep computes the face halo from ghost cells and cannot reach
+    the three face rows upwind CT reconstructs: ASX_ERR_UNSUPPORTED."""
+    _torch()
+    from asterx_amd.patch import StateBuf
+    h, d = _pair(curved=True)
+    par = abi.params(use_uct=1)
+    out = StateBuf(d)
+    assert _gpu().lib.asx_substep(C.byref(d.struct()), None, C.byref(out.struct()), 0.0,
+                                  1.0, 0.01, C.byref(par), None) == 4
+    assert _gpu().lib.asx_substep_bc(C.byref(d.struct()), None, C.byref(out.struct()), 0.0,
+                                     1.0, 0.01, C.byref(par), 7, None) == 3

@@ -143,15 +143,16 @@ def cut(p, d, idx):
     return out
 
 
-def run_case(name, backend, device=None, iterations=None):
-    """Evolve a case; returns {iteration: {var: values on the golden cut}}."""
+def run_case(name, backend, device=None, iterations=None, fused=False):
+    """Evolve a case; returns {iteration: {var: values on the golden cut}}.
+    fused: the RHS group through the fused substep with SYNCed faces."""
     c = CASES[name]
     gold = np.load(os.path.join(GOLDEN, "tsv_%s.npz" % name))
     dirs = cut_dirs(c)
     its = list(gold["it_" + dirs[0]])
     p = make_patch(c, device)
     par = case_params(c)
-    ev = Evolution(backend, p, par, bc=c["bc"])
+    ev = Evolution(backend, p, par, bc=c["bc"], fused=fused)
     ev.initial(_check_prims_for(device))
     dt = c["dtfac"] * min(p.dx)
     step = ev.rk4_step if c["rk"] == "rk4" else ev.rk2_step
