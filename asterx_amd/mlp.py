@@ -163,11 +163,21 @@ def evaluate(L, par, gamma, blob, nz=1 << 20, seed=99, kguess=8, reps=3, stream=
             if mode == "mlp":
                 r["fallback_rate"] = float(fb.cpu().numpy().mean())
             out["%s_%s" % (sname, mode)] = r
-    # accuracy of the guess itself
+    # accuracy of the guess itself, and the forward pass's launch time
     g2 = torch.empty((2, nz), **f64)
     abi.check("asx_mlp_forward", L.asx_mlp_forward(nz, _p(z.cons), _p(z.g), _p(bt), _p(g2),
                                                    stream))
     torch.cuda.synchronize()
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    for _ in range(reps):
+        abi.check("asx_mlp_forward", L.asx_mlp_forward(nz, _p(z.cons), _p(z.g), _p(bt),
+                                                       _p(g2), stream))
+    torch.cuda.synchronize()
+    nl, ms = C.c_int64(), C.c_double()
+    abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["mlp"], C.byref(nl),
+                                                  C.byref(ms)))
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    out["forward_ms"] = ms.value / max(1, nl.value)
     t = z.target.cpu().numpy()
     gh = g2.cpu().numpy()
     out["guess_abs_err"] = dict(logW_median=float(np.median(np.abs(gh[0] - t[0]))),

@@ -33,6 +33,14 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_PEAK_TFS = 78.6           # MI355X FP64, TFLOP/s
+BF16_PEAK_TFS = 2500.0         # MI355X dense BF16 matrix, TFLOP/s
+# Algorithmic FLOPs of the ideal-gas con2prim (Noble, 1e-10): per Newton
+# iteration and for the rest of a zone, pinned by the flop-counting oracle
+# build (tests/test_c2p_flops.py).
+C2P_FLOPS_PER_ITER = 95
+C2P_FLOPS_SETUP = 445
+# The guess network 5 -> 64 -> 64 -> 2: 2 (5 64 + 64 64 + 64 2) per zone
+MLP_FLOPS_FORWARD = 9088
 
 # The primitives the reference SYNCs after con2prim (AsterX/schedule.ccl:78,
 # with Bvec): what the next substep may read in the ghost zones.
@@ -50,7 +58,6 @@ def parse():
     ap.add_argument("--flux", default="HLLE")
     ap.add_argument("--cpu", choices=["on", "off"], default="on",
                     help="time the CPU oracle on the same inputs")
-    ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--tune", choices=["on", "off"], default="on",
                     help="pick launch-shape variants with asx_autotune_substep "
@@ -73,6 +80,12 @@ def parse():
     ap.add_argument("--tab-c2p-zones", type=int, default=1 << 22)
     ap.add_argument("--tab-bracket", type=float, default=0.03,
                     help="relative half-width of the guided Palenzuela bracket")
+    ap.add_argument("--variants", choices=["on", "off"], default="on",
+                    help="also time WENO-Z + LxF, MP5 + HLLE and MP5 + LxF on the torus")
+    ap.add_argument("--n256", choices=["on", "off"], default="on",
+                    help="also time the torus workload on a 256^3 patch")
+    ap.add_argument("--plain64", choices=["on", "off"], default="on",
+                    help="also time the plain 64^3 Balsara-1 workload (flat metric, PPM)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a device-to-device copy of 1 GiB five times "
                          "(reference kernel for the HBM byte counters)")
@@ -314,6 +327,25 @@ def bench_c2p_guess(args, L):
                      "fallback_rate": round(b["fallback_rate"], 5),
                      "flags_plain": a["flags"], "flags_mlp": b["flags"]}
     out["guess_abs_err"] = ev["guess_abs_err"]
+    # FP64 roofline of the solve: pinned algorithmic FLOPs per zone
+    # (C2P_FLOPS_SETUP + C2P_FLOPS_PER_ITER x the mean primary-solver
+    # iterations; tests/test_c2p_flops.py) x zones/s, and the HBM fraction
+    # of its 228 B per zone (cons 8, metric 6, seed 5 read; 8 primitives and
+    # the flag written, 8 B each; 4 B of iterations)
+    for seed in ("cold", "perturbed"):
+        for mode in ("plain", "mlp"):
+            r = ev["%s_%s" % (seed, mode)]
+            fl = C2P_FLOPS_SETUP + C2P_FLOPS_PER_ITER * r["iters_mean"]
+            tf = r["zones_per_s"] * fl / 1e12
+            out[seed]["%s_fp64" % mode] = {
+                "flops_per_zone": round(fl, 1), "achieved_tflops": round(tf, 3),
+                "peak_tflops": FP64_PEAK_TFS, "frac": round(tf / FP64_PEAK_TFS, 4),
+                "hbm_frac": round(r["zones_per_s"] * 228 / 1e9 / HBM_PEAK_GBS, 4)}
+    fwd = ev["forward_ms"]
+    tf = MLP_FLOPS_FORWARD * args.c2p_zones / (fwd * 1e-3) / 1e12
+    out["mlp_forward"] = {"avg_ms": round(fwd, 4), "flops_per_zone": MLP_FLOPS_FORWARD,
+                          "achieved_tflops": round(tf, 2), "peak_tflops": BF16_PEAK_TFS,
+                          "frac": round(tf / BF16_PEAK_TFS, 4)}
     return out
 
 
@@ -590,37 +622,311 @@ def bench_regrid(p, par, L, sh):
             "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
-def cpu_baseline(host_patch, par, dt, steps):
-    """The CPU oracle (OpenMP over k-planes) on the same patch and inputs:
-    stage-1 substep + SYNC + B from A + con2prim + SYNC, per substep."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_substep_once(L, p, out, par, dt):
+    """One stage-1 substep of the reference's schedule on the CPU oracle:
+    RHS + forward-Euler combination, SYNC of the state, B from A,
+    con2prim, SYNC of the synced primitives."""
+    from asterx_amd import abi
+    bc = abi.BC["copy"]
+    ps = p.struct()
+    os_ = out.struct()
+    abi.check("orc_substep", L.orc_substep(C.byref(ps), None, C.byref(os_),
+                                          0.0, 1.0, dt, C.byref(par)))
+    for n in abi.STATE:
+        p.arr[n], out.arr[n] = out.arr[n], p.arr[n]
+    ps = p.struct()
+    for n in abi.STATE:
+        abi.check("orc_bc_fill", L.orc_bc_fill(C.byref(ps), abi.SLOT[n], bc))
+    abi.check("orc_bstag_from_A", L.orc_bstag_from_A(C.byref(ps)))
+    abi.check("orc_dB_from_bstag", L.orc_dB_from_bstag(C.byref(ps)))
+    abi.check("orc_con2prim", L.orc_con2prim(C.byref(ps), C.byref(par)))
+    for n in SYNC_PRIMS:
+        abi.check("orc_bc_fill", L.orc_bc_fill(C.byref(ps), abi.SLOT[n], bc))
+
+
+def time_oracle(host_patch, par, dt, threads, reps=5):
+    """Median of `reps` stage-1 substeps (after one untimed) of the CPU oracle
+    on host_patch with `threads` OpenMP threads; seconds."""
     import numpy as np
     import oracle
-    from asterx_amd import abi
     from asterx_amd.patch import StateBuf
     L = oracle.lib()
-    p = host_patch
-    out = StateBuf(p)
-    bc = abi.BC["copy"]
-    t_all = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        ps = p.struct()
-        os_ = out.struct()
-        abi.check("orc_substep", L.orc_substep(C.byref(ps), None, C.byref(os_),
-                                              0.0, 1.0, dt, C.byref(par)))
-        for n in abi.STATE:
-            p.arr[n], out.arr[n] = out.arr[n], p.arr[n]
-        ps = p.struct()
-        for n in abi.STATE:
-            abi.check("orc_bc_fill", L.orc_bc_fill(C.byref(ps), abi.SLOT[n], bc))
-        abi.check("orc_bstag_from_A", L.orc_bstag_from_A(C.byref(ps)))
-        abi.check("orc_dB_from_bstag", L.orc_dB_from_bstag(C.byref(ps)))
-        abi.check("orc_con2prim", L.orc_con2prim(C.byref(ps), C.byref(par)))
-        for n in SYNC_PRIMS:
-            abi.check("orc_bc_fill", L.orc_bc_fill(C.byref(ps), abi.SLOT[n], bc))
-        t_all.append(time.perf_counter() - t0)
-        assert np.isfinite(p.arr["dens"]).all()
-    return t_all
+    old = L.orc_set_threads(threads)
+    out = StateBuf(host_patch)
+    ts = []
+    try:
+        for r in range(reps + 1):
+            t0 = time.perf_counter()
+            oracle_substep_once(L, host_patch, out, par, dt)
+            if r:
+                ts.append(time.perf_counter() - t0)
+            assert np.isfinite(host_patch.arr["dens"]).all()
+    finally:
+        L.orc_set_threads(old)
+    return sorted(ts)[len(ts) // 2]
+
+
+def host_window(host, m):
+    """An m^3-cell window (with ghosts) at the centre of a host patch, every
+    slot of the oracle's schedule allocated."""
+    from asterx_amd import batch
+    from asterx_amd.driver import all_slots
+    o = tuple((host.n[d] - m) // 2 for d in range(3))
+    names = [s for s in all_slots() if s in host.arr]
+    w = batch.cut_patch(host, (m, m, m), o, names)
+    for s in all_slots():
+        if s not in w:
+            w.alloc(s)
+    return w
+
+
+def cpu_baseline(host, par, dt, label, window_1core=64):
+    """The CPU oracle (g++ -O3 -march=x86-64-v3, OpenMP over k-planes and
+    zones, strict IEEE) on the same inputs: one stage-1 substep incl. SYNC,
+    B from A and con2prim, median of 5 after one warm-up, on all host cores
+    (the whole patch) and on one core (a window of window_1core^3 cells)."""
+    import oracle
+    cores = oracle.lib().orc_set_threads(0)
+    zones = host.n[0] * host.n[1] * host.n[2]
+    t_all = time_oracle(host, par, dt, cores)
+    m = min(window_1core, *host.n)
+    w = host_window(host, m)
+    t_one = time_oracle(w, par, dt, 1)
+    return {"value": round(zones / t_all, 1), "unit": "zone-updates/s", "cores": cores,
+            "kind": "port",
+            "one_core_value": round(m ** 3 / t_one, 1),
+            "cpu_model": cpu_model(),
+            "sample": "CPU oracle (g++ -O3 -march=x86-64-v3 -fopenmp, strict IEEE) on the "
+                      "same %s inputs: one stage-1 substep incl. SYNC, B from A and con2prim; "
+                      "median of 5 after a warm-up: %d threads on the whole patch (%.2f s "
+                      "each), 1 thread on a %d^3 window (%.2f s each)" %
+                      (label, cores, t_all, m, t_one)}
+
+
+def substep_bytes_per_zone(par, first_stage):
+    """Algorithmic HBM bytes of a whole substep per interior zone: the
+    stencil half (stencil_bytes_per_zone) plus con2prim's saved-primitive
+    reads (6) and its writes of 12 primitives, 6 saved primitives, 3 z, 3 s,
+    the flag and the cell mask (32 fields): 736 B, 648 B in a first stage."""
+    return stencil_bytes_per_zone(par, first_stage) + 8 * 32
+
+
+SUBSTEP_KERNELS = ("bstag", "face_x", "face_y", "face_z", "aux", "update", "c2p",
+                   "c2p_rest", "bc")
+
+
+def read_kernel_times(L, K):
+    from asterx_amd import abi
+    kt = {}
+    for name, kc in abi.KCLASS.items():
+        nl, ms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(kc, C.byref(nl), C.byref(ms)))
+        if nl.value:
+            kt[name] = (nl.value, ms.value)
+    return kt
+
+
+def build_torus_patch(n, recon, flux, stream, be):
+    import torch
+    from asterx_amd import abi, torus
+    from asterx_amd.patch import Patch
+    nn, ng, x0, dx = torus.torus_patch_geometry(n=n)
+    spec = torus.TorusSpec()
+    par = torus.eos_params(spec, reconstruction_method=recon, flux_type=flux)
+    slots = abi.ADM + abi.PRIMS + abi.CONS + abi.DB + abi.SAVED + abi.ZS + \
+        ["con2prim_flag", "aster_mask_cc"] + abi.DBSTAG + abi.AVEC
+    with torch.cuda.stream(stream):
+        p = Patch(nn, ng, x0, dx, slots, device="cuda:0")
+        torus.build_torus(p, be, par, spec)
+    torch.cuda.synchronize()
+    # the fused substep keeps its face data in library scratch; con2prim
+    # derives dB from A directly, so the staggered field is not kept.
+    for s in abi.DBSTAG:
+        del p.arr[s]
+    return p, par, 0.2 * dx[0]
+
+
+def tune_variants(be, p, st, par, dt, sh):
+    """Launch shapes from asx_autotune_substep (SSPRK3 stage-2 coefficients
+    into a scratch state; setup, untimed)."""
+    import torch
+    from asterx_amd import abi
+    from asterx_amd.patch import StateBuf
+    scratch = StateBuf(p)
+    ps, u0s, outs = p.struct(), state_struct(p, st.bufs[0]), scratch.struct()
+    abi.check("asx_autotune_substep", be.lib.asx_autotune_substep(
+        C.byref(ps), C.byref(u0s), C.byref(outs), 0.75, 0.25, 0.25 * dt,
+        C.byref(par), 3, sh))
+    torch.cuda.synchronize()
+    del scratch
+    variants = {}
+    for name in ("face_x", "face_y", "face_z", "update"):
+        v = C.c_int32()
+        abi.check("asx_get_variant", be.lib.asx_get_variant(abi.KCLASS[name], C.byref(v)))
+        variants[name] = v.value
+    return variants
+
+
+def time_substeps(st, L, K, warmup):
+    """warmup untimed substeps, then K timed with per-kernel HIP events:
+    (ms per substep of wall time, kernel times, first stages among the K)."""
+    import torch
+    from asterx_amd import abi
+    for _ in range(warmup):
+        st.substep()
+    torch.cuda.synchronize()
+    first = sum(1 for k in range(K) if (k + st.stage) % 3 == 0)
+    abi.check("asx_timing_enable", L.asx_timing_enable(1))
+    t0 = time.perf_counter()
+    for _ in range(K):
+        st.substep()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kt = read_kernel_times(L, K)
+    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    return 1e3 * wall / K, kt, first
+
+
+def substep_roofline(par, zones, K, first, kt, ms_step, workload):
+    """HBM roofline of the whole substep: its algorithmic bytes (736 B per
+    zone, 648 in a first stage, averaged over the timed stages) over the GPU
+    time of its launches per substep (HIP events), beside the same bytes
+    over the wall time per substep."""
+    sub_b = (first * substep_bytes_per_zone(par, True) +
+             (K - first) * substep_bytes_per_zone(par, False)) / K
+    gpu_ms = sum(kt[k][1] for k in SUBSTEP_KERNELS if k in kt) / K
+    per = sub_b * zones
+    ach = per / (gpu_ms * 1e-3) / 1e9
+    traffic = None
+    parts = [read_traffic(k, workload) for k in ("face_x", "face_y", "face_z", "aux",
+                                                 "update", "c2p")]
+    if all(t is not None for t in parts):
+        traffic = sum(parts)
+    return {"bound": "hbm", "scope": "whole substep: RHS + RK combination, SYNC, B from A "
+                                     "+ con2prim, SYNC",
+            "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "bytes_per_substep": int(per), "bytes_per_zone": round(sub_b, 1),
+            "gpu_ms_per_substep": round(gpu_ms, 4),
+            "wall_frac": round(per / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def dominant_kernel(par, n, zones, K, first, kt, workload):
+    """The kernel class with the largest summed time, its algorithmic bytes
+    per launch (each grid function it reads or writes once per point) over
+    its average HIP-event duration."""
+    nf = [(n[d] + 1) * (n[(d + 1) % 3] + 2) * (n[(d + 2) % 3] + 2) for d in range(3)]
+    upd_b = (first * update_bytes(par, True) + (K - first) * update_bytes(par, False)) / K
+    per_launch = {"face_x": face_bytes(par) * nf[0], "face_y": face_bytes(par) * nf[1],
+                  "face_z": face_bytes(par) * nf[2], "update": upd_b * zones,
+                  "c2p": c2p_bytes_per_zone(True) * zones}
+    dom = max((k for k in kt if k in per_launch), key=lambda k: kt[k][1])
+    nl, ms = kt[dom]
+    avg = ms / nl
+    ach = per_launch[dom] / (avg * 1e-3) / 1e9
+    return {"kernel": dom, "avg_ms": round(avg, 4), "bytes_per_launch": int(per_launch[dom]),
+            "achieved": round(ach, 2), "frac": round(ach / HBM_PEAK_GBS, 4),
+            "traffic": read_traffic(dom, workload),
+            "bytes_definition": "face: 9 reconstructed cell fields + normal B + 10 vertex "
+                                "metric fields read, 10 fluxes written per face; update: "
+                                "27 face fluxes, 7 aux, 16 ADM, 9 prims, 11 (+11) state "
+                                "read, 11 written per zone; c2p: 60 fields per zone"}
+
+
+def bench_torus_variant(args, n, recon, flux, steps, warmup, tune, sh, stream, be):
+    """The torus workload at n^3 with one reconstruction / Riemann solver:
+    zone-updates/s and the substep roofline (the patch is freed after)."""
+    import torch
+    p, par, dt = build_torus_patch(n, recon, flux, stream, be)
+    st = Stepper(p, par, be, dt, sh)
+    if tune:
+        tune_variants(be, p, st, par, dt, sh)
+    ms_step, kt, first = time_substeps(st, be.lib, steps, warmup)
+    zones = n ** 3
+    out = {"recon": recon, "flux": flux, "n": n, "steps": steps,
+           "value": round(zones / (ms_step * 1e-3), 1), "unit": "zone-updates/s",
+           "ms_per_step": round(ms_step, 4),
+           "roofline": substep_roofline(par, zones, steps, first, kt, ms_step,
+                                        "torus%d_%s_%s" % (n, recon, flux)),
+           "kernels_ms_per_step": {k: round(v[1] / steps, 4) for k, v in kt.items()}}
+    del st, p
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_plain64(args, sh, stream, be):
+    """The plain small workload: one 64^3 patch with 3 ghosts over
+    [-0.5, 0.5]^3, flat metric, ideal gas Gamma 5/3, PPM + HLLE, Noble with
+    Palenzuela backup, the Balsara-1 shock tube along x with every zone's
+    rho and p multiplied by (1 + 1e-3 u), u ~ U[-1, 1] seeded; B from a
+    staggered A.  Substeps cycled through the SSPRK3 stages at dt = 0.25 dx;
+    the CPU oracle on the same inputs beside it."""
+    import torch
+    from asterx_amd import abi, seeds
+    from asterx_amd.driver import Evolution, all_slots, check_prims_ideal
+    from asterx_amd.patch import Patch
+    n, ng = 64, 3
+    dx = 1.0 / n
+    par = abi.params(reconstruction_method="ppm", flux_type="HLLE", gl_gamma=5.0 / 3.0,
+                     c2p_prime="Noble", c2p_second="Palenzuela", eps_max=1e8, rho_max=1e8,
+                     rho_abs_min=1e-8)
+    slots = abi.ADM + abi.PRIMS + abi.CONS + abi.DB + abi.SAVED + abi.ZS + \
+        ["con2prim_flag", "aster_mask_cc"] + abi.DBSTAG + abi.AVEC
+    with torch.cuda.stream(stream):
+        p = Patch((n, n, n), ng, (-0.5, -0.5, -0.5), (dx, dx, dx), slots, device="cuda:0")
+        seeds.minkowski(p)
+        seeds.balsara(p, "Balsara1", "x", gamma=5.0 / 3.0, perturb=1e-3, seed=100)
+        seeds.post_initial(p, 5.0 / 3.0)
+        Evolution(be, p, par, bc="copy").initial(check_prims_ideal)
+    torch.cuda.synchronize()
+    for s in abi.DBSTAG:
+        del p.arr[s]
+    host = None
+    if args.cpu == "on":
+        host = p.to(None)
+        for s in all_slots():
+            if s not in host:
+                host.alloc(s)
+    dt = 0.25 * dx
+    st = Stepper(p, par, be, dt, sh)
+    K = 30
+    ms_step, kt, first = time_substeps(st, be.lib, K, 6)
+    zones = n ** 3
+    out = {"workload": "Balsara-1 along x, 64^3 +3 ghosts, flat metric, rho and p x (1 + "
+                       "1e-3 u), PPM + HLLE, Noble/Palenzuela, Gamma 5/3",
+           "zones": zones, "steps": K, "value": round(zones / (ms_step * 1e-3), 1),
+           "unit": "zone-updates/s", "ms_per_step": round(ms_step, 4),
+           "roofline": substep_roofline(par, zones, K, first, kt, ms_step, "plain64"),
+           "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()}}
+    for name in abi.CONS:
+        if not bool(torch.isfinite(p.arr[name][p.interior(name)]).all()):
+            raise RuntimeError("non-finite %s in the 64^3 run" % name)
+    del st, p
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if host is not None:
+        import oracle
+        cores = oracle.lib().orc_set_threads(0)
+        t_all = time_oracle(host, par, dt, cores)
+        t_one = time_oracle(host, par, dt, 1, reps=3)
+        out["cpu_baseline"] = {"value": round(zones / t_all, 1), "cores": cores,
+                               "one_core_value": round(zones / t_one, 1),
+                               "sample": "CPU oracle, same inputs, stage-1 substep incl. "
+                                         "con2prim: median of 5 on %d threads, of 3 on 1" %
+                                         cores}
+    return out
 
 
 def main():
@@ -632,36 +938,21 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
-    import numpy as np
     import torch
-    from asterx_amd import abi, lib, torus
+    from asterx_amd import abi, lib
     from asterx_amd.driver import all_slots
-    from asterx_amd.patch import Patch
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
-    dev = "cuda:0"
     torch.cuda.set_device(0)
     stream = torch.cuda.Stream()
     sh = C.c_void_p(stream.cuda_stream)
     be = lib.backend(sh)
-    n, ng, x0, dx = torus.torus_patch_geometry(n=args.n)
-    spec = torus.TorusSpec()
-    par = torus.eos_params(spec, reconstruction_method=args.recon,
-                           flux_type=args.flux)
-    slots = abi.ADM + abi.PRIMS + abi.CONS + abi.DB + abi.SAVED + abi.ZS + \
-        ["con2prim_flag", "aster_mask_cc"] + abi.DBSTAG + abi.AVEC
+    L = be.lib
     t0 = time.perf_counter()
-    with torch.cuda.stream(stream):
-        p = Patch(n, ng, x0, dx, slots, device=dev)
-        torus.build_torus(p, be, par, spec)
-    torch.cuda.synchronize()
-    # the fused substep keeps its face data in library scratch; con2prim
-    # derives dB from A directly, so the staggered field is not kept.
-    for s in abi.DBSTAG:
-        del p.arr[s]
+    p, par, dt = build_torus_patch(args.n, args.recon, args.flux, stream, be)
+    n = p.n
     log(args, "initial data: %.1f s" % (time.perf_counter() - t0))
     zones = n[0] * n[1] * n[2]
-    dt = 0.2 * dx[0]
     host = None
     if args.cpu == "on":
         # same inputs for the CPU oracle, copied before the GPU evolves them
@@ -671,44 +962,10 @@ def main():
                 host.alloc(s)
 
     st = Stepper(p, par, be, dt, sh)
-    variants = {}
-    if args.tune == "on":
-        # SSPRK3 stage-2 coefficients into a scratch state (setup, untimed)
-        from asterx_amd.patch import StateBuf
-        scratch = StateBuf(p)
-        ps, u0s, outs = p.struct(), state_struct(p, st.bufs[0]), scratch.struct()
-        abi.check("asx_autotune_substep", be.lib.asx_autotune_substep(
-            C.byref(ps), C.byref(u0s), C.byref(outs), 0.75, 0.25, 0.25 * dt,
-            C.byref(par), 3, sh))
-        torch.cuda.synchronize()
-        del scratch
-        for name in ("face_x", "face_y", "face_z", "update"):
-            v = C.c_int32()
-            abi.check("asx_get_variant", be.lib.asx_get_variant(abi.KCLASS[name],
-                                                                C.byref(v)))
-            variants[name] = v.value
-        log(args, "tuned variants:", variants)
-    for _ in range(args.warmup):
-        st.substep()
-    torch.cuda.synchronize()
+    variants = tune_variants(be, p, st, par, dt, sh) if args.tune == "on" else {}
+    log(args, "tuned variants:", variants)
     K = args.steps
-    L = be.lib
-    # per-kernel HIP events on the library's stream, over the timed region
-    abi.check("asx_timing_enable", L.asx_timing_enable(1))
-    first_stages = sum(1 for k in range(K) if (k + args.warmup) % 3 == 0)
-    t0 = time.perf_counter()
-    for k in range(K):
-        st.substep()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_step = 1e3 * wall / K
-    kt = {}
-    for name, kc in abi.KCLASS.items():
-        nl, ms = C.c_int64(), C.c_double()
-        abi.check("asx_timing_read", L.asx_timing_read(kc, C.byref(nl), C.byref(ms)))
-        if nl.value:
-            kt[name] = (nl.value, ms.value)
-    abi.check("asx_timing_enable", L.asx_timing_enable(0))
+    ms_step, kt, first = time_substeps(st, L, K, args.warmup)
     log(args, "kernel classes (launches, total ms, avg ms):")
     for name, (nl, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
         log(args, "  %-8s %4d %9.3f %8.4f" % (name, nl, ms, ms / nl))
@@ -719,80 +976,71 @@ def main():
         C.byref(ps), C.byref(par), C.byref(cnt), sh))
     torch.cuda.synchronize()
     log(args, "c2p flags:", list(cnt.c2p_flags))
-    log(args, "c2p iterations:", {i: v for i, v in enumerate(cnt.c2p_iters) if v})
     for name in abi.CONS:
         a = p.arr[name][p.interior(name)]
         if not bool(torch.isfinite(a).all()):
             raise RuntimeError("non-finite %s after the timed substeps" % name)
-
-    # roofline of the dominant kernel class (largest summed time): its
-    # algorithmic bytes per launch over its average launch duration
-    nf = [(n[d] + 1) * (n[(d + 1) % 3] + 2) * (n[(d + 2) % 3] + 2) for d in range(3)]
-    upd_b = (first_stages * update_bytes(par, True) +
-             (K - first_stages) * update_bytes(par, False)) / K
-    per_launch = {"face_x": face_bytes(par) * nf[0],
-                  "face_y": face_bytes(par) * nf[1],
-                  "face_z": face_bytes(par) * nf[2],
-                  "update": upd_b * zones,
-                  "c2p": c2p_bytes_per_zone(True) * zones}
     workload = "torus%d_%s_%s" % (args.n, args.recon, args.flux)
-    dom = max((k for k in kt if k in per_launch), key=lambda k: kt[k][1])
-    nl, ms = kt[dom]
-    avg = ms / nl
-    achieved = per_launch[dom] / (avg * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": read_traffic(dom, workload),
-            "avg_ms": round(avg, 4),
-            "bytes_per_launch": int(per_launch[dom]),
-            "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()},
-            "variants": variants}
-    # the whole stencil half of the substep against the per-zone figure
-    # (480 B/zone, 392 B/zone in stage 1: each field of the substep read or
-    # written once), over the summed time of its kernels
+    roof = substep_roofline(par, zones, K, first, kt, ms_step, workload)
+    roof["dominant_kernel"] = dominant_kernel(par, n, zones, K, first, kt, workload)
+    roof["kernels_ms_per_step"] = {k: round(v[1] / K, 4) for k, v in kt.items()}
+    roof["variants"] = variants
+    roof["c2p_flags"] = list(cnt.c2p_flags)
     sten_ms = sum(kt[k][1] for k in ("bstag", "face_x", "face_y", "face_z", "aux",
                                      "update") if k in kt) / K
-    sten_b = (first_stages * stencil_bytes_per_zone(par, True) +
-              (K - first_stages) * stencil_bytes_per_zone(par, False)) / K
-    roof["substep_stencil_ms"] = round(sten_ms, 4)
-    roof["substep_stencil_frac"] = round(sten_b * zones / (sten_ms * 1e-3) / 1e9 /
-                                         HBM_PEAK_GBS, 4)
+    sten_b = (first * stencil_bytes_per_zone(par, True) +
+              (K - first) * stencil_bytes_per_zone(par, False)) / K
+    roof["stencil_ms"] = round(sten_ms, 4)
+    roof["stencil_frac"] = round(sten_b * zones / (sten_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
 
     batched = None
     if args.batch == "on":
         t0 = time.perf_counter()
         batched = bench_batched(args, p, par, dt, L, sh, stream)
         log(args, "batched: %.1f s" % (time.perf_counter() - t0), batched)
+    t0 = time.perf_counter()
+    tmunu = bench_tmunu(args, p, par, L, sh, workload)
+    log(args, "tmunu: %.1f s" % (time.perf_counter() - t0), tmunu)
+    regrid = bench_regrid(p, par, L, sh)
+    log(args, "regrid:", regrid)
+    del st, p
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+    extra = {}
+    if args.variants == "on":
+        t0 = time.perf_counter()
+        extra["torus_variants"] = [
+            bench_torus_variant(args, args.n, r, f, 9, 3, False, sh, stream, be)
+            for r, f in (("wenoz", "LxF"), ("mp5", "HLLE"), ("mp5", "LxF"))]
+        log(args, "variants: %.1f s" % (time.perf_counter() - t0), extra["torus_variants"])
+    if args.n256 == "on":
+        t0 = time.perf_counter()
+        extra["torus256"] = bench_torus_variant(args, 256, args.recon, args.flux, 9, 3,
+                                                args.tune == "on", sh, stream, be)
+        log(args, "256^3: %.1f s" % (time.perf_counter() - t0), extra["torus256"])
+    if args.plain64 == "on":
+        t0 = time.perf_counter()
+        extra["plain64"] = bench_plain64(args, sh, stream, be)
+        log(args, "plain 64^3: %.1f s" % (time.perf_counter() - t0), extra["plain64"])
+
     guess = None
     if args.c2p_guess == "on":
         t0 = time.perf_counter()
         guess = bench_c2p_guess(args, L)
         log(args, "c2p guess: %.1f s" % (time.perf_counter() - t0), guess)
-
     tab = None
     if args.tab == "on":
         t0 = time.perf_counter()
         tab = bench_tabulated(args, L, sh, stream)
         log(args, "tabulated: %.1f s" % (time.perf_counter() - t0), tab)
 
-    t0 = time.perf_counter()
-    tmunu = bench_tmunu(args, p, par, L, sh, workload)
-    log(args, "tmunu: %.1f s" % (time.perf_counter() - t0), tmunu)
-    regrid = bench_regrid(p, par, L, sh)
-    log(args, "regrid:", regrid)
-
     cpu = None
     if host is not None:
-        ts = cpu_baseline(host, par, dt, args.cpu_steps)
-        cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or \
-            len(os.sched_getaffinity(0))
-        cpu = {"value": round(zones / min(ts), 1), "unit": "zone-updates/s",
-               "cores": cores, "kind": "port",
-               "sample": "CPU oracle (g++ -O2, OpenMP) on the same %d^3 torus "
-                         "inputs: %d stage-1 substep(s) incl. SYNC, B from A "
-                         "and con2prim; best of %d, %.1f s" %
-                         (args.n, len(ts), len(ts), sum(ts))}
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(host, par, dt, "%d^3 torus" % args.n)
+        log(args, "cpu baseline: %.1f s" % (time.perf_counter() - t0))
+        del host
 
     out = {"metric": "zone-updates/s per RK substep",
            "value": round(zones / (ms_step * 1e-3), 1),
@@ -807,6 +1055,7 @@ def main():
            "roofline": roof, "cpu_baseline": cpu,
            "batched": batched, "c2p_guess": guess, "tabulated": tab,
            "tmunu": tmunu, "regrid_error": regrid}
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 
