@@ -456,6 +456,40 @@ struct C2PScratch {
   unsigned long long *cnt; // 80 counters
 };
 
+// Fixed device scratch of asx_con2prim_group_noalloc: the counters (80
+// words at byte 0), the queue count (byte 1024) and the queue (one int per
+// interior zone from byte 2048), for patches of up to kFixedC2PZones
+// interior zones.  The runtime places one copy per device when it loads the
+// code object; it lives until the process ends.  In host code `scratch` is
+// only a handle for hipGetSymbolAddress: every host use goes through the
+// per-device address fixed_c2p_scratch() returns.
+constexpr int64_t kFixedC2PZones = int64_t(1) << 24; // 256^3
+constexpr size_t kFixedC2PQueueByte = 2048;
+static __device__ __align__(256) unsigned long long
+    scratch[kFixedC2PQueueByte / 8 + kFixedC2PZones / 2];
+std::map<int, unsigned char *> g_fixed_c2p;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (under g_mu) and kept.
+asx_status fixed_c2p_scratch(C2PScratch *out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ASX_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(g_mu);
+  unsigned char *&base = g_fixed_c2p[dev];
+  if (!base) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+      g_fixed_c2p.erase(dev);
+      return ASX_ERR_DEVICE;
+    }
+    base = (unsigned char *)p;
+  }
+  out->cnt = (unsigned long long *)base;
+  out->qcount = (unsigned *)(base + 1024);
+  out->queue = (int *)(base + kFixedC2PQueueByte);
+  return ASX_OK;
+}
+
 // Device scratch per (device, stream, tag), grown on demand.
 enum { SCR_MLP_GUESS = 1 };
 struct DevScratch {
@@ -839,11 +873,14 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
   if (from_A && (st = need(pt, {ASX_AVEC_X, ASX_AVEC_Y, ASX_AVEC_Z}))) return st;
   hipStream_t s = (hipStream_t)stream;
   const KPatch P = kpatch(pt);
+  // With caller-fixed buffers the launches are not timed: the timer's event
+  // pool grows by hipEventCreate.
+  auto run = [&](int cls, auto &&f) { return fixed ? f() : timed(cls, s, f); };
   if (from_A && pt->gf[ASX_DBX_STAG].data) {
     if ((st = need(pt, {ASX_DBY_STAG, ASX_DBZ_STAG}))) return st;
     const int halo[3] = {0, 0, 0};
     for (int d = 0; d < 3; ++d)
-      if ((st = dev_status(timed(ASX_KC_BSTAG, s, [&] { return launch_bstag(d, P, box(pt, 1 << d, halo),
+      if ((st = dev_status(run(ASX_KC_BSTAG, [&] { return launch_bstag(d, P, box(pt, 1 << d, halo),
                                         pt->gf[ASX_DBX_STAG + d].data, s); }))))
         return st;
   }
@@ -873,12 +910,12 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
   const C2PArgs ca = c2p_args(par);
   unsigned long long *hist = cd ? cd + 2 : nullptr;
   const bool tab = tabulated(par);
-  if ((st = dev_status(timed(ASX_KC_C2P, s, [&] {
+  if ((st = dev_status(run(ASX_KC_C2P, [&] {
          return tab ? launch_con2prim_main(P, tab_eos(par), ca, from_A, hist, queue, qcount, s)
                     : launch_con2prim_main(P, ideal_gas(par), ca, from_A, hist, queue, qcount, s);
        }))))
     return st;
-  if ((st = dev_status(timed(ASX_KC_C2P_REST, s, [&] {
+  if ((st = dev_status(run(ASX_KC_C2P_REST, [&] {
          return tab ? launch_con2prim_rest(P, tab_eos(par), ca, from_A, hist, queue, qcount, s)
                     : launch_con2prim_rest(P, ideal_gas(par), ca, from_A, hist, queue, qcount, s);
        }))))
@@ -902,6 +939,16 @@ asx_status asx_con2prim(const asx_patch *pt, const asx_params *par, asx_counters
 asx_status asx_con2prim_group(const asx_patch *pt, const asx_params *par,
                               asx_counters *cnt, void *stream) {
   return c2p_common(pt, par, true, cnt, stream);
+}
+
+asx_status asx_con2prim_group_noalloc(const asx_patch *pt, const asx_params *par,
+                                      asx_counters *cnt, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if ((int64_t)pt->n[0] * pt->n[1] * pt->n[2] > kFixedC2PZones) return ASX_ERR_SIZE;
+  C2PScratch f;
+  if ((st = fixed_c2p_scratch(&f))) return st;
+  return c2p_common(pt, par, true, cnt, stream, &f);
 }
 
 asx_status asx_state_lincomb(const asx_patch *pt, const asx_state *dst, double scale,

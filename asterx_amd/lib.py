@@ -30,6 +30,7 @@ SIGNATURES = {
     "asx_prim2con": ([_P, _V], C.c_int),
     "asx_con2prim": ([_P, _PP, _CNT, _V], C.c_int),
     "asx_con2prim_group": ([_P, _PP, _CNT, _V], C.c_int),
+    "asx_con2prim_group_noalloc": ([_P, _PP, _CNT, _V], C.c_int),
     "asx_state_lincomb": ([_P, _S, C.c_double, C.c_int32, C.POINTER(C.c_double),
                            C.POINTER(_S), _V], C.c_int),
     "asx_bc_fill": ([_P, C.c_int32, C.c_int32, _V], C.c_int),

@@ -462,6 +462,21 @@ asx_status asx_substep_bc(const asx_patch *stage, const asx_state *u0,
 asx_status asx_con2prim_group(const asx_patch *patch, const asx_params *par,
                               asx_counters *cnt, void *stream);
 
+/* asx_con2prim_group that allocates nothing: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and results as asx_con2prim_group; the queue of zones the primary
+ * solver failed, its count and the counters live in a fixed device array
+ * of the library (64 MiB + 2 KiB per device, held until the process ends)
+ * instead of library-allocated buffers, and the launches are not timed
+ * (asx_timing_*).  Patches of more than 2^24 interior zones: ASX_ERR_SIZE.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (two streams, or two host threads) race; calls in one stream
+ * order are safe.  The count and the counters are cleared by kernels on the
+ * caller's stream at the start of each call. */
+asx_status asx_con2prim_group_noalloc(const asx_patch *patch, const asx_params *par,
+                                      asx_counters *cnt, void *stream);
+
 /* Con2PrimFactory solve on a batch of independent zones (no grid): the
  * c2p_2DNoble / c2p_1DPalenzuela call of con2prim.cxx:138-306 with the
  * atmosphere and floors of con2prim.cxx:82-133 for r <= r_atmo.
