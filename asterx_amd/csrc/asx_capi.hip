@@ -389,6 +389,7 @@ int variant_count(int kc) {
   if (kc == ASX_KC_FACE_X || kc == ASX_KC_FACE_Y || kc == ASX_KC_FACE_Z) return kFaceVariants;
   if (kc == ASX_KC_UPDATE) return kUpdateVariants;
   if (kc == ASX_KC_TMUNU) return kTmunuVariants;
+  if (kc == ASX_KC_MLP) return 2;
   return 1;
 }
 int variant_of(int kc) {
@@ -1340,12 +1341,19 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
   if (nz > 0 && (!cons || !g || !seed || !blob || !prim || !flag || !iters))
     return ASX_ERR_NULL;
   if (nz == 0) return ASX_OK;
-  double *guess = (double *)dev_scratch(SCR_MLP_GUESS, stream, (size_t)nz * 2 * sizeof(double));
+  // variant 0: the network inside the primary-solver kernel; variant 1: a
+  // forward-pass kernel writing the guesses to library scratch first
+  const int v = variant_of(ASX_KC_MLP);
+  double *guess = nullptr;
+  if (v == 1) {
+    guess = (double *)dev_scratch(SCR_MLP_GUESS, stream, (size_t)nz * 2 * sizeof(double));
+    if (!guess) return ASX_ERR_DEVICE;
+  }
   C2PQueue *Q = c2p_queue(stream, nz);
-  if (!guess || !Q) return ASX_ERR_DEVICE;
+  if (!Q) return ASX_ERR_DEVICE;
   return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] {
     return launch_c2p_batch_mlp(nz, cons, g, seed, blob, kguess, prim, flag, iters, fallback,
-                                ideal_gas(par), c2p_args(par), guess, Q->q, Q->n,
+                                ideal_gas(par), c2p_args(par), guess, Q->q, Q->n, v,
                                 (hipStream_t)stream);
   }));
 }

@@ -286,7 +286,8 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
-                                double *guess, int *queue, unsigned *qcount, hipStream_t s);
+                                double *guess, int *queue, unsigned *qcount, int variant,
+                                hipStream_t s);
 
 // Independent zones with composition, any 3-parameter EOS (asx_eos.hip).
 // mode 0: (rho, T, Ye) -> out[4][n] = press, eps, c_s, entropy;

@@ -3,12 +3,14 @@
 // inference, training (forward + backward + Adam) and con2prim from the
 // guess with the plain guess as fallback.
 //
-// The network is 5 -> 64 tanh -> 64 tanh -> 2.  Its 64x64 layer, and both
-// 64x64 products of its backward pass, run on the matrix cores
-// (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32 accumulation); the first
-// and last layers (5 and 2 wide) run on the vector ALU with bf16-rounded
-// operands and fp32 accumulation.  Weights are kept in fp32 (the blob) and
-// rounded to bf16 where they enter a product.
+// The network is 5 -> 64 tanh -> 64 tanh -> 2.  Inference runs all three
+// layers on the matrix cores (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32
+// accumulation; net_wave), inside the con2prim kernel or as a kernel of its
+// own.  Training runs the 64x64 layer and both 64x64 products of its
+// backward pass on the matrix cores and the first and last layers (5 and 2
+// wide) on the vector ALU with bf16-rounded operands and fp32
+// accumulation.  Weights are kept in fp32 (the blob) and rounded to bf16
+// where they enter a product.
 #include "asx_kernels.hpp"
 #include "asx_rand.hpp"
 
@@ -239,56 +241,6 @@ ASX_DEV void layer2(const uint16_t *h1, const uint16_t *w2s, f32x4 acc[4][4], in
   }
 }
 
-// Forward pass of one wavefront (64 zones, lane = zone).  act is the
-// wavefront's 64x64 scratch tile.  Every work-item of the block calls it
-// (it synchronises the block).
-template <int NIN>
-ASX_DEV void mlp_wave(const float *blob, const uint16_t *w2s, uint16_t *act, const float xn[NIN],
-                      float y[2]) {
-  using L = MlpLayout<NIN>;
-  const int l = threadIdx.x & 63;
-  float xb[NIN];
-  for (int i = 0; i < L::In; ++i) xb[i] = bfr(xn[i]);
-  layer1<NIN>(blob, xb, act, nullptr, l);
-  __syncthreads();
-  f32x4 acc[4][4];
-  layer2(act, w2s, acc, l);
-  __syncthreads();
-  for (int rt = 0; rt < 4; ++rt)
-    for (int ob = 0; ob < 4; ++ob)
-      for (int r = 0; r < 4; ++r) {
-        const int zr = rt * 16 + 4 * (l >> 4) + r, o = ob * 16 + (l & 15);
-        act[zr * kTS + o] = f2bf(fast_tanh(acc[rt][ob][r] + blob[L::B2 + o]));
-      }
-  __syncthreads();
-  for (int o = 0; o < L::Out; ++o) {
-    float s = blob[L::B3 + o];
-    for (int k = 0; k < L::H; ++k) s += bfr(blob[L::W3 + o * L::H + k]) * bf2f(act[l * kTS + k]);
-    y[o] = s;
-  }
-}
-
-// Network outputs in target units (log W, log(Z/D)) for every zone.
-template <int NIN>
-__global__ __launch_bounds__(256) void k_mlp_forward(int64_t nz, const double *cons,
-                                                     const double *g, const float *blob,
-                                                     double *out) {
-  using L = MlpLayout<NIN>;
-  __shared__ uint16_t w2s[L::H * kTS];
-  __shared__ uint16_t act[4][L::H * kTS];
-  stage_w2<NIN>(blob, w2s, nullptr);
-  __syncthreads();
-  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = z < nz;
-  float xn[NIN], y[2];
-  zone_inputs<NIN>(live, z, nz, cons, g, blob, xn);
-  mlp_wave<NIN>(blob, w2s, act[threadIdx.x >> 6], xn, y);
-  if (live) {
-    out[z] = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
-    out[nz + z] = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
-  }
-}
-
 template <int NIN>
 __global__ __launch_bounds__(256) void k_mlp_features(int64_t nz, const double *cons,
                                                       const double *g, float *xo) {
@@ -509,13 +461,16 @@ __global__ __launch_bounds__(256) void k_mlp_adam(int nw, const float *partial, 
 // The primary-only pass carries neither the backup solver nor the network,
 // so it runs at the occupancy of the plain solver.
 // ---------------------------------------------------------------------------
-ASX_DEV void guess_zv(const double *guess, int64_t z, int64_t nz, const C2PIn &in, double &Zg,
-                      double &vsq) {
-  const double W = smax(1.0, exp(guess[z]));
+ASX_DEV void guess_zv_of(double lw, double lzd, const C2PIn &in, double &Zg, double &vsq) {
+  const double W = smax(1.0, exp(lw));
   vsq = 1.0 - 1.0 / (W * W);
   const double D = in.cv.dens / sqrt(det3(in.glo));
-  Zg = D * exp(guess[nz + z]);
+  Zg = D * exp(lzd);
   if (!(Zg > 0.0) || !isfinite(Zg) || !isfinite(vsq)) Zg = -1.0; // plain seed instead
+}
+ASX_DEV void guess_zv(const double *guess, int64_t z, int64_t nz, const C2PIn &in, double &Zg,
+                      double &vsq) {
+  guess_zv_of(guess[z], guess[nz + z], in, Zg, vsq);
 }
 
 __global__ __launch_bounds__(256) void k_c2p_guess_main(int64_t nz, const double *cons,
@@ -570,6 +525,244 @@ __global__ __launch_bounds__(256) void k_c2p_guess_rest(int64_t nz, const double
 __global__ void k_zero_count(unsigned *p) { *p = 0u; }
 
 // ---------------------------------------------------------------------------
+// The network on the matrix cores, all three layers, for the 64 zones of a
+// wavefront (lane = zone).  Every layer is evaluated as Out^T = W In^T: in
+// the 16x16x16 product lane l holds rows 4(l/16)..+3 of column l%16 of both
+// the accumulator and the B operand, so one layer's activations are, lane
+// for lane, the next layer's B operand and never leave registers.  Layer 1
+// pads its K (the inputs) to 16, layer 3 its M (the outputs) to 16; biases
+// initialise the fp32 accumulators.  Only the inputs (zone rows to B
+// columns) and the outputs (back to the zone's lane) pass through the
+// wavefront's own LDS rows.
+// ---------------------------------------------------------------------------
+// Orders a wavefront's LDS writes before its later LDS reads of other
+// lanes' data (the rows used here are private to one wavefront, so no
+// workgroup barrier is needed).
+ASX_DEV void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Two floats to bf16 with the hardware conversion (v_cvt_pk_bf16_f32,
+// round to nearest even: the rounding of f2bf for every non-NaN value).
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+ASX_DEV uint32_t pack_bf16(float a, float b) {
+  const bf16x2_t h = __builtin_convertvector(f32x2_t{a, b}, bf16x2_t);
+  return __builtin_bit_cast(uint32_t, h);
+}
+ASX_DEV bf16x4 tanh_bf16x4(const f32x4 &v) {
+  const uint32_t lo = pack_bf16(fast_tanh(v[0]), fast_tanh(v[1]));
+  const uint32_t hi = pack_bf16(fast_tanh(v[2]), fast_tanh(v[3]));
+  return bf16x4{(short)(lo & 0xffffu), (short)(lo >> 16), (short)(hi & 0xffffu),
+                (short)(hi >> 16)};
+}
+
+// The small layers' weights as bf16 in LDS, zero padded: W1 as 64 rows of 8
+// inputs, W3 as 2 rows of 64 (staged once per workgroup with W2).
+struct NetSmall {
+  uint16_t w1[64 * 8];
+  uint16_t w3[2 * 64];
+};
+template <int NIN> ASX_DEV void stage_small(const float *blob, NetSmall &S) {
+  using L = MlpLayout<NIN>;
+  static_assert(L::H == 64 && L::Out == 2, "layer shapes");
+  for (int q = threadIdx.x; q < 64 * 8; q += blockDim.x) {
+    const int j = q / 8, i = q % 8;
+    S.w1[q] = i < NIN ? f2bf(blob[L::W1 + j * L::In + i]) : (uint16_t)0;
+  }
+  for (int q = threadIdx.x; q < 2 * 64; q += blockDim.x) S.w3[q] = f2bf(blob[L::W3 + q]);
+}
+
+// Normalised outputs y[2] of this lane's zone from its normalised inputs
+// xn.  w2s: W2 as bf16 in LDS (stage_w2); S: W1 and W3 (stage_small); xrow:
+// the wavefront's 64 input rows; yrow: its 64 output pairs.  Every lane of
+// the wavefront calls it.  Layer 3 consumes each 16-row block of layer 2's
+// activations as it is produced, so only layer 1's activations (32 VGPRs)
+// are held whole.
+template <int NIN>
+ASX_DEV void net_wave(const float *blob, const uint16_t *w2s, const NetSmall &S, uint4 *xrow,
+                      float (*yrow)[2], const float xn[NIN], float y[2]) {
+  using L = MlpLayout<NIN>;
+  static_assert(NIN <= 8 && L::Out <= 4, "inputs padded to 8 of K = 16; outputs in rows 0..3");
+  const int l = threadIdx.x & 63, r16 = l & 15, q4 = l >> 4, k4 = 4 * q4;
+  const bf16x4 zero4{0, 0, 0, 0};
+  uint32_t pk[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p)
+    pk[p] = pack_bf16(2 * p < NIN ? xn[2 * p] : 0.0f, 2 * p + 1 < NIN ? xn[2 * p + 1] : 0.0f);
+  xrow[l] = make_uint4(pk[0], pk[1], pk[2], pk[3]);
+  wave_lds_sync();
+  // layer 1: A = rows of W1 (inputs k4..k4+3, zero beyond 8), B = X^T,
+  // rows k4..k4+3 (inputs) of column zt*16 + l%16 (zone)
+  bf16x4 hb[4][4];
+  {
+    bf16x4 xb[4];
+#pragma unroll
+    for (int zt = 0; zt < 4; ++zt) {
+      const uint4 v = xrow[zt * 16 + r16];
+      const uint32_t u0 = q4 == 0 ? v.x : q4 == 1 ? v.z : 0u;
+      const uint32_t u1 = q4 == 0 ? v.y : q4 == 1 ? v.w : 0u;
+      xb[zt] = bf16x4{(short)(u0 & 0xffffu), (short)(u0 >> 16), (short)(u1 & 0xffffu),
+                      (short)(u1 >> 16)};
+    }
+#pragma unroll
+    for (int jt = 0; jt < 4; ++jt) {
+      const bf16x4 aw = q4 < 2 ? *reinterpret_cast<const bf16x4 *>(S.w1 + (jt * 16 + r16) * 8 + k4)
+                               : zero4;
+      const int j = jt * 16 + k4;
+      const f32x4 b{blob[L::B1 + j], blob[L::B1 + j + 1], blob[L::B1 + j + 2],
+                    blob[L::B1 + j + 3]};
+#pragma unroll
+      for (int zt = 0; zt < 4; ++zt) hb[jt][zt] = tanh_bf16x4(mfma16(aw, xb[zt], b));
+    }
+  }
+  // layer 2 (A = rows of W2 from LDS, B = layer 1's activations), each
+  // 16-row block fed at once to layer 3 (A = W3 rows o = l%16 < 2, B = that
+  // block); layer 3's rows o = k4 + r: outputs 0 and 1 sit in lanes 0..15
+  f32x4 yv[4];
+  {
+    f32x4 b3{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (q4 == 0 && r < L::Out) b3[r] = blob[L::B3 + r];
+#pragma unroll
+    for (int zt = 0; zt < 4; ++zt) yv[zt] = b3;
+  }
+#pragma unroll
+  for (int jt = 0; jt < 4; ++jt) {
+    const int j = jt * 16 + k4;
+    const f32x4 b{blob[L::B2 + j], blob[L::B2 + j + 1], blob[L::B2 + j + 2],
+                  blob[L::B2 + j + 3]};
+    f32x4 acc[4] = {b, b, b, b};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+      const bf16x4 aw = frag(w2s, jt * 16 + r16, kb * 16 + k4);
+#pragma unroll
+      for (int zt = 0; zt < 4; ++zt) acc[zt] = mfma16(aw, hb[kb][zt], acc[zt]);
+    }
+    const bf16x4 a3 = r16 < L::Out
+                          ? *reinterpret_cast<const bf16x4 *>(S.w3 + r16 * 64 + jt * 16 + k4)
+                          : zero4;
+#pragma unroll
+    for (int zt = 0; zt < 4; ++zt) yv[zt] = mfma16(a3, tanh_bf16x4(acc[zt]), yv[zt]);
+  }
+#pragma unroll
+  for (int zt = 0; zt < 4; ++zt)
+    if (q4 == 0) {
+      yrow[zt * 16 + r16][0] = yv[zt][0];
+      yrow[zt * 16 + r16][1] = yv[zt][1];
+    }
+  wave_lds_sync();
+  y[0] = yrow[l][0];
+  y[1] = yrow[l][1];
+}
+
+// Normalised inputs of zone z from its conserved variables and metric.
+template <int NIN>
+ASX_DEV void zone_inputs_of(bool live, const double *c, const Sym3 &gl, const float *blob,
+                            float xn[NIN]) {
+  using L = MlpLayout<NIN>;
+  float x[NIN] = {};
+  if (live) net_features<NIN>(c, gl, x);
+#pragma unroll
+  for (int i = 0; i < NIN; ++i)
+    xn[i] = live ? (x[i] - blob[L::InMean + i]) * blob[L::InIstd + i] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Con2prim from the network's guess in one kernel: the wavefront evaluates
+// the network for its 64 zones (net_wave), then each lane runs Noble from
+// its zone's guess for at most kguess iterations, as k_c2p_guess_main,
+// queueing failures for k_c2p_guess_rest.  One chunk of 256 zones per
+// workgroup: a loop over chunks made the compiler keep ~100 more VGPRs live
+// (occupancy 1); without it the kernel fits the solver's 3 waves per SIMD
+// (139 VGPRs, no spills) and the network runs in that budget.
+// ---------------------------------------------------------------------------
+template <int NIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_c2p_guess_fused(
+    int64_t nz, const double *cons, const double *g, const double *seed, const float *blob,
+    int kguess, double *prim, double *flag, int32_t *iters, int32_t *fallback, IdealGas eos,
+    C2PArgs a, int *queue, unsigned *qcount) {
+  using L = MlpLayout<NIN>;
+  __shared__ uint16_t w2s[L::H * kTS];
+  __shared__ uint4 xs[4][64];
+  __shared__ float ys[4][64][2];
+  __shared__ NetSmall S;
+  stage_w2<NIN>(blob, w2s, nullptr);
+  stage_small<NIN>(blob, S);
+  __syncthreads();
+  const int wv = threadIdx.x >> 6;
+  {
+    const int64_t base = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t z = base + threadIdx.x;
+    const bool live = z < nz;
+    float xn[NIN], y[2];
+    {
+      double c[L::NCons];
+      Sym3 gl{1.0, 0.0, 0.0, 1.0, 0.0, 1.0};
+      if (live) {
+#pragma unroll
+        for (int q = 0; q < L::NCons; ++q) c[q] = cons[q * nz + z];
+        gl = Sym3{g[z], g[nz + z], g[2 * nz + z], g[3 * nz + z], g[4 * nz + z], g[5 * nz + z]};
+      }
+      zone_inputs_of<NIN>(live, c, gl, blob, xn);
+    }
+    net_wave<NIN>(blob, w2s, S, xs[wv], ys[wv], xn, y);
+    bool fail = false;
+    if (live) {
+      const double lw = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
+      const double lzd = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
+      C2PIn in;
+      batch_zone_in(z, nz, cons, g, seed, a, in);
+      double Zg, vsq;
+      guess_zv_of(lw, lzd, in, Zg, vsq);
+      C2PArgs ak = a;
+      ak.max_iter = kguess;
+      C2POut o;
+      if (con2prim_zone<ASX_C2P_NOBLE>(eos, ak, in, o, Zg, vsq)) {
+        batch_zone_out(z, nz, o, prim, flag);
+        if (fallback) fallback[z] = 0;
+      } else {
+        fail = true;
+      }
+      iters[z] = o.iters;
+    }
+    queue_push(fail, (int)z, queue, qcount);
+  }
+}
+
+// Network outputs in target units (log W, log(Z/D)) for every zone, all
+// three layers on the matrix cores (net_wave); workgroups loop over chunks
+// of 256 zones.
+template <int NIN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_forward(int64_t nz, const double *cons,
+                                                     const double *g, const float *blob,
+                                                     double *out) {
+  using L = MlpLayout<NIN>;
+  __shared__ uint16_t w2s[L::H * kTS];
+  __shared__ uint4 xs[4][64];
+  __shared__ float ys[4][64][2];
+  __shared__ NetSmall S;
+  stage_w2<NIN>(blob, w2s, nullptr);
+  stage_small<NIN>(blob, S);
+  __syncthreads();
+  const int wv = threadIdx.x >> 6;
+  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nz;
+       base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t z = base + threadIdx.x;
+    const bool live = z < nz;
+    float xn[NIN], y[2];
+    zone_inputs<NIN>(live, z, nz, cons, g, blob, xn);
+    net_wave<NIN>(blob, w2s, S, xs[wv], ys[wv], xn, y);
+    if (live) {
+      out[z] = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
+      out[nz + z] = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static dim3 grid256(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
@@ -601,7 +794,9 @@ template <int NIN>
 hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
                               const float *blob, double *out, hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mlp_forward<NIN>, grid256(nz), dim3(256), 0, s, nz, cons, g, blob, out);
+  const int64_t nb = (nz + 255) / 256;
+  hipLaunchKernelGGL(k_mlp_forward<NIN>, dim3((unsigned)(nb < 2048 ? nb : 2048)), dim3(256), 0,
+                     s, nz, cons, g, blob, out);
   return hipGetLastError();
 }
 template <int NIN>
@@ -639,13 +834,21 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
-                                double *guess, int *queue, unsigned *qcount, hipStream_t s) {
+                                double *guess, int *queue, unsigned *qcount, int variant,
+                                hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipError_t e = launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, guess, s);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
-  hipLaunchKernelGGL(k_c2p_guess_main, grid256(nz), dim3(256), 0, s, nz, cons, g, seed, guess,
-                     kguess, prim, flag, iters, fallback, eos, a, queue, qcount);
+  if (variant == 1) {
+    // forward pass into guess[2][nz], then the primary solve from it
+    hipError_t e = launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, guess, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
+    hipLaunchKernelGGL(k_c2p_guess_main, grid256(nz), dim3(256), 0, s, nz, cons, g, seed,
+                       guess, kguess, prim, flag, iters, fallback, eos, a, queue, qcount);
+  } else {
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
+    hipLaunchKernelGGL(k_c2p_guess_fused<kMlpIdeal>, grid256(nz), dim3(256), 0, s, nz, cons, g,
+                       seed, blob, kguess, prim, flag, iters, fallback, eos, a, queue, qcount);
+  }
   const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
   hipLaunchKernelGGL(k_c2p_guess_rest, dim3(g2), dim3(256), 0, s, nz, cons, g, seed, prim,
                      flag, iters, fallback, eos, a, queue, qcount);

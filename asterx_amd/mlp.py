@@ -116,10 +116,14 @@ def save(blob, path=DEFAULT_WEIGHTS):
     np.save(path, np.asarray(blob, dtype=np.float32), allow_pickle=False)
 
 
-def evaluate(L, par, gamma, blob, nz=1 << 20, seed=99, kguess=8, reps=3, stream=None):
+def evaluate(L, par, gamma, blob, nz=1 << 20, seed=99, kguess=8, reps=3, stream=None,
+             separate=False):
     """Plain and learned-guess con2prim on nz fresh zones, for the perturbed
     seed (previous-step primitives) and the cold start: zones/s (median of
-    reps timed runs), iteration histograms, fallback rate, accuracy."""
+    reps timed runs), iteration histograms, fallback rate, accuracy.  The
+    learned guess runs with the network inside the solver kernel ("mlp");
+    separate=True adds the form with a forward-pass kernel of its own
+    ("mlp_separate")."""
     import torch
     z = Zones(L, nz, seed, par, gamma, stream=stream)
     bt = torch.from_numpy(np.asarray(blob, dtype=np.float32)).cuda()
@@ -129,8 +133,12 @@ def evaluate(L, par, gamma, blob, nz=1 << 20, seed=99, kguess=8, reps=3, stream=
     it = torch.empty(nz, dtype=torch.int32, device="cuda")
     fb = torch.empty(nz, dtype=torch.int32, device="cuda")
     out = {}
+    modes = ("plain", "mlp", "mlp_separate") if separate else ("plain", "mlp")
     for sname, sd in (("perturbed", z.seed), ("cold", z.cold)):
-        for mode in ("plain", "mlp"):
+        for mode in modes:
+            abi.check("asx_set_variant", L.asx_set_variant(
+                abi.KCLASS["mlp"], 1 if mode == "mlp_separate" else 0))
+
             def run():
                 if mode == "plain":
                     return L.asx_c2p_batch(nz, _p(z.cons), _p(z.g), _p(sd), _p(prim),
@@ -160,9 +168,10 @@ def evaluate(L, par, gamma, blob, nz=1 << 20, seed=99, kguess=8, reps=3, stream=
                      flags={int(k): int(v) for k, v in
                             zip(*np.unique(fl, return_counts=True))},
                      rho_rel_err_max=float(np.max(np.abs(rho - rho_true) / rho_true)))
-            if mode == "mlp":
+            if mode != "plain":
                 r["fallback_rate"] = float(fb.cpu().numpy().mean())
             out["%s_%s" % (sname, mode)] = r
+    abi.check("asx_set_variant", L.asx_set_variant(abi.KCLASS["mlp"], 0))
     # accuracy of the guess itself, and the forward pass's launch time
     g2 = torch.empty((2, nz), **f64)
     abi.check("asx_mlp_forward", L.asx_mlp_forward(nz, _p(z.cons), _p(z.g), _p(bt), _p(g2),

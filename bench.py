@@ -315,13 +315,18 @@ def bench_c2p_guess(args, L):
     blob = mlp.load()
     par = abi.params(gl_gamma=5.0 / 3.0, eps_max=1e8, rho_max=1e8, rho_abs_min=1e-12,
                      vw_lim=1e3, B_lim=1e3, c2p_tol=1e-10)
-    ev = mlp.evaluate(L, par, 5.0 / 3.0, blob, nz=args.c2p_zones, reps=3)
-    out = {"zones": args.c2p_zones, "kguess": 8, "tolerance": 1e-10}
+    ev = mlp.evaluate(L, par, 5.0 / 3.0, blob, nz=args.c2p_zones, reps=3, separate=True)
+    out = {"zones": args.c2p_zones, "kguess": 8, "tolerance": 1e-10,
+           "mlp": "network on the matrix cores inside the solver kernel",
+           "mlp_separate": "forward-pass kernel, guesses through memory, then the solver"}
     for seed in ("cold", "perturbed"):
         a, b = ev["%s_plain" % seed], ev["%s_mlp" % seed]
+        c = ev["%s_mlp_separate" % seed]
         out[seed] = {"plain_zones_per_s": round(a["zones_per_s"], 1),
                      "mlp_zones_per_s": round(b["zones_per_s"], 1),
                      "speedup": round(b["zones_per_s"] / a["zones_per_s"], 3),
+                     "mlp_separate_zones_per_s": round(c["zones_per_s"], 1),
+                     "speedup_separate": round(c["zones_per_s"] / a["zones_per_s"], 3),
                      "iters_mean_plain": round(a["iters_mean"], 3),
                      "iters_mean_mlp": round(b["iters_mean"], 3),
                      "fallback_rate": round(b["fallback_rate"], 5),

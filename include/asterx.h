@@ -322,7 +322,9 @@ asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
 /* Launch-shape variants and autotuning (no counterpart in the reference).
  * The face kernels (ASX_KC_FACE_*) and the fused update (ASX_KC_UPDATE) are
  * built in several workgroup tile shapes; results do not depend on the
- * variant.  The choice is per device (default 0).  asx_autotune_substep
+ * variant.  ASX_KC_MLP has two forms of asx_c2p_batch_mlp (network fused
+ * into the solver kernel, or a separate forward pass), whose guesses differ
+ * by the fp32 summation order only.  The choice is per device (default 0).  asx_autotune_substep
  * runs asx_substep with the given arguments once untimed and `reps` times
  * timed for every variant, keeps the fastest per kernel class, and leaves the
  * caller's timing state as it was.  It overwrites `out`. */
@@ -544,8 +546,11 @@ asx_status asx_mlp_train_step(int64_t nz, const double *cons, const double *g,
  * from the network's (Z, v^2) for at most kguess iterations; a zone not
  * converged by then runs the full chain from the plain seed, as
  * asx_c2p_batch would (fallback[z] = 1, optional).  iters counts both.
- * nz < 2^31.  The guesses and the queue of unconverged zones live in
- * library scratch per device and stream, grown on demand. */
+ * nz < 2^31.  Variant 0 of ASX_KC_MLP (default) evaluates the network on
+ * the matrix cores inside the primary-solver kernel, so the guesses never
+ * reach memory; variant 1 runs the forward pass as a kernel of its own and
+ * keeps the guesses in library scratch.  The queue of unconverged zones
+ * lives in library scratch per device and stream, grown on demand. */
 asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                              const double *seed, const float *blob, int32_t kguess,
                              double *prim, double *flag, int32_t *iters, int32_t *fallback,

@@ -143,15 +143,20 @@ def test_training_reduces_loss():
     assert last < 0.5, hist
 
 
-def test_c2p_from_guess_matches_plain():
+@pytest.mark.parametrize("variant", [0, 1])
+def test_c2p_from_guess_matches_plain(variant):
     """Zones that fall back run the reference chain from the plain seed and
     agree bit for bit with asx_c2p_batch; zones converged from the guess
-    agree with it to the solver tolerance, with the same flags."""
+    agree with it to the solver tolerance, with the same flags.  Variant 0
+    evaluates the network inside the solver kernel, variant 1 in a kernel of
+    its own."""
     torch = _torch()
     par = _par()
     nz = 20000
     z = _zones(nz, 9, par)
     L = _L()
+    KC_MLP = 11
+    abi.check("asx_set_variant", L.asx_set_variant(KC_MLP, variant))
     blob = torch.from_numpy(_rand_blob(2)).cuda()
     f64 = dict(dtype=torch.float64, device="cuda")
     i32 = dict(dtype=torch.int32, device="cuda")
@@ -172,6 +177,7 @@ def test_c2p_from_guess_matches_plain():
         torch.cuda.synchronize()
         res[mode] = (prim.cpu().numpy(), flag.cpu().numpy(), it.cpu().numpy(),
                      fb.cpu().numpy())
+    abi.check("asx_set_variant", L.asx_set_variant(KC_MLP, 0))
     pp, fp, ip, _ = res["plain"]
     pg, fg, ig, fbg = res["guess"]
     f = fbg == 1
@@ -179,10 +185,14 @@ def test_c2p_from_guess_matches_plain():
     assert np.array_equal(pg[:, f], pp[:, f]) and np.array_equal(fg[f], fp[f])
     ok = (~f) & (fp == 1)
     assert np.array_equal(fg[ok], fp[ok])
+    # both solves stop at Noble's 1e-10 step criterion from different
+    # starting points; the converged (Z, v^2) agree far below it
     rel = np.abs(pg[:, ok] - pp[:, ok]) / (np.abs(pp[:, ok]) + 1e-300)
-    assert np.nanmax(rel[[0, 1, 2, 6]]) < 1e-6
+    worst = np.nanmax(rel[[0, 1, 2, 6]], axis=1)
+    assert worst.max() < 1e-9, worst
     # velocities: compare against the Lorentz factor scale
-    assert np.nanmax(np.abs(pg[3:6, ok] - pp[3:6, ok])) < 1e-6
+    dv = np.nanmax(np.abs(pg[3:6, ok] - pp[3:6, ok]))
+    assert dv < 1e-9, dv
 
 
 def test_trained_network_gains_over_plain_guess():
