@@ -108,7 +108,8 @@ BC = {"linear": 0, "periodic": 1, "copy": 2}
 # kernel classes of asx_timing_read
 KCLASS = {"bstag": 0, "face_x": 1, "face_y": 2, "face_z": 3, "aux": 4,
           "update": 5, "c2p": 6, "bc": 7, "rhs": 8, "other": 9,
-          "c2p_rest": 10, "mlp": 11, "graph": 12, "tmunu": 13, "regrid": 14}
+          "c2p_rest": 10, "mlp": 11, "graph": 12, "tmunu": 13, "regrid": 14,
+          "persist": 15}
 C2P_FLAG = {"INIT": 0, "PRIME": 1, "SECOND": 2, "ENTROPY": 3, "ATMO": 4,
             "AVG": 5, "FAIL": 6}
 

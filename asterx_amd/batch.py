@@ -100,8 +100,12 @@ class Plan:
     def stage(self, s, stream=None):
         abi.check("asx_batch_stage", self.L.asx_batch_stage(self.h, s, stream))
 
-    def step(self, nsteps=1, graph=True, stream=None):
-        abi.check("asx_batch_step", self.L.asx_batch_step(self.h, nsteps, int(graph), stream))
+    def step(self, nsteps=1, graph=True, stream=None, persistent=False):
+        """nsteps SSPRK3 steps: the captured graph (graph=True), plain
+        launches (graph=False), or one persistent launch per stage
+        (persistent=True)."""
+        mode = 2 if persistent else int(graph)
+        abi.check("asx_batch_step", self.L.asx_batch_step(self.h, nsteps, mode, stream))
 
     def autotune(self, reps=3, stream=None):
         abi.check("asx_batch_autotune", self.L.asx_batch_autotune(self.h, reps, stream))

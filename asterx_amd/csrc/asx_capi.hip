@@ -1594,6 +1594,8 @@ struct asx_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_var[4] = {-1, -1, -1, -1};
+  PersistArgs *pargs = nullptr; // one per SSPRK3 stage, device memory
+  int pgrid = 0;                // blocks of a persistent stage launch
 };
 
 namespace {
@@ -1845,6 +1847,71 @@ asx_status plan_capture(asx_batch *b) {
   return ASX_OK;
 }
 
+// The three stages' argument blocks of the persistent form (built once):
+// launch lists of the persistent kernel's fixed shapes (face variant 0 of
+// every direction, update tiles of kPersistUpdateTile = variant 1).
+asx_status plan_persist_prepare(asx_batch *b) {
+  if (b->pargs) return ASX_OK;
+  const asx_params &par = b->par;
+  const RhsArgs R = rhs_args(&par);
+  if (!persist_supported(par.reconstruction_method, R.fd_order) ||
+      par.c2p_prime != ASX_C2P_NOBLE || b->user_bstag)
+    return ASX_ERR_UNSUPPORTED;
+  int ex[3];
+  update_tile_extent(1, ex);
+  if (ex[0] != kPersistUpdateTile[0] || ex[1] != kPersistUpdateTile[1] ||
+      ex[2] != kPersistUpdateTile[2])
+    return ASX_ERR_DEVICE;
+  BatchLaunch *L;
+  asx_status st;
+  auto list = [&](int k, int v, BatchList &out) {
+    if ((st = plan_launch(b, k, v, &L))) return false;
+    out = BatchList{L->box, L->map, L->nmap};
+    return true;
+  };
+  PersistArgs a;
+  std::memset(&a, 0, sizeof(a));
+  for (int d = 0; d < 3; ++d)
+    if (!list(BK_FACE + d, 0, a.face[d]) || !list(BK_BC + d, 0, a.bc[d])) return st;
+  if (!list(BK_AUX, 0, a.aux) || !list(BK_UPDATE, 1, a.update) || !list(BK_C2P, 0, a.c2p))
+    return st;
+  a.A = plan_flux_args(&par);
+  a.R = R;
+  a.eos = ideal_gas(&par);
+  a.ca = c2p_args(&par);
+  a.state.nf = ASX_NUM_STATE;
+  for (int q = 0; q < ASX_NUM_STATE; ++q) {
+    a.state.slot[q] = kStateSlots[q];
+    a.state.cent[q] = centering(kStateSlots[q]);
+  }
+  a.prims.nf = kNumSyncPrims;
+  for (int q = 0; q < kNumSyncPrims; ++q) {
+    a.prims.slot[q] = kSyncPrims[q];
+    a.prims.cent[q] = centering(kSyncPrims[q]);
+  }
+  a.bc_kind = b->bc_kind;
+  a.zbase = b->zbase;
+  a.np = b->np;
+  a.queue = b->queue;
+  a.qcount = b->qcount;
+  std::vector<PersistArgs> v(3, a);
+  for (int s = 0; s < 3; ++s) {
+    const int bi = kStageIn[s], bo = kStageOut[s];
+    v[s].T = b->stab[bi];
+    v[s].U = b->utab[bo];
+    v[s].u0 = b->st[0];
+    v[s].out = b->st[bo];
+    v[s].c0 = kRK3[s][0];
+    v[s].c1 = kRK3[s][1];
+    v[s].c2 = kRK3[s][2] * b->dt;
+  }
+  const int grid = persist_grid(par.reconstruction_method, R.fd_order);
+  if (grid < 8) return ASX_ERR_DEVICE;
+  if ((st = plan_upload(b, v, &b->pargs))) return st;
+  b->pgrid = grid;
+  return ASX_OK;
+}
+
 // a plan runs on the device it was created on
 asx_status plan_device(const asx_batch *b) {
   int d = -1;
@@ -1997,9 +2064,20 @@ asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, vo
   hipStream_t s = (hipStream_t)stream;
   asx_status st = plan_device(plan);
   if (st) return st;
+  if (use_graph < 0 || use_graph > 2) return ASX_ERR_PARAM;
   if (!use_graph) {
     for (int n = 0; n < nsteps && !st; ++n)
       for (int k = 0; k < 3 && !st; ++k) st = plan_stage(plan, k, s);
+    return st;
+  }
+  if (use_graph == 2) {
+    if ((st = plan_persist_prepare(plan))) return st;
+    const int m = plan->par.reconstruction_method, o = rhs_args(&plan->par).fd_order;
+    for (int n = 0; n < nsteps && !st; ++n)
+      for (int k = 0; k < 3 && !st; ++k)
+        st = dev_status(timed(ASX_KC_PERSIST, s, [&] {
+          return launch_stage_persist(plan->pargs + k, m, o, plan->pgrid, s);
+        }));
     return st;
   }
   if ((st = plan_capture(plan))) return st;

@@ -1,0 +1,560 @@
+// Device code of AsterX_Fluxes shared by the face, vertex and curl kernels
+// (asx_flux.hip) and the batch plan's persistent stage kernel
+// (asx_plan.hip): reconstruction, the numerical flux of one face, the face
+// tile with its LDS hand-over, the vertex auxiliaries and the staggered
+// field of one point.
+#pragma once
+#include "asx_kernels.hpp"
+
+namespace asx {
+
+// ---------------------------------------------------------------------------
+// CalcFlux<D> (AsterX/src/fluxes.cxx:216-795) for faces normal to D.
+//
+// Work decomposition: one work-item per cell of a tile that is one cell
+// longer along D than the faces it produces.  Each work-item reconstructs
+// its cell once, both sides (ReconX's per-cell parabola / WENO weights are
+// shared by the two faces of the cell), keeps the low side in registers as
+// the right state of its own face and hands the high side to the next cell
+// along D through LDS, where it is the left state of that face.  Tiles
+// overlap by one cell along D.
+// ---------------------------------------------------------------------------
+
+// The reconstructed primitives of one side of a cell: rho, entropy, Ye,
+// pressure (or temperature), the two transverse B components (D+1, D+2)
+// and the velocity-like vector (v, z or s).
+struct Side {
+  double rho, ent, ye, th, b1, b2, v[3];
+};
+constexpr int kSideVars = 9;
+
+// Face values (lo = the cell's face at -D, up = at +D) of one grid function
+// from its 5-cell stencil along D.  Equal, value for value, to the two
+// sides that ReconX's reconstruct() yields for the faces of this cell.
+template <int M>
+ASX_DEV void cell_recon(const double *p, int oc, int sc, const double ps[5],
+                        const double vs[5], bool is_rho, const RParams &rp,
+                        double &lo, double &up) {
+  double s[5];
+  constexpr int m0 = M == ASX_RECON_GODUNOV ? 2
+                     : (M == ASX_RECON_MINMOD || M == ASX_RECON_MONOCENTRAL) ? 1
+                                                                            : 0;
+#pragma unroll
+  for (int m = m0; m <= 4 - m0; ++m) s[m] = p[oc + (m - 2) * sc];
+  if (M == ASX_RECON_GODUNOV) {
+    lo = s[2];
+    up = s[2];
+  } else if (M == ASX_RECON_MINMOD) {
+    const double sl = minmod(s[3] - s[2], s[2] - s[1]);
+    up = s[2] + sl / 2;
+    lo = s[2] - sl / 2;
+  } else if (M == ASX_RECON_MONOCENTRAL) {
+    const double sl = monocentral(s[3] - s[2], s[2] - s[1]);
+    up = s[2] + sl / 2;
+    lo = s[2] - sl / 2;
+  } else if (M == ASX_RECON_PPM) {
+    ppm_cell(s[0], s[1], s[2], s[3], s[4], ps[0], ps[1], ps[3], ps[4], vs[1], vs[3],
+             is_rho, rp, lo, up);
+  } else if (M == ASX_RECON_WENOZ) {
+    wenoz_cell(s[0], s[1], s[2], s[3], s[4], rp.weno_eps, lo, up);
+  } else {
+    up = mp5_side(s[0], s[1], s[2], s[3], s[4], rp.mp5_alpha);
+    lo = mp5_side(s[4], s[3], s[2], s[1], s[0], rp.mp5_alpha);
+  }
+}
+ASX_DEV void cell_recon_lo(int m, const double *p, int oc, int sc, double &lo,
+                           double &up) {
+  const double z[5] = {0, 0, 0, 0, 0};
+  const RParams rp{};
+  if (m == ASX_RECON_MINMOD)
+    cell_recon<ASX_RECON_MINMOD>(p, oc, sc, z, z, false, rp, lo, up);
+  else if (m == ASX_RECON_MONOCENTRAL)
+    cell_recon<ASX_RECON_MONOCENTRAL>(p, oc, sc, z, z, false, rp, lo, up);
+  else
+    cell_recon<ASX_RECON_GODUNOV>(p, oc, sc, z, z, false, rp, lo, up);
+}
+
+// Both sides of cell oc with the low-order fallback of fluxes.cxx:258-330:
+// a side whose rho, entropy, Ye or pressure/temperature is not positive is
+// reconstructed again, every variable, with the low-order method.
+template <int D, int M>
+ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Side &up,
+                        bool &flo, bool &fup) {
+  constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
+  const int sc = step(P, 0, D);
+  double ps[5] = {0, 0, 0, 0, 0}, vs[5] = {0, 0, 0, 0, 0};
+  if (M == ASX_RECON_PPM) {
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+      ps[m] = P.g[ASX_PRESS][oc + (m - 2) * sc];
+      vs[m] = P.g[ASX_VELX + D][oc + (m - 2) * sc];
+    }
+  }
+  auto rc = [&](int slot, bool is_rho, double &l, double &u) {
+    cell_recon<M>(P.g[slot], oc, sc, ps, vs, is_rho, A.rp, l, u);
+  };
+  const int thermo = A.with_T ? ASX_TEMPERATURE : ASX_PRESS;
+  rc(ASX_RHO, true, lo.rho, up.rho);
+  rc(ASX_ENTROPY, false, lo.ent, up.ent);
+  rc(ASX_YE, false, lo.ye, up.ye);
+  rc(thermo, false, lo.th, up.th);
+  flo = (lo.rho <= 0.0) || (lo.ent <= 0.0) || (lo.ye <= 0.0) || (lo.th <= 0.0);
+  fup = (up.rho <= 0.0) || (up.ent <= 0.0) || (up.ye <= 0.0) || (up.th <= 0.0);
+  const bool any = flo || fup;
+  auto fix = [&](int slot, double &l, double &u) {
+    if (any) {
+      double a, b;
+      cell_recon_lo(A.lo_method, P.g[slot], oc, sc, a, b);
+      if (flo) l = a;
+      if (fup) u = b;
+    }
+  };
+  fix(ASX_RHO, lo.rho, up.rho);
+  fix(ASX_ENTROPY, lo.ent, up.ent);
+  fix(ASX_YE, lo.ye, up.ye);
+  fix(thermo, lo.th, up.th);
+  rc(ASX_BVECX + D1, false, lo.b1, up.b1);
+  fix(ASX_BVECX + D1, lo.b1, up.b1);
+  rc(ASX_BVECX + D2, false, lo.b2, up.b2);
+  fix(ASX_BVECX + D2, lo.b2, up.b2);
+  const int v0 = A.rec_var == ASX_RECVAR_V   ? ASX_VELX
+                 : A.rec_var == ASX_RECVAR_Z ? ASX_ZVEC_X
+                                             : ASX_SVEC_X;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    rc(v0 + c, false, lo.v[c], up.v[c]);
+    fix(v0 + c, lo.v[c], up.v[c]);
+  }
+}
+
+// Numerical flux of every conserved variable at face (i, j, k) from the
+// left (0) and right (1) states.
+// dB_stag on the D-face at (i, j, k): the curl of the edge vector
+// potential with first differences (ComputeStaggeredB<D>,
+// AsterX/src/computeBfromA.cxx:32-55).  One expression for the curl kernel
+// and for the face kernels that take it inline, so both give the same value.
+template <int D> ASX_DEV double bstag_value(const KPatch &P, int i, int j, int k) {
+  const double *idx = P.idx;
+  const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
+  if (D == 0) {
+    const int oz = off(P, 3, i, j, k), oy = off(P, 5, i, j, k);
+    return idx[1] * (Az[oz + P.sj[3]] - Az[oz]) - idx[2] * (Ay[oy + P.sk[5]] - Ay[oy]);
+  } else if (D == 1) {
+    const int ox = off(P, 6, i, j, k), oz = off(P, 3, i, j, k);
+    return idx[2] * (Ax[ox + P.sk[6]] - Ax[ox]) - idx[0] * (Az[oz + 1] - Az[oz]);
+  } else {
+    const int oy = off(P, 5, i, j, k), ox = off(P, 6, i, j, k);
+    return idx[0] * (Ay[oy + 1] - Ay[oy]) - idx[1] * (Ax[ox + P.sj[6]] - Ax[ox]);
+  }
+}
+
+template <int D, bool TAB = false>
+ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
+                       const Side &L, const Side &R, bool lo0, bool lo1) {
+  ASX_CONTRACT
+  constexpr int CF = 1 << D;      // face centering
+  constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
+  const int ofc = off(P, CF, i, j, k);
+  const int ov = off(P, 7, i, j, k);
+  const int svj = P.sj[7], svk = P.sk[7];
+  const IdealGas &eos = A.eos;
+
+  // metric at the face
+  const double alp = avg_v2f<D>(P.g[ASX_ALP], ov, svj, svk);
+  double betas[3];
+  betas[0] = avg_v2f<D>(P.g[ASX_BETAX], ov, svj, svk);
+  betas[1] = avg_v2f<D>(P.g[ASX_BETAY], ov, svj, svk);
+  betas[2] = avg_v2f<D>(P.g[ASX_BETAZ], ov, svj, svk);
+  Sym3 g;
+  g.xx = avg_v2f<D>(P.g[ASX_GXX], ov, svj, svk);
+  g.xy = avg_v2f<D>(P.g[ASX_GXY], ov, svj, svk);
+  g.xz = avg_v2f<D>(P.g[ASX_GXZ], ov, svj, svk);
+  g.yy = avg_v2f<D>(P.g[ASX_GYY], ov, svj, svk);
+  g.yz = avg_v2f<D>(P.g[ASX_GYZ], ov, svj, svk);
+  g.zz = avg_v2f<D>(P.g[ASX_GZZ], ov, svj, svk);
+  const double detg = det3(g);
+  const double sqrtg = sqrt(detg);
+  const double isqrtg = 1.0 / sqrtg;
+  const double ialp = 1.0 / alp;
+
+  const double rho[2] = {L.rho, R.rho}, ent[2] = {L.ent, R.ent}, ye[2] = {L.ye, R.ye};
+  const double th[2] = {L.th, R.th};
+  const double irho[2] = {1.0 / rho[0], 1.0 / rho[1]};
+  double eps[2], press[2], cs2t[2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    if constexpr (TAB) {
+      // fluxes.cxx:298-304 and the sound speed of :555-559 from one lookup
+      // at the reconstructed (rho, T, Ye); the reference's csnd from
+      // (rho, eps) inverts eps back to that T within 1e-14 in log T
+      A.tab.state_rt(rho[f], th[f], ye[f], eps[f], press[f], cs2t[f]);
+    } else if (A.with_T) {
+      eps[f] = eos.eps_rt(th[f]);
+      press[f] = eos.press_rt(rho[f], th[f]);
+    } else {
+      press[f] = th[f];
+      eps[f] = th[f] * irho[f] * eos.igm1;
+    }
+  }
+  double rhoh[2];
+  rhoh[0] = rho[0] + rho[0] * eps[0] + press[0];
+  rhoh[1] = rho[1] + rho[1] * eps[1] + press[1];
+
+  // magnetic field: normal from the staggered field, transverse recon
+  double Bs[3][2];
+  {
+    const double bn = A.bstag_from_A ? bstag_value<D>(P, i, j, k) : P.g[ASX_DBX_STAG + D][ofc];
+    const double val = bn * isqrtg;
+    Bs[D][0] = val;
+    Bs[D][1] = val;
+    Bs[D1][0] = L.b1;
+    Bs[D1][1] = R.b1;
+    Bs[D2][0] = L.b2;
+    Bs[D2][1] = R.b2;
+  }
+
+  double vel[3][2], vlow[3][2], W[2], iW[2];
+  auto lower2 = [&](const double a[3][2], double o[3][2]) {
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const double v[3] = {a[0][f], a[1][f], a[2][f]};
+      double l[3];
+      lower(g, v, l);
+      o[0][f] = l[0];
+      o[1][f] = l[1];
+      o[2][f] = l[2];
+    }
+  };
+  auto dot2 = [](const double a[3][2], const double b[3][2], int f) {
+    return 0.0 + a[0][f] * b[0][f] + a[1][f] * b[1][f] + a[2][f] * b[2][f];
+  };
+  double rv[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    rv[c][0] = L.v[c];
+    rv[c][1] = R.v[c];
+  }
+  if (A.rec_var == ASX_RECVAR_V) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      vel[c][0] = rv[c][0];
+      vel[c][1] = rv[c][1];
+    }
+    lower2(vel, vlow);
+    iW[0] = sqrt(1 - dot2(vlow, vel, 0));
+    iW[1] = sqrt(1 - dot2(vlow, vel, 1));
+    W[0] = 1 / iW[0];
+    W[1] = 1 / iW[1];
+  } else if (A.rec_var == ASX_RECVAR_Z) {
+    double zl[3][2];
+    lower2(rv, zl);
+    W[0] = sqrt(1 + dot2(zl, rv, 0));
+    W[1] = sqrt(1 + dot2(zl, rv, 1));
+    iW[0] = 1 / W[0];
+    iW[1] = 1 / W[1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        vel[c][f] = rv[c][f] * iW[f];
+        vlow[c][f] = zl[c][f] * iW[f];
+      }
+  } else {
+    double sl[3][2];
+    lower2(rv, sl);
+    double fac[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const double irh = 1.0 / rhoh[f];
+      W[f] = sqrt(0.5 + sqrt(0.25 + dot2(sl, rv, f) * irh * irh));
+      iW[f] = 1 / W[f];
+      fac[f] = iW[f] * iW[f] * irh;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int f = 0; f < 2; ++f) {
+        vel[c][f] = rv[c][f] * fac[f];
+        vlow[c][f] = sl[c][f] * fac[f];
+      }
+  }
+
+  double vt[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int f = 0; f < 2; ++f) vt[c][f] = alp * vel[c][f] - betas[c];
+  double ab0[2], Bl[3][2], B2[2], bl[3][2], bsq[2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) ab0[f] = W[f] * dot2(Bs, vlow, f);
+  lower2(Bs, Bl);
+#pragma unroll
+  for (int f = 0; f < 2; ++f) B2[f] = dot2(Bs, Bl, f);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int f = 0; f < 2; ++f) bl[c][f] = Bl[c][f] * iW[f] + ab0[f] * vlow[c][f];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) bsq[f] = (B2[f] + pow2(ab0[f])) * pow2(iW[f]);
+
+  double cs2[2], h[2], dens[2], dent[2], dhW[2], dhWb[2], ppm[2], tau[2];
+  double mom[3][2], Bt[3][2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    if constexpr (TAB) {
+      cs2[f] = cs2t[f];
+    } else {
+      const double cs = eos.csnd(eps[f]);
+      cs2[f] = cs * cs;
+    }
+    h[f] = 1 + eps[f] + press[f] * irho[f];
+    dens[f] = sqrtg * rho[f] * W[f];
+    dent[f] = sqrtg * rho[f] * W[f] * ent[f];
+    dhW[f] = dens[f] * h[f] * W[f];
+    dhWb[f] = dhW[f] + sqrtg * (ab0[f] * ab0[f] + B2[f]);
+    ppm[f] = press[f] + 0.5 * bsq[f];
+    tau[f] = dhW[f] - dens[f] + sqrtg * (B2[f] - ppm[f]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      mom[c][f] = dhWb[f] * vlow[c][f] - sqrtg * ab0[f] * bl[c][f];
+      Bt[c][f] = sqrtg * Bs[c][f];
+    }
+  const double alp_sqrtg = alp * sqrtg;
+
+  // characteristic speeds (u = (gamma^-1)^{DD})
+  const double u = (D == 0   ? g.yy * g.zz - pow2(g.yz)
+                    : D == 1 ? g.xx * g.zz - pow2(g.xz)
+                             : g.xx * g.yy - pow2(g.xy)) /
+                   detg; // (gamma^-1)^{DD}
+  double l0p, l0m, l1p, l1m;
+  eig_side(alp, ialp, betas[D], u, vel[D][0], rho[0], cs2[0], W[0], h[0], bsq[0], l0p, l0m);
+  eig_side(alp, ialp, betas[D], u, vel[D][1], rho[1], cs2[1], W[1], h[1], bsq[1], l1p, l1m);
+  double cmax, cmin;
+  char_extrema(l0p, l0m, l1p, l1m, cmax, cmin);
+  const double cabs = smax(smax(smax(smax(0.0, fabs(l0p)), fabs(l0m)), fabs(l1p)),
+                           fabs(l1m));
+  const bool use_hlle = A.flux_type == ASX_FLUX_HLLE;
+  const double idc = 1.0 / (cmax - cmin);
+  auto num = [&](double U0, double U1, double F0, double F1) {
+    return use_hlle ? hlle(cmax, cmin, idc, U0, U1, F0, F1)
+                    : laxf(cabs, U0, U1, F0, F1);
+  };
+
+  double BoW[2];
+  BoW[0] = Bs[D][0] * iW[0];
+  BoW[1] = Bs[D][1] * iW[1];
+  double *const *F = P.g + ASX_FXDENS + 10 * D;
+  F[0][ofc] = num(dens[0], dens[1], dens[0] * vt[D][0], dens[1] * vt[D][1]);
+  F[6][ofc] = num(dent[0], dent[1], dent[0] * vt[D][0], dent[1] * vt[D][1]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double unit = (c == D) ? 1.0 : 0.0;
+    double fm[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+      fm[f] = mom[c][f] * vt[D][f] + alp_sqrtg * (ppm[f] * unit - bl[c][f] * BoW[f]);
+    F[1 + c][ofc] = num(mom[c][0], mom[c][1], fm[0], fm[1]);
+  }
+  {
+    double ft[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f)
+      ft[f] = tau[f] * vt[D][f] + alp_sqrtg * (ppm[f] * vel[D][f] - ab0[f] * BoW[f]);
+    F[4][ofc] = num(tau[0], tau[1], ft[0], ft[1]);
+  }
+  {
+    const double dye0 = dens[0] * ye[0], dye1 = dens[1] * ye[1];
+    F[5][ofc] = num(dye0, dye1, dye0 * vt[D][0], dye1 * vt[D][1]);
+  }
+  // induction flux: E = Btilde x vtilde, flux(B)_c = (unit x E)_c
+  double E[3][2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int cj = (c + 1) % 3, ck = (c + 2) % 3;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) E[c][f] = Bt[cj][f] * vt[ck][f] - Bt[ck][f] * vt[cj][f];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int cj = (c + 1) % 3, ck = (c + 2) % 3;
+    const double uj = (cj == D) ? 1.0 : 0.0, uk = (ck == D) ? 1.0 : 0.0;
+    double fb[2];
+#pragma unroll
+    for (int f = 0; f < 2; ++f) fb[f] = uj * E[ck][f] - uk * E[cj][f];
+    F[7 + c][ofc] = (D != c) * num(Bt[c][0], Bt[c][1], fb[0], fb[1]);
+  }
+
+  if (A.write_uct) { // fluxes.cxx:776-793
+    const double amx = cmax, amn = -1 * cmin;
+    P.g[ASX_AMAX_XFACE + 2 * D][ofc] = amx;
+    P.g[ASX_AMIN_XFACE + 2 * D][ofc] = amn;
+    // vtilde^(D+1), vtilde^(D+2); a face stores its two transverse
+    // components in increasing order (AsterX/interface.ccl:137-150)
+    constexpr int s1 = ASX_VTILDE_Y_XFACE + 2 * D + (D1 < D2 ? 0 : 1);
+    constexpr int s2 = ASX_VTILDE_Y_XFACE + 2 * D + (D1 < D2 ? 1 : 0);
+    const double ia = 1.0 / (amx + amn);
+    P.g[s1][ofc] = (amx * vt[D1][0] + amn * vt[D1][1]) * ia;
+    P.g[s2][ofc] = (amx * vt[D2][0] + amn * vt[D2][1]) * ia;
+  }
+  if (A.cnt_dev) {
+    const bool lo = lo0 || lo1;
+    const bool bad = !(dens[0] == dens[0]) || !(dens[1] == dens[1]) ||
+                     !(tau[0] == tau[0]) || !(tau[1] == tau[1]) ||
+                     rho[0] < 0.0 || rho[1] < 0.0 || press[0] < 0.0 || press[1] < 0.0;
+    const uint64_t mb = __ballot(bad), ml = __ballot(lo);
+    if ((threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) {
+      if (mb) atomicAdd((unsigned long long *)&A.cnt_dev[0], (unsigned long long)__popcll(mb));
+      if (ml) atomicAdd((unsigned long long *)&A.cnt_dev[1], (unsigned long long)__popcll(ml));
+    }
+  }
+}
+
+// Face tile shapes (x, y, z work-items; 256 per block), variant V of the
+// faces normal to D.  The extent along D is one cell more than the faces the
+// tile produces.
+template <int X_, int Y_, int Z_> struct Tile3 {
+  static constexpr int X = X_, Y = Y_, Z = Z_;
+};
+template <int D, int V> struct FaceTileT;
+template <> struct FaceTileT<0, 0> { using T = Tile3<64, 4, 1>; };
+template <> struct FaceTileT<0, 1> { using T = Tile3<64, 2, 2>; };
+template <> struct FaceTileT<0, 2> { using T = Tile3<32, 4, 2>; };
+template <> struct FaceTileT<1, 0> { using T = Tile3<16, 16, 1>; };
+template <> struct FaceTileT<1, 1> { using T = Tile3<32, 8, 1>; };
+template <> struct FaceTileT<1, 2> { using T = Tile3<16, 8, 2>; };
+template <> struct FaceTileT<2, 0> { using T = Tile3<16, 1, 16>; };
+template <> struct FaceTileT<2, 1> { using T = Tile3<32, 1, 8>; };
+template <> struct FaceTileT<2, 2> { using T = Tile3<16, 2, 8>; };
+// narrower rows: fewer idle work-items on extents such as n + 2
+template <> struct FaceTileT<0, 3> { using T = Tile3<16, 4, 4>; };
+template <> struct FaceTileT<0, 4> { using T = Tile3<16, 8, 2>; };
+template <> struct FaceTileT<1, 3> { using T = Tile3<8, 16, 2>; };
+template <> struct FaceTileT<1, 4> { using T = Tile3<8, 32, 1>; };
+template <> struct FaceTileT<2, 3> { using T = Tile3<8, 2, 16>; };
+template <> struct FaceTileT<2, 4> { using T = Tile3<8, 4, 8>; };
+template <int D, int V> using FaceTile = typename FaceTileT<D, V>::T;
+
+// LDS of a face tile: the high side of every cell, handed to the next cell
+// along D, and its low-order flag.
+struct FaceLDS {
+  double sh[kSideVars][256];
+  int shf[256];
+};
+
+// One tile of a face box (lo, cnt), tile index b of a grid of ntx x nty x ntz
+// tiles (ordered x, y, z).  Every work-item of the workgroup calls it; the
+// caller owns the LDS (one FaceLDS serves tiles of every direction).
+template <int D, int M, int V, bool TAB = false>
+ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], const int cnt[3],
+                       int b, int ntx, int nty, FaceLDS &S) {
+  using T = FaceTile<D, V>;
+  static_assert(T::X * T::Y * T::Z == 256, "face tile");
+  constexpr int TD = D == 0 ? T::X : (D == 1 ? T::Y : T::Z);
+  constexpr int SD = D == 0 ? 1 : (D == 1 ? T::X : T::X * T::Y); // tile stride along D
+  double (*sh)[256] = S.sh;
+  int *shf = S.shf;
+  const int bx = b % ntx, by = (b / ntx) % nty, bz = b / (ntx * nty);
+  const int t = threadIdx.x;
+  const int tx = t % T::X, ty = (t / T::X) % T::Y, tz = t / (T::X * T::Y);
+  // cell (and face) coordinates relative to the face box; along D the tile
+  // starts one cell below its first face
+  const int fx = bx * (D == 0 ? TD - 1 : T::X) + tx - (D == 0);
+  const int fy = by * (D == 1 ? TD - 1 : T::Y) + ty - (D == 1);
+  const int fz = bz * (D == 2 ? TD - 1 : T::Z) + tz - (D == 2);
+  const int td = D == 0 ? tx : (D == 1 ? ty : tz);
+  const int fd = D == 0 ? fx : (D == 1 ? fy : fz);
+  const bool need = fx < cnt[0] && fy < cnt[1] && fz < cnt[2];
+  const int i = lo3[0] + fx, j = lo3[1] + fy, k = lo3[2] + fz;
+  Side lo, up;
+  bool flo = false, fup = false;
+  if (need) {
+    recon_cell<D, M>(P, A, off(P, 0, i, j, k), lo, up, flo, fup);
+    sh[0][t] = up.rho;
+    sh[1][t] = up.ent;
+    sh[2][t] = up.ye;
+    sh[3][t] = up.th;
+    sh[4][t] = up.b1;
+    sh[5][t] = up.b2;
+    sh[6][t] = up.v[0];
+    sh[7][t] = up.v[1];
+    sh[8][t] = up.v[2];
+    shf[t] = fup;
+  }
+  __syncthreads();
+  if (need && td >= 1 && fd >= 0) {
+    const int tl = t - SD;
+    Side L;
+    L.rho = sh[0][tl];
+    L.ent = sh[1][tl];
+    L.ye = sh[2][tl];
+    L.th = sh[3][tl];
+    L.b1 = sh[4][tl];
+    L.b2 = sh[5][tl];
+    L.v[0] = sh[6][tl];
+    L.v[1] = sh[7][tl];
+    L.v[2] = sh[8][tl];
+    face_flux<D, TAB>(P, A, i, j, k, L, lo, shf[tl] != 0, flo);
+  }
+}
+
+#ifndef ASX_FACE_WAVES
+#define ASX_FACE_WAVES 3 // waves per SIMD the register budget is sized for
+#endif
+// the table's face kernels keep the lookup's 8 corners x 3 variables and
+// the temperature bracket live beside the reconstruction: a 256-register
+// budget (2 waves per SIMD) instead of spilling ~300 B per lane at 3
+#define ASX_FACE_WAVES_TAB 2
+// CalcAuxForAvecPsi (AsterX/src/fluxes.cxx:803-825) at vertex (i, j, k)
+ASX_DEV void aux_point(const KPatch &P, int i, int j, int k) {
+  ASX_CONTRACT
+  const int ov = off(P, 7, i, j, k);
+  double A[3];
+  {
+    const int o = off(P, 6, i, j, k);
+    A[0] = (0.0 + P.g[ASX_AVEC_X][o] + P.g[ASX_AVEC_X][o - 1]) / 2.0;
+  }
+  {
+    const int o = off(P, 5, i, j, k);
+    A[1] = (0.0 + P.g[ASX_AVEC_Y][o] + P.g[ASX_AVEC_Y][o - P.sj[5]]) / 2.0;
+  }
+  {
+    const int o = off(P, 3, i, j, k);
+    A[2] = (0.0 + P.g[ASX_AVEC_Z][o] + P.g[ASX_AVEC_Z][o - P.sk[3]]) / 2.0;
+  }
+  const Sym3 g{P.g[ASX_GXX][ov], P.g[ASX_GXY][ov], P.g[ASX_GXZ][ov],
+               P.g[ASX_GYY][ov], P.g[ASX_GYZ][ov], P.g[ASX_GZZ][ov]};
+  const double beta[3] = {P.g[ASX_BETAX][ov], P.g[ASX_BETAY][ov], P.g[ASX_BETAZ][ov]};
+  const double detg = det3(g);
+  const double sqrtg = sqrt(detg);
+  const Sym3 ug = inv3(g, detg);
+  double Aup[3];
+  lower(ug, A, Aup);
+  const double alp = P.g[ASX_ALP][ov], psi = P.g[ASX_PSI][ov];
+  P.g[ASX_FX][ov] = alp * sqrtg * Aup[0];
+  P.g[ASX_FY][ov] = alp * sqrtg * Aup[1];
+  P.g[ASX_FZ][ov] = alp * sqrtg * Aup[2];
+  P.g[ASX_FBETAX][ov] = beta[0] * psi;
+  P.g[ASX_FBETAY][ov] = beta[1] * psi;
+  P.g[ASX_FBETAZ][ov] = beta[2] * psi;
+  P.g[ASX_G][ov] = alp * psi / sqrtg - dot3(beta, A);
+}
+// ComputeStaggeredB<D> (AsterX/src/computeBfromA.cxx:32-55)
+template <int D>
+ASX_DEV void bstag_point(const KPatch &P, int i, int j, int k, double *out) {
+  out[off(P, 1 << D, i, j, k)] = bstag_value<D>(P, i, j, k);
+}
+ASX_DEV bool batch_point(const BatchBox &bb, int chunk, int &i, int &j, int &k) {
+  const int t = chunk * 256 + (int)threadIdx.x;
+  const int nxy = bb.cnt[0] * bb.cnt[1];
+  if (t >= nxy * bb.cnt[2]) return false;
+  i = bb.lo[0] + t % bb.cnt[0];
+  j = bb.lo[1] + (t / bb.cnt[0]) % bb.cnt[1];
+  k = bb.lo[2] + t / nxy;
+  return true;
+}
+
+
+} // namespace asx

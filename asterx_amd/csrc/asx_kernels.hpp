@@ -220,6 +220,41 @@ hipError_t launch_con2prim_rest_batch(const KPatch *tab, const int *zbase, int n
                                       const IdealGas &eos, const C2PArgs &a, int64_t nzones,
                                       const int *queue, const unsigned *qcount,
                                       hipStream_t s);
+// A whole SSPRK3 stage of a batch as one persistent cooperative launch
+// (asx_plan.hip): the launch lists of each phase and everything the phases
+// read, in device memory.  Faces use tile variant 0 of every direction, the
+// update 32x4x2 tiles (kPersistUpdateTile), con2prim Noble as the primary.
+struct BatchList {
+  const BatchBox *box;
+  const BatchRef *map;
+  int n;
+};
+struct PersistArgs {
+  const KPatch *T;  // stage input views (faces, aux and update read them)
+  const KPatch *U;  // views of the output buffer (fills and con2prim)
+  const StatePtrs *u0, *out;
+  FluxArgs A;
+  RhsArgs R;
+  double c0, c1, c2;
+  IdealGas eos;
+  C2PArgs ca;
+  BatchList face[3], aux, update, bc[3], c2p;
+  BCSlots state, prims; // grid functions filled after the update / con2prim
+  int bc_kind;
+  const int *zbase;
+  int np;
+  int *queue;
+  unsigned *qcount;
+};
+constexpr int kPersistUpdateTile[3] = {32, 4, 2};
+// the reconstruction methods and fd orders the persistent stage is built for
+bool persist_supported(int method, int fd_order);
+// blocks of one launch: every block resident at once (cooperative launch),
+// a multiple of 8 (one run of the lists per XCD); 0 on error
+int persist_grid(int method, int fd_order);
+hipError_t launch_stage_persist(const PersistArgs *dev_args, int method, int fd_order,
+                                int grid, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // Independent zones: sampler, batched prim2con, learned con2prim guess
 // ---------------------------------------------------------------------------
@@ -238,6 +273,9 @@ template <int NIN> struct MlpLayout {
                        OutStd = OutMean + Out, NBlob = OutStd + Out;
   static constexpr int Partial = NParam + 1; // per-wavefront gradient + loss
   static constexpr int NCons = NIN == 5 ? 8 : 9; // rows of the cons batch read
+  // hidden activation: ReLU for the ideal-gas network (one VALU operation,
+  // so the forward pass costs the solver little), tanh for the table's
+  static constexpr bool Relu = NIN == 5;
 };
 constexpr int kMlpIdeal = 5, kMlpTab = 7;
 constexpr int kMlpNParam = MlpLayout<kMlpIdeal>::NParam, kMlpNBlob = MlpLayout<kMlpIdeal>::NBlob;

@@ -3,7 +3,8 @@
 // inference, training (forward + backward + Adam) and con2prim from the
 // guess with the plain guess as fallback.
 //
-// The network is 5 -> 64 tanh -> 64 tanh -> 2.  Inference runs all three
+// The ideal-gas network is 5 -> 64 ReLU -> 64 ReLU -> 2, the table's
+// 7 -> 64 tanh -> 64 tanh -> 2 (MlpLayout::Relu).  Inference runs all three
 // layers on the matrix cores (v_mfma_f32_16x16x16_bf16: bf16 operands, fp32
 // accumulation; net_wave), inside the con2prim kernel or as a kernel of its
 // own.  Training runs the 64x64 layer and both 64x64 products of its
@@ -147,6 +148,17 @@ ASX_DEV float fast_tanh(float x) {
   return 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
 }
 
+// The hidden activation of the network of NIN inputs and its derivative
+// expressed through the activation's value h.
+template <int NIN> ASX_DEV float act(float z) {
+  if constexpr (MlpLayout<NIN>::Relu) return z > 0.0f ? z : 0.0f;
+  else return fast_tanh(z);
+}
+template <int NIN> ASX_DEV float dact(float h) {
+  if constexpr (MlpLayout<NIN>::Relu) return h > 0.0f ? 1.0f : 0.0f;
+  else return 1.0f - h * h;
+}
+
 // bf16 round-to-nearest-even and back
 ASX_DEV uint16_t f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -214,7 +226,7 @@ ASX_DEV void layer1(const float *blob, const float xb[NIN], uint16_t *h1, uint16
   for (int j = 0; j < L::H; ++j) {
     float zj = blob[L::B1 + j];
     for (int i = 0; i < L::In; ++i) zj += bfr(blob[L::W1 + j * L::In + i]) * xb[i];
-    const uint16_t hb = f2bf(fast_tanh(zj));
+    const uint16_t hb = f2bf(act<NIN>(zj));
     h1[l * kTS + j] = hb;
     if (h1t) h1t[j * kTS + l] = hb;
   }
@@ -301,7 +313,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
     for (int ob = 0; ob < 4; ++ob)
       for (int r = 0; r < 4; ++r) {
         const int zr = rt * 16 + k4 + r, o = ob * 16 + r16;
-        const float h = fast_tanh(acc[rt][ob][r] + blob[L::B2 + o]);
+        const float h = act<NIN>(acc[rt][ob][r] + blob[L::B2 + o]);
         acc[rt][ob][r] = h;
         a2[zr * kTS + o] = f2bf(h);
       }
@@ -335,7 +347,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
     for (int q = 0; q < 64; ++q) s += lsum[q];
     out[L::NParam] = s;
   }
-  // dz2 = (W3^T dy) (1 - h2^2), in place of acc; db2 partials per lane group
+  // dz2 = (W3^T dy) act'(h2), in place of acc; db2 partials per lane group
   float db2p[4] = {0.f, 0.f, 0.f, 0.f};
   for (int rt = 0; rt < 4; ++rt)
     for (int ob = 0; ob < 4; ++ob)
@@ -344,7 +356,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
         float dh = 0.f;
         for (int o = 0; o < L::Out; ++o) dh += dys[zr][o] * bfr(blob[L::W3 + o * L::H + j]);
         const float h = acc[rt][ob][r];
-        const float dz = dh * (1.f - h * h);
+        const float dz = dh * dact<NIN>(h);
         acc[rt][ob][r] = dz;
         db2p[ob] += dz;
       }
@@ -398,7 +410,7 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
       for (int kt = 0; kt < 4; ++kt) dh1[rt][kt] = mfma16(a, b[kt], dh1[rt][kt]);
     }
   }
-  // dz1 = dH1 (1 - h1^2) with h1 recomputed in fp32; dW1 and db1 partials
+  // dz1 = dH1 act'(h1) with h1 recomputed in fp32; dW1 and db1 partials
   // per (lane group, column), reduced over the four groups
   __syncthreads();
   for (int kt = 0; kt < 4; ++kt) {
@@ -411,8 +423,8 @@ __global__ __launch_bounds__(64) void k_mlp_grad(int64_t nz, const double *cons,
         const int zr = rt * 16 + k4 + r;
         float zk = blob[L::B1 + k];
         for (int i = 0; i < L::In; ++i) zk += wk[i] * xs[zr][i];
-        const float h = fast_tanh(zk);
-        const float dz = dh1[rt][kt][r] * (1.f - h * h);
+        const float h = act<NIN>(zk);
+        const float dz = dh1[rt][kt][r] * dact<NIN>(h);
         sd += dz;
         for (int i = 0; i < L::In; ++i) sw[i] += dz * xs[zr][i];
       }
@@ -551,9 +563,9 @@ ASX_DEV uint32_t pack_bf16(float a, float b) {
   const bf16x2_t h = __builtin_convertvector(f32x2_t{a, b}, bf16x2_t);
   return __builtin_bit_cast(uint32_t, h);
 }
-ASX_DEV bf16x4 tanh_bf16x4(const f32x4 &v) {
-  const uint32_t lo = pack_bf16(fast_tanh(v[0]), fast_tanh(v[1]));
-  const uint32_t hi = pack_bf16(fast_tanh(v[2]), fast_tanh(v[3]));
+template <int NIN> ASX_DEV bf16x4 act_bf16x4(const f32x4 &v) {
+  const uint32_t lo = pack_bf16(act<NIN>(v[0]), act<NIN>(v[1]));
+  const uint32_t hi = pack_bf16(act<NIN>(v[2]), act<NIN>(v[3]));
   return bf16x4{(short)(lo & 0xffffu), (short)(lo >> 16), (short)(hi & 0xffffu),
                 (short)(hi >> 16)};
 }
@@ -614,7 +626,7 @@ ASX_DEV void net_wave(const float *blob, const uint16_t *w2s, const NetSmall &S,
       const f32x4 b{blob[L::B1 + j], blob[L::B1 + j + 1], blob[L::B1 + j + 2],
                     blob[L::B1 + j + 3]};
 #pragma unroll
-      for (int zt = 0; zt < 4; ++zt) hb[jt][zt] = tanh_bf16x4(mfma16(aw, xb[zt], b));
+      for (int zt = 0; zt < 4; ++zt) hb[jt][zt] = act_bf16x4<NIN>(mfma16(aw, xb[zt], b));
     }
   }
   // layer 2 (A = rows of W2 from LDS, B = layer 1's activations), each
@@ -645,7 +657,7 @@ ASX_DEV void net_wave(const float *blob, const uint16_t *w2s, const NetSmall &S,
                           ? *reinterpret_cast<const bf16x4 *>(S.w3 + r16 * 64 + jt * 16 + k4)
                           : zero4;
 #pragma unroll
-    for (int zt = 0; zt < 4; ++zt) yv[zt] = mfma16(a3, tanh_bf16x4(acc[zt]), yv[zt]);
+    for (int zt = 0; zt < 4; ++zt) yv[zt] = mfma16(a3, act_bf16x4<NIN>(acc[zt]), yv[zt]);
   }
 #pragma unroll
   for (int zt = 0; zt < 4; ++zt)

@@ -5,6 +5,10 @@ All computation runs in the HIP library: zones are drawn on the GPU
 the network is trained with its own forward/backward/Adam kernels
 (asx_mlp_train_step).  torch only holds the device buffers.  Every step
 draws a fresh batch, so `samples` counts distinct generated pairs.
+
+The committed weights (DEFAULT_WEIGHTS) are the ReLU network trained by
+`python -m asterx_amd.mlp --samples 1.6e10 --batch 65536 --lr 3e-3`
+(about 90 s on one MI355X).
 """
 import ctypes as C
 import math
@@ -47,18 +51,19 @@ class Zones:
 
 
 def init_blob(L, par, gamma, seed=1234, nstat=1 << 20, stream=None):
-    """Xavier-uniform weights, zero biases, and input/output normalisation
-    from the statistics of nstat generated zones."""
+    """He-uniform weights (the hidden layers are ReLU), zero biases, and
+    input/output normalisation from the statistics of nstat generated
+    zones."""
     import torch
     rng = np.random.default_rng(seed)
     blob = np.zeros(NBLOB, dtype=np.float32)
 
-    def xavier(n_out, n_in):
-        a = math.sqrt(6.0 / (n_in + n_out))
+    def he(n_out, n_in):
+        a = math.sqrt(6.0 / n_in)
         return rng.uniform(-a, a, n_out * n_in).astype(np.float32)
-    blob[0:320] = xavier(64, 5)
-    blob[384:4480] = xavier(64, 64)
-    blob[4544:4672] = xavier(2, 64)
+    blob[0:320] = he(64, 5)
+    blob[384:4480] = he(64, 64)
+    blob[4544:4672] = he(2, 64)
     z = Zones(L, nstat, seed + 7, par, gamma, stream=stream)
     x = torch.empty((5, nstat), dtype=torch.float32, device="cuda")
     abi.check("asx_mlp_features", L.asx_mlp_features(nstat, _p(z.cons), _p(z.g), _p(x),

@@ -272,6 +272,26 @@ def bench_batched(args, src, par, dt, L, sh, stream):
     nl, gms = C.c_int64(), C.c_double()
     abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["graph"], C.byref(nl),
                                                    C.byref(gms)))
+    # the same steps as one persistent cooperative launch per stage (fixed
+    # launch shapes: face variant 0, update variant 1)
+    persist = None
+    try:
+        plan.step(1, persistent=True, stream=sh)
+        torch.cuda.synchronize()
+        abi.check("asx_timing_enable", L.asx_timing_enable(1))
+        t0 = time.perf_counter()
+        plan.step(K, persistent=True, stream=sh)
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t0
+        nlp, pms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["persist"], C.byref(nlp),
+                                                       C.byref(pms)))
+        persist = {"value": round(3 * zones * K / pwall, 1), "unit": "zone-updates/s",
+                   "ms_per_rk3_step": round(1e3 * pwall / K, 4),
+                   "launch_ms_per_stage": round(pms.value / max(1, nlp.value), 4),
+                   "vs_graph": round(wall / pwall, 3)}
+    except abi.AsxError as e:
+        persist = {"skipped": str(e)}
     # per-kernel split from one step of plain launches
     abi.check("asx_timing_enable", L.asx_timing_enable(1))
     plan.step(1, graph=False, stream=sh)
@@ -296,7 +316,8 @@ def bench_batched(args, src, par, dt, L, sh, stream):
            "ms_per_rk3_step": round(1e3 * wall / K, 4),
            "graph_ms_per_step": round(gms.value / max(1, nl.value), 4),
            "plain_launch_kernels_ms_per_step": kt,
-           "variants": variants, "plan_device_bytes": dbytes}
+           "variants": variants, "plan_device_bytes": dbytes,
+           "persistent": persist}
     plan.close()
     del patches
     torch.cuda.synchronize()

@@ -314,7 +314,8 @@ enum {
   ASX_KC_GRAPH = 12,    /* replays of a batch plan's captured step */
   ASX_KC_TMUNU = 13,    /* stress-energy tensor at the vertices */
   ASX_KC_REGRID = 14,   /* regrid error estimate */
-  ASX_KC_COUNT = 15
+  ASX_KC_PERSIST = 15,  /* a batch plan's stage as one persistent launch */
+  ASX_KC_COUNT = 16
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
@@ -516,7 +517,7 @@ asx_status asx_prim2con_batch(int64_t nz, const double *prim, const double *g, d
 /* The network: 5 inputs (log D, log S^2, log tau, log B^2 in units of
  * E = D + tau, and cos(S,B); the ideal-gas con2prim is invariant under
  * D, S, tau -> l D, l S, l tau, B -> sqrt(l) B, so the guess needs only these
- * ratios) -> 64 tanh -> 64 tanh -> 2 outputs (log W, log(Z/D),
+ * ratios) -> 64 ReLU -> 64 ReLU -> 2 outputs (log W, log(Z/D),
  * Z = rho h W^2), on a
  * float32 blob of ASX_MLP_NBLOB values: W1[64][5] b1[64] W2[64][64] b2[64]
  * W3[2][64] b3[2] (ASX_MLP_NPARAM parameters), then the input means and
@@ -659,9 +660,17 @@ asx_status asx_batch_create(int32_t npatch, const asx_patch *patches, const asx_
                             asx_batch **plan);
 /* One stage (0, 1 or 2) with plain launches on `stream`. */
 asx_status asx_batch_stage(asx_batch *plan, int32_t stage, void *stream);
-/* nsteps whole steps.  use_graph: the three stages are captured once into a
- * HIP graph (re-captured when a launch-shape variant changes) and the graph
- * is replayed on `stream`; otherwise plain launches. */
+/* nsteps whole steps.  use_graph = 1: the three stages are captured once
+ * into a HIP graph (re-captured when a launch-shape variant changes) and the
+ * graph is replayed on `stream`; 0: plain launches, one per kernel class
+ * and phase; 2: each stage is ONE persistent cooperative launch whose
+ * workgroups walk every phase over every patch (faces and aux, update,
+ * state fill, con2prim, queued zones, primitive fill) with grid-wide
+ * barriers between phases, every workgroup resident for the whole stage;
+ * it uses fixed launch shapes (face variant 0, update variant 1) and is
+ * built for PPM, WENO-Z and MP5 with Noble as the primary solver and
+ * without caller dB_stag (ASX_ERR_UNSUPPORTED otherwise).  All three modes
+ * give the same results bit for bit.  Other values: ASX_ERR_PARAM. */
 asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, void *stream);
 /* Launch shapes: a plan starts with the device's current variants
  * (asx_set_variant) and keeps its own from then on.  asx_batch_autotune

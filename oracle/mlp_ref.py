@@ -4,7 +4,8 @@ Test infrastructure only: the HIP kernels' inference and gradient are
 checked against it.  Same network, parameter layout and arithmetic as the
 kernels -- inputs computed in double precision from the densitized conserved
 variables, every product with bf16-rounded operands and float32
-accumulation, tanh activations -- and the gradient of the mean squared error
+accumulation, ReLU hidden activations for the ideal-gas network (5 inputs)
+and tanh for the table network (7 inputs) -- and the gradient of the mean squared error
 of the normalised outputs with bf16 rounding taken as the identity in the
 derivatives (bf16=True).  With bf16=False every rounding is dropped and the
 arithmetic is float64, so the backpropagation itself can be checked against
@@ -124,17 +125,26 @@ def normalized_inputs(blob, cons, g, bf=True):
     return xn.astype(dt)
 
 
+def activation(nin):
+    """(act, dact) of the hidden layers, dact as a function of the value
+    h = act(z): ReLU for the ideal-gas network, tanh for the table's."""
+    if nin == 5:
+        return (lambda z: np.maximum(z, 0.0)), (lambda h: (h > 0).astype(h.dtype))
+    return np.tanh, (lambda h: 1 - h ** 2)
+
+
 def forward(blob, cons, g, bf=True):
     """Normalised outputs y[nz, 2] and the cache for backward (5 or 7
     inputs, from the blob's size)."""
     r = bf16 if bf else (lambda a: a)
     p = _params(blob, bf)
+    act, _ = activation(_nin(blob))
     xb = r(normalized_inputs(blob, cons, g, bf))
     z1 = xb @ p["W1"].T + p["b1"]
-    h1 = np.tanh(z1)
+    h1 = act(z1)
     h1b = r(h1)
     z2 = h1b @ p["W2"].T + p["b2"]
-    h2 = np.tanh(z2)
+    h2 = act(z2)
     h2b = r(h2)
     y = h2b @ p["W3"].T + p["b3"]
     return y, dict(p=p, xb=xb, h1=h1, h1b=h1b, h2=h2, h2b=h2b)
@@ -161,13 +171,14 @@ def loss_and_grad(blob, cons, g, target, bf=True):
     p = c["p"]
     gW3 = dy.T @ c["h2b"]
     gb3 = dy.sum(0)
+    _, dact = activation(_nin(blob))
     dh2 = dy @ p["W3"]
-    dz2 = dh2 * (1 - c["h2"] ** 2)
+    dz2 = dh2 * dact(c["h2"])
     dz2b = r(dz2)
     gW2 = dz2b.T @ c["h1b"]
     gb2 = dz2.sum(0)
     dh1 = dz2b @ p["W2"]
-    dz1 = dh1 * (1 - c["h1"] ** 2)
+    dz1 = dh1 * dact(c["h1"])
     gW1 = dz1.T @ c["xb"]
     gb1 = dz1.sum(0)
     grad = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])

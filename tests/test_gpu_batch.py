@@ -125,6 +125,56 @@ def test_graph_replays_match_plain_launches():
     gb.close()
 
 
+@pytest.mark.parametrize("recon", ["wenoz", "mp5"])
+def test_persistent_stage_matches_plain_launches(recon):
+    """One persistent cooperative launch per stage (asx_batch_step mode 2)
+    against plain launches with the same launch shapes (face variant 0,
+    update variant 1), bit for bit, over three steps."""
+    torch = _torch()
+    pa, par, dt = _cuts(seed=7)
+    pb, _, _ = _cuts(seed=7)
+    par = abi.asx_params.from_buffer_copy(par)
+    par.reconstruction_method = abi.RECON[recon]
+    L = _L()
+    classes = [abi.KCLASS[k] for k in ("face_x", "face_y", "face_z", "update")]
+    saved = []
+    for kc in classes:
+        v = C.c_int32()
+        abi.check("asx_get_variant", L.asx_get_variant(kc, C.byref(v)))
+        saved.append(v.value)
+    try:
+        for kc, v in zip(classes, (0, 0, 0, 1)):
+            abi.check("asx_set_variant", L.asx_set_variant(kc, v))
+        ga = batch.Plan(L, pa, par, dt)
+        gb = batch.Plan(L, pb, par, dt)
+        before = pa[0].arr["dens"].clone()
+        for r in range(3):
+            ga.step(1, persistent=True)
+            gb.step(1, graph=False)
+            torch.cuda.synchronize()
+            _assert_same(pa, pb, "persistent step %d" % r)
+        assert not torch.equal(before, pa[0].arr["dens"])
+        ga.close()
+        gb.close()
+    finally:
+        for kc, v in zip(classes, saved):
+            abi.check("asx_set_variant", L.asx_set_variant(kc, v))
+
+
+def test_persistent_stage_refuses_unsupported():
+    """Methods the persistent stage is not built for are refused, not run."""
+    _torch()
+    pa, par, dt = _cuts(seed=9)
+    par = abi.asx_params.from_buffer_copy(par)
+    par.reconstruction_method = abi.RECON["minmod"]
+    plan = batch.Plan(_L(), pa, par, dt)
+    st = _L().asx_batch_step(plan.h, 1, 2, None)
+    assert abi.STATUS[st] == "ASX_ERR_UNSUPPORTED"
+    st = _L().asx_batch_step(plan.h, 1, 3, None)
+    assert abi.STATUS[st] == "ASX_ERR_PARAM"
+    plan.close()
+
+
 def test_results_independent_of_variants():
     torch = _torch()
     L = _L()

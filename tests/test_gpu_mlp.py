@@ -196,7 +196,7 @@ def test_c2p_from_guess_matches_plain(variant):
 
 
 def test_trained_network_gains_over_plain_guess():
-    """The committed network (asterx_amd/data, trained on 1e9 generated
+    """The committed network (asterx_amd/data, trained on 1.6e10 generated
     pairs) against the method without it on fresh zones of the workload:
     con2prim from its guess needs fewer Noble iterations than from either
     plain seed (cold start and the previous-step seed), falls back on less
