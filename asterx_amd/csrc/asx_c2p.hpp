@@ -332,6 +332,9 @@ ASX_DEV void vel_from_Z(double Z, double W, double Bsq, double BiSi, const Cons 
 
 // Noble 2D Newton on (Z, v^2), c2p_2DNoble.hxx:275-594.  z0/v0 (if >= 0)
 // replace the seed-derived initial guess (used by the learned guess).
+// SEEDLESS: the guess is given (zguess >= 0) and the seed is not read at
+// all (the saved primitives need not be loaded); same iterates otherwise.
+template <bool SEEDLESS = false>
 ASX_DEV void noble_solve(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a,
                          Prim &pv, Prim &seed, Cons &cv, const Sym3 &glo,
                          Report &rep, double zguess = -1.0, double vguess = -1.0) {
@@ -357,8 +360,9 @@ ASX_DEV void noble_solve(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a
   lower(glo, seed.B, B_low);
   const double Bsq = dot3(B_low, seed.B);
   const double BiSi = dot3(cv.mom, seed.B);
-  double vsq_seed;
-  if (a.use_z) {
+  double vsq_seed = 0.0;
+  if (SEEDLESS) {
+  } else if (a.use_z) {
     double z_up[3], z_low[3];
     for (int i = 0; i < 3; ++i) z_up[i] = seed.vel[i] * seed.w_lor;
     lower(glo, z_up, z_low);
@@ -384,11 +388,16 @@ ASX_DEV void noble_solve(const IdealGas &eos, const Atmo &atmo, const C2PArgs &a
     prim_nan(pv); cv.set_nan();
     return;
   }
-  seed.rho = cv.dens / seed.w_lor;
-  const double eps_last = smax(seed.eps, atmo.eps);
-  seed.press = eos.press_re(seed.rho, eps_last);
-  const double Z_seed = (seed.rho + eps_last * seed.rho + seed.press) * seed.w_lor * seed.w_lor;
-  double x0 = fabs(Z_seed), x1 = vsq_seed;
+  double x0 = zguess, x1 = vguess;
+  if (!SEEDLESS) {
+    seed.rho = cv.dens / seed.w_lor;
+    const double eps_last = smax(seed.eps, atmo.eps);
+    seed.press = eos.press_re(seed.rho, eps_last);
+    const double Z_seed =
+        (seed.rho + eps_last * seed.rho + seed.press) * seed.w_lor * seed.w_lor;
+    x0 = fabs(Z_seed);
+    x1 = vsq_seed;
+  }
   if (zguess >= 0.0) {
     x0 = zguess;
     x1 = vguess;
@@ -835,45 +844,53 @@ struct C2POut {
 // relative half-width vguess of zguess when the residual changes sign
 // there; otherwise the primary-only form fails (the zone is queued for the
 // full chain) and the full chain takes the reference's bracket.
-template <int PRIME, class E>
+// SEEDLESS (Noble primary only, zguess >= 0, no excision or mask): the
+// saved primitives are neither read nor derived; the zone is the same
+// otherwise.
+template <int PRIME, class E, bool SEEDLESS = false>
 ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
                            C2POut &out, double zguess = -1.0, double vguess = -1.0) {
+  static_assert(!SEEDLESS || PRIME == ASX_C2P_NOBLE, "seedless: the Noble primary only");
   const Atmo atmo = make_atmo(eos, a, in.r);
   const Sym3 &glo = in.glo;
   double mask_local = in.mask;
   const double sqrt_detg = sqrt(det3(glo));
-  double v_up[3] = {in.saved_vel[0], in.saved_vel[1], in.saved_vel[2]};
-  double v_low[3];
-  lower(glo, v_up, v_low);
-  double zsq;
-  const double vsq = dot3(v_low, v_up);
-  if (vsq >= 1.0) {
-    const double wlim = sqrt(1.0 + a.vw_lim * a.vw_lim);
-    const double vlim = a.vw_lim / wlim;
-    const double s = vlim / sqrt(vsq);
-    v_up[0] *= s; v_up[1] *= s; v_up[2] *= s;
-    zsq = a.vw_lim * a.vw_lim;
-  } else {
-    zsq = vsq / (1.0 - vsq);
-  }
   Cons cv = in.cv;
   double Bup[3];
   for (int i = 0; i < 3; ++i) Bup[i] = cv.dB[i] / sqrt_detg;
   Prim pv;
   prim_nan(pv);
   Prim seed;
-  seed.rho = in.saved_rho;
-  seed.eps = in.saved_eps;
-  seed.Ye = in.saved_Ye;
-  {
-    // p and the entropy of the saved state; the EOS takes eps by reference
-    // (a table may clamp it), the seed keeps the value as loaded
-    double e = in.saved_eps, temp;
-    eos.state_re(in.saved_rho, e, in.saved_Ye, seed.press, temp, seed.entropy);
+  if (SEEDLESS) {
+    prim_nan(seed);
+  } else {
+    double v_up[3] = {in.saved_vel[0], in.saved_vel[1], in.saved_vel[2]};
+    double v_low[3];
+    lower(glo, v_up, v_low);
+    double zsq;
+    const double vsq = dot3(v_low, v_up);
+    if (vsq >= 1.0) {
+      const double wlim = sqrt(1.0 + a.vw_lim * a.vw_lim);
+      const double vlim = a.vw_lim / wlim;
+      const double s = vlim / sqrt(vsq);
+      v_up[0] *= s; v_up[1] *= s; v_up[2] *= s;
+      zsq = a.vw_lim * a.vw_lim;
+    } else {
+      zsq = vsq / (1.0 - vsq);
+    }
+    seed.rho = in.saved_rho;
+    seed.eps = in.saved_eps;
+    seed.Ye = in.saved_Ye;
+    {
+      // p and the entropy of the saved state; the EOS takes eps by reference
+      // (a table may clamp it), the seed keeps the value as loaded
+      double e = in.saved_eps, temp;
+      eos.state_re(in.saved_rho, e, in.saved_Ye, seed.press, temp, seed.entropy);
+    }
+    seed.temperature = in.temperature;
+    seed.vel[0] = v_up[0]; seed.vel[1] = v_up[1]; seed.vel[2] = v_up[2];
+    seed.w_lor = sqrt(1.0 + zsq);
   }
-  seed.temperature = in.temperature;
-  seed.vel[0] = v_up[0]; seed.vel[1] = v_up[1]; seed.vel[2] = v_up[2];
-  seed.w_lor = sqrt(1.0 + zsq);
   seed.B[0] = Bup[0]; seed.B[1] = Bup[1]; seed.B[2] = Bup[2];
   bool flag_local = true;
   int code = ASX_C2P_FLAG_INIT;
@@ -886,6 +903,8 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
     call_c2p = false;
   }
   if (in.alp < a.alp_thresh) mask_local = 0.0;
+  // the BH-interior paths start from the seed: the full chain takes them
+  if (SEEDLESS && mask_local != 1.0) return false;
   out.mask_cc = mask_local;
   if (a.excise && mask_local != 1.0) {
     bh_interior<false>(eos, atmo, a, seed, cv, glo);
@@ -899,7 +918,8 @@ ASX_DEV bool con2prim_zone(const E &eos, const C2PArgs &a, const C2PIn &in,
     code = ASX_C2P_FLAG_PRIME;
     const int prime = PRIME < 0 ? a.prime : PRIME;
     if constexpr (std::is_same<E, IdealGas>::value) {
-      if (prime == ASX_C2P_NOBLE) noble_solve(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
+      if (prime == ASX_C2P_NOBLE)
+        noble_solve<SEEDLESS>(eos, atmo, a, pv, seed, cv, glo, rep1, zguess, vguess);
       else if (prime == ASX_C2P_ENTROPY) entropy_solve(eos, atmo, a, pv, cv, glo, rep1);
       else palenzuela_solve(eos, atmo, a, pv, cv, glo, rep1, zguess, vguess, PRIME >= 0);
     } else {
@@ -968,6 +988,9 @@ ASX_DEV void queue_push(bool fail, int value, int *queue, unsigned *qcount) {
 
 // Independent zone z of a batch (layout of asx_c2p_batch: cons[8][nz],
 // g[6][nz], seed[5][nz]); no grid position, lapse 1, no mask.
+// SEED = false: the saved primitives are not read (left NaN), for the
+// seedless primary solve from a given guess.
+template <bool SEED = true>
 ASX_DEV void batch_zone_in(int64_t z, int64_t nz, const double *cons, const double *g,
                            const double *seed, const C2PArgs &a, C2PIn &in) {
   in.r = 0.0;
@@ -984,12 +1007,18 @@ ASX_DEV void batch_zone_in(int64_t z, int64_t nz, const double *cons, const doub
   in.cv.dB[2] = cons[7 * nz + z];
   in.cv.DYe = in.cv.dens * a.Ye_atmo;
   in.cv.DEnt = 0.0;
-  in.saved_rho = seed[z];
-  in.saved_eps = seed[nz + z];
   in.saved_Ye = a.Ye_atmo;
-  in.saved_vel[0] = seed[2 * nz + z];
-  in.saved_vel[1] = seed[3 * nz + z];
-  in.saved_vel[2] = seed[4 * nz + z];
+  if (SEED) {
+    in.saved_rho = seed[z];
+    in.saved_eps = seed[nz + z];
+    in.saved_vel[0] = seed[2 * nz + z];
+    in.saved_vel[1] = seed[3 * nz + z];
+    in.saved_vel[2] = seed[4 * nz + z];
+  } else {
+    const double nan = __builtin_nan("");
+    in.saved_rho = in.saved_eps = nan;
+    in.saved_vel[0] = in.saved_vel[1] = in.saved_vel[2] = nan;
+  }
   in.temperature = 0.0;
 }
 // prim[8][nz] = rho, eps, press, vel^i, w_lor, temperature; flag[nz]

@@ -492,7 +492,7 @@ asx_status fixed_c2p_scratch(C2PScratch *out) {
 }
 
 // Device scratch per (device, stream, tag), grown on demand.
-enum { SCR_MLP_GUESS = 1 };
+enum { SCR_MLP_GUESS = 1, SCR_MLP_IMAGE = 2 };
 struct DevScratch {
   void *p = nullptr;
   size_t bytes = 0;
@@ -1304,8 +1304,11 @@ asx_status asx_mlp_forward(int64_t nz, const double *cons, const double *g, cons
                            double *out, void *stream) {
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !blob || !out)) return ASX_ERR_NULL;
+  if (nz == 0) return ASX_OK;
+  void *img = dev_scratch(SCR_MLP_IMAGE, stream, kNetImageBytes);
+  if (!img) return ASX_ERR_DEVICE;
   return dev_status(timed(ASX_KC_MLP, (hipStream_t)stream, [&] {
-    return launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, out, (hipStream_t)stream);
+    return launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, img, out, (hipStream_t)stream);
   }));
 }
 
@@ -1349,11 +1352,12 @@ asx_status asx_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
     guess = (double *)dev_scratch(SCR_MLP_GUESS, stream, (size_t)nz * 2 * sizeof(double));
     if (!guess) return ASX_ERR_DEVICE;
   }
+  void *img = dev_scratch(SCR_MLP_IMAGE, stream, kNetImageBytes);
   C2PQueue *Q = c2p_queue(stream, nz);
-  if (!Q) return ASX_ERR_DEVICE;
+  if (!img || !Q) return ASX_ERR_DEVICE;
   return dev_status(timed(ASX_KC_C2P, (hipStream_t)stream, [&] {
     return launch_c2p_batch_mlp(nz, cons, g, seed, blob, kguess, prim, flag, iters, fallback,
-                                ideal_gas(par), c2p_args(par), guess, Q->q, Q->n, v,
+                                ideal_gas(par), c2p_args(par), guess, img, Q->q, Q->n, v,
                                 (hipStream_t)stream);
   }));
 }
@@ -1489,8 +1493,11 @@ asx_status asx_mlp_ye_forward(int64_t nz, const double *cons, const double *g,
                               const float *blob, double *out, void *stream) {
   if (nz < 0) return ASX_ERR_SIZE;
   if (nz > 0 && (!cons || !g || !blob || !out)) return ASX_ERR_NULL;
+  if (nz == 0) return ASX_OK;
+  void *img = dev_scratch(SCR_MLP_IMAGE, stream, kNetImageBytes);
+  if (!img) return ASX_ERR_DEVICE;
   return dev_status(timed(ASX_KC_MLP, (hipStream_t)stream, [&] {
-    return launch_mlp_forward<kMlpTab>(nz, cons, g, blob, out, (hipStream_t)stream);
+    return launch_mlp_forward<kMlpTab>(nz, cons, g, blob, img, out, (hipStream_t)stream);
   }));
 }
 
@@ -1525,12 +1532,13 @@ asx_status asx_c2p_batch_ye_mlp(int64_t nz, const double *cons, const double *g,
   if (nz > 0 && (!cons || !g || !seed || !blob || !prim || !flag || !iters)) return ASX_ERR_NULL;
   if (nz == 0) return ASX_OK;
   double *guess = (double *)dev_scratch(SCR_MLP_GUESS, stream, (size_t)nz * 2 * sizeof(double));
+  void *img = dev_scratch(SCR_MLP_IMAGE, stream, kNetImageBytes);
   C2PQueue *Q = c2p_queue(stream, nz);
-  if (!guess || !Q) return ASX_ERR_DEVICE;
+  if (!guess || !img || !Q) return ASX_ERR_DEVICE;
   hipStream_t s = (hipStream_t)stream;
   return dev_status(timed(ASX_KC_C2P, s, [&] {
     return launch_c2p_ye_mlp(nz, cons, g, seed, blob, bracket, prim, flag, iters, tab_eos(par),
-                             c2p_args(par), guess, Q->q, Q->n, s);
+                             c2p_args(par), guess, img, Q->q, Q->n, s);
   }));
 }
 

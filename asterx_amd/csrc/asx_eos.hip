@@ -221,9 +221,10 @@ hipError_t launch_c2p_ye(int64_t nz, const double *cons, const double *g, const 
 hipError_t launch_c2p_ye_mlp(int64_t nz, const double *cons, const double *g,
                              const double *seed, const float *blob, double xrel, double *prim,
                              double *flag, int32_t *iters, const TabEOS &eos, const C2PArgs &a,
-                             double *guess, int *queue, unsigned *qcount, hipStream_t s) {
+                             double *guess, void *img, int *queue, unsigned *qcount,
+                             hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  hipError_t e = launch_mlp_forward<kMlpTab>(nz, cons, g, blob, guess, s);
+  hipError_t e = launch_mlp_forward<kMlpTab>(nz, cons, g, blob, img, guess, s);
   if (e != hipSuccess) return e;
   return c2p_ye_t<ASX_C2P_PALENZUELA>(nz, cons, g, seed, prim, flag, iters, eos, a, queue,
                                       qcount, s, guess, xrel);

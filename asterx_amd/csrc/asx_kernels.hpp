@@ -292,9 +292,12 @@ hipError_t launch_prim2con_batch(int64_t nz, const double *prim, const double *g
 template <int NIN>
 hipError_t launch_mlp_features(int64_t nz, const double *cons, const double *g, float *x,
                                hipStream_t s);
+// img: device scratch of kNetImageBytes for the network's inference image
+// (bf16 weights and fp32 biases, rebuilt from blob on every call)
+constexpr size_t kNetImageBytes = 10512;
 template <int NIN>
 hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
-                              const float *blob, double *out, hipStream_t s);
+                              const float *blob, void *img, double *out, hipStream_t s);
 template <int NIN>
 hipError_t launch_mlp_grad(int64_t nz, const double *cons, const double *g,
                            const double *target, const float *blob, float *partial,
@@ -308,7 +311,8 @@ hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float 
 hipError_t launch_c2p_ye_mlp(int64_t nz, const double *cons, const double *g,
                              const double *seed, const float *blob, double xrel, double *prim,
                              double *flag, int32_t *iters, const TabEOS &eos, const C2PArgs &a,
-                             double *guess, int *queue, unsigned *qcount, hipStream_t s);
+                             double *guess, void *img, int *queue, unsigned *qcount,
+                             hipStream_t s);
 // zones with composition: box = rho lo, hi, T lo, hi, Ye lo, hi, W max
 struct Box7 {
   double v[7];
@@ -324,8 +328,8 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
-                                double *guess, int *queue, unsigned *qcount, int variant,
-                                hipStream_t s);
+                                double *guess, void *img, int *queue, unsigned *qcount,
+                                int variant, hipStream_t s);
 
 // Independent zones with composition, any 3-parameter EOS (asx_eos.hip).
 // mode 0: (rho, T, Ye) -> out[4][n] = press, eps, c_s, entropy;

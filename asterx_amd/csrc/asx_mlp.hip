@@ -570,31 +570,54 @@ template <int NIN> ASX_DEV bf16x4 act_bf16x4(const f32x4 &v) {
                 (short)(hi >> 16)};
 }
 
-// The small layers' weights as bf16 in LDS, zero padded: W1 as 64 rows of 8
-// inputs, W3 as 2 rows of 64 (staged once per workgroup with W2).
-struct NetSmall {
+// The network's inference weights as the matrix cores read them: W2 as bf16
+// rows padded to kTS, W1 as bf16 rows of 8 inputs (zero padded), W3 as two
+// bf16 rows, the biases in fp32.  Built once per call from the fp32 blob
+// (k_net_image) and copied into LDS by every workgroup with 16-byte loads.
+struct NetImage {
+  uint16_t w2[64 * kTS];
   uint16_t w1[64 * 8];
   uint16_t w3[2 * 64];
+  float b1[64], b2[64], b3[4];
 };
-template <int NIN> ASX_DEV void stage_small(const float *blob, NetSmall &S) {
+static_assert(sizeof(NetImage) % 16 == 0, "NetImage is copied as uint4");
+static_assert(sizeof(NetImage) == kNetImageBytes, "scratch size of the image");
+template <int NIN> __global__ __launch_bounds__(256) void k_net_image(const float *blob,
+                                                                      NetImage *img) {
   using L = MlpLayout<NIN>;
   static_assert(L::H == 64 && L::Out == 2, "layer shapes");
-  for (int q = threadIdx.x; q < 64 * 8; q += blockDim.x) {
-    const int j = q / 8, i = q % 8;
-    S.w1[q] = i < NIN ? f2bf(blob[L::W1 + j * L::In + i]) : (uint16_t)0;
+  const int t = threadIdx.x;
+  for (int q = t; q < 64 * kTS; q += blockDim.x) {
+    const int j = q / kTS, k = q % kTS;
+    img->w2[q] = k < 64 ? f2bf(blob[L::W2 + j * 64 + k]) : (uint16_t)0;
   }
-  for (int q = threadIdx.x; q < 2 * 64; q += blockDim.x) S.w3[q] = f2bf(blob[L::W3 + q]);
+  for (int q = t; q < 64 * 8; q += blockDim.x) {
+    const int j = q / 8, i = q % 8;
+    img->w1[q] = i < NIN ? f2bf(blob[L::W1 + j * L::In + i]) : (uint16_t)0;
+  }
+  for (int q = t; q < 2 * 64; q += blockDim.x) img->w3[q] = f2bf(blob[L::W3 + q]);
+  if (t < 64) {
+    img->b1[t] = blob[L::B1 + t];
+    img->b2[t] = blob[L::B2 + t];
+  }
+  if (t < 4) img->b3[t] = t < L::Out ? blob[L::B3 + t] : 0.0f;
+}
+// Every work-item of the workgroup copies its share; the caller then
+// synchronises the workgroup.
+ASX_DEV void load_image(const NetImage *img, NetImage &S) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(img);
+  uint4 *dst = reinterpret_cast<uint4 *>(&S);
+  for (int q = threadIdx.x; q < (int)(sizeof(NetImage) / 16); q += blockDim.x) dst[q] = src[q];
 }
 
 // Normalised outputs y[2] of this lane's zone from its normalised inputs
-// xn.  w2s: W2 as bf16 in LDS (stage_w2); S: W1 and W3 (stage_small); xrow:
-// the wavefront's 64 input rows; yrow: its 64 output pairs.  Every lane of
-// the wavefront calls it.  Layer 3 consumes each 16-row block of layer 2's
+// xn.  S: the weights in LDS (load_image); xrow: the wavefront's 64 input
+// rows; yrow: its 64 output pairs.  Every lane of the wavefront calls it.  Layer 3 consumes each 16-row block of layer 2's
 // activations as it is produced, so only layer 1's activations (32 VGPRs)
 // are held whole.
 template <int NIN>
-ASX_DEV void net_wave(const float *blob, const uint16_t *w2s, const NetSmall &S, uint4 *xrow,
-                      float (*yrow)[2], const float xn[NIN], float y[2]) {
+ASX_DEV void net_wave(const NetImage &S, uint4 *xrow, float (*yrow)[2], const float xn[NIN],
+                      float y[2]) {
   using L = MlpLayout<NIN>;
   static_assert(NIN <= 8 && L::Out <= 4, "inputs padded to 8 of K = 16; outputs in rows 0..3");
   const int l = threadIdx.x & 63, r16 = l & 15, q4 = l >> 4, k4 = 4 * q4;
@@ -622,9 +645,7 @@ ASX_DEV void net_wave(const float *blob, const uint16_t *w2s, const NetSmall &S,
     for (int jt = 0; jt < 4; ++jt) {
       const bf16x4 aw = q4 < 2 ? *reinterpret_cast<const bf16x4 *>(S.w1 + (jt * 16 + r16) * 8 + k4)
                                : zero4;
-      const int j = jt * 16 + k4;
-      const f32x4 b{blob[L::B1 + j], blob[L::B1 + j + 1], blob[L::B1 + j + 2],
-                    blob[L::B1 + j + 3]};
+      const f32x4 b = *reinterpret_cast<const f32x4 *>(S.b1 + jt * 16 + k4);
 #pragma unroll
       for (int zt = 0; zt < 4; ++zt) hb[jt][zt] = act_bf16x4<NIN>(mfma16(aw, xb[zt], b));
     }
@@ -634,22 +655,17 @@ ASX_DEV void net_wave(const float *blob, const uint16_t *w2s, const NetSmall &S,
   // block); layer 3's rows o = k4 + r: outputs 0 and 1 sit in lanes 0..15
   f32x4 yv[4];
   {
-    f32x4 b3{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (q4 == 0 && r < L::Out) b3[r] = blob[L::B3 + r];
+    const f32x4 b3 = q4 == 0 ? *reinterpret_cast<const f32x4 *>(S.b3) : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int zt = 0; zt < 4; ++zt) yv[zt] = b3;
   }
 #pragma unroll
   for (int jt = 0; jt < 4; ++jt) {
-    const int j = jt * 16 + k4;
-    const f32x4 b{blob[L::B2 + j], blob[L::B2 + j + 1], blob[L::B2 + j + 2],
-                  blob[L::B2 + j + 3]};
+    const f32x4 b = *reinterpret_cast<const f32x4 *>(S.b2 + jt * 16 + k4);
     f32x4 acc[4] = {b, b, b, b};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
-      const bf16x4 aw = frag(w2s, jt * 16 + r16, kb * 16 + k4);
+      const bf16x4 aw = frag(S.w2, jt * 16 + r16, kb * 16 + k4);
 #pragma unroll
       for (int zt = 0; zt < 4; ++zt) acc[zt] = mfma16(aw, hb[kb][zt], acc[zt]);
     }
@@ -686,23 +702,21 @@ ASX_DEV void zone_inputs_of(bool live, const double *c, const Sym3 &gl, const fl
 // Con2prim from the network's guess in one kernel: the wavefront evaluates
 // the network for its 64 zones (net_wave), then each lane runs Noble from
 // its zone's guess for at most kguess iterations, as k_c2p_guess_main,
-// queueing failures for k_c2p_guess_rest.  One chunk of 256 zones per
-// workgroup: a loop over chunks made the compiler keep ~100 more VGPRs live
-// (occupancy 1); without it the kernel fits the solver's 3 waves per SIMD
-// (139 VGPRs, no spills) and the network runs in that budget.
+// queueing failures for k_c2p_guess_rest.  The guided solve reads no seed
+// (con2prim_zone's SEEDLESS form).  One chunk of 256 zones per workgroup: a
+// loop over chunks made the compiler keep ~100 more VGPRs live (occupancy
+// 1); without it the kernel runs at 4 waves per SIMD (120 VGPRs, no spills).
 // ---------------------------------------------------------------------------
 template <int NIN>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_c2p_guess_fused(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_c2p_guess_fused(
     int64_t nz, const double *cons, const double *g, const double *seed, const float *blob,
-    int kguess, double *prim, double *flag, int32_t *iters, int32_t *fallback, IdealGas eos,
+    const NetImage *img, int kguess, double *prim, double *flag, int32_t *iters, int32_t *fallback, IdealGas eos,
     C2PArgs a, int *queue, unsigned *qcount) {
   using L = MlpLayout<NIN>;
-  __shared__ uint16_t w2s[L::H * kTS];
+  __shared__ NetImage S;
   __shared__ uint4 xs[4][64];
   __shared__ float ys[4][64][2];
-  __shared__ NetSmall S;
-  stage_w2<NIN>(blob, w2s, nullptr);
-  stage_small<NIN>(blob, S);
+  load_image(img, S);
   __syncthreads();
   const int wv = threadIdx.x >> 6;
   {
@@ -720,19 +734,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
       }
       zone_inputs_of<NIN>(live, c, gl, blob, xn);
     }
-    net_wave<NIN>(blob, w2s, S, xs[wv], ys[wv], xn, y);
+    net_wave<NIN>(S, xs[wv], ys[wv], xn, y);
     bool fail = false;
     if (live) {
       const double lw = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
       const double lzd = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
+      // the guided solve never reads the seed; a guess that is not usable
+      // (non-finite) sends the zone to the full chain from the seed
       C2PIn in;
-      batch_zone_in(z, nz, cons, g, seed, a, in);
+      batch_zone_in<false>(z, nz, cons, g, seed, a, in);
       double Zg, vsq;
       guess_zv_of(lw, lzd, in, Zg, vsq);
       C2PArgs ak = a;
       ak.max_iter = kguess;
       C2POut o;
-      if (con2prim_zone<ASX_C2P_NOBLE>(eos, ak, in, o, Zg, vsq)) {
+      o.iters = 0;
+      if (Zg >= 0.0 && con2prim_zone<ASX_C2P_NOBLE, IdealGas, true>(eos, ak, in, o, Zg, vsq)) {
         batch_zone_out(z, nz, o, prim, flag);
         if (fallback) fallback[z] = 0;
       } else {
@@ -745,32 +762,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 // Network outputs in target units (log W, log(Z/D)) for every zone, all
-// three layers on the matrix cores (net_wave); workgroups loop over chunks
-// of 256 zones.
+// three layers on the matrix cores (net_wave), one chunk of 256 zones per
+// workgroup.
 template <int NIN>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_forward(int64_t nz, const double *cons,
                                                      const double *g, const float *blob,
-                                                     double *out) {
+                                                     const NetImage *img, double *out) {
   using L = MlpLayout<NIN>;
-  __shared__ uint16_t w2s[L::H * kTS];
+  __shared__ NetImage S;
   __shared__ uint4 xs[4][64];
   __shared__ float ys[4][64][2];
-  __shared__ NetSmall S;
-  stage_w2<NIN>(blob, w2s, nullptr);
-  stage_small<NIN>(blob, S);
+  load_image(img, S);
   __syncthreads();
   const int wv = threadIdx.x >> 6;
-  for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < nz;
-       base += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t z = base + threadIdx.x;
-    const bool live = z < nz;
-    float xn[NIN], y[2];
-    zone_inputs<NIN>(live, z, nz, cons, g, blob, xn);
-    net_wave<NIN>(blob, w2s, S, xs[wv], ys[wv], xn, y);
-    if (live) {
-      out[z] = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
-      out[nz + z] = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
-    }
+  const int64_t z = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = z < nz;
+  float xn[NIN], y[2];
+  zone_inputs<NIN>(live, z, nz, cons, g, blob, xn);
+  net_wave<NIN>(S, xs[wv], ys[wv], xn, y);
+  if (live) {
+    out[z] = (double)(y[0] * blob[L::OutStd] + blob[L::OutMean]);
+    out[nz + z] = (double)(y[1] * blob[L::OutStd + 1] + blob[L::OutMean + 1]);
   }
 }
 
@@ -804,11 +816,12 @@ hipError_t launch_mlp_features(int64_t nz, const double *cons, const double *g, 
 }
 template <int NIN>
 hipError_t launch_mlp_forward(int64_t nz, const double *cons, const double *g,
-                              const float *blob, double *out, hipStream_t s) {
+                              const float *blob, void *img, double *out, hipStream_t s) {
   if (nz <= 0) return hipSuccess;
-  const int64_t nb = (nz + 255) / 256;
-  hipLaunchKernelGGL(k_mlp_forward<NIN>, dim3((unsigned)(nb < 2048 ? nb : 2048)), dim3(256), 0,
-                     s, nz, cons, g, blob, out);
+  NetImage *im = static_cast<NetImage *>(img);
+  hipLaunchKernelGGL(k_net_image<NIN>, dim3(1), dim3(256), 0, s, blob, im);
+  hipLaunchKernelGGL(k_mlp_forward<NIN>, grid256(nz), dim3(256), 0, s, nz, cons, g, blob, im,
+                     out);
   return hipGetLastError();
 }
 template <int NIN>
@@ -833,7 +846,7 @@ hipError_t launch_mlp_adam(int nwaves, const float *partial, float *blob, float 
   template hipError_t launch_mlp_features<N>(int64_t, const double *, const double *, float *, \
                                              hipStream_t);                                     \
   template hipError_t launch_mlp_forward<N>(int64_t, const double *, const double *,          \
-                                            const float *, double *, hipStream_t);             \
+                                            const float *, void *, double *, hipStream_t);     \
   template hipError_t launch_mlp_grad<N>(int64_t, const double *, const double *,             \
                                          const double *, const float *, float *, hipStream_t); \
   template hipError_t launch_mlp_adam<N>(int, const float *, float *, float *, float *, float, \
@@ -846,20 +859,23 @@ hipError_t launch_c2p_batch_mlp(int64_t nz, const double *cons, const double *g,
                                 const double *seed, const float *blob, int kguess,
                                 double *prim, double *flag, int32_t *iters,
                                 int32_t *fallback, const IdealGas &eos, const C2PArgs &a,
-                                double *guess, int *queue, unsigned *qcount, int variant,
-                                hipStream_t s) {
+                                double *guess, void *img, int *queue, unsigned *qcount,
+                                int variant, hipStream_t s) {
   if (nz <= 0) return hipSuccess;
   if (variant == 1) {
     // forward pass into guess[2][nz], then the primary solve from it
-    hipError_t e = launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, guess, s);
+    hipError_t e = launch_mlp_forward<kMlpIdeal>(nz, cons, g, blob, img, guess, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
     hipLaunchKernelGGL(k_c2p_guess_main, grid256(nz), dim3(256), 0, s, nz, cons, g, seed,
                        guess, kguess, prim, flag, iters, fallback, eos, a, queue, qcount);
   } else {
+    NetImage *im = static_cast<NetImage *>(img);
+    hipLaunchKernelGGL(k_net_image<kMlpIdeal>, dim3(1), dim3(256), 0, s, blob, im);
     hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, qcount);
     hipLaunchKernelGGL(k_c2p_guess_fused<kMlpIdeal>, grid256(nz), dim3(256), 0, s, nz, cons, g,
-                       seed, blob, kguess, prim, flag, iters, fallback, eos, a, queue, qcount);
+                       seed, blob, im, kguess, prim, flag, iters, fallback, eos, a, queue,
+                       qcount);
   }
   const unsigned g2 = (unsigned)((nz + 255) / 256 < 2048 ? (nz + 255) / 256 : 2048);
   hipLaunchKernelGGL(k_c2p_guess_rest, dim3(g2), dim3(256), 0, s, nz, cons, g, seed, prim,
