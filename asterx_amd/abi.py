@@ -109,7 +109,7 @@ BC = {"linear": 0, "periodic": 1, "copy": 2}
 KCLASS = {"bstag": 0, "face_x": 1, "face_y": 2, "face_z": 3, "aux": 4,
           "update": 5, "c2p": 6, "bc": 7, "rhs": 8, "other": 9,
           "c2p_rest": 10, "mlp": 11, "graph": 12, "tmunu": 13, "regrid": 14,
-          "persist": 15}
+          "persist": 15, "amr": 16}
 C2P_FLAG = {"INIT": 0, "PRIME": 1, "SECOND": 2, "ENTROPY": 3, "ATMO": 4,
             "AVG": 5, "FAIL": 6}
 
@@ -152,6 +152,10 @@ class asx_eos_table(C.Structure):
                 ("eps_min", C.c_double), ("eps_max", C.c_double),
                 ("x_lo", C.c_double * 3), ("x_step", C.c_double * 3),
                 ("x_hi", C.c_double * 3)]
+
+
+class asx_box(C.Structure):
+    _fields_ = [("lo", C.c_int32 * 3), ("hi", C.c_int32 * 3)]
 
 
 class asx_counters(C.Structure):

@@ -1,5 +1,6 @@
 // C ABI of the substep library (include/asterx.h): argument validation,
 // parameter resolution and kernel launches.  Host code only.
+#include <algorithm>
 #include <array>
 #include <atomic>
 #include <cmath>
@@ -1035,6 +1036,198 @@ asx_status asx_bc_fill_many(const asx_patch *pt, int32_t nslot, const int32_t *s
     L.cent[q] = c;
   }
   return dev_status(timed(ASX_KC_BC, (hipStream_t)stream, [&] { return launch_bc_many(kpatch(pt), L, kind, (hipStream_t)stream); }));
+}
+
+}  // extern "C"
+
+// Fine patch f one level finer than c (refinement 2): off[d] = fine index of
+// f's first interior point counted from twice c's first interior point.
+static asx_status amr_geometry(const asx_patch *c, const asx_patch *f, int off[3]) {
+  for (int d = 0; d < 3; ++d) {
+    if (std::fabs(2.0 * f->dx[d] - c->dx[d]) > 1e-12 * c->dx[d]) return ASX_ERR_LAYOUT;
+    const double r = (f->x0[d] - c->x0[d]) / f->dx[d];
+    const double ri = std::nearbyint(r);
+    if (!(std::fabs(r - ri) <= 1e-6) || std::fabs(ri) > 1e9) return ASX_ERR_LAYOUT;
+    off[d] = (int)ri;
+    if ((off[d] & 1) || (f->n[d] & 1)) return ASX_ERR_LAYOUT;
+  }
+  return ASX_OK;
+}
+
+static asx_status amr_slots(const asx_patch *a, const asx_patch *b, int32_t nslot,
+                            const int32_t *slots) {
+  if (nslot < 0 || nslot > kMaxAmrSlots) return ASX_ERR_PARAM;
+  if (nslot > 0 && !slots) return ASX_ERR_NULL;
+  for (int q = 0; q < nslot; ++q) {
+    if (slots[q] < 0 || slots[q] >= ASX_NUM_GF || centering(slots[q]) < 0) return ASX_ERR_PARAM;
+    asx_status st = need(a, {slots[q]});
+    if (st || (st = need(b, {slots[q]}))) return st;
+  }
+  return ASX_OK;
+}
+
+static void amr_strides(const asx_patch *pt, int sj[8], int sk[8]) {
+  for (int c = 0; c < 8; ++c) {
+    int64_t a, b;
+    strides(pt->n, pt->ng, c, a, b);
+    sj[c] = (int)a;
+    sk[c] = (int)b;
+  }
+}
+
+extern "C" {
+
+asx_status asx_prolongate(const asx_patch *coarse, const asx_patch *fine, int32_t nslot,
+                          const int32_t *slots, int32_t order, int32_t region, void *stream) {
+  asx_status st = check_geom(coarse);
+  if (st || (st = check_geom(fine))) return st;
+  if (region < 0 || region > 1) return ASX_ERR_PARAM;
+  if (order != 1 && order != 3 && order != 5) return ASX_ERR_UNSUPPORTED;
+  if ((st = amr_slots(coarse, fine, nslot, slots))) return st;
+  ProlongArgs A{};
+  if ((st = amr_geometry(coarse, fine, A.off))) return st;
+  A.nslot = nslot;
+  A.order = order;
+  A.region = region;
+  A.fng = fine->ng;
+  A.cng = coarse->ng;
+  for (int d = 0; d < 3; ++d) A.fn[d] = fine->n[d];
+  amr_strides(fine, A.fsj, A.fsk);
+  amr_strides(coarse, A.csj, A.csk);
+  int64_t most = 0;
+  for (int q = 0; q < nslot; ++q) {
+    const int c = centering(slots[q]);
+    A.src[q] = coarse->gf[slots[q]].data;
+    A.dst[q] = fine->gf[slots[q]].data;
+    A.cent[q] = c;
+    // every coarse point a stencil reads must exist (every fine index along
+    // each axis is reached, by the ghost slabs of the other axes too)
+    int E3[3];
+    for (int d = 0; d < 3; ++d) {
+      const int vc = (c >> d) & 1;
+      const int E = fine->n[d] + vc + 2 * fine->ng, Ec = coarse->n[d] + vc + 2 * coarse->ng;
+      for (int i = 0; i < E; ++i) {
+        int s0, n;
+        axis_span(A.off[d] + i - fine->ng, vc, order, s0, n);
+        if (s0 + coarse->ng < 0 || s0 + coarse->ng + n > Ec) return ASX_ERR_GHOSTS;
+      }
+      E3[d] = E;
+    }
+    most = std::max(most, BlockShell(A.off, E3, fine->ng).total(region == 1));
+  }
+  return dev_status(timed(ASX_KC_AMR, (hipStream_t)stream, [&] {
+    return launch_prolong(A, most, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_restrict(const asx_patch *fine, const asx_patch *coarse, int32_t nslot,
+                        const int32_t *slots, void *stream) {
+  asx_status st = check_geom(coarse);
+  if (st || (st = check_geom(fine))) return st;
+  if ((st = amr_slots(coarse, fine, nslot, slots))) return st;
+  RestrictArgs A{};
+  if ((st = amr_geometry(coarse, fine, A.off))) return st;
+  A.nslot = nslot;
+  A.fng = fine->ng;
+  A.cng = coarse->ng;
+  amr_strides(fine, A.fsj, A.fsk);
+  amr_strides(coarse, A.csj, A.csk);
+  int64_t most = 0;
+  for (int q = 0; q < nslot; ++q) {
+    const int c = centering(slots[q]);
+    A.src[q] = fine->gf[slots[q]].data;
+    A.dst[q] = coarse->gf[slots[q]].data;
+    A.cent[q] = c;
+    // coarse points (counted from the coarse interior origin) whose fine
+    // points are the fine patch's own: vertex axis 2I - off in [0, n_f],
+    // cell axis both children in [0, n_f - 1]; then inside the coarse interior
+    Box &B = A.box[q];
+    for (int d = 0; d < 3; ++d) {
+      const int vc = (c >> d) & 1;
+      const int lo = std::max(A.off[d] / 2, 0);
+      const int hi = std::min(vc ? (A.off[d] + fine->n[d]) / 2 : A.off[d] / 2 + fine->n[d] / 2 - 1,
+                              coarse->n[d] + vc - 1);
+      B.lo[d] = lo + coarse->ng;
+      B.cnt[d] = std::max(0, hi - lo + 1);
+    }
+    most = std::max(most, B.htotal());
+  }
+  return dev_status(timed(ASX_KC_AMR, (hipStream_t)stream, [&] {
+    return launch_restrict(A, most, (hipStream_t)stream);
+  }));
+}
+
+asx_status asx_regrid_boxes(const asx_patch *pt, double threshold, int32_t n_error_buf,
+                            const int32_t blocking_factor[3], const int32_t max_grid_size[3],
+                            double grid_eff, int32_t max_boxes, asx_box *boxes,
+                            int32_t *nboxes, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (!blocking_factor || !max_grid_size || !nboxes || (max_boxes > 0 && !boxes))
+    return ASX_ERR_NULL;
+  if (n_error_buf < 0 || !(grid_eff > 0.0 && grid_eff <= 1.0) || max_boxes < 0)
+    return ASX_ERR_PARAM;
+  if ((st = need(pt, {ASX_REGRID_ERROR}))) return st;
+  TagArgs A{};
+  int unit[3], per[3];
+  for (int d = 0; d < 3; ++d) {
+    const int bf = blocking_factor[d];
+    if (bf < 1 || (bf & (bf - 1)) || max_grid_size[d] % bf) return ASX_ERR_PARAM;
+    A.blk[d] = std::max(1, bf / 2);
+    if (pt->n[d] % A.blk[d]) return ASX_ERR_PARAM;
+    A.nb[d] = pt->n[d] / A.blk[d];
+    A.n[d] = pt->n[d];
+    unit[d] = 2 * A.blk[d];             // fine cells per block
+    per[d] = max_grid_size[d] / unit[d]; // blocks per piece at most
+    if (per[d] < 1) return ASX_ERR_PARAM;
+  }
+  int64_t sj, sk;
+  strides(pt->n, pt->ng, 0, sj, sk);
+  A.err = pt->gf[ASX_REGRID_ERROR].data;
+  A.sj = (int)sj;
+  A.sk = (int)sk;
+  A.ng = pt->ng;
+  A.buf = n_error_buf;
+  A.thr = threshold;
+  const int64_t nb = (int64_t)A.nb[0] * A.nb[1] * A.nb[2];
+  hipStream_t s = (hipStream_t)stream;
+  uint8_t *dtags = nullptr;
+  if (hipMalloc(&dtags, nb) != hipSuccess) return ASX_ERR_DEVICE;
+  A.tags = dtags;
+  std::vector<uint8_t> tags(nb);
+  hipError_t e = timed(ASX_KC_AMR, s, [&] { return launch_regrid_tags(A, s); });
+  if (e == hipSuccess) e = hipMemcpyAsync(tags.data(), dtags, nb, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  const hipError_t ef = hipFree(dtags);
+  if (e != hipSuccess || ef != hipSuccess) return ASX_ERR_DEVICE;
+  // clusters in blocks, each cut into pieces of at most per[d] blocks
+  // (equal to within one block, the larger ones first), in fine cells
+  int64_t found = 0;
+  for (const ClusterBox &b : cluster_tags(tags.data(), A.nb, grid_eff)) {
+    int np[3], len[3];
+    for (int d = 0; d < 3; ++d) {
+      len[d] = b.hi[d] - b.lo[d] + 1;
+      np[d] = (len[d] + per[d] - 1) / per[d];
+    }
+    for (int pz = 0; pz < np[2]; ++pz)
+      for (int py = 0; py < np[1]; ++py)
+        for (int px = 0; px < np[0]; ++px) {
+          const int p[3] = {px, py, pz};
+          if (found < max_boxes) {
+            asx_box &o = boxes[found];
+            for (int d = 0; d < 3; ++d) {
+              const int base = len[d] / np[d], extra = len[d] % np[d];
+              const int a = b.lo[d] + p[d] * base + std::min(p[d], extra);
+              const int w = base + (p[d] < extra ? 1 : 0);
+              o.lo[d] = a * unit[d];
+              o.hi[d] = (a + w) * unit[d] - 1;
+            }
+          }
+          ++found;
+        }
+  }
+  *nboxes = (int32_t)std::min<int64_t>(found, INT32_MAX);
+  return found > max_boxes ? ASX_ERR_SIZE : ASX_OK;
 }
 
 // The fused substep.  bc_kind < 0: the face data of the one-cell transverse

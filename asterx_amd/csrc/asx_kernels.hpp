@@ -2,6 +2,8 @@
 // translation units and the C ABI layer.
 #pragma once
 
+#include <vector>
+
 #include "asx_c2p.hpp"
 #include "asx_device.hpp"
 
@@ -146,6 +148,126 @@ struct RegridArgs {
   int slot[kMaxRegridVars], cent[kMaxRegridVars];
 };
 hipError_t launch_regrid(const KPatch &P, const Box &B, const RegridArgs &A, hipStream_t s);
+
+// Refinement by 2 between a coarse and a fine patch (asx_amr.hip).  off[d]:
+// fine index of the fine patch's first interior point counted from twice
+// the coarse patch's first interior point; strides per centering code.
+constexpr int kMaxAmrSlots = 64;
+__host__ __device__ inline int floor2(int v) { return (v - (v & 1)) / 2; }
+// Coarse points of the prolongation stencil of fine index I along one axis
+// (I counted from twice the coarse patch's first interior point): the first
+// coarse index (counted from that point) and the number of points.  A
+// vertex axis takes the coinciding point (even I) or the order + 1 points
+// centred on the midpoint; a cell axis the order cells centred on the parent.
+__host__ __device__ inline void axis_span(int I, bool vertex, int order, int &start, int &n) {
+  if (vertex) {
+    if ((I & 1) == 0) {
+      start = I / 2;
+      n = 1;
+    } else {
+      start = floor2(I - 1) - (order - 1) / 2;
+      n = order + 1;
+    }
+  } else {
+    start = floor2(I) - (order - 1) / 2;
+    n = order;
+  }
+}
+// The coarse blocks of a fine grid function (block a holds the fine points
+// 2a and 2a+1 of each axis, fine indices counted from twice the coarse
+// patch's first interior point) over its extent E[d] with fng ghosts and
+// interior offset off[d]: nbk[d] blocks per axis, of which the first gl[d]
+// and the last gh[d] hold ghost points.  decode() maps t to block indices
+// over all blocks, or over the shell of blocks holding ghost points: the
+// low and high z planes, then the y rows of the other planes, then the x
+// ends of the remaining rows.
+struct BlockShell {
+  int nbk[3], gl[3], gh[3];
+  __host__ __device__ BlockShell(const int off[3], const int E[3], int fng) {
+    for (int d = 0; d < 3; ++d) {
+      const int a0 = floor2(off[d] - fng);
+      nbk[d] = floor2(off[d] - fng + E[d] - 1) - a0 + 1;
+      gl[d] = off[d] / 2 - a0;
+      gl[d] = gl[d] < 0 ? 0 : gl[d] > nbk[d] ? nbk[d] : gl[d];
+      const int bh = floor2(E[d] - 2 * fng + off[d]) - a0; // first block with a high ghost
+      gh[d] = nbk[d] - (bh < gl[d] ? gl[d] : bh > nbk[d] ? nbk[d] : bh);
+    }
+  }
+  __host__ __device__ int64_t total(bool shell) const {
+    const int64_t all = (int64_t)nbk[0] * nbk[1] * nbk[2];
+    if (!shell) return all;
+    return all - (int64_t)(nbk[0] - gl[0] - gh[0]) * (nbk[1] - gl[1] - gh[1]) *
+                     (nbk[2] - gl[2] - gh[2]);
+  }
+  __host__ __device__ void decode(int64_t t, bool shell, int b[3]) const {
+    if (!shell) {
+      b[0] = (int)(t % nbk[0]);
+      b[1] = (int)((t / nbk[0]) % nbk[1]);
+      b[2] = (int)(t / ((int64_t)nbk[0] * nbk[1]));
+      return;
+    }
+    const int64_t plane = (int64_t)nbk[0] * nbk[1];
+    const int tz = gl[2] + gh[2], ty = gl[1] + gh[1], tx = gl[0] + gh[0];
+    if (t < tz * plane) {
+      const int p = (int)(t / plane), r = (int)(t % plane);
+      b[2] = p < gl[2] ? p : nbk[2] - gh[2] + (p - gl[2]);
+      b[1] = r / nbk[0];
+      b[0] = r % nbk[0];
+      return;
+    }
+    t -= tz * plane;
+    const int mz = nbk[2] - tz;
+    if (t < (int64_t)ty * nbk[0] * mz) {
+      b[0] = (int)(t % nbk[0]);
+      const int q = (int)(t / nbk[0]), row = q % ty;
+      b[1] = row < gl[1] ? row : nbk[1] - gh[1] + (row - gl[1]);
+      b[2] = gl[2] + q / ty;
+      return;
+    }
+    t -= (int64_t)ty * nbk[0] * mz;
+    const int my = nbk[1] - ty;
+    const int col = (int)(t % tx), q = (int)(t / tx);
+    b[0] = col < gl[0] ? col : nbk[0] - gh[0] + (col - gl[0]);
+    b[1] = gl[1] + q % my;
+    b[2] = gl[2] + q / my;
+  }
+};
+
+struct ProlongArgs {
+  const double *src[kMaxAmrSlots];
+  double *dst[kMaxAmrSlots];
+  int cent[kMaxAmrSlots];
+  int nslot, order, region; // region 0: every fine point, 1: ghost points only
+  int off[3], fn[3], fng, cng;
+  int fsj[8], fsk[8], csj[8], csk[8];
+};
+struct RestrictArgs {
+  const double *src[kMaxAmrSlots];
+  double *dst[kMaxAmrSlots];
+  int cent[kMaxAmrSlots];
+  Box box[kMaxAmrSlots]; // coarse points written, per slot
+  int nslot;
+  int off[3], fng, cng;
+  int fsj[8], fsk[8], csj[8], csk[8];
+};
+// max_blocks: the largest number of coarse blocks (2 x 2 x 2 fine points)
+// of one slot; max_points: the largest coarse box of one slot
+hipError_t launch_prolong(const ProlongArgs &A, int64_t max_blocks, hipStream_t s);
+hipError_t launch_restrict(const RestrictArgs &A, int64_t max_points, hipStream_t s);
+
+// Regrid tags on blocks of blk[d] interior cells (cell layout strides sj, sk)
+struct TagArgs {
+  const double *err;
+  uint8_t *tags;
+  int sj, sk, ng, n[3], blk[3], nb[3], buf;
+  double thr;
+};
+hipError_t launch_regrid_tags(const TagArgs &A, hipStream_t s);
+// Berger-Rigoutsos clusters of the block tags (inclusive block indices)
+struct ClusterBox {
+  int lo[3], hi[3];
+};
+std::vector<ClusterBox> cluster_tags(const uint8_t *tags, const int nb[3], double grid_eff);
 hipError_t launch_sources(const KPatch &P, const Box &B, const RhsArgs &A,
                           hipStream_t s);
 hipError_t launch_rhs(const KPatch &P, const RhsArgs &A, hipStream_t s);

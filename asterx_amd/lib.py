@@ -35,6 +35,12 @@ SIGNATURES = {
                            C.POINTER(_S), _V], C.c_int),
     "asx_bc_fill": ([_P, C.c_int32, C.c_int32, _V], C.c_int),
     "asx_bc_fill_many": ([_P, C.c_int32, C.POINTER(C.c_int32), C.c_int32, _V], C.c_int),
+    "asx_prolongate": ([_P, _P, C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32, _V],
+                       C.c_int),
+    "asx_restrict": ([_P, _P, C.c_int32, C.POINTER(C.c_int32), _V], C.c_int),
+    "asx_regrid_boxes": ([_P, C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                          C.POINTER(C.c_int32), C.c_double, C.c_int32,
+                          C.POINTER(abi.asx_box), C.POINTER(C.c_int32), _V], C.c_int),
     "asx_timing_enable": ([C.c_int32], C.c_int),
     "asx_timing_read": ([C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double)],
                         C.c_int),

@@ -72,6 +72,8 @@ def parse():
                     help="also time con2prim with the learned guess against "
                          "the plain seed on 2^24 independent zones")
     ap.add_argument("--c2p-zones", type=int, default=1 << 24)
+    ap.add_argument("--amr", choices=["on", "off"], default="on",
+                    help="prolongation, restriction and regrid boxes on the torus")
     ap.add_argument("--tab", choices=["on", "off"], default="on",
                     help="also run the tabulated-EOS workload (torus on the table and "
                          "con2prim with composition)")
@@ -648,6 +650,108 @@ def bench_regrid(p, par, L, sh):
             "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
 
 
+def bench_amr(p, par, L, sh, reps=10):
+    """Refinement by 2 on the evolved torus: a fine patch of 2m^3 cells
+    over the central m^3 coarse cells (m = n/2, 3 ghosts), the 11 evolved
+    grid functions (cell, edge and vertex centred).  Prolongation of order 3
+    (every fine point: a new patch; ghost points only: the coarse-fine
+    boundary fill of a substep), restriction of the fine patch back, and the
+    regrid box table from the first gradient of rho.  Algorithmic bytes:
+    prolongation 8 B per fine point written plus the coarse points of the
+    region read once (1/8 per fine point); restriction 8 B per coarse point
+    written plus its fine points read once.  HIP events on the library's
+    stream; the box table (device tags, host clustering) by wall clock."""
+    import numpy as np
+    import torch
+    from asterx_amd import abi
+    from asterx_amd.patch import Patch
+    m = (p.n[0] // 4) * 2
+    lo = [(p.n[d] - m) // 2 for d in range(3)]
+    fdx = tuple(d / 2 for d in p.dx)
+    names = list(abi.STATE)
+    with torch.cuda.device(0):
+        f = Patch((2 * m,) * 3, p.ng, tuple(p.x0[d] + lo[d] * p.dx[d] for d in range(3)), fdx,
+                  slots=names, device="cuda:0")
+    slots = (C.c_int32 * len(names))(*[abi.SLOT[n] for n in names])
+    ps, fs = p.struct(), f.struct()
+
+    def timed(fn):
+        for _ in range(2):
+            abi.check("amr", fn())
+        torch.cuda.synchronize()
+        abi.check("asx_timing_enable", L.asx_timing_enable(1))
+        for _ in range(reps):
+            abi.check("amr", fn())
+        torch.cuda.synchronize()
+        nl, ms = C.c_int64(), C.c_double()
+        abi.check("asx_timing_read", L.asx_timing_read(abi.KCLASS["amr"], C.byref(nl),
+                                                      C.byref(ms)))
+        abi.check("asx_timing_enable", L.asx_timing_enable(0))
+        return ms.value / nl.value
+
+    def points(n, ng, name, own):
+        e = abi.extents(n, ng, abi.centering(name))
+        if own:
+            return (e[0] - 2 * ng) * (e[1] - 2 * ng) * (e[2] - 2 * ng)
+        return e[0] * e[1] * e[2]
+
+    out = {"workload": "fine %d^3 over the central %d^3 cells of the %d^3 torus, 11 evolved "
+                       "grid functions" % (2 * m, m, p.n[0])}
+    for region, key in ((0, "prolongate_patch"), (1, "prolongate_ghosts")):
+        ms = timed(lambda: L.asx_prolongate(C.byref(ps), C.byref(fs), len(names), slots, 3,
+                                            region, sh))
+        pts = sum(points(f.n, f.ng, n, False) - (points(f.n, f.ng, n, True) if region else 0)
+                  for n in names)
+        ach = 9 * pts / (ms * 1e-3) / 1e9
+        out[key] = {"order": 3, "fine_points": pts, "avg_ms": round(ms, 4),
+                    "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+    ms = timed(lambda: L.asx_restrict(C.byref(fs), C.byref(ps), len(names), slots, sh))
+    cpts, byt = 0, 0
+    for n in names:
+        c = abi.centering(n)
+        cp = 1
+        for d in range(3):
+            cp *= m + ((c >> d) & 1)
+        kids = 1
+        for d in range(3):
+            kids *= 1 if (c >> d) & 1 else 2
+        cpts += cp
+        byt += 8 * cp * (1 + kids)
+    ach = byt / (ms * 1e-3) / 1e9
+    out["restrict"] = {"coarse_points": cpts, "avg_ms": round(ms, 4),
+                       "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4)}
+    # box table from the first gradient of rho
+    p.alloc("regrid_error")
+    ps = p.struct()
+    rs = (C.c_int32 * 1)(*abi.regrid_slots("HydroBaseX::rho"))
+    abi.check("asx_estimate_error", L.asx_estimate_error(C.byref(ps), C.byref(par), 1, rs, sh))
+    torch.cuda.synchronize()
+    err = p.numpy("regrid_error")[p.ng:-p.ng, p.ng:-p.ng, p.ng:-p.ng]
+    thr = float(np.quantile(err, 0.9))
+    bf, mgs = (C.c_int32 * 3)(8, 8, 8), (C.c_int32 * 3)(64, 64, 64)
+    cap = 1 << 16
+    boxes = (abi.asx_box * cap)()
+    nb = C.c_int32()
+    t = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        abi.check("asx_regrid_boxes", L.asx_regrid_boxes(C.byref(ps), thr, 1, bf, mgs, 0.7,
+                                                        cap, boxes, C.byref(nb), sh))
+        t.append(time.perf_counter() - t0)
+    fine_cells = 0
+    for q in range(nb.value):
+        b = boxes[q]
+        fine_cells += (b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1)
+    tagged = float((err > thr).mean())
+    out["regrid_boxes"] = {"threshold": "90th percentile of the first gradient of rho",
+                           "n_error_buf": 1, "blocking_factor": 8, "max_grid_size": 64,
+                           "grid_eff": 0.7, "boxes": nb.value, "tagged_frac": round(tagged, 4),
+                           "covered_frac": round(fine_cells / (8.0 * err.size), 4),
+                           "median_ms": round(1e3 * float(np.median(t[1:])), 2)}
+    del p.arr["regrid_error"]
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -1029,6 +1133,11 @@ def main():
     log(args, "tmunu: %.1f s" % (time.perf_counter() - t0), tmunu)
     regrid = bench_regrid(p, par, L, sh)
     log(args, "regrid:", regrid)
+    amr = None
+    if args.amr == "on":
+        t0 = time.perf_counter()
+        amr = bench_amr(p, par, L, sh)
+        log(args, "amr: %.1f s" % (time.perf_counter() - t0), amr)
     del st, p
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
@@ -1080,9 +1189,33 @@ def main():
                       "zones": zones, "substep": "SSPRK3 stages cycled"},
            "roofline": roof, "cpu_baseline": cpu,
            "batched": batched, "c2p_guess": guess, "tabulated": tab,
-           "tmunu": tmunu, "regrid_error": regrid}
+           "tmunu": tmunu, "regrid_error": regrid, "amr": amr}
     out.update(extra)
     print(json.dumps(out), flush=True)
+    reap_children()
+
+
+def reap_children():
+    """End every child process this run left (none is started on purpose),
+    so that no process outlives bench.py; each is named on stderr."""
+    try:
+        import psutil
+    except ImportError:
+        return
+    kids = psutil.Process().children(recursive=True)
+    for k in kids:
+        try:
+            print("bench.py: ending leftover child %d (%s)" % (k.pid, " ".join(k.cmdline())),
+                  file=sys.stderr, flush=True)
+            k.terminate()
+        except psutil.Error:
+            pass
+    _, alive = psutil.wait_procs(kids, timeout=5)
+    for k in alive:
+        try:
+            k.kill()
+        except psutil.Error:
+            pass
 
 
 if __name__ == "__main__":

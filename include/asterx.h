@@ -315,7 +315,8 @@ enum {
   ASX_KC_TMUNU = 13,    /* stress-energy tensor at the vertices */
   ASX_KC_REGRID = 14,   /* regrid error estimate */
   ASX_KC_PERSIST = 15,  /* a batch plan's stage as one persistent launch */
-  ASX_KC_COUNT = 16
+  ASX_KC_AMR = 16,      /* prolongation, restriction and regrid tags */
+  ASX_KC_COUNT = 17
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
@@ -424,6 +425,66 @@ asx_status asx_bc_fill(const asx_patch *patch, int32_t slot, int32_t kind,
  * as asx_bc_fill for each listed slot; ASX_ERR_PARAM for nslot > 64. */
 asx_status asx_bc_fill_many(const asx_patch *patch, int32_t nslot,
                             const int32_t *slots, int32_t kind, void *stream);
+
+/* Mesh refinement by a factor 2 between two patches of adjacent levels, the
+ * CarpetX operators the AsterX regression runs select with
+ * CarpetX::prolongation_type = "ddf" (e.g. AsterX/test/Balsara1_shocktube_
+ * xdir_ppm.par:59) and CarpetX::max_num_levels > 1.  The fine patch's dx is
+ * half the coarse one's, and its interior starts on a coarse cell face:
+ * (fine.x0 - coarse.x0) / fine.dx is an even integer on every axis and every
+ * fine n[d] is even (ASX_ERR_LAYOUT otherwise).
+ *
+ * asx_prolongate: coarse -> fine for each listed grid function (at most 64),
+ * per axis by its centering ("discrete differential forms"): along a vertex
+ * axis a fine point on a coarse point takes its value, a fine point between
+ * two takes the centred Lagrange interpolant of degree `order` (order + 1
+ * coarse points); along a cell axis each of the two children of a coarse
+ * cell takes the average over its half of the polynomial of degree
+ * order - 1 whose cell averages match the order coarse cells centred on the
+ * parent (so the two children average to the parent).  3-D by tensor
+ * product, x innermost.  region 0 writes every point of the fine patch (a
+ * new patch after regridding), region 1 only its ghost points (the points
+ * outside the patch's own cells and boundary vertices: the coarse-fine
+ * boundary fill before a substep).  order 1, 3 or 5 (others:
+ * ASX_ERR_UNSUPPORTED); a stencil reaching past the coarse patch's points:
+ * ASX_ERR_GHOSTS.  CarpetX's source is not part of the reference tree; the
+ * operators are restated from the discrete-differential-form rule above.
+ *
+ * asx_restrict: fine -> coarse on the coarse patch's own points covered by
+ * the fine patch's own points: along a vertex axis the coinciding fine point,
+ * along a cell axis the mean of the two children; tensor product. */
+asx_status asx_prolongate(const asx_patch *coarse, const asx_patch *fine, int32_t nslot,
+                          const int32_t *slots, int32_t order, int32_t region,
+                          void *stream);
+asx_status asx_restrict(const asx_patch *fine, const asx_patch *coarse, int32_t nslot,
+                        const int32_t *slots, void *stream);
+
+/* Refined boxes for the next level from regrid_error (the box table AMReX
+ * builds for CarpetX at a regrid, CarpetX::regrid_error_threshold,
+ * CarpetX::blocking_factor_*, CarpetX::max_grid_size_*; AsterX/test/
+ * Balsara1_shocktube_xdir_ppm.par:52-57).  A cell is tagged when its
+ * regrid_error exceeds `threshold`; tags grow by n_error_buf cells (inside
+ * the interior), are coarsened to blocks of blocking_factor/2 cells and
+ * clustered by Berger & Rigoutsos (1991): a box whose tagged fraction is at
+ * least grid_eff (or one block) is kept, otherwise it is cut at the empty
+ * plane nearest its middle, else at the strongest inflection of the tag
+ * signature, else in half along its longest axis.  The boxes are returned
+ * in FINE cell indices (interior cell i of the patch covers fine cells 2i,
+ * 2i+1), inclusive, aligned to blocking_factor and cut to at most
+ * max_grid_size fine cells per axis.  Each blocking_factor[d] must be a
+ * power of two dividing max_grid_size[d], each n[d] a multiple of
+ * max(1, blocking_factor[d] / 2), n_error_buf >= 0 and 0 < grid_eff <= 1
+ * (ASX_ERR_PARAM otherwise).  At most max_boxes are written; *nboxes is the
+ * number found (ASX_ERR_SIZE when it exceeds max_boxes).  Waits for the
+ * stream. */
+typedef struct {
+  int32_t lo[3], hi[3];
+} asx_box;
+asx_status asx_regrid_boxes(const asx_patch *patch, double threshold, int32_t n_error_buf,
+                            const int32_t blocking_factor[3],
+                            const int32_t max_grid_size[3], double grid_eff,
+                            int32_t max_boxes, asx_box *boxes, int32_t *nboxes,
+                            void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* Fused substep (the performance path; no reference counterpart as one    */
