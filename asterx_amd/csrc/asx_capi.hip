@@ -688,6 +688,7 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
   A.write_uct = uct;
   A.bstag_from_A = 0; // AsterX_Fluxes reads dB_stag (schedule.ccl:173-194)
   A.cnt_dev = cd;
+  A.with_aux = 0;
   for (int d = 0; d < 3; ++d) {
     int halo[3] = {face_halo, face_halo, face_halo};
     halo[d] = 0;
@@ -696,12 +697,13 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
+    if (d == 2 && aux) { // CalcAuxForAvecPsi rides on the z-face launch
+      const int h = 2 * face_halo;
+      const int ha[3] = {h, h, h};
+      A.with_aux = 1;
+      A.aux = box(pt, 7, ha);
+    }
     if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s, tab); })))) return st;
-  }
-  if (aux) {
-    const int h = 2 * face_halo;
-    const int halo[3] = {h, h, h};
-    if ((st = dev_status(timed(ASX_KC_AUX, s, [&] { return launch_aux(P, box(pt, 7, halo), s); })))) return st;
   }
   if (cnt) {
     unsigned long long hbuf[2];
@@ -1302,6 +1304,7 @@ static asx_status substep_impl(const asx_patch *stage, const asx_state *u0,
   // ComputedBstagFromA would store on the interior faces)
   A.bstag_from_A = 1;
   A.cnt_dev = nullptr;
+  A.with_aux = 0;
   const int fh = bc ? 0 : 1;
   for (int d = 0; d < 3; ++d) {
     int halo[3] = {fh, fh, fh};
@@ -1311,12 +1314,13 @@ static asx_status substep_impl(const asx_patch *stage, const asx_state *u0,
       A.lo[q] = b.lo[q];
       A.cnt[q] = b.cnt[q];
     }
+    if (d == 2) { // the vertex aux ride on the z-face launch
+      const int h = 2 * fh;
+      const int ha[3] = {h, h, h};
+      A.with_aux = 1;
+      A.aux = box(stage, 7, ha);
+    }
     if ((st = dev_status(timed(ASX_KC_FACE_X + d, s, [&] { return launch_flux(d, P, A, par->reconstruction_method, variant_of(ASX_KC_FACE_X + d), s, tab); })))) return st;
-  }
-  {
-    const int h = 2 * fh;
-    const int halo[3] = {h, h, h};
-    if ((st = dev_status(timed(ASX_KC_AUX, s, [&] { return launch_aux(P, box(stage, 7, halo), s); })))) return st;
   }
   if (bc) {
     BCList L;
@@ -1917,6 +1921,8 @@ FluxArgs plan_flux_args(const asx_params *par) {
   A.bstag_from_A = 1; // the faces take the curl of the stage's A
   A.cnt_dev = nullptr;
   for (int q = 0; q < 3; ++q) A.lo[q] = A.cnt[q] = 0;
+  A.with_aux = 0; // the batch launches aux over its own (patch, chunk) list
+  A.aux = Box{};
   return A;
 }
 

@@ -148,30 +148,46 @@ template <int D> ASX_DEV double bstag_value(const KPatch &P, int i, int j, int k
   }
 }
 
+// The metric at face (i, j, k) (v2f averages of the vertex lapse, shift and
+// 3-metric) and the staggered normal field there.  The face tile loads them
+// before it reconstructs, so that their latency hides behind the
+// reconstruction.
+struct FaceGeom {
+  double alp, beta[3];
+  Sym3 g;
+  double bn;
+};
+template <int D>
+ASX_DEV void face_geom(const KPatch &P, const FluxArgs &A, int i, int j, int k, FaceGeom &G) {
+  const int ov = off(P, 7, i, j, k);
+  const int svj = P.sj[7], svk = P.sk[7];
+  G.alp = avg_v2f<D>(P.g[ASX_ALP], ov, svj, svk);
+  G.beta[0] = avg_v2f<D>(P.g[ASX_BETAX], ov, svj, svk);
+  G.beta[1] = avg_v2f<D>(P.g[ASX_BETAY], ov, svj, svk);
+  G.beta[2] = avg_v2f<D>(P.g[ASX_BETAZ], ov, svj, svk);
+  G.g.xx = avg_v2f<D>(P.g[ASX_GXX], ov, svj, svk);
+  G.g.xy = avg_v2f<D>(P.g[ASX_GXY], ov, svj, svk);
+  G.g.xz = avg_v2f<D>(P.g[ASX_GXZ], ov, svj, svk);
+  G.g.yy = avg_v2f<D>(P.g[ASX_GYY], ov, svj, svk);
+  G.g.yz = avg_v2f<D>(P.g[ASX_GYZ], ov, svj, svk);
+  G.g.zz = avg_v2f<D>(P.g[ASX_GZZ], ov, svj, svk);
+  G.bn = A.bstag_from_A ? bstag_value<D>(P, i, j, k)
+                        : P.g[ASX_DBX_STAG + D][off(P, 1 << D, i, j, k)];
+}
+
 template <int D, bool TAB = false>
 ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
-                       const Side &L, const Side &R, bool lo0, bool lo1) {
+                       const FaceGeom &G, const Side &L, const Side &R, bool lo0, bool lo1) {
   ASX_CONTRACT
   constexpr int CF = 1 << D;      // face centering
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
   const int ofc = off(P, CF, i, j, k);
-  const int ov = off(P, 7, i, j, k);
-  const int svj = P.sj[7], svk = P.sk[7];
   const IdealGas &eos = A.eos;
 
   // metric at the face
-  const double alp = avg_v2f<D>(P.g[ASX_ALP], ov, svj, svk);
-  double betas[3];
-  betas[0] = avg_v2f<D>(P.g[ASX_BETAX], ov, svj, svk);
-  betas[1] = avg_v2f<D>(P.g[ASX_BETAY], ov, svj, svk);
-  betas[2] = avg_v2f<D>(P.g[ASX_BETAZ], ov, svj, svk);
-  Sym3 g;
-  g.xx = avg_v2f<D>(P.g[ASX_GXX], ov, svj, svk);
-  g.xy = avg_v2f<D>(P.g[ASX_GXY], ov, svj, svk);
-  g.xz = avg_v2f<D>(P.g[ASX_GXZ], ov, svj, svk);
-  g.yy = avg_v2f<D>(P.g[ASX_GYY], ov, svj, svk);
-  g.yz = avg_v2f<D>(P.g[ASX_GYZ], ov, svj, svk);
-  g.zz = avg_v2f<D>(P.g[ASX_GZZ], ov, svj, svk);
+  const double alp = G.alp;
+  const double betas[3] = {G.beta[0], G.beta[1], G.beta[2]};
+  const Sym3 g = G.g;
   const double detg = det3(g);
   const double sqrtg = sqrt(detg);
   const double isqrtg = 1.0 / sqrtg;
@@ -203,8 +219,7 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
   // magnetic field: normal from the staggered field, transverse recon
   double Bs[3][2];
   {
-    const double bn = A.bstag_from_A ? bstag_value<D>(P, i, j, k) : P.g[ASX_DBX_STAG + D][ofc];
-    const double val = bn * isqrtg;
+    const double val = G.bn * isqrtg;
     Bs[D][0] = val;
     Bs[D][1] = val;
     Bs[D1][0] = L.b1;
@@ -468,6 +483,9 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
   const int fd = D == 0 ? fx : (D == 1 ? fy : fz);
   const bool need = fx < cnt[0] && fy < cnt[1] && fz < cnt[2];
   const int i = lo3[0] + fx, j = lo3[1] + fy, k = lo3[2] + fz;
+  const bool flux = need && td >= 1 && fd >= 0;
+  FaceGeom G;
+  if (flux) face_geom<D>(P, A, i, j, k, G);
   Side lo, up;
   bool flo = false, fup = false;
   if (need) {
@@ -484,7 +502,7 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     shf[t] = fup;
   }
   __syncthreads();
-  if (need && td >= 1 && fd >= 0) {
+  if (flux) {
     const int tl = t - SD;
     Side L;
     L.rho = sh[0][tl];
@@ -496,7 +514,7 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     L.v[0] = sh[6][tl];
     L.v[1] = sh[7][tl];
     L.v[2] = sh[8][tl];
-    face_flux<D, TAB>(P, A, i, j, k, L, lo, shf[tl] != 0, flo);
+    face_flux<D, TAB>(P, A, i, j, k, G, L, lo, shf[tl] != 0, flo);
   }
 }
 

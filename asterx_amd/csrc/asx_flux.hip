@@ -2,6 +2,8 @@
 // gfx950.  Vertex and curl kernels: one work-item per point, x-fastest so a
 // wavefront reads 64 consecutive doubles of every stencil row.  Face
 // kernels: cell tiles with an LDS hand-over between neighbouring cells.
+#include <algorithm>
+
 #include "asx_face.hpp"
 
 namespace asx {
@@ -11,8 +13,19 @@ __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(TAB ? ASX_FACE_WAVES_TAB : ASX_FACE_WAVES)))
 void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
   __shared__ FaceLDS S;
-  face_tile<D, M, V, TAB>(P, A, A.lo, A.cnt, xcd_tile((int)blockIdx.x, ntx * nty * ntz), ntx,
-                          nty, S);
+  // workgroup u takes face tile u and, when the launch carries the vertex
+  // auxiliaries, their 256-vertex chunk u: the aux loads and stores then
+  // overlap the face arithmetic instead of running as a launch of their own
+  const int u = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (u < ntx * nty * ntz) face_tile<D, M, V, TAB>(P, A, A.lo, A.cnt, u, ntx, nty, S);
+  if (A.with_aux) {
+    const int64_t t = (int64_t)u * 256 + threadIdx.x;
+    if (t < A.aux.total()) {
+      int i, j, k;
+      A.aux.ijk(t, i, j, k);
+      aux_point(P, i, j, k);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_aux(KPatch P, Box B) {
@@ -107,7 +120,9 @@ static hipError_t launch_flux_dv(const KPatch &P, const FluxArgs &A, int method,
     const int per = (q == D) ? ex[q] - 1 : ex[q];
     nt[q] = (A.cnt[q] + per - 1) / per;
   }
-  const dim3 grid((unsigned)(nt[0] * nt[1] * nt[2])), block(256);
+  int64_t nb = (int64_t)nt[0] * nt[1] * nt[2];
+  if (A.with_aux) nb = std::max<int64_t>(nb, (A.aux.htotal() + 255) / 256);
+  const dim3 grid((unsigned)nb), block(256);
   if (tab) {
     switch (method) {
     case ASX_RECON_PPM: hipLaunchKernelGGL((k_face<D, ASX_RECON_PPM, V, true>), grid, block, 0, s, P, A, nt[0], nt[1], nt[2]); break;

@@ -112,6 +112,8 @@ struct FluxArgs {
   int bstag_from_A; // normal face field as the curl of A at the face (no dB_stag read)
   int lo[3], cnt[3];
   unsigned long long *cnt_dev; // [nan_faces, lo_faces] or null
+  int with_aux; // the face launch also computes the vertex auxiliaries of box aux
+  Box aux;
 };
 
 struct RhsArgs {

@@ -940,7 +940,7 @@ def substep_roofline(par, zones, K, first, kt, ms_step, workload):
     ach = per / (gpu_ms * 1e-3) / 1e9
     traffic = None
     parts = [read_traffic(k, workload) for k in ("face_x", "face_y", "face_z", "aux",
-                                                 "update", "c2p")]
+                                                 "update", "c2p") if k in kt]
     if all(t is not None for t in parts):
         traffic = sum(parts)
     return {"bound": "hbm", "scope": "whole substep: RHS + RK combination, SYNC, B from A "
