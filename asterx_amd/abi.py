@@ -115,7 +115,8 @@ C2P_FLAG = {"INIT": 0, "PRIME": 1, "SECOND": 2, "ENTROPY": 3, "ATMO": 4,
 
 STATUS = {0: "ASX_OK", 1: "ASX_ERR_NULL", 2: "ASX_ERR_LAYOUT",
           3: "ASX_ERR_PARAM", 4: "ASX_ERR_UNSUPPORTED", 5: "ASX_ERR_GHOSTS",
-          6: "ASX_ERR_DEVICE", 7: "ASX_ERR_SIZE"}
+          6: "ASX_ERR_DEVICE", 7: "ASX_ERR_SIZE", 8: "ASX_ERR_POISON"}
+POISON_BITS = 0x7ff4dead0badf00d
 
 
 class AsxError(RuntimeError):

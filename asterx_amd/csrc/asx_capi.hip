@@ -523,6 +523,102 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// ---------------------------------------------------------------------------
+// Debug poison (CarpetX::poison_undefined_values): outputs filled with
+// ASX_POISON_BITS before an entry point runs, its declared region counted
+// afterwards.
+// ---------------------------------------------------------------------------
+std::atomic<int> g_poison{0};
+std::mutex g_pmu;
+int32_t g_poison_slot = -1;
+int64_t g_poison_points = 0;
+constexpr int kPoisonBatch = 64;
+std::map<int, unsigned long long *> g_pcnt;
+
+unsigned long long *poison_buf(int dev) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_pcnt.find(dev);
+  if (it != g_pcnt.end()) return it->second;
+  unsigned long long *p = nullptr;
+  if (hipMalloc(&p, kPoisonBatch * sizeof(unsigned long long)) != hipSuccess) return nullptr;
+  g_pcnt[dev] = p;
+  return p;
+}
+
+// The outputs of one call.  out(): a grid function the call only writes,
+// filled on every point and checked on box w; scratch(): library memory
+// filled only.
+struct Poisoner {
+  struct Out {
+    double *g;
+    int64_t n;
+    int sj, sk, slot;
+    Box w;
+  };
+  bool on = g_poison.load() != 0;
+  std::vector<Out> outs;
+  std::vector<std::pair<double *, int64_t>> extra;
+  void out(const asx_patch *pt, int slot, const Box &w) {
+    if (!on || !pt->gf[slot].data) return;
+    const int c = centering(slot);
+    int64_t sj, sk;
+    strides(pt->n, pt->ng, c, sj, sk);
+    outs.push_back({pt->gf[slot].data, npoints(pt->n, pt->ng, c), (int)sj, (int)sk, slot, w});
+  }
+  void out_range(const asx_patch *pt, int a, int b, const Box &w) {
+    for (int q = a; q <= b; ++q) out(pt, q, w);
+  }
+  // array g laid out as slot `slot` of a patch of geometry pt
+  void out_array(double *g, const asx_patch *pt, int slot, const Box &w) {
+    if (!on || !g) return;
+    const int c = centering(slot);
+    int64_t sj, sk;
+    strides(pt->n, pt->ng, c, sj, sk);
+    outs.push_back({g, npoints(pt->n, pt->ng, c), (int)sj, (int)sk, slot, w});
+  }
+  void scratch(double *p, int64_t n) {
+    if (on && p) extra.push_back({p, n});
+  }
+  asx_status fill(hipStream_t s) const {
+    for (const Out &o : outs)
+      if (launch_poison_fill(o.g, o.n, ASX_POISON_BITS, s) != hipSuccess) return ASX_ERR_DEVICE;
+    for (const auto &e : extra)
+      if (launch_poison_fill(e.first, e.second, ASX_POISON_BITS, s) != hipSuccess)
+        return ASX_ERR_DEVICE;
+    return ASX_OK;
+  }
+  // st: the call's own status; the check runs only after a successful call
+  asx_status check(hipStream_t s, asx_status st) const {
+    if (!on || st != ASX_OK || outs.empty()) return st;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return ASX_ERR_DEVICE;
+    unsigned long long *cnt = poison_buf(dev);
+    if (!cnt) return ASX_ERR_DEVICE;
+    for (size_t a = 0; a < outs.size(); a += kPoisonBatch) {
+      const int m = (int)std::min<size_t>(kPoisonBatch, outs.size() - a);
+      hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(kPoisonBatch), 0, s, cnt, m);
+      for (int q = 0; q < m; ++q) {
+        const Out &o = outs[a + q];
+        if (launch_poison_count(o.g, o.sj, o.sk, o.w, ASX_POISON_BITS, cnt + q, s) != hipSuccess)
+          return ASX_ERR_DEVICE;
+      }
+      unsigned long long h[kPoisonBatch];
+      if (hipMemcpyAsync(h, cnt, m * sizeof(unsigned long long), hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return ASX_ERR_DEVICE;
+      for (int q = 0; q < m; ++q)
+        if (h[q]) {
+          std::lock_guard<std::mutex> lk(g_pmu);
+          g_poison_slot = outs[a + q].slot;
+          g_poison_points = (int64_t)h[q];
+          return ASX_ERR_POISON;
+        }
+    }
+    return ASX_OK;
+  }
+};
+
 // Scratch slots the fused substep computes: fluxes, upwind-CT face data,
 // aux, and dB_stag (written only by asx_substep_bc with use_uct; the faces
 // take their normal field from A).
@@ -676,6 +772,24 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
     if (!cd) return ASX_ERR_DEVICE;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(128), 0, s, cd, 80);
   }
+  Poisoner pz;
+  for (int d = 0; d < 3; ++d) {
+    int halo[3] = {face_halo, face_halo, face_halo};
+    halo[d] = 0;
+    const Box b = box(pt, 1 << d, halo);
+    pz.out_range(pt, ASX_FXDENS + 10 * d, ASX_FXDENS + 10 * d + 9, b);
+    if (uct) {
+      pz.out_range(pt, ASX_VTILDE_Y_XFACE + 2 * d, ASX_VTILDE_Y_XFACE + 2 * d + 1, b);
+      pz.out(pt, ASX_AMAX_XFACE + 2 * d, b);
+      pz.out(pt, ASX_AMIN_XFACE + 2 * d, b);
+    }
+  }
+  if (aux) {
+    const int h = 2 * face_halo;
+    const int ha[3] = {h, h, h};
+    pz.out_range(pt, ASX_FX, ASX_G, box(pt, 7, ha));
+  }
+  if ((st = pz.fill(s))) return st;
   FluxArgs A;
   A.rp = rparams(par);
   A.eos = ideal_gas(par);
@@ -713,7 +827,7 @@ asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_h
     cnt->nan_faces += (int64_t)hbuf[0];
     cnt->lo_faces += (int64_t)hbuf[1];
   }
-  return ASX_OK;
+  return pz.check(s, ASX_OK);
 }
 
 static RhsArgs rhs_args(const asx_params *par) {
@@ -742,8 +856,12 @@ asx_status asx_source_terms(const asx_patch *pt, const asx_params *par, void *st
     return st;
   if ((st = need_range(pt, ASX_DENSRHS, ASX_DYE_RHS))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_sources(kpatch(pt), box(pt, 0, halo), rhs_args(par),
+  Poisoner pz;
+  pz.out_range(pt, ASX_DENSRHS, ASX_DYE_RHS, box(pt, 0, halo));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
+  st = dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_sources(kpatch(pt), box(pt, 0, halo), rhs_args(par),
                                    (hipStream_t)stream); }));
+  return pz.check((hipStream_t)stream, st);
 }
 
 asx_status asx_tmunu(const asx_patch *pt, const asx_params *par, void *stream) {
@@ -791,9 +909,13 @@ asx_status asx_estimate_error(const asx_patch *pt, const asx_params *par, int32_
   }
   if ((st = need(pt, {ASX_REGRID_ERROR}))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(timed(ASX_KC_REGRID, (hipStream_t)stream, [&] {
+  Poisoner pz;
+  pz.out(pt, ASX_REGRID_ERROR, box(pt, 0, halo));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
+  st = dev_status(timed(ASX_KC_REGRID, (hipStream_t)stream, [&] {
     return launch_regrid(kpatch(pt), box(pt, 0, halo), A, (hipStream_t)stream);
   }));
+  return pz.check((hipStream_t)stream, st);
 }
 
 asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {
@@ -814,7 +936,13 @@ asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {
     if (pt->ng < recon_ghosts(par->reconstruction_method)) return ASX_ERR_GHOSTS;
   }
   if (pt->ng < 2) return ASX_ERR_GHOSTS;
-  return dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_rhs(kpatch(pt), rhs_args(par), (hipStream_t)stream); }));
+  // the hydro right-hand sides are added to, not written: not poisoned
+  Poisoner pz;
+  const int h0[3] = {0, 0, 0};
+  for (int q = ASX_AVEC_X_RHS; q <= ASX_PSI_RHS; ++q) pz.out(pt, q, box(pt, centering(q), h0));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
+  st = dev_status(timed(ASX_KC_RHS, (hipStream_t)stream, [&] { return launch_rhs(kpatch(pt), rhs_args(par), (hipStream_t)stream); }));
+  return pz.check((hipStream_t)stream, st);
 }
 
 asx_status asx_bstag_from_A(const asx_patch *pt, void *stream) {
@@ -825,11 +953,14 @@ asx_status asx_bstag_from_A(const asx_patch *pt, void *stream) {
     return st;
   const KPatch P = kpatch(pt);
   const int halo[3] = {0, 0, 0};
+  Poisoner pz;
+  for (int d = 0; d < 3; ++d) pz.out(pt, ASX_DBX_STAG + d, box(pt, 1 << d, halo));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
   for (int d = 0; d < 3; ++d)
     if ((st = dev_status(timed(ASX_KC_BSTAG, (hipStream_t)stream, [&] { return launch_bstag(d, P, box(pt, 1 << d, halo), pt->gf[ASX_DBX_STAG + d].data,
                                       (hipStream_t)stream); }))))
       return st;
-  return ASX_OK;
+  return pz.check((hipStream_t)stream, ASX_OK);
 }
 
 asx_status asx_dB_from_bstag(const asx_patch *pt, void *stream) {
@@ -838,7 +969,11 @@ asx_status asx_dB_from_bstag(const asx_patch *pt, void *stream) {
   if ((st = need(pt, {ASX_DBX_STAG, ASX_DBY_STAG, ASX_DBZ_STAG, ASX_DBX, ASX_DBY, ASX_DBZ})))
     return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
+  Poisoner pz;
+  pz.out_range(pt, ASX_DBX, ASX_DBZ, box(pt, 0, halo));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
+  st = dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
+  return pz.check((hipStream_t)stream, st);
 }
 
 asx_status asx_B_from_dB(const asx_patch *pt, void *stream) {
@@ -847,7 +982,11 @@ asx_status asx_B_from_dB(const asx_patch *pt, void *stream) {
   if ((st = need_range(pt, ASX_GXX, ASX_GZZ))) return st;
   if ((st = need(pt, {ASX_DBX, ASX_DBY, ASX_DBZ, ASX_BVECX, ASX_BVECY, ASX_BVECZ}))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_B_from_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
+  Poisoner pz;
+  pz.out_range(pt, ASX_BVECX, ASX_BVECZ, box(pt, 0, halo));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
+  st = dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_B_from_dB(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
+  return pz.check((hipStream_t)stream, st);
 }
 
 asx_status asx_prim2con(const asx_patch *pt, void *stream) {
@@ -859,7 +998,11 @@ asx_status asx_prim2con(const asx_patch *pt, void *stream) {
     return st;
   if ((st = need_range(pt, ASX_DENS, ASX_DBZ))) return st;
   const int halo[3] = {0, 0, 0};
-  return dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_prim2con(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
+  Poisoner pz;
+  pz.out_range(pt, ASX_DENS, ASX_DBZ, box(pt, 0, halo));
+  if ((st = pz.fill((hipStream_t)stream))) return st;
+  st = dev_status(timed(ASX_KC_OTHER, (hipStream_t)stream, [&] { return launch_prim2con(kpatch(pt), box(pt, 0, halo), (hipStream_t)stream); }));
+  return pz.check((hipStream_t)stream, st);
 }
 
 static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool from_A,
@@ -1013,6 +1156,55 @@ asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms) {
   *launches = g_tn[kclass];
   *ms = g_tms[kclass];
   return st;
+}
+
+asx_status asx_set_poison(int32_t on) {
+  g_poison.store(on ? 1 : 0);
+  std::lock_guard<std::mutex> lk(g_pmu);
+  g_poison_slot = -1;
+  g_poison_points = 0;
+  return ASX_OK;
+}
+
+asx_status asx_poison_report(int32_t *slot, int64_t *points) {
+  if (!slot || !points) return ASX_ERR_NULL;
+  std::lock_guard<std::mutex> lk(g_pmu);
+  *slot = g_poison_slot;
+  *points = g_poison_points;
+  return ASX_OK;
+}
+
+asx_status asx_poison_check(const asx_patch *pt, int32_t slot, const int32_t lo[3],
+                            const int32_t cnt[3], int64_t *points, void *stream) {
+  asx_status st = check_geom(pt);
+  if (st) return st;
+  if (!lo || !cnt || !points) return ASX_ERR_NULL;
+  if (slot < 0 || slot >= ASX_NUM_GF || centering(slot) < 0) return ASX_ERR_PARAM;
+  if ((st = need(pt, {slot}))) return st;
+  const int c = centering(slot);
+  Box B;
+  for (int d = 0; d < 3; ++d) {
+    const int E = pt->n[d] + ((c >> d) & 1) + 2 * pt->ng;
+    if (lo[d] < 0 || cnt[d] < 0 || lo[d] + cnt[d] > E) return ASX_ERR_SIZE;
+    B.lo[d] = lo[d];
+    B.cnt[d] = cnt[d];
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ASX_ERR_DEVICE;
+  unsigned long long *cd = poison_buf(dev);
+  if (!cd) return ASX_ERR_DEVICE;
+  hipStream_t s = (hipStream_t)stream;
+  int64_t sj, sk;
+  strides(pt->n, pt->ng, c, sj, sk);
+  unsigned long long h = 0;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, cd, 1);
+  if (launch_poison_count(pt->gf[slot].data, (int)sj, (int)sk, B, ASX_POISON_BITS, cd, s) !=
+          hipSuccess ||
+      hipMemcpyAsync(&h, cd, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return ASX_ERR_DEVICE;
+  *points = (int64_t)h;
+  return ASX_OK;
 }
 
 asx_status asx_bc_fill_many(const asx_patch *pt, int32_t nslot, const int32_t *slots,
@@ -1287,6 +1479,24 @@ static asx_status substep_impl(const asx_patch *stage, const asx_state *u0,
   hipStream_t s = (hipStream_t)stream;
   Scratch *sc = scratch_for(stage, stream);
   if (!sc) return ASX_ERR_DEVICE;
+  Poisoner pz;
+  if (pz.on) {
+    // the out state (unless it shares an array with an input) and the face scratch
+    bool alias = false;
+    for (int q = 0; q < ASX_NUM_STATE; ++q) {
+      for (int r = 0; r < ASX_NUM_GF; ++r) alias |= out->gf[q].data == stage->gf[r].data;
+      if (c0 != 0.0)
+        for (int r = 0; r < ASX_NUM_STATE; ++r) alias |= out->gf[q].data == u0->gf[r].data;
+    }
+    const int h0[3] = {0, 0, 0};
+    if (!alias)
+      for (int q = 0; q < ASX_NUM_STATE; ++q)
+        pz.out_array(out->gf[q].data, stage, kStateSlots[q],
+                     box(stage, centering(kStateSlots[q]), h0));
+    for (int q = 0; q < ASX_NUM_GF; ++q)
+      if (sc->slot[q]) pz.scratch(sc->slot[q], npoints(stage->n, stage->ng, centering(q)));
+    if ((st = pz.fill(s))) return st;
+  }
   KPatch P = kpatch(stage);
   for (int q = 0; q < ASX_NUM_GF; ++q)
     if (sc->slot[q]) P.g[q] = sc->slot[q];
@@ -1352,7 +1562,8 @@ static asx_status substep_impl(const asx_patch *stage, const asx_state *u0,
     pu0.g[q] = (c0 != 0.0) ? u0->gf[q].data : nullptr;
     pout.g[q] = out->gf[q].data;
   }
-  return dev_status(timed(ASX_KC_UPDATE, s, [&] { return launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, variant_of(ASX_KC_UPDATE), s); }));
+  st = dev_status(timed(ASX_KC_UPDATE, s, [&] { return launch_update(P, rhs_args(par), pu0, pout, c0, c1, c2, variant_of(ASX_KC_UPDATE), s); }));
+  return pz.check(s, st);
 }
 
 asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_state *out,

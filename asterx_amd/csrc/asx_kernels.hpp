@@ -257,6 +257,12 @@ struct RestrictArgs {
 hipError_t launch_prolong(const ProlongArgs &A, int64_t max_blocks, hipStream_t s);
 hipError_t launch_restrict(const RestrictArgs &A, int64_t max_points, hipStream_t s);
 
+// Debug poison: fill n doubles with a bit pattern; add to *cnt the points of
+// box B (strides sj, sk) holding it
+hipError_t launch_poison_fill(double *p, int64_t n, uint64_t bits, hipStream_t s);
+hipError_t launch_poison_count(const double *p, int sj, int sk, const Box &B, uint64_t bits,
+                               unsigned long long *cnt, hipStream_t s);
+
 // Regrid tags on blocks of blk[d] interior cells (cell layout strides sj, sk)
 struct TagArgs {
   const double *err;

@@ -39,7 +39,9 @@ typedef enum {
   ASX_ERR_UNSUPPORTED = 4, /* valid in the reference, not built here        */
   ASX_ERR_GHOSTS = 5,      /* too few ghost zones for the stencil           */
   ASX_ERR_DEVICE = 6,      /* HIP runtime error                             */
-  ASX_ERR_SIZE = 7         /* size out of range                             */
+  ASX_ERR_SIZE = 7,        /* size out of range                             */
+  ASX_ERR_POISON = 8       /* debug poison: a declared output point was not
+                              written (asx_set_poison)                      */
 } asx_status;
 
 /* ------------------------------------------------------------------------ */
@@ -320,6 +322,28 @@ enum {
 };
 asx_status asx_timing_enable(int32_t on);
 asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
+
+/* Debug poison, the device form of CarpetX::poison_undefined_values = yes
+ * (FishboneMoncriefIDX/par/FMdisk_unigrid.par:26).  While enabled, the
+ * entry points named below first fill every grid function they only write
+ * (not those they also read) with ASX_POISON_BITS on every point, ghosts
+ * included, then run, then count the points of the region they declare
+ * written that still hold the pattern; any such point makes the call return
+ * ASX_ERR_POISON, and asx_poison_report gives the first such slot and its
+ * count.  Points outside the declared region keep the pattern, so a later
+ * entry point that reads them computes NaN.  Poisoned: asx_fluxes (fluxes,
+ * UCT face data, aux), asx_source_terms (hydro RHS), asx_rhs (A and Psi
+ * RHS), asx_bstag_from_A, asx_dB_from_bstag, asx_B_from_dB, asx_prim2con,
+ * asx_estimate_error, asx_substep and asx_substep_bc (the out state, when it
+ * shares no array with the inputs; the library's face scratch is poisoned
+ * too).  Each checked call waits for its stream.  asx_poison_check counts
+ * the pattern in any box of one slot (lo, cnt in point indices of that
+ * slot's layout). */
+#define ASX_POISON_BITS 0x7ff4dead0badf00dULL /* a signalling NaN */
+asx_status asx_set_poison(int32_t on);
+asx_status asx_poison_report(int32_t *slot, int64_t *points);
+asx_status asx_poison_check(const asx_patch *patch, int32_t slot, const int32_t lo[3],
+                            const int32_t cnt[3], int64_t *points, void *stream);
 
 /* Launch-shape variants and autotuning (no counterpart in the reference).
  * The face kernels (ASX_KC_FACE_*) and the fused update (ASX_KC_UPDATE) are
