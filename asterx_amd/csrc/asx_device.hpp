@@ -222,8 +222,9 @@ ASX_DEV void ppm_cell(double gImm, double gIm, double gI, double gIp,
 // WENO-Z face values of the centre cell (ReconX/src/wenoz.hxx:23-99).
 // Same smoothness indicators, weights and stencils; the nonlinear weights
 // a_k = 1 + tau5/(b_k + eps) are carried over the common denominator
-// prod_k (b_k + eps), so each face value costs one division instead of the
-// reference's seven (identical up to rounding).  The (b_k + eps) are first
+// prod_k (b_k + eps), and the two face values share one reciprocal, so a
+// cell costs one division instead of the reference's fourteen (identical up
+// to rounding).  The (b_k + eps) are first
 // rescaled by a power of two (exact) so the triple products cannot overflow.
 ASX_DEV void wenoz_cell(double gImm, double gIm, double gI, double gIp,
                         double gIpp, double weno_eps, double &m, double &p) {
@@ -247,14 +248,19 @@ ASX_DEV void wenoz_cell(double gImm, double gIm, double gI, double gIp,
   const double w0 = 3.0 / 10.0, w1 = 3.0 / 5.0, w2 = 1.0 / 10.0;
   const double am0 = w0 * n0, am1 = w1 * n1, am2 = w2 * n2;
   const double ap0 = w2 * n0, ap1 = w1 * n1, ap2 = w0 * n2;
-  m = (am2 * (2.0 * gIpp - 7.0 * gIp + 11.0 * gI) +
-       am1 * (-1.0 * gIp + 5.0 * gI + 2.0 * gIm) +
-       am0 * (2.0 * gI + 5.0 * gIm - 1.0 * gImm)) /
-      (6.0 * (am0 + am1 + am2));
-  p = (ap0 * (2.0 * gImm - 7.0 * gIm + 11.0 * gI) +
-       ap1 * (-1.0 * gIm + 5.0 * gI + 2.0 * gIp) +
-       ap2 * (2.0 * gI + 5.0 * gIp - 1.0 * gIpp)) /
-      (6.0 * (ap0 + ap1 + ap2));
+  const double nm = am2 * (2.0 * gIpp - 7.0 * gIp + 11.0 * gI) +
+                    am1 * (-1.0 * gIp + 5.0 * gI + 2.0 * gIm) +
+                    am0 * (2.0 * gI + 5.0 * gIm - 1.0 * gImm);
+  const double np = ap0 * (2.0 * gImm - 7.0 * gIm + 11.0 * gI) +
+                    ap1 * (-1.0 * gIm + 5.0 * gI + 2.0 * gIp) +
+                    ap2 * (2.0 * gI + 5.0 * gIp - 1.0 * gIpp);
+  const double dm = 6.0 * (am0 + am1 + am2), dp = 6.0 * (ap0 + ap1 + ap2);
+  // one reciprocal for both sides: after the rescaling the weights' sums
+  // lie within ~1e-130 .. 1e2, so their product neither overflows nor
+  // underflows
+  const double r = 1.0 / (dm * dp);
+  m = nm * (dp * r);
+  p = np * (dm * r);
 }
 
 // MP5 value on the gIp side of cell gI (ReconX/src/mp5.hxx:26-77)
@@ -338,22 +344,32 @@ ASX_DEV void recon_lo(int m, const double s[6], double &o0, double &o1) {
 // ---------------------------------------------------------------------------
 // Riemann fluxes and characteristic speeds (AsterX)
 // ---------------------------------------------------------------------------
-// AsterX/src/eigenvalues.hxx:19-79 for one side: returns (lambda+, lambda-)
-ASX_DEV void eig_side(double alp, double ialp, double beta, double u, double vel,
-                      double rho, double cs2, double w, double h, double bsq,
-                      double &lp, double &lm) {
+// Characteristic speeds (lambda+, lambda-) of the left (0) and right (1)
+// states (AsterX/src/eigenvalues.hxx:19-79, the roots of a2 l^2 + a1 l + a0
+// over alpha), one reciprocal serving both 1/(2 a2 alpha) (a2 > 0)
+ASX_DEV void eig_sides(double alp, double ialp, double beta, double u, const double vel[2],
+                       const double rho[2], const double cs2[2], const double w[2],
+                       const double h[2], const double bsq[2], double &l0p, double &l0m,
+                       double &l1p, double &l1m) {
   ASX_CONTRACT
-  const double a0 = (bsq + cs2 * h * rho) * (pow2(beta) - pow2(alp) * u) -
-                    (-1 + cs2) * h * rho * pow2(beta - alp * vel) * pow2(w);
-  const double a1 = 2 * beta * (bsq + cs2 * h * rho) -
-                    2 * (-1 + cs2) * h * rho * (beta - alp * vel) * pow2(w);
-  const double a2 = bsq + h * rho * (cs2 + pow2(w) - cs2 * pow2(w));
-  double det = pow2(a1) - 4 * a2 * a0;
-  if (det < 0) det = 0;
-  const double sd = sqrt(det);
-  const double f = ialp / (2 * a2); // 1 / (2 a2 alpha)
-  lp = (-a1 + sd) * f;
-  lm = (-a1 - sd) * f;
+  double a1[2], a2[2], sd[2];
+#pragma unroll
+  for (int f = 0; f < 2; ++f) {
+    const double a0 = (bsq[f] + cs2[f] * h[f] * rho[f]) * (pow2(beta) - pow2(alp) * u) -
+                      (-1 + cs2[f]) * h[f] * rho[f] * pow2(beta - alp * vel[f]) * pow2(w[f]);
+    a1[f] = 2 * beta * (bsq[f] + cs2[f] * h[f] * rho[f]) -
+            2 * (-1 + cs2[f]) * h[f] * rho[f] * (beta - alp * vel[f]) * pow2(w[f]);
+    a2[f] = bsq[f] + h[f] * rho[f] * (cs2[f] + pow2(w[f]) - cs2[f] * pow2(w[f]));
+    double det = pow2(a1[f]) - 4 * a2[f] * a0;
+    if (det < 0) det = 0;
+    sd[f] = sqrt(det);
+  }
+  const double r = ialp / (4 * a2[0] * a2[1]); // 1 / (2 a2 alpha) = 2 a2' r
+  const double f0 = 2 * a2[1] * r, f1 = 2 * a2[0] * r;
+  l0p = (-a1[0] + sd[0]) * f0;
+  l0m = (-a1[0] - sd[0]) * f0;
+  l1p = (-a1[1] + sd[1]) * f1;
+  l1m = (-a1[1] - sd[1]) * f1;
 }
 // max/min over {0, l0+, l0+, l0-, l0-, l1+, l1+, l1-, l1-} (fluxes.hxx:35-42)
 ASX_DEV void char_extrema(double l0p, double l0m, double l1p, double l1m,

@@ -190,8 +190,10 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
   const Sym3 g = G.g;
   const double detg = det3(g);
   const double sqrtg = sqrt(detg);
-  const double isqrtg = 1.0 / sqrtg;
-  const double ialp = 1.0 / alp;
+  // one reciprocal for 1/sqrt(gamma) and 1/alpha (both O(1) on any slice)
+  const double ras = 1.0 / (alp * sqrtg);
+  const double isqrtg = alp * ras;
+  const double ialp = sqrtg * ras;
 
   const double rho[2] = {L.rho, R.rho}, ent[2] = {L.ent, R.ent}, ye[2] = {L.ye, R.ye};
   const double th[2] = {L.th, R.th};
@@ -258,15 +260,17 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
     lower2(vel, vlow);
     iW[0] = sqrt(1 - dot2(vlow, vel, 0));
     iW[1] = sqrt(1 - dot2(vlow, vel, 1));
-    W[0] = 1 / iW[0];
-    W[1] = 1 / iW[1];
+    const double rw = 1 / (iW[0] * iW[1]); // 1/W lies in (0, 1]
+    W[0] = iW[1] * rw;
+    W[1] = iW[0] * rw;
   } else if (A.rec_var == ASX_RECVAR_Z) {
     double zl[3][2];
     lower2(rv, zl);
     W[0] = sqrt(1 + dot2(zl, rv, 0));
     W[1] = sqrt(1 + dot2(zl, rv, 1));
-    iW[0] = 1 / W[0];
-    iW[1] = 1 / W[1];
+    const double rw = 1 / (W[0] * W[1]);
+    iW[0] = W[1] * rw;
+    iW[1] = W[0] * rw;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -339,14 +343,14 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
     }
   const double alp_sqrtg = alp * sqrtg;
 
-  // characteristic speeds (u = (gamma^-1)^{DD})
+  // characteristic speeds (u = (gamma^-1)^{DD}, the cofactor over det
+  // gamma = sqrt(gamma)^2, by the reciprocal already at hand)
   const double u = (D == 0   ? g.yy * g.zz - pow2(g.yz)
                     : D == 1 ? g.xx * g.zz - pow2(g.xz)
-                             : g.xx * g.yy - pow2(g.xy)) /
-                   detg; // (gamma^-1)^{DD}
+                             : g.xx * g.yy - pow2(g.xy)) *
+                   (isqrtg * isqrtg);
   double l0p, l0m, l1p, l1m;
-  eig_side(alp, ialp, betas[D], u, vel[D][0], rho[0], cs2[0], W[0], h[0], bsq[0], l0p, l0m);
-  eig_side(alp, ialp, betas[D], u, vel[D][1], rho[1], cs2[1], W[1], h[1], bsq[1], l1p, l1m);
+  eig_sides(alp, ialp, betas[D], u, vel[D], rho, cs2, W, h, bsq, l0p, l0m, l1p, l1m);
   double cmax, cmin;
   char_extrema(l0p, l0m, l1p, l1m, cmax, cmin);
   const double cabs = smax(smax(smax(smax(0.0, fabs(l0p)), fabs(l0m)), fabs(l1p)),
@@ -410,7 +414,7 @@ ASX_DEV void face_flux(const KPatch &P, const FluxArgs &A, int i, int j, int k,
     // components in increasing order (AsterX/interface.ccl:137-150)
     constexpr int s1 = ASX_VTILDE_Y_XFACE + 2 * D + (D1 < D2 ? 0 : 1);
     constexpr int s2 = ASX_VTILDE_Y_XFACE + 2 * D + (D1 < D2 ? 1 : 0);
-    const double ia = 1.0 / (amx + amn);
+    const double ia = idc; // amx + amn is cmax - cmin
     P.g[s1][ofc] = (amx * vt[D1][0] + amn * vt[D1][1]) * ia;
     P.g[s2][ofc] = (amx * vt[D2][0] + amn * vt[D2][1]) * ia;
   }
