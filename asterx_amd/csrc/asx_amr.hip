@@ -154,7 +154,10 @@ ASX_DEV void prolong_block(const ProlongArgs &A, int s, int64_t t) {
 template <int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_prolong(ProlongArgs A) {
   const int s = (int)blockIdx.y;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // contiguous runs of blocks per XCD: neighbouring blocks share most of
+  // their coarse rows, which then meet in one L2
+  const int64_t t =
+      (int64_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
   switch (A.cent[s]) {
   case 0: prolong_block<P, 0>(A, s, t); break;
   case 1: prolong_block<P, 1>(A, s, t); break;
@@ -172,7 +175,8 @@ __global__ __launch_bounds__(256) void k_restrict(RestrictArgs A) {
   const int s = (int)blockIdx.y;
   const int c = A.cent[s];
   const Box &B = A.box[s];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t =
+      (int64_t)xcd_tile((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= B.total()) return;
   int ic[3];
   B.ijk(t, ic[0], ic[1], ic[2]);
