@@ -336,9 +336,10 @@ asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms);
  * RHS), asx_bstag_from_A, asx_dB_from_bstag, asx_B_from_dB, asx_prim2con,
  * asx_estimate_error, asx_substep and asx_substep_bc (the out state, when it
  * shares no array with the inputs; the library's face scratch is poisoned
- * too).  Each checked call waits for its stream.  asx_poison_check counts
- * the pattern in any box of one slot (lo, cnt in point indices of that
- * slot's layout). */
+ * too).  Each checked call waits for its stream; the count buffer is per
+ * device, so checked calls must not overlap on one device (as for
+ * asx_counters).  asx_poison_check counts the pattern in any box of one
+ * slot (lo, cnt in point indices of that slot's layout). */
 #define ASX_POISON_BITS 0x7ff4dead0badf00dULL /* a signalling NaN */
 asx_status asx_set_poison(int32_t on);
 asx_status asx_poison_report(int32_t *slot, int64_t *points);
