@@ -1385,14 +1385,16 @@ asx_status asx_regrid_boxes(const asx_patch *pt, double threshold, int32_t n_err
   A.thr = threshold;
   const int64_t nb = (int64_t)A.nb[0] * A.nb[1] * A.nb[2];
   hipStream_t s = (hipStream_t)stream;
+  // the block tags live on the caller's stream only (stream-ordered
+  // allocation: no device-wide synchronisation)
   uint8_t *dtags = nullptr;
-  if (hipMalloc(&dtags, nb) != hipSuccess) return ASX_ERR_DEVICE;
+  if (hipMallocAsync((void **)&dtags, nb, s) != hipSuccess) return ASX_ERR_DEVICE;
   A.tags = dtags;
   std::vector<uint8_t> tags(nb);
   hipError_t e = timed(ASX_KC_AMR, s, [&] { return launch_regrid_tags(A, s); });
   if (e == hipSuccess) e = hipMemcpyAsync(tags.data(), dtags, nb, hipMemcpyDeviceToHost, s);
+  const hipError_t ef = hipFreeAsync(dtags, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  const hipError_t ef = hipFree(dtags);
   if (e != hipSuccess || ef != hipSuccess) return ASX_ERR_DEVICE;
   // clusters in blocks, each cut into pieces of at most per[d] blocks
   // (equal to within one block, the larger ones first), in fine cells

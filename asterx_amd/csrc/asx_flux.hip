@@ -28,14 +28,6 @@ void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
   }
 }
 
-__global__ __launch_bounds__(256) void k_aux(KPatch P, Box B) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= B.total()) return;
-  int i, j, k;
-  B.ijk(t, i, j, k);
-  aux_point(P, i, j, k);
-}
-
 template <int D>
 __global__ __launch_bounds__(256) void k_bstag(KPatch P, Box B, double *out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -163,12 +155,6 @@ hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, in
   if (d == 0) return launch_flux_d<0>(P, A, method, variant, s, tab);
   if (d == 1) return launch_flux_d<1>(P, A, method, variant, s, tab);
   return launch_flux_d<2>(P, A, method, variant, s, tab);
-}
-
-hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s) {
-  if (B.htotal() <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_aux, B.grid(), dim3(256), 0, s, P, B);
-  return hipGetLastError();
 }
 
 hipError_t launch_bstag(int d, const KPatch &P, const Box &B, double *out,

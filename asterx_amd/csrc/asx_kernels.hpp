@@ -131,7 +131,6 @@ struct StatePtrs {
 hipError_t launch_flux(int d, const KPatch &P, const FluxArgs &A, int method, int variant,
                        hipStream_t s, bool tab = false);
 bool flux_tab_method(int method);
-hipError_t launch_aux(const KPatch &P, const Box &B, hipStream_t s);
 hipError_t launch_bstag(int d, const KPatch &P, const Box &B, double *out,
                         hipStream_t s);
 hipError_t launch_dB(const KPatch &P, const Box &B, hipStream_t s);

@@ -305,7 +305,8 @@ enum {
   ASX_KC_FACE_X = 1, /* face reconstruction + numerical flux, x faces */
   ASX_KC_FACE_Y = 2,
   ASX_KC_FACE_Z = 3,
-  ASX_KC_AUX = 4,    /* vertex auxiliaries of the A/Psi gauge */
+  ASX_KC_AUX = 4,    /* vertex auxiliaries of the A/Psi gauge (batch plans;
+                        on one patch they ride on the z-face launch) */
   ASX_KC_UPDATE = 5, /* fused RHS + RK combination */
   ASX_KC_C2P = 6,    /* con2prim: every zone, primary solver (patch or batch) */
   ASX_KC_BC = 7,     /* boundary fills */
