@@ -173,6 +173,43 @@ def test_poisoned_substep_matches():
         assert (_bits(b)[outside] == abi.POISON_BITS).all(), name
 
 
+@pytest.mark.parametrize("uct", [0, 1])
+def test_poisoned_substep_bc_matches(uct):
+    """asx_substep_bc fills the face data's ghost points by the boundary
+    kind and (with upwind CT) reconstructs face data three rows into them:
+    with the face scratch poisoned, a ghost point the fill missed would
+    reach the result as NaN."""
+    torch = _torch()
+    from asterx_amd import lib
+    from asterx_amd.patch import StateBuf
+    L = lib.lib()
+    h = make_state(curved=True)
+    par = abi.params(reconstruction_method="wenoz", flux_type="HLLE", gl_gamma=5.0 / 3.0,
+                     use_uct=uct, lorenz_damp_fac=0.2)
+    res = []
+    for poison in (0, 1):
+        d = h.to("cuda")
+        u0 = StateBuf(d, fill=0.0)
+        for name in abi.STATE:
+            u0.arr[name] = d[name].clone() * 1.01
+        out = StateBuf(d, fill=0.0)
+        abi.check("asx_set_poison", L.asx_set_poison(poison))
+        try:
+            st = L.asx_substep_bc(C.byref(d.struct()), C.byref(u0.struct()),
+                                  C.byref(out.struct()), 0.75, 0.25, 0.0025, C.byref(par),
+                                  abi.BC["copy"], None)
+        finally:
+            L.asx_set_poison(0)
+        torch.cuda.synchronize()
+        abi.check("asx_substep_bc", st)
+        res.append(out)
+    for name in abi.STATE:
+        sl = h.interior(name)
+        a, b = res[0].arr[name].cpu().numpy(), res[1].arr[name].cpu().numpy()
+        assert np.array_equal(_bits(a[sl]), _bits(b[sl])), name
+        assert not np.isnan(b[sl]).any(), name
+
+
 def test_poison_check_and_report():
     torch = _torch()
     from asterx_amd import lib
