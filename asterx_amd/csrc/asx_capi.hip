@@ -2081,10 +2081,10 @@ asx_status plan_launch(asx_batch *b, int k, int v, BatchLaunch **out) {
       }
       update_tile_extent(v, tile);
     } else if (k >= BK_BC && k < BK_BC + 3) {
-      const int d = k - BK_BC; // ghost points normal to d, widest centering
+      const int d = k - BK_BC; // bc_point's pass normal to d, widest centering
       for (int q = 0; q < 3; ++q) {
         bb.lo[q] = 0;
-        bb.cnt[q] = pt.n[q] + 2 * pt.ng + 1;
+        bb.cnt[q] = q > d ? pt.n[q] + 1 : pt.n[q] + 2 * pt.ng + 1;
       }
       bb.cnt[d] = 2 * pt.ng;
     } else if (k >= BK_UBSTAG && k < BK_UBSTAG + 3) {

@@ -397,10 +397,21 @@ ASX_DEV void bc_point(double *g, int c, int n0, int n1, int n2, int ng, int kind
   const int n[3] = {n0, n1, n2};
   const int e[3] = {n0 + 2 * ng + (c & 1), n1 + 2 * ng + ((c >> 1) & 1),
                     n2 + 2 * ng + ((c >> 2) & 1)};
-  int w[3] = {e[0], e[1], e[2]};
+  // The axes run in the order x, y, z, and each pass reads only interior
+  // rows / planes along its own axis, so the x pass covers the interior rows
+  // in y and z and the y pass the interior planes in z: the ghost rows and
+  // planes of a later axis are written by that axis's pass.  (Covering them
+  // too, as a single pass per axis over the full extent would, only writes
+  // values the later pass overwrites.)
+  int w[3] = {e[0], e[1], e[2]}, o[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = D + 1; a < 3; ++a) {
+    w[a] = e[a] - 2 * ng;
+    o[a] = ng;
+  }
   w[D] = 2 * ng;
   if (t >= w[0] * w[1] * w[2]) return;
-  int x[3] = {t % w[0], (t / w[0]) % w[1], t / (w[0] * w[1])};
+  int x[3] = {t % w[0] + o[0], (t / w[0]) % w[1] + o[1], t / (w[0] * w[1]) + o[2]};
   const int lo = ng, hi = ng + n[D] - 1 + ((c >> D) & 1);
   const int q = x[D] < ng ? x[D] : hi + 1 + (x[D] - ng);
   const int64_t ex = e[0], exy = (int64_t)e[0] * e[1];
@@ -425,9 +436,11 @@ ASX_DEV void bc_point(double *g, int c, int n0, int n1, int n2, int ng, int kind
   }
   l[q * sd] = v;
 }
-// ghost points normal to d of the widest centering (vertex on every axis)
+// points of bc_point's pass normal to d for the widest centering (vertex on
+// every axis): the full extent on the axes before d, the interior after it
 __host__ inline int bc_points(const int n[3], int ng, int d) {
   int w[3] = {n[0] + 2 * ng + 1, n[1] + 2 * ng + 1, n[2] + 2 * ng + 1};
+  for (int a = d + 1; a < 3; ++a) w[a] = n[a] + 1;
   w[d] = 2 * ng;
   return w[0] * w[1] * w[2];
 }
