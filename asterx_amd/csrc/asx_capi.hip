@@ -10,11 +10,23 @@
 #include <tuple>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "asx_kernels.hpp"
 
 using namespace asx;
 
 namespace {
+
+// A roctx range over one entry point's host call: `rocprofv3 --marker-trace`
+// shows it beside the kernels the call launched (a batch plan's graph
+// replays and persistent stages get one each).
+struct Range {
+  explicit Range(const char *name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range &) = delete;
+  Range &operator=(const Range &) = delete;
+};
 
 constexpr int kStateSlots[ASX_NUM_STATE] = {
     ASX_DENS, ASX_MOMX, ASX_MOMY, ASX_MOMZ, ASX_TAU, ASX_DENT,
@@ -736,6 +748,7 @@ void asx_params_default(asx_params *p) {
 
 asx_status asx_fluxes(const asx_patch *pt, const asx_params *par, int32_t face_halo,
                       asx_counters *cnt, void *stream) {
+  const Range range("asx_fluxes");
   asx_status st = check_geom(pt);
   if (st) return st;
   if ((st = check_par(par))) return st;
@@ -919,6 +932,7 @@ asx_status asx_estimate_error(const asx_patch *pt, const asx_params *par, int32_
 }
 
 asx_status asx_rhs(const asx_patch *pt, const asx_params *par, void *stream) {
+  const Range range("asx_rhs");
   asx_status st = check_geom(pt);
   if (st) return st;
   if ((st = check_par(par))) return st;
@@ -1080,11 +1094,13 @@ static asx_status c2p_common(const asx_patch *pt, const asx_params *par, bool fr
 
 asx_status asx_con2prim(const asx_patch *pt, const asx_params *par, asx_counters *cnt,
                         void *stream) {
+  const Range range("asx_con2prim");
   return c2p_common(pt, par, false, cnt, stream);
 }
 
 asx_status asx_con2prim_group(const asx_patch *pt, const asx_params *par,
                               asx_counters *cnt, void *stream) {
+  const Range range("asx_con2prim_group");
   return c2p_common(pt, par, true, cnt, stream);
 }
 
@@ -1273,6 +1289,7 @@ extern "C" {
 
 asx_status asx_prolongate(const asx_patch *coarse, const asx_patch *fine, int32_t nslot,
                           const int32_t *slots, int32_t order, int32_t region, void *stream) {
+  const Range range("asx_prolongate");
   asx_status st = check_geom(coarse);
   if (st || (st = check_geom(fine))) return st;
   if (region < 0 || region > 1) return ASX_ERR_PARAM;
@@ -1316,6 +1333,7 @@ asx_status asx_prolongate(const asx_patch *coarse, const asx_patch *fine, int32_
 
 asx_status asx_restrict(const asx_patch *fine, const asx_patch *coarse, int32_t nslot,
                         const int32_t *slots, void *stream) {
+  const Range range("asx_restrict");
   asx_status st = check_geom(coarse);
   if (st || (st = check_geom(fine))) return st;
   if ((st = amr_slots(coarse, fine, nslot, slots))) return st;
@@ -1355,6 +1373,7 @@ asx_status asx_regrid_boxes(const asx_patch *pt, double threshold, int32_t n_err
                             const int32_t blocking_factor[3], const int32_t max_grid_size[3],
                             double grid_eff, int32_t max_boxes, asx_box *boxes,
                             int32_t *nboxes, void *stream) {
+  const Range range("asx_regrid_boxes");
   asx_status st = check_geom(pt);
   if (st) return st;
   if (!blocking_factor || !max_grid_size || !nboxes || (max_boxes > 0 && !boxes))
@@ -1571,6 +1590,7 @@ static asx_status substep_impl(const asx_patch *stage, const asx_state *u0,
 asx_status asx_substep(const asx_patch *stage, const asx_state *u0, const asx_state *out,
                        double c0, double c1, double c2, const asx_params *par,
                        void *stream) {
+  const Range range("asx_substep");
   return substep_impl(stage, u0, out, c0, c1, c2, par, -1, stream);
 }
 
@@ -1578,6 +1598,7 @@ asx_status asx_substep_bc(const asx_patch *stage, const asx_state *u0, const asx
                           double c0, double c1, double c2, const asx_params *par,
                           int32_t bc_kind, void *stream) {
   if (bc_kind < 0) return ASX_ERR_PARAM;
+  const Range range("asx_substep_bc");
   return substep_impl(stage, u0, out, c0, c1, c2, par, bc_kind, stream);
 }
 
@@ -2496,13 +2517,17 @@ asx_status asx_batch_step(asx_batch *plan, int32_t nsteps, int32_t use_graph, vo
     for (int n = 0; n < nsteps && !st; ++n)
       for (int k = 0; k < 3 && !st; ++k)
         st = dev_status(timed(ASX_KC_PERSIST, s, [&] {
+          const Range range("asx_batch_step persistent stage");
           return launch_stage_persist(plan->pargs + k, m, o, plan->pgrid, s);
         }));
     return st;
   }
   if ((st = plan_capture(plan))) return st;
   for (int n = 0; n < nsteps && !st; ++n)
-    st = dev_status(timed(ASX_KC_GRAPH, s, [&] { return hipGraphLaunch(plan->exec, s); }));
+    st = dev_status(timed(ASX_KC_GRAPH, s, [&] {
+      const Range range("asx_batch_step graph replay");
+      return hipGraphLaunch(plan->exec, s);
+    }));
   return st;
 }
 
