@@ -1174,6 +1174,29 @@ asx_status asx_timing_read(int32_t kclass, int64_t *launches, double *ms) {
   return st;
 }
 
+asx_status asx_release_scratch(void *stream) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return ASX_ERR_DEVICE;
+  if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) return ASX_ERR_DEVICE;
+  std::lock_guard<std::mutex> lk(g_mu);
+  bool ok = true;
+  for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+    if (std::get<0>(it->first) == dev && std::get<1>(it->first) == stream) {
+      for (double *b : it->second.bufs) ok &= hipFree(b) == hipSuccess;
+      it = g_scratch.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  auto q = g_c2pq.find({dev, stream});
+  if (q != g_c2pq.end()) {
+    if (q->second.q) ok &= hipFree(q->second.q) == hipSuccess;
+    if (q->second.n) ok &= hipFree(q->second.n) == hipSuccess;
+    g_c2pq.erase(q);
+  }
+  return ok ? ASX_OK : ASX_ERR_DEVICE;
+}
+
 asx_status asx_set_poison(int32_t on) {
   g_poison.store(on ? 1 : 0);
   std::lock_guard<std::mutex> lk(g_pmu);

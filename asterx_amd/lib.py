@@ -35,6 +35,7 @@ SIGNATURES = {
                            C.POINTER(_S), _V], C.c_int),
     "asx_bc_fill": ([_P, C.c_int32, C.c_int32, _V], C.c_int),
     "asx_bc_fill_many": ([_P, C.c_int32, C.POINTER(C.c_int32), C.c_int32, _V], C.c_int),
+    "asx_release_scratch": ([_V], C.c_int),
     "asx_set_poison": ([C.c_int32], C.c_int),
     "asx_poison_report": ([C.POINTER(C.c_int32), C.POINTER(C.c_int64)], C.c_int),
     "asx_poison_check": ([_P, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32),

@@ -552,6 +552,14 @@ asx_status asx_substep_bc(const asx_patch *stage, const asx_state *u0,
 asx_status asx_con2prim_group(const asx_patch *patch, const asx_params *par,
                               asx_counters *cnt, void *stream);
 
+/* Frees the device buffers the library keeps for `stream` on the current
+ * device -- the face scratch of asx_substep / asx_substep_bc for every patch
+ * shape used on that stream and the con2prim queue -- after waiting for the
+ * stream (no counterpart in the reference).  For drivers whose patch shapes
+ * change (after a regrid) or that retire a stream; the next call on the
+ * stream allocates again.  Must not overlap with calls on that stream. */
+asx_status asx_release_scratch(void *stream);
+
 /* asx_con2prim_group that allocates nothing: no counterpart in the
  * reference.  For callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  The same

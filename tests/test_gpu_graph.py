@@ -106,3 +106,29 @@ def test_graph_replay_matches_eager_substeps():
         for n in a:
             assert np.array_equal(a[n].view(np.uint64), b[n].view(np.uint64)), (rep, n)
             assert np.isfinite(a[n][h.interior(n)]).all(), (rep, n)
+
+
+def test_release_scratch_then_substeps_again():
+    """asx_release_scratch frees the stream's face scratch and con2prim
+    queue; the next calls allocate them again and give the same bits."""
+    torch = _torch()
+    from asterx_amd import lib
+    h = make_state(curved=True, uct=False)
+    par = abi.params(reconstruction_method="wenoz", flux_type="HLLE", gl_gamma=5.0 / 3.0)
+    dt = 0.2 * min(h.dx)
+    stream = torch.cuda.Stream()
+    snaps = []
+    for release in (False, True):
+        with torch.cuda.stream(stream):
+            r = Run(h.to("cuda"), par, dt, stream)
+            for q in range(3):
+                r.substep()
+                if release:
+                    abi.check("asx_release_scratch",
+                              lib.lib().asx_release_scratch(r.sh))
+        torch.cuda.synchronize()
+        snaps.append(r.snapshot())
+    for n in snaps[0]:
+        assert np.array_equal(snaps[0][n].view(np.uint64), snaps[1][n].view(np.uint64)), n
+    # a stream the library holds nothing for: nothing to free
+    abi.check("asx_release_scratch", lib.lib().asx_release_scratch(None))
