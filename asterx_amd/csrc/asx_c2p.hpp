@@ -491,6 +491,9 @@ template <class E> struct PalF {
   const Atmo &atmo;
   const C2PArgs &a;
   double q, r, s, t, dens, ye;
+  // the table's temperature cell of the previous evaluation (successive
+  // Brent iterates of one zone move T little; the solve is exact either way)
+  mutable int thint = -1;
   ASX_DEV double operator()(double x) const {
     double Wm2 = 1.0 - (x * x * r + (2.0 * x + s) * t * t) / (x * x * (x + s) * (x + s));
     Wm2 = fmin(fmax(Wm2, 1e-10), 1);
@@ -502,7 +505,7 @@ template <class E> struct PalF {
       eps = smax(eps, atmo.eps);
       eps = smin(eps, eos.eps_max);
     }
-    const double P = eos.press_re(rho, eps, ye);
+    const double P = eos.press_re(rho, eps, ye, &thint);
     return (x - (1.0 + eps + P / rho) * W);
   }
 };
@@ -668,7 +671,7 @@ ASX_DEV void palenzuela_solve(const E &eos, const Atmo &atmo, const C2PArgs &a,
     }
     vel_from_Z(x * pv.rho * W, W, Bsq, BiSi, cv, gup, glo, a.use_z, pv);
     pv.Ye = cv.DYe / cv.dens;
-    eos.state_re(pv.rho, pv.eps, pv.Ye, pv.press, pv.temperature, pv.entropy);
+    eos.state_re(pv.rho, pv.eps, pv.Ye, pv.press, pv.temperature, pv.entropy, &fn.thint);
     pv.B[0] = cv.dB[0]; pv.B[1] = cv.dB[1]; pv.B[2] = cv.dB[2];
   }
   if (pv.rho <= 0.0) {

@@ -413,7 +413,9 @@ struct IdealGas {
   }
   // The 3-parameter interface shared with TabEOS (EOSX eos_3p.hxx): Ye is
   // not used by the ideal gas, and eps is never changed.
-  ASX_DEV double press_re(double rho, double &eps, double) const { return press_re(rho, eps); }
+  ASX_DEV double press_re(double rho, double &eps, double, int * = nullptr) const {
+    return press_re(rho, eps);
+  }
   ASX_DEV double temp_re(double, double &eps, double) const { return temp_re(eps); }
   ASX_DEV double kappa_re(double rho, double &eps, double) const { return kappa(rho, eps); }
   ASX_DEV double csnd_re(double, double &eps, double) const { return csnd(eps); }
@@ -425,7 +427,7 @@ struct IdealGas {
   }
   // press_re, temp_re and kappa_re at one (rho, eps, Ye)
   ASX_DEV void state_re(double rho, double &eps, double ye, double &press, double &temp,
-                        double &kap) const {
+                        double &kap, int * = nullptr) const {
     press = press_re(rho, eps, ye);
     temp = temp_re(rho, eps, ye);
     kap = kappa_re(rho, eps, ye);

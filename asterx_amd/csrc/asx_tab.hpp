@@ -90,23 +90,53 @@ struct TabEOS {
   ASX_DEV double xmin1() const { return ax[1][0]; }
   ASX_DEV double xmax1() const { return ax[1][n[1] - 1]; }
 
-  // logtemp_from_valid_rho_eps_ye (eos_3p_tabulated3d.hxx:186-218)
-  ASX_DEV double logtemp_re(double rho, double &eps, double ye) const {
+  // logtemp_from_valid_rho_eps_ye (eos_3p_tabulated3d.hxx:186-218).  hint
+  // (optional) is the T cell of the previous solve for the same zone: when
+  // that cell, away from the table's ends, still brackets the value, the
+  // root is solved there with two node values instead of the end values and
+  // the bisection.  For a table whose eps grows with T the bracketing cell is
+  // unique, so the result is the same bits either way.
+  ASX_DEV double logtemp_re(double rho, double &eps, double ye, int *hint = nullptr) const {
     const double r = crho(rho);
     eps = fmax(eps + shift, eps_min + shift) - shift;
     const double lrho = log(r);
     const double leps = log(eps + shift);
+    if (hint && *hint >= 0 && *hint <= n[1] - 3) {
+      const int i0 = find(0, lrho), i2 = find(2, ye), h = *hint;
+      const int s1 = NV * n[0], s2 = NV * n[0] * n[1];
+      const double *p = tab + EPS + NV * i0 + s2 * i2;
+      const double r0 = ax[0][i0], r1 = ax[0][i0 + 1];
+      const double y0 = ax[2][i2], y1 = ax[2][i2 + 1];
+      const double eh = enode(p + s1 * h, s2, r0, r1, y0, y1, lrho, ye);
+      const double eh1 = enode(p + s1 * (h + 1), s2, r0, r1, y0, y1, lrho, ye);
+      // at h = 0 the bisection takes the end value, equal to E_0: strictly
+      // above it, no clamp applies
+      if (eh <= leps && leps < eh1 && (h > 0 || leps > eh)) {
+        const double t0 = ax[1][h], t1 = ax[1][h + 1];
+        return t0 + (leps - eh) * (t1 - t0) / (eh1 - eh);
+      }
+    }
     const double vmin = interp(EPS, lrho, xmin1(), ye);
     const double vmax = interp(EPS, lrho, xmax1(), ye);
     if (leps <= vmin) {
       eps = exp(vmin) - shift;
+      if (hint) *hint = -1;
       return xmin1();
     }
     if (leps >= vmax) {
       eps = exp(vmax) - shift;
+      if (hint) *hint = -1;
       return xmax1();
     }
-    return logtemp_nodes(lrho, leps, ye, vmin, vmax);
+    return logtemp_nodes(lrho, leps, ye, vmin, vmax, hint);
+  }
+  // E_j at the node row q of the T axis: log(eps + shift) interpolated in
+  // log rho, then in Ye (the reference's order)
+  ASX_DEV static double enode(const double *q, int s2, double r0, double r1, double y0,
+                              double y1, double lrho, double ye) {
+    const double a0 = lin(q[0], q[NV], r0, r1, lrho);
+    const double a1 = lin(q[s2], q[s2 + NV], r0, r1, lrho);
+    return lin(a0, a1, y0, y1, ye);
   }
   // The root in log T of interp(EPS, lrho, lt, ye) = leps for
   // vmin < leps < vmax (the interpolant's values at the ends of the T
@@ -119,7 +149,7 @@ struct TabEOS {
   // iteration on the same function stops within 2 eps |lt| + 1e-14 of this
   // root, after about twice as many dependent steps of twice the reads.
   ASX_DEV double logtemp_nodes(double lrho, double leps, double ye, double vmin,
-                               double vmax) const {
+                               double vmax, int *hint = nullptr) const {
     const int i0 = find(0, lrho), i2 = find(2, ye);
     const int s1 = NV * n[0], s2 = NV * n[0] * n[1];
     const double *p = tab + EPS + NV * i0 + s2 * i2;
@@ -129,10 +159,7 @@ struct TabEOS {
     double elo = vmin, ehi = vmax;
     while (hi - lo > 1) {
       const int mid = (lo + hi) >> 1;
-      const double *q = p + s1 * mid;
-      const double a0 = lin(q[0], q[NV], r0, r1, lrho);
-      const double a1 = lin(q[s2], q[s2 + NV], r0, r1, lrho);
-      const double em = lin(a0, a1, y0, y1, ye);
+      const double em = enode(p + s1 * mid, s2, r0, r1, y0, y1, lrho, ye);
       if (em <= leps) {
         lo = mid;
         elo = em;
@@ -141,6 +168,7 @@ struct TabEOS {
         ehi = em;
       }
     }
+    if (hint) *hint = lo;
     const double t0 = ax[1][lo], t1 = ax[1][hi];
     return t0 + (leps - elo) * (t1 - t0) / (ehi - elo);
   }
@@ -148,9 +176,9 @@ struct TabEOS {
   ASX_DEV double press_rt(double rho, double temp, double ye) const {
     return exp(interp(PRESS, log(rho), log(temp), ye));
   }
-  ASX_DEV double press_re(double rho, double &eps, double ye) const {
+  ASX_DEV double press_re(double rho, double &eps, double ye, int *hint = nullptr) const {
     const double lr = log(crho(rho));
-    const double lt = logtemp_re(rho, eps, ye);
+    const double lt = logtemp_re(rho, eps, ye, hint);
     return exp(interp(PRESS, lr, lt, ye));
   }
   ASX_DEV double eps_rt(double rho, double temp, double ye) const {
@@ -173,8 +201,8 @@ struct TabEOS {
   // where that clamp applied, the later solves may land an ulp away from
   // the table edge, which is the only difference
   ASX_DEV void state_re(double rho, double &eps, double ye, double &press, double &temp,
-                        double &kap) const {
-    const double lt = logtemp_re(rho, eps, ye);
+                        double &kap, int *hint = nullptr) const {
+    const double lt = logtemp_re(rho, eps, ye, hint);
     press = exp(interp(PRESS, log(crho(rho)), lt, ye));
     temp = exp(lt);
     kap = entropy_rt(rho, temp, ye);
