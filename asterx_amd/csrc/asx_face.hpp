@@ -28,19 +28,24 @@ struct Side {
 };
 constexpr int kSideVars = 9;
 
+// Points of a cell's stencil along D that method M reads: s[m] holds the
+// value at oc + (m - 2) sc.
+template <int M> struct StencilSpan {
+  static constexpr int m0 = M == ASX_RECON_GODUNOV ? 2
+                            : (M == ASX_RECON_MINMOD || M == ASX_RECON_MONOCENTRAL) ? 1
+                                                                                   : 0;
+};
+template <int M> ASX_DEV void cell_stencil(const double *p, int oc, int sc, double s[5]) {
+  constexpr int m0 = StencilSpan<M>::m0;
+#pragma unroll
+  for (int m = m0; m <= 4 - m0; ++m) s[m] = p[oc + (m - 2) * sc];
+}
 // Face values (lo = the cell's face at -D, up = at +D) of one grid function
 // from its 5-cell stencil along D.  Equal, value for value, to the two
 // sides that ReconX's reconstruct() yields for the faces of this cell.
 template <int M>
-ASX_DEV void cell_recon(const double *p, int oc, int sc, const double ps[5],
-                        const double vs[5], bool is_rho, const RParams &rp,
-                        double &lo, double &up) {
-  double s[5];
-  constexpr int m0 = M == ASX_RECON_GODUNOV ? 2
-                     : (M == ASX_RECON_MINMOD || M == ASX_RECON_MONOCENTRAL) ? 1
-                                                                            : 0;
-#pragma unroll
-  for (int m = m0; m <= 4 - m0; ++m) s[m] = p[oc + (m - 2) * sc];
+ASX_DEV void cell_recon_s(const double s[5], const double ps[5], const double vs[5],
+                          bool is_rho, const RParams &rp, double &lo, double &up) {
   if (M == ASX_RECON_GODUNOV) {
     lo = s[2];
     up = s[2];
@@ -62,16 +67,31 @@ ASX_DEV void cell_recon(const double *p, int oc, int sc, const double ps[5],
     lo = mp5_side(s[4], s[3], s[2], s[1], s[0], rp.mp5_alpha);
   }
 }
-ASX_DEV void cell_recon_lo(int m, const double *p, int oc, int sc, double &lo,
-                           double &up) {
+template <int M>
+ASX_DEV void cell_recon(const double *p, int oc, int sc, const double ps[5],
+                        const double vs[5], bool is_rho, const RParams &rp,
+                        double &lo, double &up) {
+  double s[5];
+  cell_stencil<M>(p, oc, sc, s);
+  cell_recon_s<M>(s, ps, vs, is_rho, rp, lo, up);
+}
+// The low-order method m (Godunov, minmod or monocentral) on a stencil that
+// holds at least s[1..3].
+ASX_DEV void cell_recon_lo_s(int m, const double s[5], double &lo, double &up) {
   const double z[5] = {0, 0, 0, 0, 0};
   const RParams rp{};
   if (m == ASX_RECON_MINMOD)
-    cell_recon<ASX_RECON_MINMOD>(p, oc, sc, z, z, false, rp, lo, up);
+    cell_recon_s<ASX_RECON_MINMOD>(s, z, z, false, rp, lo, up);
   else if (m == ASX_RECON_MONOCENTRAL)
-    cell_recon<ASX_RECON_MONOCENTRAL>(p, oc, sc, z, z, false, rp, lo, up);
+    cell_recon_s<ASX_RECON_MONOCENTRAL>(s, z, z, false, rp, lo, up);
   else
-    cell_recon<ASX_RECON_GODUNOV>(p, oc, sc, z, z, false, rp, lo, up);
+    cell_recon_s<ASX_RECON_GODUNOV>(s, z, z, false, rp, lo, up);
+}
+ASX_DEV void cell_recon_lo(int m, const double *p, int oc, int sc, double &lo,
+                           double &up) {
+  double s[5];
+  cell_stencil<ASX_RECON_MINMOD>(p, oc, sc, s);
+  cell_recon_lo_s(m, s, lo, up);
 }
 
 // Both sides of cell oc with the low-order fallback of fluxes.cxx:258-330:
@@ -90,40 +110,45 @@ ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Si
       vs[m] = P.g[ASX_VELX + D][oc + (m - 2) * sc];
     }
   }
-  auto rc = [&](int slot, bool is_rho, double &l, double &u) {
-    cell_recon<M>(P.g[slot], oc, sc, ps, vs, is_rho, A.rp, l, u);
-  };
   const int thermo = A.with_T ? ASX_TEMPERATURE : ASX_PRESS;
-  rc(ASX_RHO, true, lo.rho, up.rho);
-  rc(ASX_ENTROPY, false, lo.ent, up.ent);
-  rc(ASX_YE, false, lo.ye, up.ye);
-  rc(thermo, false, lo.th, up.th);
-  flo = (lo.rho <= 0.0) || (lo.ent <= 0.0) || (lo.ye <= 0.0) || (lo.th <= 0.0);
-  fup = (up.rho <= 0.0) || (up.ent <= 0.0) || (up.ye <= 0.0) || (up.th <= 0.0);
-  const bool any = flo || fup;
-  auto fix = [&](int slot, double &l, double &u) {
-    if (any) {
-      double a, b;
-      cell_recon_lo(A.lo_method, P.g[slot], oc, sc, a, b);
-      if (flo) l = a;
-      if (fup) u = b;
-    }
-  };
-  fix(ASX_RHO, lo.rho, up.rho);
-  fix(ASX_ENTROPY, lo.ent, up.ent);
-  fix(ASX_YE, lo.ye, up.ye);
-  fix(thermo, lo.th, up.th);
-  rc(ASX_BVECX + D1, false, lo.b1, up.b1);
-  fix(ASX_BVECX + D1, lo.b1, up.b1);
-  rc(ASX_BVECX + D2, false, lo.b2, up.b2);
-  fix(ASX_BVECX + D2, lo.b2, up.b2);
   const int v0 = A.rec_var == ASX_RECVAR_V   ? ASX_VELX
                  : A.rec_var == ASX_RECVAR_Z ? ASX_ZVEC_X
                                              : ASX_SVEC_X;
+  // every stencil is loaded before any is used: the fallback branches below
+  // would otherwise hold each variable's loads back until the previous
+  // variable is done.  The stencils cover s[1..3] (what the low-order
+  // fallback reads) for every method but Godunov.
+  const int slot[kSideVars] = {ASX_RHO, ASX_ENTROPY, ASX_YE, thermo, ASX_BVECX + D1,
+                               ASX_BVECX + D2, v0, v0 + 1, v0 + 2};
+  double st[kSideVars][5];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    rc(v0 + c, false, lo.v[c], up.v[c]);
-    fix(v0 + c, lo.v[c], up.v[c]);
+  for (int q = 0; q < kSideVars; ++q) cell_stencil<M>(P.g[slot[q]], oc, sc, st[q]);
+  double *const lv[kSideVars] = {&lo.rho, &lo.ent, &lo.ye, &lo.th, &lo.b1,
+                                 &lo.b2, &lo.v[0], &lo.v[1], &lo.v[2]};
+  double *const uv[kSideVars] = {&up.rho, &up.ent, &up.ye, &up.th, &up.b1,
+                                 &up.b2, &up.v[0], &up.v[1], &up.v[2]};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) cell_recon_s<M>(st[q], ps, vs, q == 0, A.rp, *lv[q], *uv[q]);
+  flo = (lo.rho <= 0.0) || (lo.ent <= 0.0) || (lo.ye <= 0.0) || (lo.th <= 0.0);
+  fup = (up.rho <= 0.0) || (up.ent <= 0.0) || (up.ye <= 0.0) || (up.th <= 0.0);
+  const bool any = flo || fup;
+  // a side whose rho, entropy, Ye or pressure/temperature is not positive
+  // takes the low-order method for every variable
+  auto fix = [&](int q) {
+    if (any) {
+      double a, b;
+      if (M == ASX_RECON_GODUNOV) cell_recon_lo(A.lo_method, P.g[slot[q]], oc, sc, a, b);
+      else cell_recon_lo_s(A.lo_method, st[q], a, b);
+      if (flo) *lv[q] = a;
+      if (fup) *uv[q] = b;
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < 4; ++q) fix(q);
+#pragma unroll
+  for (int q = 4; q < kSideVars; ++q) {
+    cell_recon_s<M>(st[q], ps, vs, false, A.rp, *lv[q], *uv[q]);
+    fix(q);
   }
 }
 
