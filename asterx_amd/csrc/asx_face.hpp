@@ -158,25 +158,45 @@ ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Si
 // potential with first differences (ComputeStaggeredB<D>,
 // AsterX/src/computeBfromA.cxx:32-55).  One expression for the curl kernel
 // and for the face kernels that take it inline, so both give the same value.
-template <int D> ASX_DEV double bstag_value(const KPatch &P, int i, int j, int k) {
-  const double *idx = P.idx;
-  const double *Ax = P.g[ASX_AVEC_X], *Ay = P.g[ASX_AVEC_Y], *Az = P.g[ASX_AVEC_Z];
-  if (D == 0) {
-    const int oz = off(P, 3, i, j, k), oy = off(P, 5, i, j, k);
-    return idx[1] * (Az[oz + P.sj[3]] - Az[oz]) - idx[2] * (Ay[oy + P.sk[5]] - Ay[oy]);
-  } else if (D == 1) {
-    const int ox = off(P, 6, i, j, k), oz = off(P, 3, i, j, k);
-    return idx[2] * (Ax[ox + P.sk[6]] - Ax[ox]) - idx[0] * (Az[oz + 1] - Az[oz]);
-  } else {
-    const int oy = off(P, 5, i, j, k), ox = off(P, 6, i, j, k);
-    return idx[0] * (Ay[oy + 1] - Ay[oy]) - idx[1] * (Ax[ox + P.sj[6]] - Ax[ox]);
-  }
+template <int D>
+ASX_DEV double curl_of(const double *idx, double b0, double b1, double b2, double b3) {
+  constexpr int J = (D + 1) % 3, K = (D + 2) % 3;
+  // (A_K at +J minus at 0) / dx_J - (A_J at +K minus at 0) / dx_K
+  return idx[J] * (b1 - b0) - idx[K] * (b3 - b2);
 }
+template <int D> ASX_DEV double bstag_value(const KPatch &P, int i, int j, int k) {
+  constexpr int J = (D + 1) % 3, K = (D + 2) % 3;
+  constexpr int CK = 7 ^ (1 << K), CJ = 7 ^ (1 << J); // centerings of A_K, A_J
+  const double *AK = P.g[ASX_AVEC_X + K], *AJ = P.g[ASX_AVEC_X + J];
+  const int ok = off(P, CK, i, j, k), oj = off(P, CJ, i, j, k);
+  return curl_of<D>(P.idx, AK[ok], AK[ok + step(P, CK, J)], AJ[oj], AJ[oj + step(P, CJ, K)]);
+}
+// The normal field on the D-face at (i, j, k) as loaded, without a branch
+// (so that its loads issue with the reconstruction's): the curl's four
+// points of A when the field is taken from A, else the stored field (read
+// four times).  Every point exists wherever cell (i, j, k) does.
+template <int D> struct FaceBn {
+  double b[4];
+  ASX_DEV FaceBn(const KPatch &P, const FluxArgs &A, int i, int j, int k) {
+    constexpr int J = (D + 1) % 3, K = (D + 2) % 3;
+    constexpr int CK = 7 ^ (1 << K), CJ = 7 ^ (1 << J);
+    const bool c = A.bstag_from_A;
+    const double *AK = c ? P.g[ASX_AVEC_X + K] : P.g[ASX_DBX_STAG + D];
+    const double *AJ = c ? P.g[ASX_AVEC_X + J] : AK;
+    const int ok = c ? off(P, CK, i, j, k) : off(P, 1 << D, i, j, k);
+    const int oj = c ? off(P, CJ, i, j, k) : ok;
+    const int sJ = c ? step(P, CK, J) : 0, sK = c ? step(P, CJ, K) : 0;
+    b[0] = AK[ok], b[1] = AK[ok + sJ], b[2] = AJ[oj], b[3] = AJ[oj + sK];
+  }
+  ASX_DEV double value(const KPatch &P, const FluxArgs &A) const {
+    return A.bstag_from_A ? curl_of<D>(P.idx, b[0], b[1], b[2], b[3]) : b[0];
+  }
+};
 
 // The metric at face (i, j, k) (v2f averages of the vertex lapse, shift and
-// 3-metric) and the staggered normal field there.  The face tile loads them
-// before it reconstructs, so that their latency hides behind the
-// reconstruction.
+// 3-metric) and the staggered normal field there.  The face tile loads the
+// metric before it reconstructs and the normal field with the
+// reconstruction's stencils (FaceBn).
 struct FaceGeom {
   double alp, beta[3];
   Sym3 g;
@@ -196,8 +216,7 @@ ASX_DEV void face_geom(const KPatch &P, const FluxArgs &A, int i, int j, int k, 
   G.g.yy = avg_v2f<D>(P.g[ASX_GYY], ov, svj, svk);
   G.g.yz = avg_v2f<D>(P.g[ASX_GYZ], ov, svj, svk);
   G.g.zz = avg_v2f<D>(P.g[ASX_GZZ], ov, svj, svk);
-  G.bn = A.bstag_from_A ? bstag_value<D>(P, i, j, k)
-                        : P.g[ASX_DBX_STAG + D][off(P, 1 << D, i, j, k)];
+  G.bn = 0.0; // set by the face tile from FaceBn
 }
 
 template <int D, bool TAB = false>
@@ -517,8 +536,11 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
   if (flux) face_geom<D>(P, A, i, j, k, G);
   Side lo, up;
   bool flo = false, fup = false;
+  double bn = 0.0;
   if (need) {
+    const FaceBn<D> fb(P, A, i, j, k);
     recon_cell<D, M>(P, A, off(P, 0, i, j, k), lo, up, flo, fup);
+    bn = fb.value(P, A);
     sh[0][t] = up.rho;
     sh[1][t] = up.ent;
     sh[2][t] = up.ye;
@@ -543,6 +565,7 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     L.v[0] = sh[6][tl];
     L.v[1] = sh[7][tl];
     L.v[2] = sh[8][tl];
+    G.bn = bn;
     face_flux<D, TAB>(P, A, i, j, k, G, L, lo, shf[tl] != 0, flo);
   }
 }
