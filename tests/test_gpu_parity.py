@@ -91,6 +91,25 @@ def test_fluxes(recon, flux):
     _cmp(h, d, abi.AUX, halo=2)
 
 
+@pytest.mark.parametrize("recon", ["wenoz", "ppm"])
+@pytest.mark.parametrize("lo", ["Godunov", "minmod", "monocentral"])
+def test_fluxes_loworder(recon, lo):
+    """Every low-order method of the per-face-side fallback
+    (ReconX/param.ccl loworder_method; fluxes.cxx:258-330), taken from the
+    stencil the high-order reconstruction already loaded."""
+    _torch()
+    h, d = _pair(zero_entropy_block=True)
+    par = abi.params(reconstruction_method=recon, flux_type="HLLE",
+                     gl_gamma=5.0 / 3.0, loworder_method=lo)
+    g = _gpu()
+    cnt = abi.asx_counters()
+    g._call("fluxes", C.byref(d.struct()), C.byref(par), 1, C.byref(cnt), None)
+    _sync()
+    oracle.lib().orc_fluxes(C.byref(h.struct()), C.byref(par), 1)
+    assert cnt.lo_faces > 0
+    _cmp(h, d, abi.FLUX + abi.UCT, halo=1)
+
+
 @pytest.mark.parametrize("rec_var", ["z_vec", "s_vec"])
 def test_fluxes_recvar(rec_var):
     _torch()
