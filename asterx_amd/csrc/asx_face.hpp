@@ -97,9 +97,14 @@ ASX_DEV void cell_recon_lo(int m, const double *p, int oc, int sc, double &lo,
 // Both sides of cell oc with the low-order fallback of fluxes.cxx:258-330:
 // a side whose rho, entropy, Ye or pressure/temperature is not positive is
 // reconstructed again, every variable, with the low-order method.
-template <int D, int M>
+struct NoHook {
+  ASX_DEV void operator()() const {}
+};
+// `mid` runs once every stencil load has been issued, before any is used
+// (the face tile writes its staged metric to LDS there).
+template <int D, int M, class Mid = NoHook>
 ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Side &up,
-                        bool &flo, bool &fup) {
+                        bool &flo, bool &fup, Mid &&mid = NoHook{}) {
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
   const int sc = step(P, 0, D);
   double ps[5] = {0, 0, 0, 0, 0}, vs[5] = {0, 0, 0, 0, 0};
@@ -123,6 +128,7 @@ ASX_DEV void recon_cell(const KPatch &P, const FluxArgs &A, int oc, Side &lo, Si
   double st[kSideVars][5];
 #pragma unroll
   for (int q = 0; q < kSideVars; ++q) cell_stencil<M>(P.g[slot[q]], oc, sc, st[q]);
+  mid();
   double *const lv[kSideVars] = {&lo.rho, &lo.ent, &lo.ye, &lo.th, &lo.b1,
                                  &lo.b2, &lo.v[0], &lo.v[1], &lo.v[2]};
   double *const uv[kSideVars] = {&up.rho, &up.ent, &up.ye, &up.th, &up.b1,
@@ -498,6 +504,17 @@ template <> struct FaceTileT<1, 3> { using T = Tile3<8, 16, 2>; };
 template <> struct FaceTileT<1, 4> { using T = Tile3<8, 32, 1>; };
 template <> struct FaceTileT<2, 3> { using T = Tile3<8, 2, 16>; };
 template <> struct FaceTileT<2, 4> { using T = Tile3<8, 4, 8>; };
+// variants 5-7: the tile's vertex metric staged in LDS (MetricBox)
+template <> struct FaceTileT<0, 5> { using T = Tile3<16, 4, 4>; };
+template <> struct FaceTileT<0, 6> { using T = Tile3<16, 8, 2>; };
+template <> struct FaceTileT<0, 7> { using T = Tile3<8, 8, 4>; };
+template <> struct FaceTileT<1, 5> { using T = Tile3<16, 8, 2>; };
+template <> struct FaceTileT<1, 6> { using T = Tile3<8, 16, 2>; };
+template <> struct FaceTileT<1, 7> { using T = Tile3<8, 8, 4>; };
+template <> struct FaceTileT<2, 5> { using T = Tile3<16, 2, 8>; };
+template <> struct FaceTileT<2, 6> { using T = Tile3<8, 2, 16>; };
+template <> struct FaceTileT<2, 7> { using T = Tile3<8, 4, 8>; };
+constexpr int kFaceStagedVariant = 5; // variants from here on stage the metric
 template <int D, int V> using FaceTile = typename FaceTileT<D, V>::T;
 
 // LDS of a face tile: the high side of every cell, handed to the next cell
@@ -507,12 +524,34 @@ struct FaceLDS {
   int shf[256];
 };
 
+// The vertex metric a face tile reads, staged in LDS: the ten fields (lapse,
+// shift, 3-metric) on the box of vertices the tile's faces average over --
+// along D the low-face vertex of every cell but the first, across it the
+// tile's cells plus one.  Loaded cooperatively (x fastest, about 16 loads
+// per work-item instead of 40) at the start of the tile, written to LDS
+// after the reconstruction and summed after the hand-over barrier, so the
+// metric's latency overlaps the reconstruction's.  Only for tile shapes
+// (variants kFaceStagedVariant and up), whose box leaves room for three
+// workgroups per CU (<= 33 KB beside the hand-over); the others load the
+// metric per face.
+template <int D, int V> struct MetricBox {
+  using T = FaceTile<D, V>;
+  static constexpr int BX = D == 0 ? T::X - 1 : T::X + 1;
+  static constexpr int BY = D == 1 ? T::Y - 1 : T::Y + 1;
+  static constexpr int BZ = D == 2 ? T::Z - 1 : T::Z + 1;
+  static constexpr int VB = BX * BY * BZ, N = 10 * VB;
+  static constexpr int NV = (VB + 255) / 256; // vertices per work-item and field
+  static constexpr bool ok = V >= kFaceStagedVariant;
+  static_assert(!ok || N * 8 <= 33 * 1024, "staged metric box too large for 3 workgroups per CU");
+};
+
 // One tile of a face box (lo, cnt), tile index b of a grid of ntx x nty x ntz
 // tiles (ordered x, y, z).  Every work-item of the workgroup calls it; the
 // caller owns the LDS (one FaceLDS serves tiles of every direction).
 template <int D, int M, int V, bool TAB = false>
 ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], const int cnt[3],
-                       int b, int ntx, int nty, FaceLDS &S) {
+                       int b, int ntx, int nty, FaceLDS &S, double *MS = nullptr) {
+  // MS: LDS for MetricBox<D, V>::N doubles, or null (metric loaded per face)
   using T = FaceTile<D, V>;
   static_assert(T::X * T::Y * T::Z == 256, "face tile");
   constexpr int TD = D == 0 ? T::X : (D == 1 ? T::Y : T::Z);
@@ -532,14 +571,55 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
   const bool need = fx < cnt[0] && fy < cnt[1] && fz < cnt[2];
   const int i = lo3[0] + fx, j = lo3[1] + fy, k = lo3[2] + fz;
   const bool flux = need && td >= 1 && fd >= 0;
+  using MB = MetricBox<D, V>;
+  const bool staged = MB::ok && MS != nullptr;
+  // staged: this work-item's share of the tile's vertex metric (vertices
+  // t and t + 256 of the box, every field), issued before the
+  // reconstruction's loads; indices clamped to the box and the arrays, so no
+  // branch (elements no face reads may hold any in-range value)
+  constexpr int NV = MB::ok ? MB::NV : 1;
+  double mv[10][NV];
+  int me[NV];
+  if (staged) {
+    const int org[3] = {lo3[0] + bx * (D == 0 ? TD - 1 : T::X),
+                        lo3[1] + by * (D == 1 ? TD - 1 : T::Y),
+                        lo3[2] + bz * (D == 2 ? TD - 1 : T::Z)};
+    const int ex = P.n[0] + 2 * P.ng, ey = P.n[1] + 2 * P.ng, ez = P.n[2] + 2 * P.ng;
+    int mo[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const int e = min(q * 256 + t, MB::VB - 1);
+      const int vx = min(org[0] + e % MB::BX, ex);
+      const int vy = min(org[1] + (e / MB::BX) % MB::BY, ey);
+      const int vz = min(org[2] + e / (MB::BX * MB::BY), ez);
+      me[q] = e;
+      mo[q] = vx + vy * P.sj[7] + vz * P.sk[7];
+    }
+#pragma unroll
+    for (int f = 0; f < 10; ++f)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) mv[f][q] = P.g[ASX_ALP + f][mo[q]];
+  }
   FaceGeom G;
-  if (flux) face_geom<D>(P, A, i, j, k, G);
+  if (flux && !staged) face_geom<D>(P, A, i, j, k, G);
   Side lo, up;
   bool flo = false, fup = false;
   double bn = 0.0;
+  // the staged values go to LDS once the stencils' loads are out (their
+  // registers are then free for the reconstruction)
+  auto put = [&] {
+    if (staged) {
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int f = 0; f < 10; ++f)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) MS[f * MB::VB + me[q]] = mv[f][q];
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  };
   if (need) {
     const FaceBn<D> fb(P, A, i, j, k);
-    recon_cell<D, M>(P, A, off(P, 0, i, j, k), lo, up, flo, fup);
+    recon_cell<D, M>(P, A, off(P, 0, i, j, k), lo, up, flo, fup, put);
     bn = fb.value(P, A);
     sh[0][t] = up.rho;
     sh[1][t] = up.ent;
@@ -551,9 +631,32 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     sh[7][t] = up.v[1];
     sh[8][t] = up.v[2];
     shf[t] = fup;
+  } else {
+    put();
   }
   __syncthreads();
   if (flux) {
+    if (staged) {
+      // avg_v2f of each field from LDS, in its order of summation
+      constexpr int J = (D + 1) % 3, K = (D + 2) % 3;
+      constexpr int s1 = J == 0 ? 1 : (J == 1 ? MB::BX : MB::BX * MB::BY);
+      constexpr int s2 = K == 0 ? 1 : (K == 1 ? MB::BX : MB::BX * MB::BY);
+      const int c0 = ((tz - (D == 2)) * MB::BY + (ty - (D == 1))) * MB::BX + (tx - (D == 0));
+      double v[10];
+#pragma unroll
+      for (int f = 0; f < 10; ++f) {
+        const double *m = MS + f * MB::VB + c0;
+        double s = 0.0;
+        s += m[0];
+        s += m[s1];
+        s += m[s2];
+        s += m[s1 + s2];
+        v[f] = s / 4.0;
+      }
+      G.alp = v[0];
+      G.beta[0] = v[1], G.beta[1] = v[2], G.beta[2] = v[3];
+      G.g = Sym3{v[4], v[5], v[6], v[7], v[8], v[9]};
+    }
     const int tl = t - SD;
     Side L;
     L.rho = sh[0][tl];

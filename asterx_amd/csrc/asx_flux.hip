@@ -13,11 +13,14 @@ __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(TAB ? ASX_FACE_WAVES_TAB : ASX_FACE_WAVES)))
 void k_face(KPatch P, FluxArgs A, int ntx, int nty, int ntz) {
   __shared__ FaceLDS S;
+  using MB = MetricBox<D, V>;
+  __shared__ double MS[MB::ok ? MB::N : 1];
   // workgroup u takes face tile u and, when the launch carries the vertex
   // auxiliaries, their 256-vertex chunk u: the aux loads and stores then
   // overlap the face arithmetic instead of running as a launch of their own
   const int u = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (u < ntx * nty * ntz) face_tile<D, M, V, TAB>(P, A, A.lo, A.cnt, u, ntx, nty, S);
+  if (u < ntx * nty * ntz)
+    face_tile<D, M, V, TAB>(P, A, A.lo, A.cnt, u, ntx, nty, S, MB::ok ? MS : nullptr);
   if (A.with_aux) {
     const int64_t t = (int64_t)u * 256 + threadIdx.x;
     if (t < A.aux.total()) {
@@ -142,6 +145,9 @@ static hipError_t launch_flux_d(const KPatch &P, const FluxArgs &A, int method, 
   case 2: return launch_flux_dv<D, 2>(P, A, method, s, tab);
   case 3: return launch_flux_dv<D, 3>(P, A, method, s, tab);
   case 4: return launch_flux_dv<D, 4>(P, A, method, s, tab);
+  case 5: return launch_flux_dv<D, 5>(P, A, method, s, tab);
+  case 6: return launch_flux_dv<D, 6>(P, A, method, s, tab);
+  case 7: return launch_flux_dv<D, 7>(P, A, method, s, tab);
   default: return launch_flux_dv<D, 0>(P, A, method, s, tab);
   }
 }
@@ -190,7 +196,10 @@ void k_face_batch(const KPatch *__restrict__ tab, FluxArgs A, const BatchBox *__
   const BatchRef r = map[xcd_tile((int)blockIdx.x, nmap)];
   const BatchBox &bb = box[r.pid];
   __shared__ FaceLDS S;
-  face_tile<D, M, V>(tab[r.pid], A, bb.lo, bb.cnt, r.chunk, bb.nt[0], bb.nt[1], S);
+  using MB = MetricBox<D, V>;
+  __shared__ double MS[MB::ok ? MB::N : 1];
+  face_tile<D, M, V>(tab[r.pid], A, bb.lo, bb.cnt, r.chunk, bb.nt[0], bb.nt[1], S,
+                     MB::ok ? MS : nullptr);
 }
 
 __global__ __launch_bounds__(256) void k_aux_batch(const KPatch *__restrict__ tab,
@@ -235,6 +244,9 @@ static void launch_face_batch_d(const KPatch *tab, const FluxArgs &A, const Batc
   case 2: launch_face_batch_dv<D, 2>(tab, A, box, map, nmap, method, s); break;
   case 3: launch_face_batch_dv<D, 3>(tab, A, box, map, nmap, method, s); break;
   case 4: launch_face_batch_dv<D, 4>(tab, A, box, map, nmap, method, s); break;
+  case 5: launch_face_batch_dv<D, 5>(tab, A, box, map, nmap, method, s); break;
+  case 6: launch_face_batch_dv<D, 6>(tab, A, box, map, nmap, method, s); break;
+  case 7: launch_face_batch_dv<D, 7>(tab, A, box, map, nmap, method, s); break;
   default: launch_face_batch_dv<D, 0>(tab, A, box, map, nmap, method, s); break;
   }
 }
@@ -251,6 +263,9 @@ template <int D> static void face_ext_d(int variant, int ex[3]) {
   case 2: face_ext<D, 2>(ex); break;
   case 3: face_ext<D, 3>(ex); break;
   case 4: face_ext<D, 4>(ex); break;
+  case 5: face_ext<D, 5>(ex); break;
+  case 6: face_ext<D, 6>(ex); break;
+  case 7: face_ext<D, 7>(ex); break;
   default: face_ext<D, 0>(ex); break;
   }
 }

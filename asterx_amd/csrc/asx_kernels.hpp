@@ -91,7 +91,7 @@ template <int TX, int TY, int TZ, int YS> struct TileGridBand {
 // Launch-shape variants of the tunable kernels (selected per call; see
 // asx_set_variant / asx_autotune_substep in the C ABI).
 constexpr int kUpdateVariants = 7;
-constexpr int kFaceVariants = 5;
+constexpr int kFaceVariants = 8; // 5-7 stage the tile's vertex metric in LDS
 constexpr int kTmunuVariants = 9;
 
 // Batched launches over many patches: the workgroups of one launch are
