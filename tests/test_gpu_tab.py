@@ -281,6 +281,37 @@ def test_fluxes_on_table(recon, flux):
     assert not bad, bad
 
 
+def test_fluxes_on_table_every_face_variant():
+    """The table's face kernels (their own register budget) under every
+    launch-shape variant -- including those that stage the tile's vertex
+    metric in LDS -- give the same bits as variant 0."""
+    _torch()
+    dtab, htab = _tables()
+    kw = dict(reconstruction_method="wenoz", flux_type="HLLE", reconstruct_with_temperature=1,
+              loworder_method="minmod")
+    from asterx_amd import lib
+    L = lib.lib()
+    kcs = [abi.KCLASS[n] for n in ("face_x", "face_y", "face_z")]
+    ref = None
+    try:
+        for v in range(L.asx_variant_count(kcs[0])):
+            for kc in kcs:
+                abi.check("asx_set_variant", L.asx_set_variant(kc, v))
+            _, d, _ = _patch_pair(_par(htab), zero_entropy_block=True)
+            lib.backend()._call("fluxes", C.byref(d.struct()), C.byref(_par(dtab, **kw)), 1,
+                                None, None)
+            _torch().cuda.synchronize()
+            got = {n: d.numpy(n).copy() for n in abi.FLUX}
+            if ref is None:
+                ref = got
+            else:
+                for n in abi.FLUX:
+                    assert np.array_equal(got[n].view(np.uint64), ref[n].view(np.uint64)), (v, n)
+    finally:
+        for kc in kcs:
+            L.asx_set_variant(kc, 0)
+
+
 def test_fluxes_on_table_refuse_pressure_reconstruction():
     _torch()
     dtab, _ = _tables()
