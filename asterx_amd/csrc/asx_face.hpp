@@ -539,8 +539,13 @@ template <int D, int V> struct MetricBox {
   static constexpr int BX = D == 0 ? T::X - 1 : T::X + 1;
   static constexpr int BY = D == 1 ? T::Y - 1 : T::Y + 1;
   static constexpr int BZ = D == 2 ? T::Z - 1 : T::Z + 1;
-  static constexpr int VB = BX * BY * BZ, N = 10 * VB;
-  static constexpr int NV = (VB + 255) / 256; // vertices per work-item and field
+  // rows padded to an even length when that still fits: each work-item then
+  // loads an x-adjacent pair of vertices per field (one 16-byte load)
+  static constexpr int BXE = BX + (BX & 1);
+  static constexpr bool PAIR = 10 * BXE * BY * BZ * 8 <= 33 * 1024;
+  static constexpr int BXS = PAIR ? BXE : BX; // LDS row stride
+  static constexpr int VB = BXS * BY * BZ, N = 10 * VB;
+  static constexpr int NV = PAIR ? (VB / 2 + 255) / 256 : (VB + 255) / 256;
   static constexpr bool ok = V >= kFaceStagedVariant;
   static_assert(!ok || N * 8 <= 33 * 1024, "staged metric box too large for 3 workgroups per CU");
 };
@@ -578,8 +583,10 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
   // reconstruction's loads; indices clamped to the box and the arrays, so no
   // branch (elements no face reads may hold any in-range value)
   constexpr int NV = MB::ok ? MB::NV : 1;
-  double mv[10][NV];
+  constexpr int NE = MB::ok && MB::PAIR ? 2 : 1; // vertices per load
+  double mv[10][NV][NE];
   int me[NV];
+  bool hi[NV]; // a pair starting at the last vertex: loaded one vertex lower
   if (staged) {
     const int org[3] = {lo3[0] + bx * (D == 0 ? TD - 1 : T::X),
                         lo3[1] + by * (D == 1 ? TD - 1 : T::Y),
@@ -588,17 +595,25 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     int mo[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) {
-      const int e = min(q * 256 + t, MB::VB - 1);
-      const int vx = min(org[0] + e % MB::BX, ex);
-      const int vy = min(org[1] + (e / MB::BX) % MB::BY, ey);
-      const int vz = min(org[2] + e / (MB::BX * MB::BY), ez);
+      // first LDS element of this load (x fastest; pairs start at even x)
+      const int e = NE * min(q * 256 + t, MB::VB / NE - 1);
+      const int x = org[0] + e % MB::BXS;
+      const int vy = min(org[1] + (e / MB::BXS) % MB::BY, ey);
+      const int vz = min(org[2] + e / (MB::BXS * MB::BY), ez);
+      const int vx = min(x, ex - (NE - 1));
+      hi[q] = NE == 2 && x > vx;
       me[q] = e;
       mo[q] = vx + vy * P.sj[7] + vz * P.sk[7];
     }
 #pragma unroll
     for (int f = 0; f < 10; ++f)
 #pragma unroll
-      for (int q = 0; q < NV; ++q) mv[f][q] = P.g[ASX_ALP + f][mo[q]];
+      for (int q = 0; q < NV; ++q) {
+        const double *g = P.g[ASX_ALP + f] + mo[q];
+        // an x-adjacent pair is one 16-byte load
+#pragma unroll
+        for (int h = 0; h < NE; ++h) mv[f][q][h] = g[h];
+      }
   }
   FaceGeom G;
   if (flux && !staged) face_geom<D>(P, A, i, j, k, G);
@@ -613,7 +628,13 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
 #pragma unroll
       for (int f = 0; f < 10; ++f)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) MS[f * MB::VB + me[q]] = mv[f][q];
+        for (int q = 0; q < NV; ++q) {
+          // a pair moved down one vertex: its second value is the
+          // requested first one (the second slot is then past the array
+          // and read by no face)
+          MS[f * MB::VB + me[q]] = hi[q] ? mv[f][q][NE - 1] : mv[f][q][0];
+          if (NE == 2) MS[f * MB::VB + me[q] + 1] = mv[f][q][NE - 1];
+        }
       __builtin_amdgcn_sched_barrier(0);
     }
   };
@@ -639,9 +660,9 @@ ASX_DEV void face_tile(const KPatch &P, const FluxArgs &A, const int lo3[3], con
     if (staged) {
       // avg_v2f of each field from LDS, in its order of summation
       constexpr int J = (D + 1) % 3, K = (D + 2) % 3;
-      constexpr int s1 = J == 0 ? 1 : (J == 1 ? MB::BX : MB::BX * MB::BY);
-      constexpr int s2 = K == 0 ? 1 : (K == 1 ? MB::BX : MB::BX * MB::BY);
-      const int c0 = ((tz - (D == 2)) * MB::BY + (ty - (D == 1))) * MB::BX + (tx - (D == 0));
+      constexpr int s1 = J == 0 ? 1 : (J == 1 ? MB::BXS : MB::BXS * MB::BY);
+      constexpr int s2 = K == 0 ? 1 : (K == 1 ? MB::BXS : MB::BXS * MB::BY);
+      const int c0 = ((tz - (D == 2)) * MB::BY + (ty - (D == 1))) * MB::BXS + (tx - (D == 0));
       double v[10];
 #pragma unroll
       for (int f = 0; f < 10; ++f) {
