@@ -895,9 +895,13 @@ def tune_variants(be, p, st, par, dt, sh):
     from asterx_amd.patch import StateBuf
     scratch = StateBuf(p)
     ps, u0s, outs = p.struct(), state_struct(p, st.bufs[0]), scratch.struct()
+    # one untimed substep first (clocks and caches as in the timed run), then
+    # ten substeps per variant: the faces' variants differ by a few percent
+    abi.check("asx_substep", be.lib.asx_substep(
+        C.byref(ps), C.byref(u0s), C.byref(outs), 0.75, 0.25, 0.25 * dt, C.byref(par), sh))
     abi.check("asx_autotune_substep", be.lib.asx_autotune_substep(
         C.byref(ps), C.byref(u0s), C.byref(outs), 0.75, 0.25, 0.25 * dt,
-        C.byref(par), 3, sh))
+        C.byref(par), 10, sh))
     torch.cuda.synchronize()
     del scratch
     variants = {}
