@@ -261,7 +261,7 @@ def bench_batched(args, src, par, dt, L, sh, stream):
     torch.cuda.synchronize()
     plan = batch.Plan(L, patches, par, dt)
     zones, dbytes = plan.info()
-    plan.autotune(reps=2, stream=sh)
+    plan.autotune(reps=4, stream=sh)
     variants = plan.variants()
     plan.step(1, graph=True, stream=sh)  # records the graph
     torch.cuda.synchronize()
@@ -452,6 +452,8 @@ def bench_tabulated(args, L, sh, stream):
     zn = n[0] * n[1] * n[2]
     dt = 0.2 * dx[0]
     st = Stepper(p, par, be, dt, sh)
+    # launch shapes for the table's face kernels on this patch
+    variants = tune_variants(be, p, st, par, dt, sh) if args.tune == "on" else {}
     for _ in range(args.warmup):
         st.substep()
     torch.cuda.synchronize()
@@ -481,7 +483,8 @@ def bench_tabulated(args, L, sh, stream):
                              "Ye), Palenzuela on T" % args.tab_n,
                  "zones": zn, "steps": K, "ms_per_step": round(1e3 * wall / K, 4),
                  "value": round(zn * K / wall, 1), "unit": "zone-updates/s",
-                 "kernels_ms_per_step": kt, "c2p_flags": list(cnt.c2p_flags)}
+                 "kernels_ms_per_step": kt, "c2p_flags": list(cnt.c2p_flags),
+                 "variants": variants}
     del st, p
     torch.cuda.synchronize()
     torch.cuda.empty_cache()
@@ -1073,6 +1076,9 @@ def bench_plain64(args, sh, stream, be):
                 host.alloc(s)
     dt = 0.25 * dx
     st = Stepper(p, par, be, dt, sh)
+    # launch shapes for this patch size (the ones tuned on the larger
+    # patches are not the best at 64^3)
+    variants = tune_variants(be, p, st, par, dt, sh) if args.tune == "on" else {}
     K = 30
     ms_eager, kt, first = time_substeps(st, be.lib, K, 6)
     ms_graph, launch = graph_timing(st, stream, K)
@@ -1084,7 +1090,8 @@ def bench_plain64(args, sh, stream, be):
            "unit": "zone-updates/s", "ms_per_step": round(ms_step, 4),
            "eager_ms_per_step": round(ms_eager, 4), "launch": launch,
            "roofline": substep_roofline(par, zones, K, first, kt, ms_step, "plain64"),
-           "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()}}
+           "kernels_ms_per_step": {k: round(v[1] / K, 4) for k, v in kt.items()},
+           "variants": variants}
     for name in abi.CONS:
         if not bool(torch.isfinite(p.arr[name][p.interior(name)]).all()):
             raise RuntimeError("non-finite %s in the 64^3 run" % name)
