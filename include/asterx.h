@@ -350,7 +350,9 @@ asx_status asx_poison_check(const asx_patch *patch, int32_t slot, const int32_t 
 /* Launch-shape variants and autotuning (no counterpart in the reference).
  * The face kernels (ASX_KC_FACE_*) and the fused update (ASX_KC_UPDATE) are
  * built in several workgroup tile shapes; results do not depend on the
- * variant.  ASX_KC_MLP has two forms of asx_c2p_batch_mlp (network fused
+ * variant.  Face variants 5-7 also stage the tile's vertex metric in LDS
+ * (up to 52 KB of LDS per workgroup with the face hand-over, three
+ * workgroups per CU).  ASX_KC_MLP has two forms of asx_c2p_batch_mlp (network fused
  * into the solver kernel, or a separate forward pass), whose guesses differ
  * by the fp32 summation order only.  The choice is per device (default 0).  asx_autotune_substep
  * runs asx_substep with the given arguments once untimed and `reps` times
